@@ -1,0 +1,281 @@
+"""ctypes binding of the CPU oracle (oracle/*.c) — TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg,
+always as the checker; the product package never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+PIN, ANGLE, GRIP = 0, 1, 2
+ARM_REACH = 24 + 7 * 13.5
+MAX_ARMS = 8
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        d, i, u32 = C.c_double, C.c_int, C.c_uint32
+        P = C.c_void_p
+        dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int)
+        sig = {
+            "or_mb32": (u32, [u32]),
+            "or_mulberry32_stream": (None, [u32, i, dp]),
+            "or_ctl_u": (d, [u32, u32, u32, u32]),
+            "or_js_hypot": (d, [d, d]),
+            "or_make_map": (i, [d, d, i, i, d, d, u32, dp, i]),
+            "or_sdf": (i, [dp, i, d, d, dp]),
+            "or_world_new": (P, [i, i, d]),
+            "or_world_free": (None, [P]),
+            "or_world_set_map": (None, [P, d, d, dp, i]),
+            "or_world_set_limits": (None, [P, d, d, d, d]),
+            "or_spawn_kraken": (i, [P, d, d, d, i, i, i, d]),
+            "or_find_spawn": (i, [P, d, C.POINTER(u32), dp]),
+            "or_rng_next": (d, [C.POINTER(u32)]),
+            "or_ensure_colored": (None, [P]),
+            "or_counts": (i, [P, ip]),
+            "or_color_info": (None, [P, C.POINTER(u32), C.POINTER(u32)]),
+            "or_get_bodies": (None, [P, dp, dp, dp, dp, dp, dp, dp, dp]),
+            "or_set_bodies": (None, [P, dp, dp, dp, dp, dp, dp]),
+            "or_get_pred": (None, [P, dp, dp, dp]),
+            "or_set_pred": (None, [P, dp, dp, dp]),
+            "or_get_cons": (None, [P, ip, dp]),
+            "or_set_cons_dynamic": (None, [P, ip, dp, dp, dp, dp, dp]),
+            "or_get_points": (None, [P, ip, dp]),
+            "or_creature_info": (i, [P, i, ip]),
+            "or_arm_tip": (i, [P, i, i]),
+            "or_arm_grip_pos": (i, [P, i, i]),
+            "or_get_cd": (None, [P, dp]),
+            "or_set_cd": (None, [P, dp]),
+            "or_attach_grip": (None, [P, i, i, d]),
+            "or_apply_control": (None, [P, i, d, dp, dp, dp]),
+            "or_apply_law": (None, [P, u32, u32]),
+            "or_step": (None, [P]),
+            "or_substep": (None, [P]),
+            "or_solve_collisions": (None, [P]),
+            "or_get_stats": (None, [P, dp]),
+            "or_f32ify": (None, [P]),
+            "or_events": (i, [P, ip, i]),
+            "or_clear_events": (None, [P]),
+            "or_set_step_counter": (None, [P, i]),
+            "or_bench_threads": (d, [P, i, i, u32]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def mulberry32(seed, n):
+    out = np.zeros(n)
+    lib().or_mulberry32_stream(seed & 0xFFFFFFFF, n, _dp(out))
+    return out
+
+
+def ctl_u(seed, creature, step, slot):
+    return lib().or_ctl_u(seed & 0xFFFFFFFF, creature, step, slot)
+
+
+def make_map(width, height, seed=1, cols=4, rows=3, density=0.48, wall=14.0):
+    cap = 4 * (cols + 1) * (rows + 1) * 2 + 16
+    buf = np.zeros(cap * 4)
+    n = lib().or_make_map(width, height, cols, rows, density, wall, seed & 0xFFFFFFFF, _dp(buf), cap)
+    assert n <= cap
+    return buf[: 4 * n].reshape(n, 4).copy()
+
+
+def sdf(rects, x, y):
+    rects = np.ascontiguousarray(rects, dtype=np.float64).reshape(-1, 4)
+    out = np.zeros(3)
+    ok = lib().or_sdf(_dp(rects), len(rects), x, y, _dp(out))
+    return tuple(out) if ok else None
+
+
+class World:
+    """One oracle world (f64 by default, f32 emulation after f32ify())."""
+
+    def __init__(self, substeps=4, iterations=8, h=1 / 240):
+        self.p = lib().or_world_new(substeps, iterations, h)
+        self.S, self.I, self.h = substeps, iterations, h
+
+    def __del__(self):
+        if getattr(self, "p", None):
+            lib().or_world_free(self.p)
+            self.p = None
+
+    def set_map(self, width, height, rects):
+        rects = np.ascontiguousarray(rects, dtype=np.float64).reshape(-1, 4)
+        self.rects = rects
+        lib().or_world_set_map(self.p, width, height, _dp(rects), len(rects))
+
+    def spawn(self, x, y, angle, arms=6, segs=7, ring=8, curl=0.28):
+        r = lib().or_spawn_kraken(self.p, x, y, angle, arms, segs, ring, curl)
+        assert r >= 0
+        return r
+
+    def find_spawn(self, clearance, rng_state):
+        st = C.c_uint32(rng_state[0])
+        xy = np.zeros(2)
+        lib().or_find_spawn(self.p, clearance, C.byref(st), _dp(xy))
+        rng_state[0] = st.value
+        return xy
+
+    def color(self):
+        lib().or_ensure_colored(self.p)
+
+    def counts(self):
+        o = np.zeros(4, dtype=np.int32)
+        lib().or_counts(self.p, _ip(o))
+        return tuple(int(v) for v in o)
+
+    @property
+    def n(self):
+        return self.counts()[0]
+
+    def color_info(self):
+        off = (C.c_uint32 * 16)()
+        cnt = (C.c_uint32 * 16)()
+        lib().or_color_info(self.p, off, cnt)
+        return list(off), list(cnt)
+
+    def bodies(self):
+        n = self.n
+        a = [np.zeros(n) for _ in range(8)]
+        lib().or_get_bodies(self.p, *[_dp(x) for x in a])
+        return dict(zip(["px", "py", "ang", "vx", "vy", "av", "im", "ii"], a))
+
+    def set_bodies(self, px, py, ang, vx, vy, av):
+        arrs = [np.ascontiguousarray(v, dtype=np.float64) for v in (px, py, ang, vx, vy, av)]
+        lib().or_set_bodies(self.p, *[_dp(x) for x in arrs])
+
+    def pred(self):
+        n = self.n
+        a = [np.zeros(n) for _ in range(3)]
+        lib().or_get_pred(self.p, *[_dp(x) for x in a])
+        return dict(zip(["ppx", "ppy", "angP"], a))
+
+    def set_pred(self, ppx, ppy, angP):
+        arrs = [np.ascontiguousarray(v, dtype=np.float64) for v in (ppx, ppy, angP)]
+        lib().or_set_pred(self.p, *[_dp(x) for x in arrs])
+
+    def cons(self):
+        nc = self.counts()[1]
+        I = np.zeros(nc * 6, dtype=np.int32)
+        D = np.zeros(nc * 15)
+        lib().or_get_cons(self.p, _ip(I), _dp(D))
+        I = I.reshape(nc, 6)
+        D = D.reshape(nc, 15)
+        names_i = ["type", "a", "b", "owner", "color", "broken"]
+        names_d = ["lax", "lay", "lbx", "lby", "compliance", "rest", "target", "minA", "maxA",
+                   "lambda", "lambdaX", "lambdaY", "breakForce", "force", "id"]
+        out = {k: I[:, j].copy() for j, k in enumerate(names_i)}
+        out.update({k: D[:, j].copy() for j, k in enumerate(names_d)})
+        out["id"] = out["id"].astype(np.int64)
+        return out
+
+    def set_cons_dynamic(self, broken=None, lbx=None, lby=None, compliance=None, target=None, breakForce=None):
+        def dp(v):
+            return None if v is None else _dp(np.ascontiguousarray(v, dtype=np.float64))
+        keep = [np.ascontiguousarray(v, dtype=np.float64) if v is not None else None
+                for v in (lbx, lby, compliance, target, breakForce)]
+        b = None if broken is None else np.ascontiguousarray(broken, dtype=np.int32)
+        lib().or_set_cons_dynamic(self.p, None if b is None else _ip(b),
+                                  *[None if v is None else _dp(v) for v in keep])
+
+    def points(self):
+        np_ = self.counts()[2]
+        b = np.zeros(np_, dtype=np.int32)
+        d = np.zeros(np_ * 3)
+        lib().or_get_points(self.p, _ip(b), _dp(d))
+        d = d.reshape(np_, 3)
+        return {"body": b, "lx": d[:, 0].copy(), "ly": d[:, 1].copy(), "r": d[:, 2].copy()}
+
+    def creature(self, ci):
+        o = np.zeros(6, dtype=np.int32)
+        lib().or_creature_info(self.p, ci, _ip(o))
+        return dict(zip(["mantle", "arms", "segs", "ring", "first_body", "nbodies"], (int(v) for v in o)))
+
+    def arm_tip(self, ci, a):
+        return lib().or_arm_tip(self.p, ci, a)
+
+    def grip_pos(self, ci, a):
+        return lib().or_arm_grip_pos(self.p, ci, a)
+
+    def cd(self):
+        ncr = self.counts()[3]
+        out = np.zeros(ncr * MAX_ARMS)
+        lib().or_get_cd(self.p, _dp(out))
+        return out.reshape(ncr, MAX_ARMS)
+
+    def set_cd(self, cd):
+        cd = np.ascontiguousarray(cd, dtype=np.float64)
+        lib().or_set_cd(self.p, _dp(cd))
+
+    def attach_grip(self, ci, a, bf):
+        lib().or_attach_grip(self.p, ci, a, bf)
+
+    def apply_control(self, ci, tone, shoulder, curl, adhesion):
+        s = np.ascontiguousarray(shoulder, dtype=np.float64)
+        c = np.ascontiguousarray(curl, dtype=np.float64)
+        a = np.ascontiguousarray(adhesion, dtype=np.float64)
+        lib().or_apply_control(self.p, ci, tone, _dp(s), _dp(c), _dp(a))
+
+    def apply_law(self, seed, step):
+        lib().or_apply_law(self.p, seed & 0xFFFFFFFF, step)
+
+    def step(self):
+        lib().or_step(self.p)
+
+    def substep(self):
+        lib().or_substep(self.p)
+
+    def solve_collisions(self):
+        lib().or_solve_collisions(self.p)
+
+    def stats(self):
+        o = np.zeros(5)
+        lib().or_get_stats(self.p, _dp(o))
+        return dict(zip(["maxSpeed", "maxAngSpeed", "nan", "collisions", "broken"], o))
+
+    def f32ify(self):
+        lib().or_f32ify(self.p)
+
+    def events(self):
+        n = lib().or_events(self.p, None, 0)
+        buf = np.zeros(max(n, 1) * 4, dtype=np.int32)
+        lib().or_events(self.p, _ip(buf), n)
+        return [tuple(int(v) for v in buf[4 * i: 4 * i + 4]) for i in range(n)]
+
+    def clear_events(self):
+        lib().or_clear_events(self.p)
+
+    def set_step_counter(self, s):
+        lib().or_set_step_counter(self.p, s)
+
+    def bench_threads(self, threads, steps, seed=1):
+        return lib().or_bench_threads(self.p, threads, steps, seed & 0xFFFFFFFF)
