@@ -1,0 +1,203 @@
+"""Throughput benchmark: batched XPBD creature steps on one MI355X.
+
+Workload (uniform batch): 4096 identical creatures (mantle + 6 arms x 7
+segments: 43 bodies, 90 constraints, 92 collision points each) on
+makeMap(1200, 800, seed 1), +-20 px spawn jitter, S=4 substeps x I=8
+iterations, h = 1/240 s, device-side seeded control law with the adhesion
+reflex. One bench step = one simulation step of the whole batch.
+
+value = creature-steps/s of the persistent batched path (state resident in
+HBM when the timed region starts); the plain per-colour-launch path runs the
+same workload beside it. Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps 1000] [--warmup 20]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_PEAK_TFLOPS = 157.3       # MI355X FP32 vector peak
+HBM_PEAK_GBS = 8000.0
+
+
+def flops_per_creature_step(S, I, pins, motors, bodies, points, grips_attached, rects_near, grips_total):
+    """Algorithmic FLOPs of one creature-step (per-op counts of the WGSL
+    bodies: pin 85, motor 42, attached grip 66, point transform 11, rect
+    test 12, integrate+finalize 45 per body, grip break 5 per grip)."""
+    it = 85 * pins + 42 * motors + 66 * grips_attached + points * (11 + 12 * rects_near)
+    return S * (I * it + 45 * bodies) + 5 * grips_total
+
+
+def rects_within(w, radius=16.0):
+    """Mean number of rects within `radius` px of each collision point."""
+    b = w.p_body
+    ca, sa = np.cos(w.ang[b]), np.sin(w.ang[b])
+    x = w.px[b] + w.p_lx * ca - w.p_ly * sa
+    y = w.py[b] + w.p_lx * sa + w.p_ly * ca
+    r = w.rects.astype(np.float64)
+    cnt = np.zeros(len(x))
+    for rx, ry, rw, rh in r:
+        dx = x - np.clip(x, rx, rx + rw)
+        dy = y - np.clip(y, ry, ry + rh)
+        cnt += (dx * dx + dy * dy) <= radius * radius
+    return float(cnt.mean())
+
+
+def cpu_baseline(seconds=12.0):
+    """The oracle (C restatement of xpbd.js, f64) stepping 64-creature worlds
+    under the same control law on the host cores, one world per thread."""
+    from oracle import pyoracle as po
+    from kraken_3_gpu_amd import world as W
+    threads = max(1, min(16, os.cpu_count() or 1))
+    w = W.world_uniform(64, seed=1)
+    o = po.World(w.S, w.I, w.h)
+    o.set_map(w.width, w.height, w.rects.astype(np.float64))
+    for (x, y, a, arms, segs, ring, curl) in w.spawns:
+        o.spawn(x, y, a, arms, segs, ring, curl)
+    o.color()
+    rate1 = o.bench_threads(1, 2, 5)           # calibration: creature-steps/s on one thread
+    steps = max(2, int(seconds * rate1 / 64 / 1.0))
+    rate = o.bench_threads(threads, steps, 5)
+    return {"value": rate, "unit": "creature-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{threads} threads x 64 creatures x {steps} steps (uniform batch recipe, f64 oracle)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--creatures", type=int, default=4096)
+    ap.add_argument("--chunk", type=int, default=50, help="simulation steps per persistent launch")
+    ap.add_argument("--plain-steps", type=int, default=100)
+    ap.add_argument("--wpb", type=int, default=0, help="creatures per workgroup (0 = default)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (creatures shard across "
+                                   "GPUs with no per-step exchange, see DESIGN.md)"}))
+        sys.exit(2)
+
+    from kraken_3_gpu_amd import _lib as K
+    from kraken_3_gpu_amd import gpuworld as G
+    from kraken_3_gpu_amd import world as W
+
+    seed = 2024
+    law = W.default_law(seed)
+    w = W.world_uniform(args.creatures, seed=1)
+    n_cr = w.n_creatures
+    g = G.create(w, path=K.KX_PATH_PERSISTENT, creatures_per_block=args.wpb)
+    G.upload(g)
+    step = 0
+    while step < args.warmup:
+        n = min(args.chunk, args.warmup - step)
+        G.rollout(g, law, step, n)
+        step += n
+    G.sync(g)
+    # algorithmic accounting from the warmed-up state
+    G.download(g)
+    grips = w.type == K.KX_GRIP
+    g_att = float((w.broken[grips] == 0).sum()) / n_cr
+    r_near = rects_within(w)
+
+    chunk_ms = []
+    G.sync(g)
+    t0 = time.perf_counter()
+    done = 0
+    while done < args.steps:
+        n = min(args.chunk, args.steps - done)
+        G.rollout(g, law, step, n)
+        ms, _ = G.last_timing(g)
+        chunk_ms.append((ms, n))
+        step += n
+        done += n
+    G.sync(g)
+    wall = time.perf_counter() - t0
+    value = n_cr * args.steps / wall
+    dev_ms = sum(m for m, _ in chunk_ms)
+    st = G.download(g)
+
+    per_cr = dict(pins=42, motors=42, bodies=43, points=92, grips_total=6)
+    F = flops_per_creature_step(w.S, w.I, per_cr["pins"], per_cr["motors"], per_cr["bodies"], per_cr["points"],
+                                g_att, r_near, per_cr["grips_total"])
+    full = [m for m, n in chunk_ms if n == args.chunk] or [chunk_ms[0][0]]
+    launch_ms = float(np.mean(full))
+    achieved_tflops = F * n_cr * args.chunk / (launch_ms * 1e-3) / 1e12
+    # step-boundary HBM traffic of one launch: body state in+out (6 f32 x2),
+    # statics (im, ii, point table, constraint table) and the dynamic
+    # constraint fields, per creature
+    bytes_per_creature_launch = 43 * (6 * 4 * 2 + 9 * 4) + 90 * (16 * 4 + 9 * 4) + 92 * 3 * 4 + 8 * 8 * 2
+    hbm_gbs = bytes_per_creature_launch * n_cr / (launch_ms * 1e-3) / 1e9
+    lat = sorted(m for m, n in chunk_ms if n == args.chunk) or [launch_ms]
+    p50 = float(np.percentile(lat, 50))
+    p99 = float(np.percentile(lat, 99))
+
+    plain = None
+    if not args.no_plain:
+        wp = W.world_uniform(args.creatures, seed=1)
+        gp = G.create(wp, path=K.KX_PATH_PLAIN)
+        G.upload(gp)
+        G.rollout(gp, law, 0, 2)
+        G.sync(gp)
+        t1 = time.perf_counter()
+        G.rollout(gp, law, 2, args.plain_steps)
+        G.sync(gp)
+        pwall = time.perf_counter() - t1
+        pms, launches = G.last_timing(gp)
+        plain = {"value": n_cr * args.plain_steps / pwall, "steps": args.plain_steps,
+                 "ms_per_step": pwall * 1e3 / args.plain_steps, "device_ms_per_step": pms / args.plain_steps,
+                 "launches_per_step": launches / args.plain_steps, "sdf": "linear scan (WGSL shape)"}
+        G.set_plain_sdf(gp, True)
+        t1 = time.perf_counter()
+        G.rollout(gp, law, 2 + args.plain_steps, args.plain_steps)
+        G.sync(gp)
+        plain["value_grid_sdf"] = n_cr * args.plain_steps / (time.perf_counter() - t1)
+        G.destroy(gp)
+
+    out = {
+        "metric": "creature-steps/sec on 1 MI355X (batched vs plain) at <=1e-3 px; p99 rollout ms",
+        "value": value,
+        "unit": "creature-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"uniform batch: {n_cr} creatures (6 arms x 7 segments), makeMap 1200x800 seed 1, "
+                               f"S=4 x I=8, h=1/240, seeded control law + adhesion reflex",
+                   "creatures": n_cr, "sim_steps": args.steps, "steps_per_launch": args.chunk,
+                   "path": "persistent (one wavefront per creature, LDS-resident)"},
+        "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved_tflops / FP32_PEAK_TFLOPS, "traffic": None,
+                     "flops_per_creature_step": F, "grips_attached_mean": g_att, "rects_within_16px_mean": r_near,
+                     "kernel": "kx_k_persist", "launch_ms": launch_ms,
+                     "hbm_algorithmic_gbs": hbm_gbs, "hbm_peak_gbs": HBM_PEAK_GBS},
+        "p50_rollout_ms": p50,
+        "p99_rollout_ms": p99,
+        "rollout_unit": f"{args.chunk} steps x {n_cr} creatures per launch",
+        "device_ms_total": dev_ms,
+        "final_stats": st,
+    }
+    if plain:
+        out["plain"] = plain
+        out["speedup_vs_plain"] = value / plain["value"]
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,196 @@
+/*
+ * kxpbd — MI355X (gfx950) batched 2-D XPBD step for populations of
+ * cephalopod bodies. C ABI of libkxpbd.so.
+ *
+ * Plain pointers and sizes only; every array is caller-owned host memory in
+ * structure-of-arrays form. Functions return 0 on success or a negative
+ * KX_E* code; kx_last_error() gives a thread-local message. Calls on one
+ * handle must be serialised by the caller; different handles may be used
+ * from different threads. Reference citations are relative to the
+ * reference repository (lamachen/Kraken_3_GPU).
+ *
+ * Constraint order: kx_create colours the constraints it is given (those with
+ * colour < 0) with the greedy smallest-free-colour rule and then sorts them
+ * stably by colour, exactly as ensureColored does (src/physics/xpbd.js:
+ * 108-126). Every per-constraint array that later crosses this boundary
+ * (upload, pack_controls, download) is in that colour-sorted "handle order";
+ * kx_constraint_order() returns the permutation.
+ */
+#ifndef KXPBD_H
+#define KXPBD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KXPBD_ABI_VERSION 1
+
+/* error codes */
+#define KX_OK 0
+#define KX_EINVAL -22      /* bad argument / failed sanity guard */
+#define KX_ENOMEM -12      /* host or device allocation failed */
+#define KX_EDEVICE -5      /* HIP runtime error */
+#define KX_ENOTSUP -95     /* entry point not available for this handle */
+
+/* constraint types (src/physics/xpbd.js:148-150) */
+#define KX_PIN 0
+#define KX_ANGLE 1
+#define KX_GRIP 2
+
+/* solver paths */
+#define KX_PATH_PLAIN 0        /* one launch per colour pass, WGSL-shaped */
+#define KX_PATH_PERSISTENT 1   /* one launch per step chunk, creature state in LDS */
+
+/* World description: the static part of an XpbdWorld plus the map.
+ * Replaces what gpuworld.create() reads from `cpu` and `map`
+ * (src/gpu/gpuworld.js:34-171, writeStatic 209-247). */
+typedef struct {
+  int32_t n_bodies, n_constraints, n_points, n_rects;
+  const float *inv_mass, *inv_inertia;                 /* [n_bodies] */
+  const int32_t *c_type, *c_a, *c_b, *c_owner;         /* [n_constraints]; c_b = -1: world anchor */
+  const float *c_lax, *c_lay, *c_lbx, *c_lby;          /* anchors (lbx,lby world coords if b<0) */
+  const float *c_rest, *c_min, *c_max;                 /* angle rest / limits (+-inf allowed) */
+  const int32_t *c_color;                              /* optional; NULL or <0 entries: colour here */
+  const int32_t *p_body;                               /* [n_points] */
+  const float *p_lx, *p_ly, *p_r;
+  const float *rects;                                  /* [n_rects*4] x, y, w, h */
+} kx_world_desc;
+
+/* Optional creature structure for the device-side control law
+ * (src/sim/core.js:183-220 applyControl). Indices are desc indices. */
+typedef struct {
+  int32_t n_creatures, n_arms;
+  const int32_t *creature_arm_off;   /* [n_creatures+1] */
+  const int32_t *arm_tip;            /* [n_arms] tip body */
+  const int32_t *arm_grip;           /* [n_arms] grip constraint */
+  const int32_t *arm_motor_off;      /* [n_arms+1] */
+  const int32_t *motors;             /* motor constraints, shoulder first, then along the chain */
+} kx_creature_desc;
+
+/* Solver parameters (src/physics/xpbd.js:46-54; gpuworld.js Params 212-223). */
+typedef struct {
+  int32_t substeps, iterations;
+  double h;                          /* seconds per substep */
+  double lin_drag, ang_drag, max_speed, max_ang_speed;
+  int32_t path;                      /* KX_PATH_* */
+  int32_t creatures_per_block;       /* persistent path: 1,2,4,8; 0 = default (4) */
+} kx_params;
+
+/* Body state in SoA (gpuworld.js:250-263 upload; 417-441 mirrorInto).
+ * ppx/ppy/angP and fx/fy/tq may be NULL (upload: taken as px/py/ang and 0). */
+typedef struct {
+  float *px, *py, *vx, *vy, *ang, *av;
+  float *ppx, *ppy, *angP;
+  float *fx, *fy, *tq;
+} kx_body_state;
+
+/* Constraint state in handle order (gpuworld.js:264-281 upload; packControls
+ * 331-341; mirrorInto 443-451). Any pointer may be NULL (skipped). */
+typedef struct {
+  float *lbx, *lby, *compliance, *target, *break_force;
+  uint8_t *broken;
+  float *lambda, *lambda_x, *lambda_y;
+} kx_cons_state;
+
+/* Diagnostics of a download (gpuworld.js:452-455). */
+typedef struct {
+  double max_speed, max_ang_speed;
+  int64_t nan;
+} kx_stats;
+
+/* Device-side control law for rollouts: per step s and creature c,
+ * u = mulberry32-chain(seed, c, s, slot); motor m of arm a gets target
+ * (2u-1)*limit (shoulder_limit for m = 0, joint_limit otherwise), every motor
+ * gets compliance 1/(stiff_min + tone*(stiff_max-stiff_min)), and the
+ * adhesion a = u(slot 256+a) drives the reflex of core.js:199-218 with
+ * break force grip_min + (grip_max-grip_min)*a. */
+typedef struct {
+  uint32_t seed;
+  double tone, shoulder_limit, joint_limit;
+  double stiff_min, stiff_max, grip_min, grip_max;
+  double slip_max, cooldown, attach_min, release_below;
+} kx_law;
+
+typedef struct kx_handle kx_handle;
+
+/* ── lifecycle (gpuworld.js create 34-171 / destroy 466-472) ─────────── */
+int  kx_create(const kx_world_desc *desc, const kx_creature_desc *creatures,
+               const kx_params *params, kx_handle **out);
+void kx_destroy(kx_handle *h);             /* idempotent on NULL */
+const char *kx_last_error(void);
+int  kx_abi_version(void);
+
+/* ── layout queries ──────────────────────────────────────────────────── */
+int kx_counts(const kx_handle *h, int32_t *n_bodies, int32_t *n_constraints,
+              int32_t *n_points, int32_t *n_creatures);
+/* order[k] = desc index of the k-th constraint in handle order; color[k] its colour */
+int kx_constraint_order(const kx_handle *h, int32_t *order, int32_t *color);
+/* colour offsets/counts in handle order (colorInfo, gpuworld.js:286-297) */
+int kx_color_info(const kx_handle *h, uint32_t *off16, uint32_t *cnt16);
+/* pure host colouring, what kx_create applies (xpbd.js:98-126): colours the
+ * constraints whose color_in entry is < 0 (color_in may be NULL) and returns
+ * color_out[n_constraints] plus the stable colour sort order_out[k] = desc
+ * index of the k-th constraint in handle order. No device needed. */
+int kx_color_constraints(int32_t n_bodies, int32_t n_constraints, const int32_t *c_a, const int32_t *c_b,
+                         const int32_t *color_in, int32_t *color_out, int32_t *order_out);
+/* stable grouping of collision points by body (gpuworld.js:189-207):
+ * order[n_points], range[2*n_bodies] = offsets then counts */
+int kx_group_points(int32_t n_points, const int32_t *p_body, int32_t n_bodies,
+                    int32_t *order, int32_t *range);
+
+/* ── state transfer (gpuworld.js upload 250-298, uploadControls 312-315,
+ *    download 385-396) ──────────────────────────────────────────────── */
+int kx_upload(kx_handle *h, const kx_body_state *bodies, const kx_cons_state *cons);
+int kx_pack_controls(kx_handle *h, const kx_cons_state *controls);
+int kx_download(kx_handle *h, kx_body_state *bodies, kx_cons_state *cons, kx_stats *stats);
+/* reflex cooldown per creature arm, [n_creatures * 8] doubles */
+int kx_get_cooldown(kx_handle *h, double *cd);
+int kx_set_cooldown(kx_handle *h, const double *cd);
+
+/* ── stepping (gpuworld.js step 374-382): S substeps + grip break per step;
+ *    asynchronous on the handle's stream. ─────────────────────────────── */
+int kx_step(kx_handle *h, int32_t nsteps);
+/* nsteps of {control law + reflex, S substeps, grip break}; steps are keyed
+ * step0 .. step0+nsteps-1 in the control PRNG. Needs a creature desc.
+ * events (optional, device-resident until kx_sync): per step and creature a
+ * u32 with bit (3*arm + kind) set for kind 0 attach, 1 release, 2 break. */
+int kx_rollout(kx_handle *h, const kx_law *law, int32_t step0, int32_t nsteps,
+               uint32_t *events_host);
+int kx_sync(kx_handle *h);
+/* capture `nsteps` rollout steps into a hipGraph once and replay it `reps`
+ * times (step keys advance by nsteps per replay) */
+int kx_rollout_graph(kx_handle *h, const kx_law *law, int32_t step0, int32_t nsteps, int32_t reps);
+/* average device time (ms) of the last kx_step/kx_rollout launch sequence,
+ * measured with HIP events on the handle's stream */
+int kx_last_timing(const kx_handle *h, double *ms_total, int32_t *launches);
+int kx_set_path(kx_handle *h, int32_t path, int32_t creatures_per_block);
+/* plain path only: 0 = linear scan over every rect per query (the shape of
+ * src/gpu/solver.wgsl:114-147), 1 = the exact cull grid the persistent
+ * path always uses */
+int kx_set_plain_sdf(kx_handle *h, int32_t use_grid);
+
+/* ── world builder for synthetic workloads (src/body/kraken.js spawnKraken
+ *    64-148; src/world/map.js makeMap 22-71, findSpawn 120-128) ────────── */
+typedef struct kx_builder kx_builder;
+kx_builder *kx_builder_new(void);
+void kx_builder_free(kx_builder *b);
+int  kx_builder_map(kx_builder *b, double width, double height, int32_t cols, int32_t rows,
+                    double density, double wall, uint32_t seed);
+int  kx_builder_find_spawn(kx_builder *b, double clearance, uint32_t *rng_state, double *xy);
+int  kx_builder_spawn(kx_builder *b, double x, double y, double angle, int32_t arms,
+                      int32_t segs, int32_t ring, double curl);
+/* views valid until the next builder call */
+int  kx_builder_desc(kx_builder *b, kx_world_desc *desc, kx_creature_desc *cr);
+/* initial state: px,py,ang of every body (f64 as built; the caller rounds) */
+int  kx_builder_state(const kx_builder *b, double *px, double *py, double *ang);
+/* initial constraint dynamics in desc order: lbx, lby, compliance, target,
+ * break force, broken (spawnKraken: grips broken with anchor at spawn) */
+int  kx_builder_cons_init(const kx_builder *b, double *lbx, double *lby, double *compliance,
+                          double *target, double *break_force, uint8_t *broken);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,775 @@
+// Host side of libkxpbd: the C ABI declared in include/kxpbd.h.
+//
+// kx_create mirrors gpuworld.create (src/gpu/gpuworld.js:34-171): sanity
+// guards (writeStatic 224-229, upload 269-271), colouring (ensureColored,
+// src/physics/xpbd.js:108-126), point grouping (groupCollPoints 189-207).
+// On top of that it derives the per-creature layout the persistent kernel
+// needs (connected components of the constraint graph) and the exact cull
+// grid for the map.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/kxpbd.h"
+#include "kx_internal.h"
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+extern "C" const char *kx_last_error(void) { return g_err.c_str(); }
+extern "C" int kx_abi_version(void) { return KXPBD_ABI_VERSION; }
+
+#define HIPCHK(expr)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess) return fail(KX_EDEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+struct kx_handle {
+  int nB = 0, nC = 0, nP = 0, nR = 0, nCr = 0, nArms = 0;
+  kx_params prm{};
+  int path = KX_PATH_PERSISTENT, wpb = 4;
+  bool persist_ok = false, has_creatures = false, forces_pending = false;
+  std::string persist_why;
+  int maxb = 0, maxc = 0, maxp = 0;
+  // host tables
+  std::vector<int> ord;        // handle order -> desc index
+  std::vector<int> colour;     // handle order colour
+  std::vector<int> h2i;        // handle order -> internal index
+  uint32_t colorOff[KX_MAX_COLORS] = {}, colorCnt[KX_MAX_COLORS] = {};
+  std::vector<KxCreature> cr;
+  std::vector<int> ctype_i;    // internal order types (host copy)
+  // device
+  KxDev d{};
+  std::vector<DevBuf> bufs;
+  int *step_base = nullptr;
+  uint32_t *ev_dev = nullptr;
+  size_t ev_cap = 0;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int last_launches = 0;
+  bool timed = false;
+  // graph cache
+  hipGraphExec_t gexec = nullptr;
+  int g_nsteps = -1, g_path = -1;
+  KxLaw g_law{};
+  // scratch host staging
+  std::vector<float> hf;
+  std::vector<int> hi;
+};
+
+template <class T>
+static int dalloc(kx_handle *h, T **p, size_t n) {
+  size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+  void *q = nullptr;
+  HIPCHK(hipMalloc(&q, bytes));
+  HIPCHK(hipMemset(q, 0, bytes));
+  h->bufs.push_back({q, bytes});
+  *p = (T *)q;
+  return 0;
+}
+template <class T>
+static int dput(T *dst, const T *src, size_t n) {
+  if (n) HIPCHK(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+#define TRY(x)            \
+  do {                    \
+    int r_ = (x);         \
+    if (r_) return r_;    \
+  } while (0)
+
+// ── colouring (xpbd.js:98-126) ──────────────────────────────────────
+static int colour_constraints(int nB, int nC, const int32_t *ca, const int32_t *cb, const int32_t *given,
+                              std::vector<int> &col) {
+  col.assign(nC, -1);
+  std::vector<uint32_t> mask(std::max(1024, nB * 2), 0u);
+  for (int k = 0; k < nC; k++) col[k] = given ? given[k] : -1;
+  // pre-existing colours occupy their bodies first, like assignColor on an
+  // already-coloured world (only uncoloured constraints are assigned)
+  for (int k = 0; k < nC; k++) {
+    if (col[k] < 0) {
+      uint32_t used = mask[ca[k]] | (cb[k] >= 0 ? mask[cb[k]] : 0u);
+      int c = 0;
+      while (c < 32 && (used & (1u << c))) c++;
+      if (c >= KX_MAX_COLORS) return fail(KX_EINVAL, "constraint %d needs colour %d >= %d", k, c, KX_MAX_COLORS);
+      mask[ca[k]] |= 1u << c;
+      if (cb[k] >= 0) mask[cb[k]] |= 1u << c;
+      col[k] = c;
+    } else if (col[k] >= KX_MAX_COLORS) {
+      return fail(KX_EINVAL, "constraint %d has colour %d >= %d", k, col[k], KX_MAX_COLORS);
+    }
+  }
+  return 0;
+}
+
+extern "C" int kx_color_constraints(int32_t nB, int32_t nC, const int32_t *ca, const int32_t *cb,
+                                    const int32_t *color_in, int32_t *color_out, int32_t *order_out) {
+  if (nB < 1 || nC < 0 || (nC && (!ca || !cb))) return fail(KX_EINVAL, "bad arguments");
+  for (int k = 0; k < nC; k++)
+    if (ca[k] < 0 || ca[k] >= nB || cb[k] < -1 || cb[k] >= nB)
+      return fail(KX_EINVAL, "constraint %d: body index %d/%d outside %d", k, ca[k], cb[k], nB);
+  std::vector<int> col;
+  TRY(colour_constraints(nB, nC, ca, cb, color_in, col));
+  std::vector<int> ord(nC);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return col[x] < col[y]; });
+  for (int k = 0; k < nC; k++) {
+    if (color_out) color_out[k] = col[k];
+    if (order_out) order_out[k] = ord[k];
+  }
+  return 0;
+}
+
+extern "C" int kx_group_points(int32_t nP, const int32_t *p_body, int32_t nB, int32_t *order, int32_t *range) {
+  std::vector<int32_t> cnt(nB, 0);
+  for (int i = 0; i < nP; i++) {
+    int b = p_body[i];
+    if (b < 0 || b >= nB) return fail(KX_EINVAL, "collPt %d refers to body %d of %d", i, b, nB);
+    cnt[b]++;
+  }
+  std::vector<int32_t> off(nB, 0);
+  int acc = 0;
+  for (int b = 0; b < nB; b++) { off[b] = acc; acc += cnt[b]; }
+  if (acc != nP) return fail(KX_EINVAL, "collRange covers %d of %d points", acc, nP);
+  std::vector<int32_t> cur(off);
+  for (int i = 0; i < nP; i++) order[cur[p_body[i]]++] = i;
+  for (int b = 0; b < nB; b++) { range[b] = off[b]; range[nB + b] = cnt[b]; }
+  return 0;
+}
+
+// union-find
+static int uf_find(std::vector<int> &p, int x) {
+  while (p[x] != x) { p[x] = p[p[x]]; x = p[x]; }
+  return x;
+}
+
+// ── exact cull grid over the rect list ──────────────────────────────
+static int build_grid(kx_handle *h, const float *rects, int nR, float rmax) {
+  float half = 0.0f;
+  float x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
+  for (int i = 0; i < nR; i++) {
+    const float *r = rects + 4 * i;
+    half = std::max(half, 0.5f * std::min(std::fabs(r[2]), std::fabs(r[3])));
+    x0 = std::min(x0, r[0]); y0 = std::min(y0, r[1]);
+    x1 = std::max(x1, r[0] + r[2]); y1 = std::max(y1, r[1] + r[3]);
+  }
+  float R = std::max(half, rmax) + 2.0f;
+  float cs = std::max(32.0f, 2.0f * R);
+  int gnx = 1, gny = 1;
+  std::vector<int> off(2, 0), ids;
+  float gx0 = 0, gy0 = 0;
+  if (nR > 0) {
+    gx0 = std::floor(x0 - R - cs);
+    gy0 = std::floor(y0 - R - cs);
+    gnx = (int)std::ceil((x1 + R + cs - gx0) / cs) + 1;
+    gny = (int)std::ceil((y1 + R + cs - gy0) / cs) + 1;
+    if ((long)gnx * gny > (1L << 24)) return fail(KX_EINVAL, "map extent too large for the cull grid");
+    std::vector<std::vector<int>> cells((size_t)gnx * gny);
+    for (int i = 0; i < nR; i++) {
+      const float *r = rects + 4 * i;
+      float ax0 = r[0] - R - 1.0f, ax1 = r[0] + r[2] + R + 1.0f;
+      float ay0 = r[1] - R - 1.0f, ay1 = r[1] + r[3] + R + 1.0f;
+      int ix0 = std::max(0, (int)std::floor((ax0 - gx0) / cs)), ix1 = std::min(gnx - 1, (int)std::floor((ax1 - gx0) / cs));
+      int iy0 = std::max(0, (int)std::floor((ay0 - gy0) / cs)), iy1 = std::min(gny - 1, (int)std::floor((ay1 - gy0) / cs));
+      for (int iy = iy0; iy <= iy1; iy++)
+        for (int ix = ix0; ix <= ix1; ix++) cells[(size_t)iy * gnx + ix].push_back(i);
+    }
+    off.assign((size_t)gnx * gny + 1, 0);
+    for (size_t c = 0; c < cells.size(); c++) {
+      off[c + 1] = off[c] + (int)cells[c].size();
+      ids.insert(ids.end(), cells[c].begin(), cells[c].end());
+    }
+  }
+  int *doff, *dids;
+  TRY(dalloc(h, &doff, off.size()));
+  TRY(dalloc(h, &dids, ids.size()));
+  TRY(dput(doff, off.data(), off.size()));
+  TRY(dput(dids, ids.data(), ids.size()));
+  h->d.cell_off = doff; h->d.cell_ids = dids;
+  h->d.gx0 = gx0; h->d.gy0 = gy0; h->d.inv_cs = 1.0f / cs; h->d.gnx = gnx; h->d.gny = gny;
+  return 0;
+}
+
+static void destroy_impl(kx_handle *h) {
+  if (!h) return;
+  if (h->st) (void)hipStreamSynchronize(h->st);
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  for (auto &b : h->bufs) (void)hipFree(b.p);
+  if (h->ev_dev) (void)hipFree(h->ev_dev);
+  if (h->e0) (void)hipEventDestroy(h->e0);
+  if (h->e1) (void)hipEventDestroy(h->e1);
+  if (h->st) (void)hipStreamDestroy(h->st);
+  delete h;
+}
+
+extern "C" void kx_destroy(kx_handle *h) { destroy_impl(h); }
+
+static int create_impl(const kx_world_desc *w, const kx_creature_desc *cd, const kx_params *p, kx_handle *h) {
+  const int nB = w->n_bodies, nC = w->n_constraints, nP = w->n_points, nR = w->n_rects;
+  // sanity guards (gpuworld.js:224-229)
+  if (!(p->iterations >= 1 && p->iterations <= 64)) return fail(KX_EINVAL, "iterations implausible: %d", p->iterations);
+  if (!(p->substeps >= 1 && p->substeps <= 64)) return fail(KX_EINVAL, "substeps implausible: %d", p->substeps);
+  if (!(nR >= 1 && nR <= 100000)) return fail(KX_EINVAL, "nRects implausible: %d", nR);
+  if (!(nB >= 1 && nB <= 4000000)) return fail(KX_EINVAL, "nBodies implausible: %d", nB);
+  if (!(p->h > 0 && p->h < 1)) return fail(KX_EINVAL, "h implausible: %g", p->h);
+  if (nC < 0 || nP < 0) return fail(KX_EINVAL, "negative counts");
+  for (int k = 0; k < nC; k++) {
+    int a = w->c_a[k], b = w->c_b[k], t = w->c_type[k];
+    if (a < 0 || a >= nB || b >= nB || b < -1)
+      return fail(KX_EINVAL, "constraint %d: body index %d/%d outside %d (would be an out-of-bounds read)", k, a, b, nB);
+    if (t < KX_PIN || t > KX_GRIP) return fail(KX_EINVAL, "constraint %d: type %d", k, t);
+    if (b == a) return fail(KX_EINVAL, "constraint %d couples body %d with itself", k, a);
+  }
+  h->nB = nB; h->nC = nC; h->nP = nP; h->nR = nR;
+  h->prm = *p;
+  h->path = p->path;
+  // colouring + stable sort (handle order)
+  std::vector<int> col;
+  TRY(colour_constraints(nB, nC, w->c_a, w->c_b, w->c_color, col));
+  h->ord.resize(nC);
+  std::iota(h->ord.begin(), h->ord.end(), 0);
+  std::stable_sort(h->ord.begin(), h->ord.end(), [&](int x, int y) { return col[x] < col[y]; });
+  h->colour.resize(nC);
+  for (int k = 0; k < nC; k++) h->colour[k] = col[h->ord[k]];
+  for (int c = 0; c < KX_MAX_COLORS; c++) h->colorCnt[c] = 0;
+  for (int k = 0; k < nC; k++) h->colorCnt[h->colour[k]]++;
+  for (int c = 0, acc = 0; c < KX_MAX_COLORS; c++) { h->colorOff[c] = acc; acc += h->colorCnt[c]; }
+  // points grouped by body
+  std::vector<int32_t> porder(nP), prange(2 * (size_t)nB);
+  TRY(kx_group_points(nP, w->p_body, nB, porder.data(), prange.data()));
+  // creatures: connected components over bodies
+  std::vector<int> par(nB);
+  std::iota(par.begin(), par.end(), 0);
+  for (int k = 0; k < nC; k++)
+    if (w->c_b[k] >= 0) {
+      int x = uf_find(par, w->c_a[k]), y = uf_find(par, w->c_b[k]);
+      if (x != y) par[std::max(x, y)] = std::min(x, y);
+    }
+  std::vector<int> comp_of(nB), comp_id(nB, -1);
+  int ncomp = 0;
+  for (int b = 0; b < nB; b++) {
+    int r = uf_find(par, b);
+    if (comp_id[r] < 0) comp_id[r] = ncomp++;
+    comp_of[b] = comp_id[r];
+  }
+  h->nCr = ncomp;
+  std::vector<int> cmin(ncomp, nB), cmax(ncomp, -1), ccount(ncomp, 0);
+  for (int b = 0; b < nB; b++) {
+    int c = comp_of[b];
+    cmin[c] = std::min(cmin[c], b); cmax[c] = std::max(cmax[c], b); ccount[c]++;
+  }
+  h->persist_ok = true;
+  for (int c = 0; c < ncomp && h->persist_ok; c++) {
+    if (cmax[c] - cmin[c] + 1 != ccount[c]) { h->persist_ok = false; h->persist_why = "a creature's bodies are not contiguous"; }
+    if (ccount[c] > 255) { h->persist_ok = false; h->persist_why = "a creature has more than 255 bodies"; }
+  }
+  // internal constraint order: creature-major, handle order inside
+  std::vector<int> ccnt(ncomp, 0);
+  for (int k = 0; k < nC; k++) ccnt[comp_of[w->c_a[h->ord[k]]]]++;
+  h->cr.assign(ncomp, KxCreature{});
+  int acc = 0;
+  for (int c = 0; c < ncomp; c++) {
+    KxCreature &K = h->cr[c];
+    K.body_off = cmin[c]; K.nb = ccount[c];
+    K.cons_off = acc; K.nc = ccnt[c]; acc += ccnt[c];
+    K.pt_off = prange[cmin[c]];
+    int np = 0;
+    for (int b = cmin[c]; b <= cmax[c]; b++) np += prange[nB + b];
+    K.np = np;
+    if (K.nc > 65535 || K.np > 65535) { h->persist_ok = false; h->persist_why = "creature too large"; }
+  }
+  h->h2i.resize(nC);
+  std::vector<int> fill(ncomp, 0), i2h(nC);
+  for (int k = 0; k < nC; k++) {
+    int c = comp_of[w->c_a[h->ord[k]]];
+    int i = h->cr[c].cons_off + fill[c]++;
+    h->h2i[k] = i; i2h[i] = k;
+  }
+  // per-creature colour offsets (non-empty colours, ascending)
+  for (int c = 0; c < ncomp; c++) {
+    KxCreature &K = h->cr[c];
+    int cnts[KX_MAX_COLORS] = {};
+    for (int j = 0; j < K.nc; j++) cnts[h->colour[i2h[K.cons_off + j]]]++;
+    int nco = 0, o = 0;
+    for (int col2 = 0; col2 < KX_MAX_COLORS; col2++) {
+      if (!cnts[col2]) continue;
+      K.coff[nco++] = (uint16_t)o;
+      o += cnts[col2];
+    }
+    K.coff[nco] = (uint16_t)o;
+    for (int r = nco + 1; r <= KX_MAX_COLORS; r++) K.coff[r] = (uint16_t)o;
+    K.ncol = nco;
+    h->maxb = std::max(h->maxb, K.nb); h->maxc = std::max(h->maxc, K.nc); h->maxp = std::max(h->maxp, K.np);
+  }
+  // creature desc → arms, motor slots
+  std::vector<int> cslot(nC, -1), carm(nC, -1), arm_tip, arm_grip;
+  std::vector<int> d2i(nC);
+  for (int k = 0; k < nC; k++) d2i[h->ord[k]] = h->h2i[k];
+  if (cd && cd->n_creatures > 0) {
+    if (cd->n_creatures != ncomp)
+      return fail(KX_EINVAL, "creature desc has %d creatures but the constraint graph has %d components", cd->n_creatures, ncomp);
+    for (int c = 0; c < ncomp; c++) {
+      int a0 = cd->creature_arm_off[c], a1 = cd->creature_arm_off[c + 1];
+      if (a1 - a0 > KX_MAX_ARMS || a1 < a0) return fail(KX_EINVAL, "creature %d: %d arms (max %d)", c, a1 - a0, KX_MAX_ARMS);
+      h->cr[c].arm_off = (int)arm_tip.size();
+      h->cr[c].narms = a1 - a0;
+      for (int a = a0; a < a1; a++) {
+        int tip = cd->arm_tip[a], g = cd->arm_grip[a];
+        if (tip < 0 || tip >= nB || comp_of[tip] != c) return fail(KX_EINVAL, "arm %d: tip body %d not in creature %d", a, tip, c);
+        if (g < 0 || g >= nC || w->c_type[g] != KX_GRIP || w->c_a[g] != tip)
+          return fail(KX_EINVAL, "arm %d: constraint %d is not the grip of its tip", a, g);
+        arm_tip.push_back(tip);
+        arm_grip.push_back(d2i[g]);
+        carm[d2i[g]] = a - a0;
+        int m0 = cd->arm_motor_off[a], m1 = cd->arm_motor_off[a + 1];
+        if (m1 - m0 > KX_SLOTS_PER_ARM) return fail(KX_EINVAL, "arm %d has %d motors (max %d)", a, m1 - m0, KX_SLOTS_PER_ARM);
+        for (int m = m0; m < m1; m++) {
+          int k = cd->motors[m];
+          if (k < 0 || k >= nC || w->c_type[k] != KX_ANGLE || comp_of[w->c_a[k]] != c)
+            return fail(KX_EINVAL, "arm %d: motor %d is not an angle constraint of creature %d", a, k, c);
+          cslot[d2i[k]] = (a - a0) * KX_SLOTS_PER_ARM + (m - m0);
+        }
+      }
+    }
+    h->has_creatures = true;
+  }
+  h->nArms = (int)arm_tip.size();
+  // ── device buffers ──
+  KxDev &d = h->d;
+  float *f;
+  TRY(dalloc(h, &d.px, nB)); TRY(dalloc(h, &d.py, nB)); TRY(dalloc(h, &d.ang, nB));
+  TRY(dalloc(h, &d.vx, nB)); TRY(dalloc(h, &d.vy, nB)); TRY(dalloc(h, &d.av, nB));
+  TRY(dalloc(h, &d.ppx, nB)); TRY(dalloc(h, &d.ppy, nB)); TRY(dalloc(h, &d.angP, nB));
+  TRY(dalloc(h, &d.fx, nB)); TRY(dalloc(h, &d.fy, nB)); TRY(dalloc(h, &d.tq, nB));
+  TRY(dalloc(h, &f, nB)); TRY(dput(f, w->inv_mass, nB)); d.im = f;
+  TRY(dalloc(h, &f, nB)); TRY(dput(f, w->inv_inertia, nB)); d.ii = f;
+  // constraint statics in internal order
+  std::vector<int> ia(nC), ib(nC), it(nC), icr(nC);
+  std::vector<float> flax(nC), flay(nC), flbx(nC), flby(nC), frest(nC), fmin(nC), fmax(nC);
+  for (int k = 0; k < nC; k++) {
+    int s = h->ord[k], i = h->h2i[k];
+    ia[i] = w->c_a[s]; ib[i] = w->c_b[s]; it[i] = w->c_type[s]; icr[i] = comp_of[w->c_a[s]];
+    flax[i] = w->c_lax[s]; flay[i] = w->c_lay[s]; flbx[i] = w->c_lbx[s]; flby[i] = w->c_lby[s];
+    frest[i] = w->c_rest[s]; fmin[i] = w->c_min[s]; fmax[i] = w->c_max[s];
+  }
+  h->ctype_i = it;
+  int *ip;
+  TRY(dalloc(h, &ip, nC)); TRY(dput(ip, ia.data(), nC)); d.ca = ip;
+  TRY(dalloc(h, &ip, nC)); TRY(dput(ip, ib.data(), nC)); d.cb = ip;
+  TRY(dalloc(h, &ip, nC)); TRY(dput(ip, it.data(), nC)); d.ctype = ip;
+  TRY(dalloc(h, &ip, nC)); TRY(dput(ip, icr.data(), nC)); d.ccr = ip;
+  TRY(dalloc(h, &ip, nC)); TRY(dput(ip, cslot.data(), nC)); d.cslot = ip;
+  TRY(dalloc(h, &ip, nC)); TRY(dput(ip, carm.data(), nC)); d.carm = ip;
+  TRY(dalloc(h, &f, nC)); TRY(dput(f, flax.data(), nC)); d.lax = f;
+  TRY(dalloc(h, &f, nC)); TRY(dput(f, flay.data(), nC)); d.lay = f;
+  TRY(dalloc(h, &f, nC)); TRY(dput(f, frest.data(), nC)); d.rest = f;
+  TRY(dalloc(h, &f, nC)); TRY(dput(f, fmin.data(), nC)); d.cmin = f;
+  TRY(dalloc(h, &f, nC)); TRY(dput(f, fmax.data(), nC)); d.cmax = f;
+  TRY(dalloc(h, &d.lbx, nC)); TRY(dput(d.lbx, flbx.data(), nC));
+  TRY(dalloc(h, &d.lby, nC)); TRY(dput(d.lby, flby.data(), nC));
+  TRY(dalloc(h, &d.comp, nC)); TRY(dalloc(h, &d.target, nC)); TRY(dalloc(h, &d.bf, nC));
+  {
+    std::vector<float> big(nC, 1e30f);
+    TRY(dput(d.bf, big.data(), nC));
+  }
+  TRY(dalloc(h, &d.broken, nC));
+  TRY(dalloc(h, &d.lam, nC)); TRY(dalloc(h, &d.lamx, nC)); TRY(dalloc(h, &d.lamy, nC));
+  TRY(dalloc(h, &ip, nC)); TRY(dput(ip, h->h2i.data(), nC)); d.col_list = ip;
+  // points
+  std::vector<float> plx(nP), ply(nP), pr(nP);
+  float rmax = 0.0f;
+  for (int q = 0; q < nP; q++) {
+    int s = porder[q];
+    plx[q] = w->p_lx[s]; ply[q] = w->p_ly[s]; pr[q] = w->p_r[s];
+    rmax = std::max(rmax, std::fabs(w->p_r[s]));
+  }
+  TRY(dalloc(h, &f, nP)); TRY(dput(f, plx.data(), nP)); d.plx = f;
+  TRY(dalloc(h, &f, nP)); TRY(dput(f, ply.data(), nP)); d.ply = f;
+  TRY(dalloc(h, &f, nP)); TRY(dput(f, pr.data(), nP)); d.pr = f;
+  TRY(dalloc(h, &ip, nB)); TRY(dput(ip, prange.data(), nB)); d.bpoff = ip;
+  TRY(dalloc(h, &ip, nB)); TRY(dput(ip, prange.data() + nB, nB)); d.bpcnt = ip;
+  // map
+  float4 *r4;
+  TRY(dalloc(h, &r4, nR));
+  TRY(dput((float *)r4, w->rects, 4 * (size_t)nR));
+  d.rects = r4; d.nr = nR;
+  TRY(build_grid(h, w->rects, nR, rmax));
+  // creatures / arms
+  KxCreature *dcr;
+  TRY(dalloc(h, &dcr, ncomp)); TRY(dput(dcr, h->cr.data(), ncomp)); d.cr = dcr;
+  TRY(dalloc(h, &ip, arm_tip.size())); TRY(dput(ip, arm_tip.data(), arm_tip.size())); d.arm_tip = ip;
+  TRY(dalloc(h, &ip, arm_grip.size())); TRY(dput(ip, arm_grip.data(), arm_grip.size())); d.arm_grip = ip;
+  TRY(dalloc(h, &d.cd, (size_t)ncomp * KX_MAX_ARMS));
+  TRY(dalloc(h, &h->step_base, 1));
+  // constants (params uniform, gpuworld.js:212-223)
+  d.h = (float)p->h;
+  d.invH = (float)(1.0 / p->h);
+  d.invH2 = (float)(1.0 / (p->h * p->h));
+  d.linDamp = (float)std::exp(-p->lin_drag * p->h);
+  d.angDamp = (float)std::exp(-p->ang_drag * p->h);
+  d.maxSpeed = (float)p->max_speed;
+  d.maxAngSpeed = (float)p->max_ang_speed;
+  d.hd = p->h;
+  d.S = p->substeps; d.I = p->iterations;
+  d.nB = nB; d.nC = nC; d.nCr = ncomp; d.nArms = h->nArms;
+  d.ncol = 0;
+  for (int c = 0; c < KX_MAX_COLORS; c++)
+    if (h->colorCnt[c]) { d.col_off[d.ncol] = h->colorOff[c]; d.col_cnt[d.ncol] = h->colorCnt[c]; d.ncol++; }
+  d.use_grid = 0;
+  d.events = nullptr;
+  HIPCHK(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
+  HIPCHK(hipEventCreate(&h->e0));
+  HIPCHK(hipEventCreate(&h->e1));
+  if (p->creatures_per_block) {
+    int c = p->creatures_per_block;
+    if (c != 1 && c != 2 && c != 4 && c != 8) return fail(KX_EINVAL, "creatures_per_block %d", c);
+    h->wpb = c;
+  }
+  if (h->path == KX_PATH_PERSISTENT && !h->persist_ok)
+    return fail(KX_ENOTSUP, "persistent path unavailable: %s", h->persist_why.c_str());
+  return 0;
+}
+
+extern "C" int kx_create(const kx_world_desc *desc, const kx_creature_desc *creatures, const kx_params *params,
+                         kx_handle **out) {
+  if (!desc || !params || !out) return fail(KX_EINVAL, "null argument");
+  *out = nullptr;
+  kx_handle *h = new (std::nothrow) kx_handle();
+  if (!h) return fail(KX_ENOMEM, "out of host memory");
+  int r = create_impl(desc, creatures, params, h);
+  if (r) { std::string keep = g_err; destroy_impl(h); g_err = keep; return r; }
+  *out = h;
+  return 0;
+}
+
+extern "C" int kx_counts(const kx_handle *h, int32_t *nb, int32_t *nc, int32_t *np, int32_t *ncr) {
+  if (!h) return fail(KX_EINVAL, "null handle");
+  if (nb) *nb = h->nB;
+  if (nc) *nc = h->nC;
+  if (np) *np = h->nP;
+  if (ncr) *ncr = h->nCr;
+  return 0;
+}
+
+extern "C" int kx_constraint_order(const kx_handle *h, int32_t *order, int32_t *color) {
+  if (!h) return fail(KX_EINVAL, "null handle");
+  for (int k = 0; k < h->nC; k++) {
+    if (order) order[k] = h->ord[k];
+    if (color) color[k] = h->colour[k];
+  }
+  return 0;
+}
+
+extern "C" int kx_color_info(const kx_handle *h, uint32_t *off, uint32_t *cnt) {
+  if (!h) return fail(KX_EINVAL, "null handle");
+  for (int c = 0; c < KX_MAX_COLORS; c++) { off[c] = h->colorOff[c]; cnt[c] = h->colorCnt[c]; }
+  return 0;
+}
+
+// ── state transfer ──────────────────────────────────────────────────
+static int put_body(kx_handle *h, float *dst, const float *src, const float *fallback) {
+  const float *s = src ? src : fallback;
+  if (s) return dput(dst, s, h->nB);
+  return 0;
+}
+static int put_cons_f(kx_handle *h, float *dst, const float *src, bool bforce) {
+  if (!src) return 0;
+  h->hf.resize(h->nC);
+  for (int k = 0; k < h->nC; k++) {
+    float v = src[k];
+    if (bforce) v = std::isfinite(v) ? std::min(v, 1e30f) : 1e30f;   // gpuworld.js:278
+    h->hf[h->h2i[k]] = v;
+  }
+  return dput(dst, h->hf.data(), h->nC);
+}
+static int put_cons_u8(kx_handle *h, int *dst, const uint8_t *src) {
+  if (!src) return 0;
+  h->hi.resize(h->nC);
+  for (int k = 0; k < h->nC; k++) h->hi[h->h2i[k]] = src[k] ? 1 : 0;
+  return dput(dst, h->hi.data(), h->nC);
+}
+
+extern "C" int kx_upload(kx_handle *h, const kx_body_state *b, const kx_cons_state *c) {
+  if (!h) return fail(KX_EINVAL, "null handle");
+  HIPCHK(hipStreamSynchronize(h->st));
+  if (b) {
+    if (!b->px || !b->py || !b->ang || !b->vx || !b->vy || !b->av) return fail(KX_EINVAL, "upload needs px,py,ang,vx,vy,av");
+    TRY(put_body(h, h->d.px, b->px, nullptr)); TRY(put_body(h, h->d.py, b->py, nullptr));
+    TRY(put_body(h, h->d.ang, b->ang, nullptr)); TRY(put_body(h, h->d.vx, b->vx, nullptr));
+    TRY(put_body(h, h->d.vy, b->vy, nullptr)); TRY(put_body(h, h->d.av, b->av, nullptr));
+    TRY(put_body(h, h->d.ppx, b->ppx, b->px)); TRY(put_body(h, h->d.ppy, b->ppy, b->py));
+    TRY(put_body(h, h->d.angP, b->angP, b->ang));
+    bool any = false;
+    for (float *fa : {b->fx, b->fy, b->tq})
+      if (fa) for (int i = 0; i < h->nB && !any; i++) any = fa[i] != 0.0f;
+    if (b->fx) TRY(put_body(h, h->d.fx, b->fx, nullptr));
+    if (b->fy) TRY(put_body(h, h->d.fy, b->fy, nullptr));
+    if (b->tq) TRY(put_body(h, h->d.tq, b->tq, nullptr));
+    h->forces_pending = any;
+  }
+  if (c) {
+    TRY(put_cons_f(h, h->d.lbx, c->lbx, false)); TRY(put_cons_f(h, h->d.lby, c->lby, false));
+    TRY(put_cons_f(h, h->d.comp, c->compliance, false)); TRY(put_cons_f(h, h->d.target, c->target, false));
+    TRY(put_cons_f(h, h->d.bf, c->break_force, true)); TRY(put_cons_u8(h, h->d.broken, c->broken));
+    TRY(put_cons_f(h, h->d.lam, c->lambda, false)); TRY(put_cons_f(h, h->d.lamx, c->lambda_x, false));
+    TRY(put_cons_f(h, h->d.lamy, c->lambda_y, false));
+  }
+  return 0;
+}
+
+extern "C" int kx_pack_controls(kx_handle *h, const kx_cons_state *c) {
+  if (!h || !c) return fail(KX_EINVAL, "null argument");
+  HIPCHK(hipStreamSynchronize(h->st));
+  TRY(put_cons_f(h, h->d.lbx, c->lbx, false)); TRY(put_cons_f(h, h->d.lby, c->lby, false));
+  TRY(put_cons_f(h, h->d.comp, c->compliance, false)); TRY(put_cons_f(h, h->d.target, c->target, false));
+  TRY(put_cons_f(h, h->d.bf, c->break_force, true)); TRY(put_cons_u8(h, h->d.broken, c->broken));
+  return 0;
+}
+
+static int get_f(kx_handle *h, float *dst, const float *src, int n) {
+  if (dst) HIPCHK(hipMemcpy(dst, src, (size_t)n * 4, hipMemcpyDeviceToHost));
+  return 0;
+}
+static int get_cons_f(kx_handle *h, float *dst, const float *src) {
+  if (!dst) return 0;
+  h->hf.resize(h->nC);
+  HIPCHK(hipMemcpy(h->hf.data(), src, (size_t)h->nC * 4, hipMemcpyDeviceToHost));
+  for (int k = 0; k < h->nC; k++) dst[k] = h->hf[h->h2i[k]];
+  return 0;
+}
+
+extern "C" int kx_download(kx_handle *h, kx_body_state *b, kx_cons_state *c, kx_stats *s) {
+  if (!h) return fail(KX_EINVAL, "null handle");
+  HIPCHK(hipStreamSynchronize(h->st));
+  std::vector<float> tvx, tvy, tav, tpx, tpy, tang;
+  if (b) {
+    TRY(get_f(h, b->px, h->d.px, h->nB)); TRY(get_f(h, b->py, h->d.py, h->nB));
+    TRY(get_f(h, b->vx, h->d.vx, h->nB)); TRY(get_f(h, b->vy, h->d.vy, h->nB));
+    TRY(get_f(h, b->ang, h->d.ang, h->nB)); TRY(get_f(h, b->av, h->d.av, h->nB));
+    TRY(get_f(h, b->ppx, h->d.ppx, h->nB)); TRY(get_f(h, b->ppy, h->d.ppy, h->nB));
+    TRY(get_f(h, b->angP, h->d.angP, h->nB));
+    TRY(get_f(h, b->fx, h->d.fx, h->nB)); TRY(get_f(h, b->fy, h->d.fy, h->nB)); TRY(get_f(h, b->tq, h->d.tq, h->nB));
+  }
+  if (s) {
+    const float *vx = b && b->vx ? b->vx : nullptr, *vy = b && b->vy ? b->vy : nullptr;
+    const float *av = b && b->av ? b->av : nullptr, *px = b && b->px ? b->px : nullptr;
+    const float *py = b && b->py ? b->py : nullptr, *an = b && b->ang ? b->ang : nullptr;
+    auto need = [&](const float *have, std::vector<float> &tmp, const float *dev) -> const float * {
+      if (have) return have;
+      tmp.resize(h->nB);
+      (void)hipMemcpy(tmp.data(), dev, (size_t)h->nB * 4, hipMemcpyDeviceToHost);
+      return tmp.data();
+    };
+    vx = need(vx, tvx, h->d.vx); vy = need(vy, tvy, h->d.vy); av = need(av, tav, h->d.av);
+    px = need(px, tpx, h->d.px); py = need(py, tpy, h->d.py); an = need(an, tang, h->d.ang);
+    // mirrorInto statistics (gpuworld.js:425-441)
+    double maxS2 = 0, maxA = 0;
+    int64_t nan = 0;
+    for (int i = 0; i < h->nB; i++) {
+      double x = vx[i], y = vy[i];
+      double s2 = x * x + y * y;
+      if (s2 > maxS2) maxS2 = s2;
+      double aa = std::fabs((double)av[i]);
+      if (aa > maxA) maxA = aa;
+      if (!std::isfinite(px[i]) || !std::isfinite(py[i]) || !std::isfinite(vx[i]) || !std::isfinite(vy[i]) ||
+          !std::isfinite(an[i]))
+        nan++;
+    }
+    s->max_speed = std::sqrt(maxS2); s->max_ang_speed = maxA; s->nan = nan;
+  }
+  if (c) {
+    TRY(get_cons_f(h, c->lbx, h->d.lbx)); TRY(get_cons_f(h, c->lby, h->d.lby));
+    TRY(get_cons_f(h, c->compliance, h->d.comp)); TRY(get_cons_f(h, c->target, h->d.target));
+    TRY(get_cons_f(h, c->break_force, h->d.bf));
+    TRY(get_cons_f(h, c->lambda, h->d.lam)); TRY(get_cons_f(h, c->lambda_x, h->d.lamx));
+    TRY(get_cons_f(h, c->lambda_y, h->d.lamy));
+    if (c->broken) {
+      h->hi.resize(h->nC);
+      HIPCHK(hipMemcpy(h->hi.data(), h->d.broken, (size_t)h->nC * 4, hipMemcpyDeviceToHost));
+      for (int k = 0; k < h->nC; k++) c->broken[k] = h->hi[h->h2i[k]] ? 1 : 0;
+    }
+  }
+  return 0;
+}
+
+extern "C" int kx_get_cooldown(kx_handle *h, double *cd) {
+  if (!h || !cd) return fail(KX_EINVAL, "null argument");
+  HIPCHK(hipStreamSynchronize(h->st));
+  HIPCHK(hipMemcpy(cd, h->d.cd, (size_t)h->nCr * KX_MAX_ARMS * 8, hipMemcpyDeviceToHost));
+  return 0;
+}
+extern "C" int kx_set_cooldown(kx_handle *h, const double *cd) {
+  if (!h || !cd) return fail(KX_EINVAL, "null argument");
+  HIPCHK(hipStreamSynchronize(h->st));
+  HIPCHK(hipMemcpy(h->d.cd, cd, (size_t)h->nCr * KX_MAX_ARMS * 8, hipMemcpyHostToDevice));
+  return 0;
+}
+
+// ── stepping ────────────────────────────────────────────────────────
+static KxLaw to_law(const kx_law *l) {
+  KxLaw L{};
+  L.seed = l->seed;
+  double t = std::max(0.0, std::min(1.0, l->tone));
+  L.comp = 1.0 / (l->stiff_min + t * (l->stiff_max - l->stiff_min));
+  L.shoulder_limit = l->shoulder_limit; L.joint_limit = l->joint_limit;
+  L.grip_min = l->grip_min; L.grip_max = l->grip_max; L.slip_max = l->slip_max;
+  L.cooldown = l->cooldown; L.attach_min = l->attach_min; L.release_below = l->release_below;
+  return L;
+}
+
+static int run_steps(kx_handle *h, const KxLaw *law, int step0, const int *step_base, int nsteps, bool events) {
+  KxDev d = h->d;
+  d.events = events ? h->ev_dev : nullptr;
+  int launches = 0;
+  int with_forces = h->forces_pending ? 1 : 0;
+  h->forces_pending = false;
+  if (h->path == KX_PATH_PERSISTENT) {
+    int r = kx_persist_launch(d, law, step0, step_base, nsteps, 0, h->wpb, with_forces, h->maxb, h->maxc, h->maxp, h->st);
+    if (r) return fail(KX_EDEVICE, "persistent launch failed: %s", hipGetErrorString(hipGetLastError()));
+    launches = 1;
+  } else {
+    for (int s = 0; s < nsteps; s++) {
+      int r = step_base ? kx_plain_step(d, law, s, step_base, with_forces && s == 0, s, h->st, &launches)
+                        : kx_plain_step(d, law, step0 + s, nullptr, with_forces && s == 0, s, h->st, &launches);
+      if (r) return fail(KX_EDEVICE, "plain launch failed: %s", hipGetErrorString(hipGetLastError()));
+    }
+  }
+  h->last_launches = launches;
+  return 0;
+}
+
+extern "C" int kx_step(kx_handle *h, int32_t nsteps) {
+  if (!h) return fail(KX_EINVAL, "null handle");
+  if (nsteps < 0) return fail(KX_EINVAL, "nsteps < 0");
+  if (nsteps == 0) return 0;
+  HIPCHK(hipEventRecord(h->e0, h->st));
+  TRY(run_steps(h, nullptr, 0, nullptr, nsteps, false));
+  HIPCHK(hipEventRecord(h->e1, h->st));
+  h->timed = true;
+  return 0;
+}
+
+extern "C" int kx_rollout(kx_handle *h, const kx_law *law, int32_t step0, int32_t nsteps, uint32_t *events_host) {
+  if (!h || !law) return fail(KX_EINVAL, "null argument");
+  if (!h->has_creatures) return fail(KX_ENOTSUP, "rollout needs a creature description at create");
+  if (nsteps < 0 || step0 < 0) return fail(KX_EINVAL, "bad step range");
+  if (nsteps == 0) return 0;
+  KxLaw L = to_law(law);
+  size_t need = (size_t)nsteps * h->nCr;
+  if (events_host) {
+    if (need > h->ev_cap) {
+      if (h->ev_dev) HIPCHK(hipFree(h->ev_dev));
+      h->ev_dev = nullptr;
+      HIPCHK(hipMalloc(&h->ev_dev, need * 4));
+      h->ev_cap = need;
+    }
+    HIPCHK(hipMemsetAsync(h->ev_dev, 0, need * 4, h->st));
+  }
+  HIPCHK(hipEventRecord(h->e0, h->st));
+  TRY(run_steps(h, &L, step0, nullptr, nsteps, events_host != nullptr));
+  HIPCHK(hipEventRecord(h->e1, h->st));
+  h->timed = true;
+  if (events_host) {
+    HIPCHK(hipStreamSynchronize(h->st));
+    HIPCHK(hipMemcpy(events_host, h->ev_dev, need * 4, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+extern "C" int kx_rollout_graph(kx_handle *h, const kx_law *law, int32_t step0, int32_t nsteps, int32_t reps) {
+  if (!h || !law) return fail(KX_EINVAL, "null argument");
+  if (!h->has_creatures) return fail(KX_ENOTSUP, "rollout needs a creature description at create");
+  if (nsteps <= 0 || reps <= 0 || step0 < 0) return fail(KX_EINVAL, "bad step range");
+  KxLaw L = to_law(law);
+  HIPCHK(hipStreamSynchronize(h->st));
+  bool rebuild = !h->gexec || h->g_nsteps != nsteps || h->g_path != h->path || memcmp(&h->g_law, &L, sizeof L) != 0;
+  if (rebuild) {
+    if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+    int r = 0;
+    if (h->path == KX_PATH_PERSISTENT) {
+      KxDev d = h->d;
+      d.events = nullptr;
+      r = kx_persist_launch(d, &L, 0, h->step_base, nsteps, 0, h->wpb, 0, h->maxb, h->maxc, h->maxp, h->st);
+    } else {
+      KxDev d = h->d;
+      d.events = nullptr;
+      int launches = 0;
+      for (int s = 0; s < nsteps && !r; s++) r = kx_plain_step(d, &L, s, h->step_base, 0, 0, h->st, &launches);
+    }
+    if (!r) r = kx_bump_counter(h->step_base, nsteps, h->st);
+    hipError_t ec = hipStreamEndCapture(h->st, &g);
+    if (r || ec != hipSuccess) return fail(KX_EDEVICE, "graph capture failed: %s", hipGetErrorString(ec));
+    HIPCHK(hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    h->g_nsteps = nsteps; h->g_path = h->path; h->g_law = L;
+  }
+  HIPCHK(hipMemcpy(h->step_base, &step0, 4, hipMemcpyHostToDevice));
+  HIPCHK(hipEventRecord(h->e0, h->st));
+  for (int r = 0; r < reps; r++) HIPCHK(hipGraphLaunch(h->gexec, h->st));
+  HIPCHK(hipEventRecord(h->e1, h->st));
+  h->timed = true;
+  h->last_launches = reps;
+  return 0;
+}
+
+extern "C" int kx_sync(kx_handle *h) {
+  if (!h) return fail(KX_EINVAL, "null handle");
+  HIPCHK(hipStreamSynchronize(h->st));
+  return 0;
+}
+
+extern "C" int kx_last_timing(const kx_handle *h, double *ms, int32_t *launches) {
+  if (!h) return fail(KX_EINVAL, "null handle");
+  if (!h->timed) return fail(KX_EINVAL, "nothing timed yet");
+  HIPCHK(hipEventSynchronize(h->e1));
+  float t = 0;
+  HIPCHK(hipEventElapsedTime(&t, h->e0, h->e1));
+  if (ms) *ms = t;
+  if (launches) *launches = h->last_launches;
+  return 0;
+}
+
+extern "C" int kx_set_path(kx_handle *h, int32_t path, int32_t wpb) {
+  if (!h) return fail(KX_EINVAL, "null handle");
+  if (path != KX_PATH_PLAIN && path != KX_PATH_PERSISTENT) return fail(KX_EINVAL, "unknown path %d", path);
+  if (path == KX_PATH_PERSISTENT && !h->persist_ok)
+    return fail(KX_ENOTSUP, "persistent path unavailable: %s", h->persist_why.c_str());
+  if (wpb != 0 && wpb != 1 && wpb != 2 && wpb != 4 && wpb != 8) return fail(KX_EINVAL, "waves per block %d", wpb);
+  HIPCHK(hipStreamSynchronize(h->st));
+  h->path = path;
+  if (wpb) h->wpb = wpb;
+  return 0;
+}
+
+// plain-path SDF mode switch (0 linear WGSL scan, 1 cull grid) — exported for
+// the benchmark's like-for-like comparison
+extern "C" int kx_set_plain_sdf(kx_handle *h, int32_t use_grid) {
+  if (!h) return fail(KX_EINVAL, "null handle");
+  HIPCHK(hipStreamSynchronize(h->st));
+  h->d.use_grid = use_grid ? 1 : 0;
+  return 0;
+}

@@ -1,0 +1,86 @@
+// Internal layout shared by the host API (kx_api.cpp) and the kernels.
+//
+// Device layout (HBM, structure of arrays):
+//   bodies       desc order; every creature's bodies are contiguous
+//   constraints  "internal order": creature-major, colour order inside a
+//                creature (the handle order restricted to one creature)
+//   points       grouped by body, stable (gpuworld.js:189-207)
+//   map          rect list + a uniform cull grid (exact for contacts)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define KX_MAX_COLORS 16
+#define KX_MAX_ARMS 8
+#define KX_SLOTS_PER_ARM 16
+#define KX_SLOT_ADH 256
+#define KX_WAVE 64
+
+// Per-creature metadata (internal indices).
+struct KxCreature {
+  int body_off, nb;       // bodies [body_off, body_off+nb)
+  int cons_off, nc;       // internal constraints [cons_off, cons_off+nc)
+  int pt_off, np;         // grouped points [pt_off, pt_off+np)
+  int arm_off, narms;     // arms [arm_off, arm_off+narms) in the arm tables
+  int ncol, tmpl;         // non-empty colour count, morphology bucket
+  uint16_t coff[KX_MAX_COLORS + 1];  // colour offsets inside the creature block
+};
+
+// Device pointers + solver constants, passed by value to every kernel.
+struct KxDev {
+  // bodies
+  float *px, *py, *ang, *vx, *vy, *av, *ppx, *ppy, *angP, *fx, *fy, *tq;
+  const float *im, *ii;
+  // constraints (internal order)
+  const int *ca, *cb;            // global body indices, cb = -1 for world anchors
+  const int *ctype;              // KX_PIN / KX_ANGLE / KX_GRIP
+  const int *ccr;                // creature of the constraint
+  const int *cslot;              // control-law slot (arm*16+m) or -1
+  const int *carm;               // arm index inside the creature (grips), else -1
+  const float *lax, *lay, *rest, *cmin, *cmax;
+  float *lbx, *lby, *comp, *target, *bf;
+  int *broken;
+  float *lam, *lamx, *lamy;
+  const int *col_list;           // handle order -> internal index (per-colour lists)
+  // points (grouped by body)
+  const float *plx, *ply, *pr;
+  const int *bpoff, *bpcnt;      // per body
+  // map
+  const float4 *rects;
+  int nr;
+  const int *cell_off;           // [gnx*gny+1]
+  const int *cell_ids;
+  float gx0, gy0, inv_cs;
+  int gnx, gny;
+  // creatures / arms
+  const KxCreature *cr;
+  const int *arm_tip;            // global body
+  const int *arm_grip;           // internal constraint
+  double *cd;                    // [nCr * KX_MAX_ARMS]
+  uint32_t *events;              // [nsteps * nCr] or null
+  // constants
+  float h, invH, invH2, linDamp, angDamp, maxSpeed, maxAngSpeed;
+  double hd;                     // h in f64 (reflex cooldown arithmetic)
+  int S, I, nB, nC, nCr, nArms;
+  int ncol;                      // non-empty colours in handle order
+  int col_off[KX_MAX_COLORS], col_cnt[KX_MAX_COLORS];
+  int use_grid;                  // plain path: 0 = linear SDF scan (WGSL shape)
+};
+
+// Device copy of the control law (kx_law).
+struct KxLaw {
+  uint32_t seed;
+  double comp;                   // 1/(stiff_min + tone*(stiff_max-stiff_min))
+  double shoulder_limit, joint_limit;
+  double grip_min, grip_max, slip_max, cooldown, attach_min, release_below;
+};
+
+// Launchers (kx_plain.hip / kx_persist.hip). All asynchronous on `st`.
+// step key = (step_base ? *step_base : 0) + step_key
+int kx_plain_step(const KxDev &d, const KxLaw *law, int step_key, const int *step_base, int with_forces,
+                  int ev_row, hipStream_t st, int *launches);
+int kx_persist_launch(const KxDev &d, const KxLaw *law, int step0, const int *step_base_dev,
+                      int nsteps, int ev_row0, int waves_per_block, int with_forces,
+                      int maxb, int maxc, int maxp, hipStream_t st);
+int kx_persist_lds_words(int NB, int NC, int NP);
+int kx_bump_counter(int *counter, int delta, hipStream_t st);

@@ -1,0 +1,156 @@
+// Plain per-colour-launch path: the launch structure of the reference's
+// WebGPU solver (src/gpu/gpuworld.js:344-382 substep/step driving
+// src/gpu/solver.wgsl), rebuilt as HIP kernels over SoA buffers in global
+// coordinates. Per step: [control law] + S x (reset, integrate,
+// I x (one launch per non-empty colour, collisions), finalize) + grip break.
+#include "kx_device.h"
+
+#define PLAIN_BLOCK 256
+
+static inline int nblk(int n) { return (n + PLAIN_BLOCK - 1) / PLAIN_BLOCK; }
+
+// Control law + reflex, one thread per arm (core.js:183-220).
+__global__ __launch_bounds__(PLAIN_BLOCK) void kx_k_arms(KxDev d, KxLaw L, int step_key, const int *step_base,
+                                                         int ev_row) {
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= d.nArms) return;
+  if (step_base) step_key += step_base[0];
+  // creature of this arm: binary search over arm_off
+  int lo = 0, hi = d.nCr - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (d.cr[mid].arm_off <= a) lo = mid; else hi = mid - 1;
+  }
+  int c = lo, arm = a - d.cr[c].arm_off;
+  int tip = d.arm_tip[a], g = d.arm_grip[a];
+  double adh = kx_ctl_u(L.seed, (uint32_t)c, (uint32_t)step_key, KX_SLOT_ADH + arm);
+  double &cd = d.cd[c * KX_MAX_ARMS + arm];
+  int broken = d.broken[g];
+  float bf = d.bf[g], lbx = d.lbx[g], lby = d.lby[g], lx = d.lamx[g], ly = d.lamy[g];
+  uint32_t ev = kx_reflex(L, d.hd * d.S, adh, cd, broken, bf, lbx, lby, lx, ly, d.vx[tip], d.vy[tip],
+                          (double)d.px[tip], (double)d.py[tip], d.ang[tip], d.lax[g], d.lay[g]);
+  d.broken[g] = broken; d.bf[g] = bf; d.lbx[g] = lbx; d.lby[g] = lby; d.lamx[g] = lx; d.lamy[g] = ly;
+  if (ev && d.events) atomicOr(&d.events[(size_t)ev_row * d.nCr + c], ev << (3 * arm));
+}
+
+// Motor targets + compliance under the law, one thread per constraint.
+__global__ __launch_bounds__(PLAIN_BLOCK) void kx_k_motors(KxDev d, KxLaw L, int step_key, const int *step_base) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= d.nC) return;
+  if (step_base) step_key += step_base[0];
+  int slot = d.cslot[k];
+  if (slot < 0) return;
+  d.target[k] = kx_law_target(L, (uint32_t)d.ccr[k], (uint32_t)step_key, slot);
+  d.comp[k] = (float)L.comp;
+}
+
+// resetLambda (solver.wgsl:156-163)
+__global__ __launch_bounds__(PLAIN_BLOCK) void kx_k_reset(KxDev d) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= d.nC) return;
+  d.lam[k] = 0.0f; d.lamx[k] = 0.0f; d.lamy[k] = 0.0f;
+}
+
+// integrate (solver.wgsl:166-205); forces consumed on the first substep
+__global__ __launch_bounds__(PLAIN_BLOCK) void kx_k_integrate(KxDev d, int with_forces) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.nB) return;
+  float vx = d.vx[i], vy = d.vy[i], av = d.av[i], ppx, ppy, angP;
+  float fx = 0.0f, fy = 0.0f, tq = 0.0f;
+  if (with_forces) { fx = d.fx[i]; fy = d.fy[i]; tq = d.tq[i]; d.fx[i] = 0.0f; d.fy[i] = 0.0f; d.tq[i] = 0.0f; }
+  kx_integrate(d, vx, vy, av, d.px[i], d.py[i], d.ang[i], d.im[i], d.ii[i], fx, fy, tq, ppx, ppy, angP);
+  d.vx[i] = vx; d.vy[i] = vy; d.av[i] = av;
+  d.ppx[i] = ppx; d.ppy[i] = ppy; d.angP[i] = angP;
+}
+
+// solveConstraints for one colour (solver.wgsl:208-350)
+__global__ __launch_bounds__(PLAIN_BLOCK) void kx_k_colour(KxDev d, int off, int cnt) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= cnt) return;
+  int k = d.col_list[off + j];
+  if (d.broken[k]) return;
+  int A = d.ca[k], B = d.cb[k];
+  float alpha = d.comp[k] * d.invH2;
+  if (d.ctype[k] == 1) {
+    float lam = d.lam[k];
+    kx_solve_angle(d.angP, d.ii, A, B, d.rest[k], d.target[k], d.cmin[k], d.cmax[k], alpha, lam);
+    d.lam[k] = lam;
+  } else {
+    float lx = d.lamx[k], ly = d.lamy[k];
+    kx_solve_pin(d.ppx, d.ppy, d.angP, d.im, d.ii, A, B, d.lax[k], d.lay[k], d.lbx[k], d.lby[k], alpha, lx, ly);
+    d.lamx[k] = lx; d.lamy[k] = ly;
+  }
+}
+
+// solveCollisions, one thread per body, its points in order (solver.wgsl:373-402)
+__global__ __launch_bounds__(PLAIN_BLOCK) void kx_k_collide(KxDev d) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.nB) return;
+  if (d.im[i] == 0.0f) return;
+  int cnt = d.bpcnt[i];
+  if (cnt == 0) return;
+  int o = d.bpoff[i];
+  float x = d.ppx[i], y = d.ppy[i];
+  float sa, ca;
+  sincosf(d.angP[i], &sa, &ca);
+  for (int q = 0; q < cnt; q++) {
+    float lx = d.plx[o + q], ly = d.ply[o + q];
+    float wx = x + (lx * ca - ly * sa);
+    float wy = y + (lx * sa + ly * ca);
+    KxHit h;
+    bool ok = d.use_grid ? kx_sdf_grid(d, wx, wy, 0.0f, 0.0f, h) : kx_sdf_linear(d, wx, wy, 0.0f, 0.0f, h);
+    if (!ok) continue;
+    float pen = d.pr[o + q] - h.d;
+    if (pen > KX_SLACK_POS) { x += h.nx * pen; y += h.ny * pen; }
+  }
+  d.ppx[i] = x; d.ppy[i] = y;
+}
+
+// finalize (solver.wgsl:405-430)
+__global__ __launch_bounds__(PLAIN_BLOCK) void kx_k_finalize(KxDev d) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= d.nB) return;
+  float px = d.px[i], py = d.py[i], ang = d.ang[i], vx, vy, av, angP = d.angP[i];
+  kx_finalize(d, px, py, ang, vx, vy, av, d.ppx[i], d.ppy[i], angP);
+  d.px[i] = px; d.py[i] = py; d.ang[i] = ang; d.angP[i] = angP;
+  d.vx[i] = vx; d.vy[i] = vy; d.av[i] = av;
+}
+
+// gripBreak (solver.wgsl:433-442): F = |lambda|/h over the last substep
+__global__ __launch_bounds__(PLAIN_BLOCK) void kx_k_grip_break(KxDev d, int ev_row) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= d.nC) return;
+  if (d.ctype[k] != 2 || d.broken[k]) return;
+  float lx = d.lamx[k], ly = d.lamy[k];
+  float f = sqrtf(lx * lx + ly * ly) * d.invH;
+  if (f > d.bf[k]) {
+    d.broken[k] = 1; d.lamx[k] = 0.0f; d.lamy[k] = 0.0f;
+    int arm = d.carm[k];
+    if (d.events && arm >= 0) atomicOr(&d.events[(size_t)ev_row * d.nCr + d.ccr[k]], 4u << (3 * arm));
+  }
+}
+
+int kx_plain_step(const KxDev &d, const KxLaw *law, int step_key, const int *step_base, int with_forces,
+                  int ev_row, hipStream_t st, int *launches) {
+  int n = 0;
+  if (law) {
+    hipLaunchKernelGGL(kx_k_motors, dim3(nblk(d.nC)), dim3(PLAIN_BLOCK), 0, st, d, *law, step_key, step_base); n++;
+    if (d.nArms > 0) {
+      hipLaunchKernelGGL(kx_k_arms, dim3(nblk(d.nArms)), dim3(PLAIN_BLOCK), 0, st, d, *law, step_key, step_base, ev_row); n++;
+    }
+  }
+  for (int s = 0; s < d.S; s++) {
+    hipLaunchKernelGGL(kx_k_reset, dim3(nblk(d.nC)), dim3(PLAIN_BLOCK), 0, st, d); n++;
+    hipLaunchKernelGGL(kx_k_integrate, dim3(nblk(d.nB)), dim3(PLAIN_BLOCK), 0, st, d, with_forces && s == 0); n++;
+    for (int it = 0; it < d.I; it++) {
+      for (int c = 0; c < d.ncol; c++) {
+        hipLaunchKernelGGL(kx_k_colour, dim3(nblk(d.col_cnt[c])), dim3(PLAIN_BLOCK), 0, st, d, d.col_off[c], d.col_cnt[c]); n++;
+      }
+      hipLaunchKernelGGL(kx_k_collide, dim3(nblk(d.nB)), dim3(PLAIN_BLOCK), 0, st, d); n++;
+    }
+    hipLaunchKernelGGL(kx_k_finalize, dim3(nblk(d.nB)), dim3(PLAIN_BLOCK), 0, st, d); n++;
+  }
+  hipLaunchKernelGGL(kx_k_grip_break, dim3(nblk(d.nC)), dim3(PLAIN_BLOCK), 0, st, d, ev_row); n++;
+  if (launches) *launches += n;
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}

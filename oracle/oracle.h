@@ -87,6 +87,7 @@ void or_apply_control(OrWorld *w, int ci, double tone, const double *shoulder,
  * tone 0.5, then the adhesion reflex; step_index keys the PRNG.
  * Appends reflex events to the event log. */
 void or_apply_law(OrWorld *w, uint32_t seed, uint32_t step_index);
+void or_apply_law_one(OrWorld *w, uint32_t seed, uint32_t step_index, int creature, uint32_t key);
 
 /* step */
 void or_step(OrWorld *w);                /* S substeps + grip break (f64 or f32 emulation) */

@@ -484,23 +484,26 @@ void or_apply_control(OrWorld *w, int ci, double tone, const double *shoulder, c
     reflex(w, ci, a, adhesion[a]);
   }
 }
-void or_apply_law(OrWorld *w, uint32_t seed, uint32_t step) {
+/* the control law for creature ci, keyed in the PRNG as creature `key`
+ * (the batch index when ci lives in a sub-world) */
+void or_apply_law_one(OrWorld *w, uint32_t seed, uint32_t step, int ci, uint32_t key) {
   double comp = tone_to_compliance(0.5);
-  for (int ci = 0; ci < w->ncr; ci++) {
-    OrCreature *K = &w->cr[ci];
-    for (int a = 0; a < K->arms; a++) {
-      OrArm *arm = &K->arm[a];
-      for (int m = 0; m < arm->nmotors; m++) {
-        double u = or_ctl_u(seed, ci, step, a * OR_SLOTS_PER_ARM + m);
-        double lim = m == 0 ? B_SHOULDER_LIMIT : B_JOINT_LIMIT;
-        double t = (2 * u - 1) * lim;
-        OrCons *c = CONS_ID(w, arm->motor[m]);
-        c->compliance = comp;
-        c->target = or_js_max(-lim, or_js_min(lim, t));
-      }
-      reflex(w, ci, a, or_ctl_u(seed, ci, step, OR_SLOT_ADH + a));
+  OrCreature *K = &w->cr[ci];
+  for (int a = 0; a < K->arms; a++) {
+    OrArm *arm = &K->arm[a];
+    for (int m = 0; m < arm->nmotors; m++) {
+      double u = or_ctl_u(seed, key, step, a * OR_SLOTS_PER_ARM + m);
+      double lim = m == 0 ? B_SHOULDER_LIMIT : B_JOINT_LIMIT;
+      double t = (2 * u - 1) * lim;
+      OrCons *c = CONS_ID(w, arm->motor[m]);
+      c->compliance = comp;
+      c->target = or_js_max(-lim, or_js_min(lim, t));
     }
+    reflex(w, ci, a, or_ctl_u(seed, key, step, OR_SLOT_ADH + a));
   }
+}
+void or_apply_law(OrWorld *w, uint32_t seed, uint32_t step) {
+  for (int ci = 0; ci < w->ncr; ci++) or_apply_law_one(w, seed, step, ci, (uint32_t)ci);
 }
 
 /* ── solver (xpbd.js:257-504) ────────────────────────────────────── */

@@ -65,6 +65,7 @@ def lib():
             "or_attach_grip": (None, [P, i, i, d]),
             "or_apply_control": (None, [P, i, d, dp, dp, dp]),
             "or_apply_law": (None, [P, u32, u32]),
+            "or_apply_law_one": (None, [P, u32, u32, i, u32]),
             "or_step": (None, [P]),
             "or_substep": (None, [P]),
             "or_solve_collisions": (None, [P]),

@@ -1,0 +1,253 @@
+"""GPU parity of both device paths against the CPU oracle.
+
+Tolerance (binary32 device vs binary64 oracle, one step from identical
+f32-rounded state): max position error <= 1e-3 px. Creatures whose discrete
+events (grip attach / release / break) differ in that step are reported
+separately and excluded from the position bound; the event sequences
+themselves must match exactly over the first step window.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import decode_events, load_state, load_state_subset, oracle_subset, oracle_twin
+from kraken_3_gpu_amd import _lib as K
+from kraken_3_gpu_amd import gpuworld as G
+from kraken_3_gpu_amd import world as W
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+PATHS = {"plain": K.KX_PATH_PLAIN, "persistent": K.KX_PATH_PERSISTENT}
+TOL_PX = 1e-3
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _setup(maker, path):
+    w = maker()
+    o = oracle_twin(w)
+    g = G.create(w, path=PATHS[path])
+    G.upload(g)
+    return w, o, g
+
+
+def _pull_state(g):
+    """Device state -> g.cpu (bodies, lambda, broken, control fields, cooldown)."""
+    w = g.cpu
+    G.download(g)
+    ctl = G.download_controls(g)
+    w.lbx[:], w.lby[:] = ctl["lbx"], ctl["lby"]
+    w.compliance[:], w.target[:], w.breakForce[:] = ctl["compliance"], ctl["target"], ctl["breakForce"]
+    G.download_cooldown(g)
+
+
+def _oracle_events(o):
+    return sorted(o.events())
+
+
+def _creatures_with_event_diff(ev_a, ev_b):
+    sa, sb = set(ev_a), set(ev_b)
+    return {e[1] for e in sa ^ sb}
+
+
+def _pos_err(w, o, creature_of_body, exclude):
+    b = o.bodies()
+    err = np.maximum(np.abs(w.px - b["px"]), np.abs(w.py - b["py"]))
+    mask = ~np.isin(creature_of_body, list(exclude)) if exclude else np.ones(len(err), bool)
+    return float(err[mask].max()) if mask.any() else 0.0, err
+
+
+@pytest.mark.parametrize("path", ["plain", "persistent"])
+def test_coloring_and_layout_match_oracle(path):
+    say(f"[case] coloring path={path}")
+    w, o, g = _setup(lambda: W.world_uneven(24, seed=3, sort_by_size=False), path)
+    c = o.cons()
+    assert np.array_equal(w.type, c["type"]) and np.array_equal(w.a, c["a"]) and np.array_equal(w.b, c["b"])
+    off, cnt = o.color_info()
+    assert w.colorOff.tolist() == off and w.colorCnt.tolist() == cnt
+    G.destroy(g)
+
+
+@pytest.mark.parametrize("path", ["plain", "persistent"])
+@pytest.mark.parametrize("warm", [0, 12, 40])
+def test_one_step_rollout_parity(path, warm):
+    """Device control law + reflex + step vs oracle, from identical state."""
+    from helpers import creature_of_bodies
+    seed = 2024
+    w, o, g = _setup(lambda: W.world_cpu64(32, seed=1), path)
+    law = W.default_law(seed)
+    if warm:
+        say(f"[case] warm-up rollout path={path} steps={warm}")
+        G.rollout(g, law, 0, warm)
+    _pull_state(g)
+    load_state(o, w)
+    o.set_step_counter(warm)
+    o.clear_events()
+    o.apply_law(seed, warm)
+    o.step()
+    say(f"[case] one-step rollout path={path} warm={warm}")
+    ev_gpu = decode_events(G.rollout(g, law, warm, 1, events=True))
+    ev_gpu = [(warm + s, c, a, k) for (s, c, a, k) in ev_gpu]
+    G.download(g)
+    ev_or = _oracle_events(o)
+    flips = _creatures_with_event_diff(ev_or, ev_gpu)
+    crb = creature_of_bodies(w)
+    err, _ = _pos_err(w, o, crb, flips)
+    say(f"    path={path} warm={warm} max|dpos|={err:.3e} px events={len(ev_or)} flips={sorted(flips)}")
+    assert len(flips) <= 1
+    assert err <= TOL_PX
+
+
+@pytest.mark.parametrize("path", ["plain", "persistent"])
+def test_host_driven_step_parity(path):
+    """The drop-in path: controls packed on the host (packControls) then
+    kx_step, exactly the gpuTrainStep order (gputrain.js:93-100)."""
+    from helpers import creature_of_bodies
+    seed = 7
+    w, o, g = _setup(lambda: W.world_cpu64(16, seed=2), path)
+    law = W.default_law(seed)
+    say(f"[case] host-driven warm-up path={path}")
+    G.rollout(g, law, 0, 8)
+    _pull_state(g)
+    load_state(o, w)
+    o.apply_law(seed, 8)                    # controls + reflex on the host (oracle)
+    c = o.cons()
+    w.compliance[:], w.target[:] = c["compliance"], c["target"]
+    w.lbx[:], w.lby[:], w.breakForce[:] = c["lbx"], c["lby"], c["breakForce"]
+    w.broken[:] = c["broken"]
+    G.upload_controls(g)
+    load_state(o, w)                        # identical f32-rounded controls on both sides
+    o.step()
+    say(f"[case] host-driven kx_step path={path}")
+    G.step(g)
+    G.download(g)
+    err, _ = _pos_err(w, o, creature_of_bodies(w), set())
+    say(f"    host-driven path={path} max|dpos|={err:.3e}")
+    assert err <= TOL_PX
+    assert np.array_equal(w.broken, o.cons()["broken"].astype(np.uint8))
+
+
+@pytest.mark.parametrize("path", ["plain", "persistent"])
+def test_event_sequence_first_window(path):
+    seed, window = 99, 24
+    w, o, g = _setup(lambda: W.world_cpu64(32, seed=1), path)
+    load_state(o, w)
+    law = W.default_law(seed)
+    for s in range(window):
+        o.apply_law(seed, s)
+        o.step()
+    say(f"[case] event window path={path} steps={window}")
+    ev_gpu = decode_events(G.rollout(g, law, 0, window, events=True))
+    ev_or = _oracle_events(o)
+    say(f"    events oracle={len(ev_or)} gpu={len(ev_gpu)}")
+    assert len(ev_or) > 50
+    assert ev_gpu == ev_or
+
+
+@pytest.mark.parametrize("path", ["plain", "persistent"])
+def test_passive_rest_is_exact(path):
+    """G1 (gpu.html:223-268): no drive -> bit-exact fixpoint, v stays 0."""
+    w, o, g = _setup(lambda: W.world_cpu64(4, seed=11), path)
+    px0, py0 = np.float32(w.px).copy(), np.float32(w.py).copy()
+    say(f"[case] passive path={path}")
+    G.step(g, 240)
+    st = G.download(g)
+    assert np.array_equal(np.float32(w.px), px0) and np.array_equal(np.float32(w.py), py0)
+    assert st["maxSpeed"] == 0.0 and st["nan"] == 0
+
+
+@pytest.mark.parametrize("path", ["plain", "persistent"])
+def test_determinism(path):
+    runs = []
+    for _ in range(2):
+        w, o, g = _setup(lambda: W.world_cpu64(16, seed=5), path)
+        say(f"[case] determinism path={path}")
+        G.rollout(g, W.default_law(3), 0, 60)
+        G.download(g)
+        runs.append((w.px.copy(), w.py.copy(), w.broken.copy()))
+        G.destroy(g)
+    assert all(np.array_equal(a, b) for a, b in zip(*runs))
+
+
+def test_graph_replay_matches_eager_launches():
+    law = W.default_law(11)
+    w1, _, g1 = _setup(lambda: W.world_uneven(64, seed=2), "persistent")
+    w2, _, g2 = _setup(lambda: W.world_uneven(64, seed=2), "persistent")
+    say("[case] graph vs eager")
+    for r in range(4):
+        G.rollout(g1, law, 5 * r, 5)
+    G.rollout_graph(g2, law, 0, 5, 4)
+    G.download(g1)
+    G.download(g2)
+    assert np.array_equal(w1.px, w2.px) and np.array_equal(w1.ang, w2.ang)
+    assert np.array_equal(w1.broken, w2.broken)
+
+
+def test_plain_and_persistent_agree_uneven():
+    """Ragged morphologies (4-8 arms x 5-9 segments, ring 6-10) through both paths."""
+    from helpers import creature_of_bodies
+    law = W.default_law(5)
+    wp, o, gp = _setup(lambda: W.world_uneven(48, seed=4), "plain")
+    wq, _, gq = _setup(lambda: W.world_uneven(48, seed=4), "persistent")
+    load_state(o, wp)
+    o.apply_law(5, 0)
+    o.step()
+    say("[case] uneven one step plain+persistent")
+    evp = decode_events(G.rollout(gp, law, 0, 1, events=True))
+    evq = decode_events(G.rollout(gq, law, 0, 1, events=True))
+    G.download(gp)
+    G.download(gq)
+    ev_or = _oracle_events(o)
+    crb = creature_of_bodies(wp)
+    for w, ev in ((wp, evp), (wq, evq)):
+        flips = _creatures_with_event_diff(ev_or, ev)
+        err, _ = _pos_err(w, o, crb, flips)
+        say(f"    uneven max|dpos|={err:.3e} flips={sorted(flips)}")
+        assert err <= TOL_PX and len(flips) <= 1
+
+
+def test_full_size_uniform_properties_and_subset_parity():
+    """4096 creatures (the uniform batch): limits, NaN, and one-step parity of
+    a 64-creature subset against the oracle."""
+    from helpers import creature_of_bodies
+    seed = 31
+    w = W.world_uniform(4096, seed=1)
+    g = G.create(w, path=K.KX_PATH_PERSISTENT)
+    G.upload(g)
+    law = W.default_law(seed)
+    say("[case] uniform 4096 rollout 60")
+    G.rollout(g, law, 0, 60)
+    st = G.download(g)
+    assert st["nan"] == 0
+    assert st["maxSpeed"] <= 360 * 1.001 and st["maxAngSpeed"] <= 18 * 1.001
+    _pull_state(g)
+    sub = list(range(0, 4096, 64))
+    o, bidx, cidx = oracle_subset(w, sub)
+    load_state_subset(o, w, sub, bidx, cidx)
+    o.set_step_counter(60)
+    o.clear_events()
+    # the subset world numbers its creatures 0..63; the control law keys on
+    # the batch index, so the oracle applies it per creature with that key
+    _apply_law_keyed(o, law.seed, 60, sub)
+    o.step()
+    say("[case] uniform 4096 one more step")
+    G.rollout(g, law, 60, 1)
+    G.download(g)
+    b = o.bodies()
+    err = np.maximum(np.abs(w.px[bidx] - b["px"]), np.abs(w.py[bidx] - b["py"]))
+    say(f"    uniform subset max|dpos|={err.max():.3e}")
+    assert err.max() <= TOL_PX
+
+
+def _apply_law_keyed(o, seed, step, keys):
+    """or_apply_law keys the PRNG by the oracle's own creature index; a
+    subset needs the batch index, so remap through a seed-preserving loop."""
+    lib = po.lib()
+    import ctypes as C
+    for local, key in enumerate(keys):
+        lib.or_apply_law_one(o.p, C.c_uint32(seed), C.c_uint32(step), local, key)
