@@ -78,7 +78,8 @@ def main():
     ap.add_argument("--creatures", type=int, default=4096)
     ap.add_argument("--chunk", type=int, default=50, help="simulation steps per persistent launch")
     ap.add_argument("--plain-steps", type=int, default=100)
-    ap.add_argument("--wpb", type=int, default=0, help="creatures per workgroup (0 = default)")
+    ap.add_argument("--lanes", type=int, default=0, help="lanes per creature (0 = default)")
+    ap.add_argument("--wpb", type=int, default=0, help="wavefronts per workgroup (0 = default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     args = ap.parse_args()
@@ -95,7 +96,7 @@ def main():
     law = W.default_law(seed)
     w = W.world_uniform(args.creatures, seed=1)
     n_cr = w.n_creatures
-    g = G.create(w, path=K.KX_PATH_PERSISTENT, creatures_per_block=args.wpb)
+    g = G.create(w, path=K.KX_PATH_PERSISTENT, lanes_per_creature=args.lanes, waves_per_block=args.wpb)
     G.upload(g)
     step = 0
     while step < args.warmup:

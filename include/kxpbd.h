@@ -75,7 +75,8 @@ typedef struct {
   double h;                          /* seconds per substep */
   double lin_drag, ang_drag, max_speed, max_ang_speed;
   int32_t path;                      /* KX_PATH_* */
-  int32_t creatures_per_block;       /* persistent path: 1,2,4,8; 0 = default (4) */
+  int32_t lanes_per_creature;        /* persistent path: 8, 16, 32 or 64; 0 = default */
+  int32_t waves_per_block;           /* persistent path: 1, 2 or 4; 0 = default */
 } kx_params;
 
 /* Body state in SoA (gpuworld.js:250-263 upload; 417-441 mirrorInto).
@@ -129,6 +130,14 @@ int kx_counts(const kx_handle *h, int32_t *n_bodies, int32_t *n_constraints,
 int kx_constraint_order(const kx_handle *h, int32_t *order, int32_t *color);
 /* colour offsets/counts in handle order (colorInfo, gpuworld.js:286-297) */
 int kx_color_info(const kx_handle *h, uint32_t *off16, uint32_t *cnt16);
+/* pure host planning of the persistent path's constraint rounds for one
+ * creature (local body indices, constraints in colour order): rounds of at
+ * most G constraints, all pin/grip (rtype 0) or all angle (rtype 1), no two
+ * sharing a body, every pair that shares a body kept in colour order.
+ * sched_out[R*G] holds constraint indices or -1; returns R (rounds), or a
+ * negative code if max_rounds is too small. */
+int kx_plan_rounds(int32_t n_bodies, int32_t n_constraints, const int32_t *c_a, const int32_t *c_b,
+                   const int32_t *c_type, int32_t G, int32_t *sched_out, int32_t *rtype_out, int32_t max_rounds);
 /* pure host colouring, what kx_create applies (xpbd.js:98-126): colours the
  * constraints whose color_in entry is < 0 (color_in may be NULL) and returns
  * color_out[n_constraints] plus the stable colour sort order_out[k] = desc
@@ -165,7 +174,13 @@ int kx_rollout_graph(kx_handle *h, const kx_law *law, int32_t step0, int32_t nst
 /* average device time (ms) of the last kx_step/kx_rollout launch sequence,
  * measured with HIP events on the handle's stream */
 int kx_last_timing(const kx_handle *h, double *ms_total, int32_t *launches);
-int kx_set_path(kx_handle *h, int32_t path, int32_t creatures_per_block);
+/* switch path; for the persistent path also its shape (0 keeps the current
+ * value): G lanes per creature (64/G creatures per wavefront) and wavefronts
+ * per workgroup; rebuilds the per-morphology round schedules for G */
+int kx_set_path(kx_handle *h, int32_t path, int32_t lanes_per_creature, int32_t waves_per_block);
+/* persistent path: rounds per iteration of each morphology template for the
+ * current shape (out[n_templates]); returns the template count */
+int kx_schedule_info(const kx_handle *h, int32_t *rounds, int32_t max);
 /* plain path only: 0 = linear scan over every rect per query (the shape of
  * src/gpu/solver.wgsl:114-147), 1 = the exact cull grid the persistent
  * path always uses */

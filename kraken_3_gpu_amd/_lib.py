@@ -36,7 +36,8 @@ class CreatureDesc(C.Structure):
 
 class Params(C.Structure):
     _fields_ = [("substeps", i32), ("iterations", i32), ("h", f64), ("lin_drag", f64), ("ang_drag", f64),
-                ("max_speed", f64), ("max_ang_speed", f64), ("path", i32), ("creatures_per_block", i32)]
+                ("max_speed", f64), ("max_ang_speed", f64), ("path", i32), ("lanes_per_creature", i32),
+                ("waves_per_block", i32)]
 
 
 class BodyState(C.Structure):
@@ -68,6 +69,7 @@ EXPORTS = {
     "kx_color_info": (i32, [C.c_void_p, u32p, u32p]),
     "kx_group_points": (i32, [i32, i32p, i32, i32p, i32p]),
     "kx_color_constraints": (i32, [i32, i32, i32p, i32p, i32p, i32p, i32p]),
+    "kx_plan_rounds": (i32, [i32, i32, i32p, i32p, i32p, i32, i32p, i32p, i32]),
     "kx_upload": (i32, [C.c_void_p, C.POINTER(BodyState), C.POINTER(ConsState)]),
     "kx_pack_controls": (i32, [C.c_void_p, C.POINTER(ConsState)]),
     "kx_download": (i32, [C.c_void_p, C.POINTER(BodyState), C.POINTER(ConsState), C.POINTER(Stats)]),
@@ -78,7 +80,8 @@ EXPORTS = {
     "kx_sync": (i32, [C.c_void_p]),
     "kx_rollout_graph": (i32, [C.c_void_p, C.POINTER(Law), i32, i32, i32]),
     "kx_last_timing": (i32, [C.c_void_p, f64p, i32p]),
-    "kx_set_path": (i32, [C.c_void_p, i32, i32]),
+    "kx_set_path": (i32, [C.c_void_p, i32, i32, i32]),
+    "kx_schedule_info": (i32, [C.c_void_p, i32p, i32]),
     "kx_set_plain_sdf": (i32, [C.c_void_p, i32]),
     "kx_builder_new": (C.c_void_p, []),
     "kx_builder_free": (None, [C.c_void_p]),

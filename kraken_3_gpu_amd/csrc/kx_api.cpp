@@ -19,6 +19,9 @@
 
 #include "../../include/kxpbd.h"
 #include "kx_internal.h"
+#include "kx_sched.h"
+
+#include <map>
 
 static thread_local std::string g_err;
 
@@ -49,7 +52,7 @@ struct DevBuf {
 struct kx_handle {
   int nB = 0, nC = 0, nP = 0, nR = 0, nCr = 0, nArms = 0;
   kx_params prm{};
-  int path = KX_PATH_PERSISTENT, wpb = 4;
+  int path = KX_PATH_PERSISTENT, G = 16, wpb = 1;
   bool persist_ok = false, has_creatures = false, forces_pending = false;
   std::string persist_why;
   int maxb = 0, maxc = 0, maxp = 0;
@@ -74,6 +77,12 @@ struct kx_handle {
   hipGraphExec_t gexec = nullptr;
   int g_nsteps = -1, g_path = -1;
   KxLaw g_law{};
+  // persistent path: morphology templates and the wave map for shape (G, wpb)
+  std::vector<TmplHost> tmpl_host;
+  std::vector<int> cr_tmpl;
+  std::vector<int> tmpl_rounds;
+  KxPersistCfg pc{};
+  void *pc_bufs[4] = {nullptr, nullptr, nullptr, nullptr};
   // scratch host staging
   std::vector<float> hf;
   std::vector<int> hi;
@@ -122,6 +131,24 @@ static int colour_constraints(int nB, int nC, const int32_t *ca, const int32_t *
     }
   }
   return 0;
+}
+
+extern "C" int kx_plan_rounds(int32_t nb, int32_t nc, const int32_t *ca, const int32_t *cb, const int32_t *ct,
+                              int32_t G, int32_t *sched_out, int32_t *rtype_out, int32_t max_rounds) {
+  if (nb < 1 || nc < 0 || G < 1 || G > 64) return fail(KX_EINVAL, "bad arguments");
+  TmplHost t;
+  t.nb = nb;
+  for (int k = 0; k < nc; k++) {
+    if (ca[k] < 0 || ca[k] >= nb || cb[k] < -1 || cb[k] >= nb) return fail(KX_EINVAL, "constraint %d out of range", k);
+    t.type.push_back(ct[k]); t.a.push_back(ca[k]); t.b.push_back(cb[k]);
+  }
+  std::vector<int> sched, rtype;
+  int R = 0;
+  kx_schedule_rounds(t, G, sched, rtype, R);
+  if (R > max_rounds) return fail(KX_EINVAL, "needs %d rounds > %d", R, max_rounds);
+  for (size_t i = 0; i < sched.size(); i++) sched_out[i] = sched[i];
+  for (int r = 0; r < R; r++) rtype_out[r] = rtype[r];
+  return R;
 }
 
 extern "C" int kx_color_constraints(int32_t nB, int32_t nC, const int32_t *ca, const int32_t *cb,
@@ -212,11 +239,68 @@ static int build_grid(kx_handle *h, const float *rects, int nR, float rmax) {
   return 0;
 }
 
+// Pack every template's schedule for G lanes, map creatures onto waves
+// (64/G creatures of one template per wave, workgroups template-uniform).
+static int build_persist(kx_handle *h) {
+  const int G = h->G, CPW = KX_WAVE / G, WPB = h->wpb;
+  std::vector<uint32_t> pool;
+  std::vector<KxTmpl> T;
+  int words = 0, NB = 1, NC = 1;
+  h->tmpl_rounds.clear();
+  for (auto &t : h->tmpl_host) {
+    KxTmpl k = kx_pack_template(t, G, pool);
+    T.push_back(k);
+    words = std::max(words, k.words);
+    NB = std::max(NB, k.nb); NC = std::max(NC, k.nc);
+    h->tmpl_rounds.push_back(k.R);
+  }
+  std::vector<std::vector<int>> members(T.size());
+  for (int c = 0; c < h->nCr; c++) members[h->cr_tmpl[c]].push_back(c);
+  std::vector<int> wave_cr, wave_tmpl;
+  for (size_t t = 0; t < T.size(); t++) {
+    auto &m = members[t];
+    if (m.empty()) continue;
+    int nw = (int)((m.size() + CPW - 1) / CPW);
+    nw = (nw + WPB - 1) / WPB * WPB;
+    for (int wv = 0; wv < nw; wv++) {
+      wave_tmpl.push_back((int)t);
+      for (int j = 0; j < CPW; j++) {
+        size_t idx = (size_t)wv * CPW + j;
+        wave_cr.push_back(idx < m.size() ? m[idx] : -1);
+      }
+    }
+  }
+  int stride = kx_persist_cr_words(NB, NC) | 1;
+  size_t lds = ((size_t)words + (size_t)WPB * CPW * stride) * 4;
+  if (lds > 160 * 1024)
+    return fail(KX_ENOTSUP, "persistent shape G=%d x %d waves needs %zu B of LDS (> 160 KiB)", G, WPB, lds);
+  for (void *&p : h->pc_bufs) { if (p) (void)hipFree(p); p = nullptr; }
+  HIPCHK(hipMalloc(&h->pc_bufs[0], std::max<size_t>(pool.size(), 1) * 4));
+  HIPCHK(hipMalloc(&h->pc_bufs[1], std::max<size_t>(T.size(), 1) * sizeof(KxTmpl)));
+  HIPCHK(hipMalloc(&h->pc_bufs[2], std::max<size_t>(wave_cr.size(), 1) * 4));
+  HIPCHK(hipMalloc(&h->pc_bufs[3], std::max<size_t>(wave_tmpl.size(), 1) * 4));
+  HIPCHK(hipMemcpy(h->pc_bufs[0], pool.data(), pool.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->pc_bufs[1], T.data(), T.size() * sizeof(KxTmpl), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->pc_bufs[2], wave_cr.data(), wave_cr.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->pc_bufs[3], wave_tmpl.data(), wave_tmpl.size() * 4, hipMemcpyHostToDevice));
+  KxPersistCfg &P = h->pc;
+  P.pool = (const uint32_t *)h->pc_bufs[0];
+  P.tmpl = (const KxTmpl *)h->pc_bufs[1];
+  P.wave_cr = (const int *)h->pc_bufs[2];
+  P.wave_tmpl = (const int *)h->pc_bufs[3];
+  P.tmpl_words = words;
+  P.cr_stride = stride;
+  P.nwaves = (int)wave_tmpl.size();
+  P.NB = NB; P.NC = NC;
+  return 0;
+}
+
 static void destroy_impl(kx_handle *h) {
   if (!h) return;
   if (h->st) (void)hipStreamSynchronize(h->st);
   if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
   for (auto &b : h->bufs) (void)hipFree(b.p);
+  for (void *p : h->pc_bufs) if (p) (void)hipFree(p);
   if (h->ev_dev) (void)hipFree(h->ev_dev);
   if (h->e0) (void)hipEventDestroy(h->e0);
   if (h->e1) (void)hipEventDestroy(h->e1);
@@ -416,12 +500,59 @@ static int create_impl(const kx_world_desc *w, const kx_creature_desc *cd, const
   TRY(dput((float *)r4, w->rects, 4 * (size_t)nR));
   d.rects = r4; d.nr = nR;
   TRY(build_grid(h, w->rects, nR, rmax));
+  // morphology templates (creature-local static structure, deduplicated)
+  if (h->persist_ok) {
+    std::map<std::string, int> seen;
+    h->cr_tmpl.assign(ncomp, 0);
+    std::vector<int> arm_of_grip(nC, -1);
+    for (int k = 0; k < nC; k++) if (carm[k] >= 0) arm_of_grip[k] = carm[k];
+    for (int c = 0; c < ncomp; c++) {
+      const KxCreature &K = h->cr[c];
+      TmplHost t;
+      t.nb = K.nb;
+      for (int j = 0; j < K.nc; j++) {
+        int i = K.cons_off + j;
+        t.type.push_back(it[i]); t.a.push_back(ia[i] - K.body_off); t.b.push_back(ib[i] < 0 ? -1 : ib[i] - K.body_off);
+        t.lax.push_back(flax[i]); t.lay.push_back(flay[i]); t.rest.push_back(frest[i]);
+        t.mn.push_back(fmin[i]); t.mx.push_back(fmax[i]);
+        if (cslot[i] >= 0) { t.mot_k.push_back(j); t.mot_slot.push_back(cslot[i]); }
+        if (it[i] == KX_GRIP) { t.grip_k.push_back(j); t.grip_arm.push_back(arm_of_grip[i]); }
+      }
+      for (int b = 0; b < K.nb; b++) {
+        int g = K.body_off + b;
+        t.im.push_back(w->inv_mass[g]); t.ii.push_back(w->inv_inertia[g]);
+        t.poff.push_back(prange[g] - K.pt_off); t.pcnt.push_back(prange[nB + g]);
+      }
+      for (int q = K.pt_off; q < K.pt_off + K.np; q++) { t.plx.push_back(plx[q]); t.ply.push_back(ply[q]); t.pr.push_back(pr[q]); }
+      for (int a = 0; a < K.narms; a++) {
+        t.arm_tip.push_back(arm_tip[K.arm_off + a] - K.body_off);
+        t.arm_grip.push_back(arm_grip[K.arm_off + a] - K.cons_off);
+      }
+      std::string key = kx_template_key(t);
+      auto f = seen.find(key);
+      if (f == seen.end()) {
+        int id = (int)h->tmpl_host.size();
+        seen.emplace(key, id);
+        h->tmpl_host.push_back(std::move(t));
+        h->cr_tmpl[c] = id;
+      } else {
+        h->cr_tmpl[c] = f->second;
+      }
+    }
+    if (p->lanes_per_creature) h->G = p->lanes_per_creature;
+    if (p->waves_per_block) h->wpb = p->waves_per_block;
+    if (h->G != 8 && h->G != 16 && h->G != 32 && h->G != 64) return fail(KX_EINVAL, "lanes_per_creature %d", h->G);
+    if (h->wpb != 1 && h->wpb != 2 && h->wpb != 4) return fail(KX_EINVAL, "waves_per_block %d", h->wpb);
+    TRY(build_persist(h));
+  }
   // creatures / arms
   KxCreature *dcr;
   TRY(dalloc(h, &dcr, ncomp)); TRY(dput(dcr, h->cr.data(), ncomp)); d.cr = dcr;
   TRY(dalloc(h, &ip, arm_tip.size())); TRY(dput(ip, arm_tip.data(), arm_tip.size())); d.arm_tip = ip;
   TRY(dalloc(h, &ip, arm_grip.size())); TRY(dput(ip, arm_grip.data(), arm_grip.size())); d.arm_grip = ip;
   TRY(dalloc(h, &d.cd, (size_t)ncomp * KX_MAX_ARMS));
+  TRY(dalloc(h, &d.org, (size_t)ncomp * 2));
+  TRY(dalloc(h, &ip, nB)); TRY(dput(ip, comp_of.data(), nB)); d.bcr = ip;
   TRY(dalloc(h, &h->step_base, 1));
   // constants (params uniform, gpuworld.js:212-223)
   d.h = (float)p->h;
@@ -442,11 +573,6 @@ static int create_impl(const kx_world_desc *w, const kx_creature_desc *cd, const
   HIPCHK(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
   HIPCHK(hipEventCreate(&h->e0));
   HIPCHK(hipEventCreate(&h->e1));
-  if (p->creatures_per_block) {
-    int c = p->creatures_per_block;
-    if (c != 1 && c != 2 && c != 4 && c != 8) return fail(KX_EINVAL, "creatures_per_block %d", c);
-    h->wpb = c;
-  }
   if (h->path == KX_PATH_PERSISTENT && !h->persist_ok)
     return fail(KX_ENOTSUP, "persistent path unavailable: %s", h->persist_why.c_str());
   return 0;
@@ -646,7 +772,7 @@ static int run_steps(kx_handle *h, const KxLaw *law, int step0, const int *step_
   int with_forces = h->forces_pending ? 1 : 0;
   h->forces_pending = false;
   if (h->path == KX_PATH_PERSISTENT) {
-    int r = kx_persist_launch(d, law, step0, step_base, nsteps, 0, h->wpb, with_forces, h->maxb, h->maxc, h->maxp, h->st);
+    int r = kx_persist_launch(h->G, h->wpb, d, law, step0, step_base, nsteps, 0, with_forces, h->pc, h->st);
     if (r) return fail(KX_EDEVICE, "persistent launch failed: %s", hipGetErrorString(hipGetLastError()));
     launches = 1;
   } else {
@@ -705,6 +831,7 @@ extern "C" int kx_rollout_graph(kx_handle *h, const kx_law *law, int32_t step0, 
   KxLaw L = to_law(law);
   HIPCHK(hipStreamSynchronize(h->st));
   bool rebuild = !h->gexec || h->g_nsteps != nsteps || h->g_path != h->path || memcmp(&h->g_law, &L, sizeof L) != 0;
+  // (kx_set_path drops the cached graph when the persistent shape changes)
   if (rebuild) {
     if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
     hipGraph_t g;
@@ -713,7 +840,7 @@ extern "C" int kx_rollout_graph(kx_handle *h, const kx_law *law, int32_t step0, 
     if (h->path == KX_PATH_PERSISTENT) {
       KxDev d = h->d;
       d.events = nullptr;
-      r = kx_persist_launch(d, &L, 0, h->step_base, nsteps, 0, h->wpb, 0, h->maxb, h->maxc, h->maxp, h->st);
+      r = kx_persist_launch(h->G, h->wpb, d, &L, 0, h->step_base, nsteps, 0, 0, h->pc, h->st);
     } else {
       KxDev d = h->d;
       d.events = nullptr;
@@ -753,16 +880,31 @@ extern "C" int kx_last_timing(const kx_handle *h, double *ms, int32_t *launches)
   return 0;
 }
 
-extern "C" int kx_set_path(kx_handle *h, int32_t path, int32_t wpb) {
+extern "C" int kx_set_path(kx_handle *h, int32_t path, int32_t G, int32_t wpb) {
   if (!h) return fail(KX_EINVAL, "null handle");
   if (path != KX_PATH_PLAIN && path != KX_PATH_PERSISTENT) return fail(KX_EINVAL, "unknown path %d", path);
   if (path == KX_PATH_PERSISTENT && !h->persist_ok)
     return fail(KX_ENOTSUP, "persistent path unavailable: %s", h->persist_why.c_str());
-  if (wpb != 0 && wpb != 1 && wpb != 2 && wpb != 4 && wpb != 8) return fail(KX_EINVAL, "waves per block %d", wpb);
+  if (G != 0 && G != 8 && G != 16 && G != 32 && G != 64) return fail(KX_EINVAL, "lanes_per_creature %d", G);
+  if (wpb != 0 && wpb != 1 && wpb != 2 && wpb != 4) return fail(KX_EINVAL, "waves_per_block %d", wpb);
   HIPCHK(hipStreamSynchronize(h->st));
   h->path = path;
-  if (wpb) h->wpb = wpb;
+  if (path == KX_PATH_PERSISTENT && ((G && G != h->G) || (wpb && wpb != h->wpb))) {
+    int oldG = h->G, oldW = h->wpb;
+    if (G) h->G = G;
+    if (wpb) h->wpb = wpb;
+    int r = build_persist(h);
+    if (r) { h->G = oldG; h->wpb = oldW; std::string keep = g_err; build_persist(h); g_err = keep; return r; }
+    if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+  }
   return 0;
+}
+
+extern "C" int kx_schedule_info(const kx_handle *h, int32_t *rounds, int32_t max) {
+  if (!h) return fail(KX_EINVAL, "null handle");
+  int n = (int)h->tmpl_rounds.size();
+  for (int i = 0; i < n && i < max; i++) rounds[i] = h->tmpl_rounds[i];
+  return n;
 }
 
 // plain-path SDF mode switch (0 linear WGSL scan, 1 cull grid) — exported for

@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "kx_sched.h"
+
 #define KX_MAX_COLORS 16
 #define KX_MAX_ARMS 8
 #define KX_SLOTS_PER_ARM 16
@@ -54,6 +56,8 @@ struct KxDev {
   int gnx, gny;
   // creatures / arms
   const KxCreature *cr;
+  const int *bcr;                // creature of every body
+  float *org;                    // [2*nCr] plain path: integer origin of each creature
   const int *arm_tip;            // global body
   const int *arm_grip;           // internal constraint
   double *cd;                    // [nCr * KX_MAX_ARMS]
@@ -79,8 +83,18 @@ struct KxLaw {
 // step key = (step_base ? *step_base : 0) + step_key
 int kx_plain_step(const KxDev &d, const KxLaw *law, int step_key, const int *step_base, int with_forces,
                   int ev_row, hipStream_t st, int *launches);
-int kx_persist_launch(const KxDev &d, const KxLaw *law, int step0, const int *step_base_dev,
-                      int nsteps, int ev_row0, int waves_per_block, int with_forces,
-                      int maxb, int maxc, int maxp, hipStream_t st);
-int kx_persist_lds_words(int NB, int NC, int NP);
+// Persistent-path launch description (see kx_persist.hip).
+struct KxPersistCfg {
+  const int *wave_cr;       // [nwaves * 64/G] creature ids, -1 = empty slot
+  const int *wave_tmpl;     // [nwaves] template of each wave (uniform per workgroup)
+  const KxTmpl *tmpl;
+  const uint32_t *pool;
+  int tmpl_words;           // LDS words reserved for the template
+  int cr_stride;            // LDS words per creature (odd, avoids bank aliasing)
+  int nwaves;
+  int NB, NC;               // per-creature array sizes in LDS
+};
+int kx_persist_launch(int G, int wpb, const KxDev &d, const KxLaw *law, int step0, const int *step_base,
+                      int nsteps, int ev_row0, int with_forces, const KxPersistCfg &P, hipStream_t st);
+int kx_persist_cr_words(int NB, int NC);
 int kx_bump_counter(int *counter, int delta, hipStream_t st);

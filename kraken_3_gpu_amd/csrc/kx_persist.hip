@@ -1,17 +1,23 @@
-// Persistent batched path: one wavefront owns one creature and keeps its
-// bodies, constraints and collision points in LDS for every substep and
-// iteration of a multi-step launch. Colour passes (the sequential
-// per-colour dispatches of gpuworld.js:353-366) become in-wave passes: a
-// wave's LDS traffic is processed in issue order, so a compiler fence between
-// passes is all the ordering they need; no workgroup barrier is used.
-// State is read from and written to HBM (SoA) only at launch boundaries.
+// Persistent batched path. One workgroup holds WPB wavefronts; each
+// wavefront steps 64/G creatures of one morphology template, G lanes per
+// creature, with every creature's bodies and constraint state in LDS for all
+// substeps and iterations of a multi-step launch. The template (constraint
+// topology, anchors, collision points, the round schedule) is copied to LDS
+// once per workgroup and shared by its waves.
+//
+// A colour pass of the reference (one sequential dispatch per colour,
+// src/gpu/gpuworld.js:353-366) becomes a list of in-wave rounds built on the
+// host (kx_sched.cpp): a wave's LDS traffic is processed in issue order, so a
+// compiler fence between rounds is all the ordering they need. State is
+// read from and written to HBM (SoA) only at launch boundaries.
 //
 // Positions live in creature-local coordinates relative to an integer
-// origin (re-centred on the first body between steps); since every global
-// binary32 coordinate is a multiple of its own ulp, the shift to and from the
-// origin is exact and the solver works with ~8x finer position ulps.
-#include "kx_device.h"
+// origin (re-centred on the creature's first body between steps); since a
+// global binary32 coordinate is a multiple of its own ulp, the shift to and
+// from the origin is exact and the solver works with finer position ulps.
 #include "../../include/kxpbd.h"
+#include "kx_device.h"
+#include "kx_sched.h"
 
 #define KX_WAVE_SYNC()                                     \
   do {                                                     \
@@ -19,128 +25,121 @@
     __builtin_amdgcn_wave_barrier();                       \
   } while (0)
 
-// LDS sub-array offsets (in 4-byte words) for launch maxima NB, NC, NP
-struct KxLds {
-  int px, py, ang, vx, vy, av, ppx, ppy, angP, im, ii, bpo;
-  int cab, brk, lax, lay, lbx, lby, comp, rest, mn, mx, tgt, bf, l0, l1, slot;
-  int plx, ply, pr;
-  int ev, stride;
-  __device__ __host__ KxLds(int NB, int NC, int NP) {
+// per-creature LDS layout (word offsets), arrays of NB bodies / NC constraints
+struct KxCrLds {
+  int px, py, ang, vx, vy, av, ppx, ppy, angP;
+  int alpha, tgt, lbx, lby, bf, l0, l1, brk, ev, words;
+  __host__ __device__ KxCrLds(int NB, int NC) {
     int o = 0;
     px = o; o += NB; py = o; o += NB; ang = o; o += NB; vx = o; o += NB; vy = o; o += NB; av = o; o += NB;
-    ppx = o; o += NB; ppy = o; o += NB; angP = o; o += NB; im = o; o += NB; ii = o; o += NB; bpo = o; o += NB;
-    cab = o; o += NC; brk = o; o += NC; lax = o; o += NC; lay = o; o += NC; lbx = o; o += NC; lby = o; o += NC;
-    comp = o; o += NC; rest = o; o += NC; mn = o; o += NC; mx = o; o += NC; tgt = o; o += NC; bf = o; o += NC;
-    l0 = o; o += NC; l1 = o; o += NC; slot = o; o += NC;
-    plx = o; o += NP; ply = o; o += NP; pr = o; o += NP;
+    ppx = o; o += NB; ppy = o; o += NB; angP = o; o += NB;
+    alpha = o; o += NC; tgt = o; o += NC; lbx = o; o += NC; lby = o; o += NC; bf = o; o += NC;
+    l0 = o; o += NC; l1 = o; o += NC; brk = o; o += NC;
     ev = o; o += 1;
-    stride = (o + 3) & ~3;
+    words = o;
   }
 };
 
-int kx_persist_lds_words(int NB, int NC, int NP) { return KxLds(NB, NC, NP).stride; }
+int kx_persist_cr_words(int NB, int NC) { return KxCrLds(NB, NC).words; }
 
-template <int WPB>
+template <int G, int WPB>
 __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, int has_law, int step0,
                                                              const int *step_base, int nsteps, int ev_row0,
-                                                             int with_forces, int NB, int NC, int NP) {
-  extern __shared__ float smem[];
+                                                             int with_forces, KxPersistCfg P) {
+  constexpr int CPW = KX_WAVE / G;
+  extern __shared__ uint32_t smem[];
   const int lane = threadIdx.x & (KX_WAVE - 1);
+  const int q = lane / G, sub = lane % G;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / KX_WAVE);
-  const int cid = __builtin_amdgcn_readfirstlane(blockIdx.x * WPB + w);
-  if (cid >= d.nCr) return;                       // whole wave leaves; no block barriers below
-  const KxLds Lo(NB, NC, NP);
-  float *S = smem + (size_t)w * Lo.stride;
-  int *Si = reinterpret_cast<int *>(S);
-  const KxCreature cr = d.cr[cid];
-  const int b0 = cr.body_off, nb = cr.nb, c0 = cr.cons_off, nc = cr.nc, p0 = cr.pt_off, np = cr.np;
+  const int gw = blockIdx.x * WPB + w;
 
+  // ── template -> LDS (the only workgroup barrier) ──
+  const int tid = P.wave_tmpl[blockIdx.x * WPB];
+  const KxTmpl T = P.tmpl[tid];
+  for (int i = threadIdx.x; i < T.words; i += blockDim.x) smem[i] = P.pool[T.base + i];
+  __syncthreads();
+  if (gw >= P.nwaves) return;
+
+  const int *t_cab = (const int *)(smem + T.o_cab);
+  const float *t_lax = (const float *)(smem + T.o_lax), *t_lay = (const float *)(smem + T.o_lay);
+  const float *t_rest = (const float *)(smem + T.o_rest), *t_mn = (const float *)(smem + T.o_mn);
+  const float *t_mx = (const float *)(smem + T.o_mx);
+  const float *t_im = (const float *)(smem + T.o_im), *t_ii = (const float *)(smem + T.o_ii);
+  const int *t_bpo = (const int *)(smem + T.o_bpo);
+  const float *t_plx = (const float *)(smem + T.o_plx), *t_ply = (const float *)(smem + T.o_ply);
+  const float *t_pr = (const float *)(smem + T.o_pr);
+  const int *t_sched = (const int *)(smem + T.o_sched), *t_rtype = (const int *)(smem + T.o_rtype);
+  const int *t_coll = (const int *)(smem + T.o_coll), *t_ccnt = (const int *)(smem + T.o_ccnt);
+  const int *t_arm_tip = (const int *)(smem + T.o_arm_tip), *t_arm_grip = (const int *)(smem + T.o_arm_grip);
+  const int *t_mot_k = (const int *)(smem + T.o_mot_k), *t_mot_slot = (const int *)(smem + T.o_mot_slot);
+  const int *t_grip_k = (const int *)(smem + T.o_grip_k), *t_grip_arm = (const int *)(smem + T.o_grip_arm);
+
+  const int cid = P.wave_cr[gw * CPW + q];
+  const bool act = cid >= 0;
+  const KxCrLds Lo(P.NB, P.NC);
+  float *S = (float *)(smem + P.tmpl_words + (size_t)(w * CPW + q) * P.cr_stride);
   float *px = S + Lo.px, *py = S + Lo.py, *ang = S + Lo.ang, *vx = S + Lo.vx, *vy = S + Lo.vy, *av = S + Lo.av;
-  float *ppx = S + Lo.ppx, *ppy = S + Lo.ppy, *angP = S + Lo.angP, *im = S + Lo.im, *ii = S + Lo.ii;
-  int *bpo = Si + Lo.bpo;
-  int *cab = Si + Lo.cab, *brk = Si + Lo.brk, *slot = Si + Lo.slot;
-  float *lax = S + Lo.lax, *lay = S + Lo.lay, *lbx = S + Lo.lbx, *lby = S + Lo.lby, *comp = S + Lo.comp;
-  float *rest = S + Lo.rest, *mn = S + Lo.mn, *mx = S + Lo.mx, *tgt = S + Lo.tgt, *bf = S + Lo.bf;
+  float *ppx = S + Lo.ppx, *ppy = S + Lo.ppy, *angP = S + Lo.angP;
+  float *alpha = S + Lo.alpha, *tgt = S + Lo.tgt, *lbx = S + Lo.lbx, *lby = S + Lo.lby, *bf = S + Lo.bf;
   float *l0 = S + Lo.l0, *l1 = S + Lo.l1;
-  float *plx = S + Lo.plx, *ply = S + Lo.ply, *pr = S + Lo.pr;
-  uint32_t *evw = reinterpret_cast<uint32_t *>(Si + Lo.ev);
+  int *brk = (int *)(S + Lo.brk);
+  uint32_t *evw = (uint32_t *)(S + Lo.ev);
+  const int nb = act ? T.nb : 0, nc = act ? T.nc : 0;
+  const int b0 = act ? d.cr[cid].body_off : 0, c0 = act ? d.cr[cid].cons_off : 0;
 
-  // integer origin: the first body's position rounded
-  float ox = rintf(d.px[b0]), oy = rintf(d.py[b0]);
+  float ox = 0.0f, oy = 0.0f;
+  if (act) { ox = rintf(d.px[b0]); oy = rintf(d.py[b0]); }
 
-  // ── load ──
-  for (int i = lane; i < nb; i += KX_WAVE) {
+  // ── load the creature's dynamic state ──
+  for (int i = sub; i < nb; i += G) {
     int g = b0 + i;
     px[i] = d.px[g] - ox; py[i] = d.py[g] - oy; ang[i] = d.ang[g];
     vx[i] = d.vx[g]; vy[i] = d.vy[g]; av[i] = d.av[g];
-    im[i] = d.im[g]; ii[i] = d.ii[g];
-    bpo[i] = (d.bpoff[g] - p0) | (d.bpcnt[g] << 16);
   }
-  for (int k = lane; k < nc; k += KX_WAVE) {
+  for (int k = sub; k < nc; k += G) {
     int g = c0 + k;
-    int a = d.ca[g] - b0, bg = d.cb[g];
-    int b = bg < 0 ? 255 : bg - b0;
-    int t = d.ctype[g];
-    cab[k] = a | (b << 8) | (t << 16);
-    brk[k] = d.broken[g];
-    lax[k] = d.lax[g]; lay[k] = d.lay[g];
-    float bx = d.lbx[g], by = d.lby[g];
-    if (bg < 0) { bx -= ox; by -= oy; }
-    lbx[k] = bx; lby[k] = by;
-    comp[k] = d.comp[g]; rest[k] = d.rest[g]; mn[k] = d.cmin[g]; mx[k] = d.cmax[g];
-    tgt[k] = d.target[g]; bf[k] = d.bf[g];
+    bool world = ((t_cab[k] >> 8) & 255) == 255;
+    alpha[k] = d.comp[g] * d.invH2; tgt[k] = d.target[g];
+    lbx[k] = world ? d.lbx[g] - ox : d.lbx[g];
+    lby[k] = world ? d.lby[g] - oy : d.lby[g];
+    bf[k] = d.bf[g]; brk[k] = d.broken[g];
     l0[k] = 0.0f; l1[k] = 0.0f;
-    int s = d.cslot[g];
-    if (s < 0 && d.carm[g] >= 0) s = -2 - d.carm[g];
-    slot[k] = s;
   }
-  for (int p = lane; p < np; p += KX_WAVE) {
-    plx[p] = d.plx[p0 + p]; ply[p] = d.ply[p0 + p]; pr[p] = d.pr[p0 + p];
-  }
-  // arms: one lane per arm keeps tip, grip and the f64 cooldown in registers
-  int my_tip = 0, my_grip = 0;
   double my_cd = 0.0;
-  if (lane < cr.narms) {
-    my_tip = d.arm_tip[cr.arm_off + lane] - b0;
-    my_grip = d.arm_grip[cr.arm_off + lane] - c0;
-    my_cd = d.cd[cid * KX_MAX_ARMS + lane];
-  }
-  uint16_t coff[KX_MAX_COLORS + 1];
-#pragma unroll
-  for (int c = 0; c <= KX_MAX_COLORS; c++) coff[c] = cr.coff[c];
-  const int ncol = cr.ncol;
+  if (act && sub < T.narms) my_cd = d.cd[cid * KX_MAX_ARMS + sub];
   const double hS = d.hd * d.S;
   KX_WAVE_SYNC();
 
   for (int step = 0; step < nsteps; step++) {
     const int key = (step_base ? step_base[0] : step0) + step;
-    if (lane == 0) evw[0] = 0u;
+    if (act && sub == 0) evw[0] = 0u;
     uint32_t my_ev = 0u;
     if (has_law) {
       // motor targets + compliance (core.js:190-198)
-      for (int k = lane; k < nc; k += KX_WAVE) {
-        int s = slot[k];
-        if (s >= 0) { tgt[k] = kx_law_target(L, (uint32_t)cid, (uint32_t)key, s); comp[k] = (float)L.comp; }
+      const float a_law = (float)L.comp * d.invH2;
+      for (int j = sub; j < (act ? T.nmot : 0); j += G) {
+        int k = t_mot_k[j];
+        tgt[k] = kx_law_target(L, (uint32_t)cid, (uint32_t)key, t_mot_slot[j]);
+        alpha[k] = a_law;
       }
-      KX_WAVE_SYNC();
-      // adhesion reflex (core.js:199-218)
-      if (lane < cr.narms) {
-        double adh = kx_ctl_u(L.seed, (uint32_t)cid, (uint32_t)key, KX_SLOT_ADH + lane);
-        int g = my_grip, t = my_tip;
+      // adhesion reflex (core.js:199-218), one lane per arm
+      if (act && sub < T.narms) {
+        double adh = kx_ctl_u(L.seed, (uint32_t)cid, (uint32_t)key, KX_SLOT_ADH + sub);
+        int g = t_arm_grip[sub], t = t_arm_tip[sub];
         int bk = brk[g];
         float f_bf = bf[g], gx = lbx[g] + ox, gy = lby[g] + oy, lx = l0[g], ly = l1[g];
         uint32_t e = kx_reflex(L, hS, adh, my_cd, bk, f_bf, gx, gy, lx, ly, vx[t], vy[t],
-                               (double)ox + (double)px[t], (double)oy + (double)py[t], ang[t], lax[g], lay[g]);
+                               (double)ox + (double)px[t], (double)oy + (double)py[t], ang[t], t_lax[g], t_lay[g]);
         brk[g] = bk; bf[g] = f_bf; lbx[g] = gx - ox; lby[g] = gy - oy; l0[g] = lx; l1[g] = ly;
-        my_ev |= e << (3 * lane);
+        my_ev |= e << (3 * sub);
       }
       KX_WAVE_SYNC();
     }
     for (int s = 0; s < d.S; s++) {
-      // resetLambda + integrate
-      for (int k = lane; k < nc; k += KX_WAVE) { l0[k] = 0.0f; l1[k] = 0.0f; }
+      // resetLambda + integrate (xpbd.js:262-287)
+      for (int k = sub; k < nc; k += G) { l0[k] = 0.0f; l1[k] = 0.0f; }
       const bool forces = with_forces && step == 0 && s == 0;
-      for (int i = lane; i < nb; i += KX_WAVE) {
+      for (int i = sub; i < nb; i += G) {
         float fx = 0.0f, fy = 0.0f, tq = 0.0f;
         if (forces) {
           int g = b0 + i;
@@ -148,54 +147,58 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
           d.fx[g] = 0.0f; d.fy[g] = 0.0f; d.tq[g] = 0.0f;
         }
         float a_vx = vx[i], a_vy = vy[i], a_av = av[i], q_x, q_y, q_a;
-        kx_integrate(d, a_vx, a_vy, a_av, px[i], py[i], ang[i], im[i], ii[i], fx, fy, tq, q_x, q_y, q_a);
+        kx_integrate(d, a_vx, a_vy, a_av, px[i], py[i], ang[i], t_im[i], t_ii[i], fx, fy, tq, q_x, q_y, q_a);
         vx[i] = a_vx; vy[i] = a_vy; av[i] = a_av; ppx[i] = q_x; ppy[i] = q_y; angP[i] = q_a;
       }
       KX_WAVE_SYNC();
       for (int it = 0; it < d.I; it++) {
-        for (int c = 0; c < ncol; c++) {
-          const int j0 = coff[c], cnt = coff[c + 1] - j0;
-          for (int j = lane; j < cnt; j += KX_WAVE) {
-            int k = j0 + j;
-            if (brk[k]) continue;
-            int v = cab[k];
-            int A = v & 255, B = (v >> 8) & 255, t = v >> 16;
-            if (B == 255) B = -1;
-            float alpha = comp[k] * d.invH2;
-            if (t == KX_ANGLE) {
+        // constraint rounds (type-homogeneous, wave-uniform branch)
+        for (int r = 0; r < T.R; r++) {
+          const int k = act ? t_sched[r * G + sub] : -1;
+          if (t_rtype[r]) {
+            if (k >= 0 && !brk[k]) {
+              int v = t_cab[k];
+              int A = v & 255, B = (v >> 8) & 255;
               float lam = l0[k];
-              kx_solve_angle(angP, ii, A, B, rest[k], tgt[k], mn[k], mx[k], alpha, lam);
+              kx_solve_angle(angP, t_ii, A, B == 255 ? -1 : B, t_rest[k], tgt[k], t_mn[k], t_mx[k], alpha[k], lam);
               l0[k] = lam;
-            } else {
+            }
+          } else {
+            if (k >= 0 && !brk[k]) {
+              int v = t_cab[k];
+              int A = v & 255, B = (v >> 8) & 255;
               float lx = l0[k], ly = l1[k];
-              kx_solve_pin(ppx, ppy, angP, im, ii, A, B, lax[k], lay[k], lbx[k], lby[k], alpha, lx, ly);
+              kx_solve_pin(ppx, ppy, angP, t_im, t_ii, A, B == 255 ? -1 : B, t_lax[k], t_lay[k], lbx[k], lby[k],
+                           alpha[k], lx, ly);
               l0[k] = lx; l1[k] = ly;
             }
           }
           KX_WAVE_SYNC();
         }
-        // collisions: one lane per body, its points in order (cumulative)
-        for (int i = lane; i < nb; i += KX_WAVE) {
-          int bp = bpo[i];
+        // collisions: this lane's bodies, each body's points in order
+        const int nbl = act ? t_ccnt[sub] : 0;
+        for (int j = 0; j < nbl; j++) {
+          int i = t_coll[sub * T.maxbl + j];
+          if (t_im[i] == 0.0f) continue;
+          int bp = t_bpo[i];
           int o = bp & 0xffff, cnt = bp >> 16;
-          if (cnt == 0 || im[i] == 0.0f) continue;
           float x = ppx[i], y = ppy[i];
           float sa, ca;
           sincosf(angP[i], &sa, &ca);
-          for (int q = 0; q < cnt; q++) {
-            float lx = plx[o + q], ly = ply[o + q];
+          for (int pq = 0; pq < cnt; pq++) {
+            float lx = t_plx[o + pq], ly = t_ply[o + pq];
             float wx = x + (lx * ca - ly * sa);
             float wy = y + (lx * sa + ly * ca);
-            KxHit h;
-            if (!kx_sdf_grid(d, wx, wy, ox, oy, h)) continue;
-            float pen = pr[o + q] - h.d;
-            if (pen > KX_SLACK_POS) { x += h.nx * pen; y += h.ny * pen; }
+            KxHit hh;
+            if (!kx_sdf_grid(d, wx, wy, ox, oy, hh)) continue;
+            float pen = t_pr[o + pq] - hh.d;
+            if (pen > KX_SLACK_POS) { x += hh.nx * pen; y += hh.ny * pen; }
           }
           ppx[i] = x; ppy[i] = y;
         }
         KX_WAVE_SYNC();
       }
-      for (int i = lane; i < nb; i += KX_WAVE) {
+      for (int i = sub; i < nb; i += G) {
         float a_px = px[i], a_py = py[i], a_ang = ang[i], a_vx, a_vy, a_av, a_angP = angP[i];
         kx_finalize(d, a_px, a_py, a_ang, a_vx, a_vy, a_av, ppx[i], ppy[i], a_angP);
         px[i] = a_px; py[i] = a_py; ang[i] = a_ang; angP[i] = a_angP;
@@ -204,51 +207,52 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
       KX_WAVE_SYNC();
     }
     // grip break (xpbd.js:496-504 / solver.wgsl:433-442)
-    for (int k = lane; k < nc; k += KX_WAVE) {
-      if ((cab[k] >> 16) != KX_GRIP || brk[k]) continue;
+    for (int j = sub; j < (act ? T.ngrip : 0); j += G) {
+      int k = t_grip_k[j];
+      if (brk[k]) continue;
       float lx = l0[k], ly = l1[k];
       float f = sqrtf(lx * lx + ly * ly) * d.invH;
       if (f > bf[k]) {
         brk[k] = 1; l0[k] = 0.0f; l1[k] = 0.0f;
-        int sl = slot[k];
-        if (sl <= -2) my_ev |= 4u << (3 * (-2 - sl));
+        int arm = t_grip_arm[j];
+        if (arm >= 0) my_ev |= 4u << (3 * arm);
       }
     }
     if (d.events) {
       if (my_ev) atomicOr(evw, my_ev);
       KX_WAVE_SYNC();
-      if (lane == 0) d.events[(size_t)(ev_row0 + step) * d.nCr + cid] = evw[0];
+      if (act && sub == 0) d.events[(size_t)(ev_row0 + step) * d.nCr + cid] = evw[0];
     }
     // re-centre the origin on the first body when it drifted
-    float sx = rintf(px[0]), sy = rintf(py[0]);
+    float sx = act ? rintf(px[0]) : 0.0f, sy = act ? rintf(py[0]) : 0.0f;
+    KX_WAVE_SYNC();
     if (fabsf(sx) > 32.0f || fabsf(sy) > 32.0f) {
-      KX_WAVE_SYNC();
-      for (int i = lane; i < nb; i += KX_WAVE) { px[i] -= sx; py[i] -= sy; ppx[i] -= sx; ppy[i] -= sy; }
-      for (int k = lane; k < nc; k += KX_WAVE) {
-        if (((cab[k] >> 8) & 255) == 255) { lbx[k] -= sx; lby[k] -= sy; }
-      }
+      for (int i = sub; i < nb; i += G) { px[i] -= sx; py[i] -= sy; ppx[i] -= sx; ppy[i] -= sy; }
+      for (int k = sub; k < nc; k += G)
+        if (((t_cab[k] >> 8) & 255) == 255) { lbx[k] -= sx; lby[k] -= sy; }
       ox += sx; oy += sy;
-      KX_WAVE_SYNC();
     }
+    KX_WAVE_SYNC();
   }
 
   // ── store ──
-  for (int i = lane; i < nb; i += KX_WAVE) {
+  for (int i = sub; i < nb; i += G) {
     int g = b0 + i;
     d.px[g] = px[i] + ox; d.py[g] = py[i] + oy; d.ang[g] = ang[i];
     d.vx[g] = vx[i]; d.vy[g] = vy[i]; d.av[g] = av[i];
     d.ppx[g] = ppx[i] + ox; d.ppy[g] = ppy[i] + oy; d.angP[g] = angP[i];
   }
-  for (int k = lane; k < nc; k += KX_WAVE) {
+  for (int k = sub; k < nc; k += G) {
     int g = c0 + k;
+    int v = t_cab[k];
     d.broken[g] = brk[k];
-    if (((cab[k] >> 8) & 255) == 255) { d.lbx[g] = lbx[k] + ox; d.lby[g] = lby[k] + oy; }
-    d.bf[g] = bf[k]; d.target[g] = tgt[k]; d.comp[g] = comp[k];
-    int t = cab[k] >> 16;
-    if (t == KX_ANGLE) { d.lam[g] = l0[k]; d.lamx[g] = 0.0f; d.lamy[g] = 0.0f; }
+    if (((v >> 8) & 255) == 255) { d.lbx[g] = lbx[k] + ox; d.lby[g] = lby[k] + oy; }
+    d.bf[g] = bf[k]; d.target[g] = tgt[k];
+    if (has_law && (v >> 16) == KX_ANGLE) d.comp[g] = (float)L.comp;
+    if ((v >> 16) == KX_ANGLE) { d.lam[g] = l0[k]; d.lamx[g] = 0.0f; d.lamy[g] = 0.0f; }
     else { d.lamx[g] = l0[k]; d.lamy[g] = l1[k]; d.lam[g] = 0.0f; }
   }
-  if (lane < cr.narms) d.cd[cid * KX_MAX_ARMS + lane] = my_cd;
+  if (act && sub < T.narms) d.cd[cid * KX_MAX_ARMS + sub] = my_cd;
 }
 
 __global__ void kx_k_bump(int *counter, int delta) {
@@ -260,31 +264,43 @@ int kx_bump_counter(int *counter, int delta, hipStream_t st) {
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 
-template <int WPB>
-static int launch_wpb(const KxDev &d, const KxLaw *law, int step0, const int *step_base, int nsteps, int ev_row0,
-                      int with_forces, int NB, int NC, int NP, hipStream_t st) {
-  KxLds Lo(NB, NC, NP);
-  size_t lds = (size_t)Lo.stride * 4 * WPB;
+template <int G, int WPB>
+static int launch_gw(const KxDev &d, const KxLaw *law, int step0, const int *step_base, int nsteps, int ev_row0,
+                     int with_forces, const KxPersistCfg &P, hipStream_t st) {
+  constexpr int CPW = KX_WAVE / G;
+  size_t lds = ((size_t)P.tmpl_words + (size_t)WPB * CPW * P.cr_stride) * 4;
+  if (lds > 160 * 1024) return -22;
   static bool attr_set = false;
   if (!attr_set) {
-    (void)hipFuncSetAttribute((const void *)kx_k_persist<WPB>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute((const void *)kx_k_persist<G, WPB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024);
     attr_set = true;
   }
   KxLaw L = law ? *law : KxLaw{};
-  int blocks = (d.nCr + WPB - 1) / WPB;
-  hipLaunchKernelGGL(kx_k_persist<WPB>, dim3(blocks), dim3(KX_WAVE * WPB), lds, st, d, L, law ? 1 : 0, step0,
-                     step_base, nsteps, ev_row0, with_forces, NB, NC, NP);
+  int blocks = (P.nwaves + WPB - 1) / WPB;
+  hipLaunchKernelGGL((kx_k_persist<G, WPB>), dim3(blocks), dim3(KX_WAVE * WPB), lds, st, d, L, law ? 1 : 0, step0,
+                     step_base, nsteps, ev_row0, with_forces, P);
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 
-int kx_persist_launch(const KxDev &d, const KxLaw *law, int step0, const int *step_base_dev, int nsteps,
-                      int ev_row0, int waves_per_block, int with_forces, int maxb, int maxc, int maxp,
-                      hipStream_t st) {
-  switch (waves_per_block) {
-    case 1: return launch_wpb<1>(d, law, step0, step_base_dev, nsteps, ev_row0, with_forces, maxb, maxc, maxp, st);
-    case 2: return launch_wpb<2>(d, law, step0, step_base_dev, nsteps, ev_row0, with_forces, maxb, maxc, maxp, st);
-    case 4: return launch_wpb<4>(d, law, step0, step_base_dev, nsteps, ev_row0, with_forces, maxb, maxc, maxp, st);
-    case 8: return launch_wpb<8>(d, law, step0, step_base_dev, nsteps, ev_row0, with_forces, maxb, maxc, maxp, st);
+template <int G>
+static int launch_g(int wpb, const KxDev &d, const KxLaw *law, int step0, const int *step_base, int nsteps,
+                    int ev_row0, int with_forces, const KxPersistCfg &P, hipStream_t st) {
+  switch (wpb) {
+    case 1: return launch_gw<G, 1>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
+    case 2: return launch_gw<G, 2>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
+    case 4: return launch_gw<G, 4>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
+    default: return -22;
+  }
+}
+
+int kx_persist_launch(int G, int wpb, const KxDev &d, const KxLaw *law, int step0, const int *step_base,
+                      int nsteps, int ev_row0, int with_forces, const KxPersistCfg &P, hipStream_t st) {
+  switch (G) {
+    case 8: return launch_g<8>(wpb, d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
+    case 16: return launch_g<16>(wpb, d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
+    case 32: return launch_g<32>(wpb, d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
+    case 64: return launch_g<64>(wpb, d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
     default: return -22;
   }
 }

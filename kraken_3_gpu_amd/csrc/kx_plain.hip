@@ -1,8 +1,10 @@
 // Plain per-colour-launch path: the launch structure of the reference's
 // WebGPU solver (src/gpu/gpuworld.js:344-382 substep/step driving
-// src/gpu/solver.wgsl), rebuilt as HIP kernels over SoA buffers in global
-// coordinates. Per step: [control law] + S x (reset, integrate,
-// I x (one launch per non-empty colour, collisions), finalize) + grip break.
+// src/gpu/solver.wgsl), rebuilt as HIP kernels over SoA buffers.
+// Per step: [control law] + to-local + S x (reset, integrate,
+// I x (one launch per non-empty colour, collisions), finalize) + grip break
+// + to-global. Between to-local and to-global the position arrays hold
+// creature-local coordinates (integer origin per creature, exact shifts).
 #include "kx_device.h"
 
 #define PLAIN_BLOCK 256
@@ -42,6 +44,30 @@ __global__ __launch_bounds__(PLAIN_BLOCK) void kx_k_motors(KxDev d, KxLaw L, int
   if (slot < 0) return;
   d.target[k] = kx_law_target(L, (uint32_t)d.ccr[k], (uint32_t)step_key, slot);
   d.comp[k] = (float)L.comp;
+}
+
+// Integer origin per creature: its first body's position, rounded.
+__global__ __launch_bounds__(PLAIN_BLOCK) void kx_k_origin(KxDev d) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.nCr) return;
+  int b = d.cr[c].body_off;
+  d.org[2 * c] = rintf(d.px[b]);
+  d.org[2 * c + 1] = rintf(d.py[b]);
+}
+
+// Shift bodies and world anchors by -origin (sign=-1) or +origin (sign=+1).
+__global__ __launch_bounds__(PLAIN_BLOCK) void kx_k_shift(KxDev d, float sign) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < d.nB) {
+    int c = d.bcr[i];
+    float ox = sign * d.org[2 * c], oy = sign * d.org[2 * c + 1];
+    d.px[i] += ox; d.py[i] += oy; d.ppx[i] += ox; d.ppy[i] += oy;
+  }
+  if (i < d.nC && d.cb[i] < 0) {
+    int c = d.ccr[i];
+    d.lbx[i] += sign * d.org[2 * c];
+    d.lby[i] += sign * d.org[2 * c + 1];
+  }
 }
 
 // resetLambda (solver.wgsl:156-163)
@@ -90,6 +116,8 @@ __global__ __launch_bounds__(PLAIN_BLOCK) void kx_k_collide(KxDev d) {
   int cnt = d.bpcnt[i];
   if (cnt == 0) return;
   int o = d.bpoff[i];
+  int c = d.bcr[i];
+  float ox = d.org[2 * c], oy = d.org[2 * c + 1];
   float x = d.ppx[i], y = d.ppy[i];
   float sa, ca;
   sincosf(d.angP[i], &sa, &ca);
@@ -98,7 +126,7 @@ __global__ __launch_bounds__(PLAIN_BLOCK) void kx_k_collide(KxDev d) {
     float wx = x + (lx * ca - ly * sa);
     float wy = y + (lx * sa + ly * ca);
     KxHit h;
-    bool ok = d.use_grid ? kx_sdf_grid(d, wx, wy, 0.0f, 0.0f, h) : kx_sdf_linear(d, wx, wy, 0.0f, 0.0f, h);
+    bool ok = d.use_grid ? kx_sdf_grid(d, wx, wy, ox, oy, h) : kx_sdf_linear(d, wx, wy, ox, oy, h);
     if (!ok) continue;
     float pen = d.pr[o + q] - h.d;
     if (pen > KX_SLACK_POS) { x += h.nx * pen; y += h.ny * pen; }
@@ -139,6 +167,9 @@ int kx_plain_step(const KxDev &d, const KxLaw *law, int step_key, const int *ste
       hipLaunchKernelGGL(kx_k_arms, dim3(nblk(d.nArms)), dim3(PLAIN_BLOCK), 0, st, d, *law, step_key, step_base, ev_row); n++;
     }
   }
+  int nbc = d.nB > d.nC ? d.nB : d.nC;
+  hipLaunchKernelGGL(kx_k_origin, dim3(nblk(d.nCr)), dim3(PLAIN_BLOCK), 0, st, d); n++;
+  hipLaunchKernelGGL(kx_k_shift, dim3(nblk(nbc)), dim3(PLAIN_BLOCK), 0, st, d, -1.0f); n++;
   for (int s = 0; s < d.S; s++) {
     hipLaunchKernelGGL(kx_k_reset, dim3(nblk(d.nC)), dim3(PLAIN_BLOCK), 0, st, d); n++;
     hipLaunchKernelGGL(kx_k_integrate, dim3(nblk(d.nB)), dim3(PLAIN_BLOCK), 0, st, d, with_forces && s == 0); n++;
@@ -151,6 +182,7 @@ int kx_plain_step(const KxDev &d, const KxLaw *law, int step_key, const int *ste
     hipLaunchKernelGGL(kx_k_finalize, dim3(nblk(d.nB)), dim3(PLAIN_BLOCK), 0, st, d); n++;
   }
   hipLaunchKernelGGL(kx_k_grip_break, dim3(nblk(d.nC)), dim3(PLAIN_BLOCK), 0, st, d, ev_row); n++;
+  hipLaunchKernelGGL(kx_k_shift, dim3(nblk(nbc)), dim3(PLAIN_BLOCK), 0, st, d, 1.0f); n++;
   if (launches) *launches += n;
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }

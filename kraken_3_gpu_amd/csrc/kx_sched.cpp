@@ -1,0 +1,157 @@
+// Morphology templates and their in-wave schedules for the persistent path.
+//
+// Creatures with identical static structure share one template (bodies'
+// inverse mass/inertia, constraint topology and anchors, collision points,
+// arms). Its constraints are scheduled into rounds for G lanes per creature:
+// the colour order (src/physics/xpbd.js:108-126 + 290-296) is a sequential
+// Gauss-Seidel order, and two constraints that share no body commute, so any
+// topological order of the "shares a body, comes earlier" relation gives the
+// same result bit for bit. Rounds are antichains of that relation of at most
+// G constraints, all pins/grips or all angle motors (no divergent paths),
+// filled by critical-path list scheduling.
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/kxpbd.h"
+#include "kx_sched.h"
+
+void kx_schedule_rounds(const TmplHost &t, int G, std::vector<int> &sched, std::vector<int> &rtype, int &R) {
+  const int nc = (int)t.type.size();
+  std::vector<std::vector<int>> succ(nc);
+  std::vector<int> npred(nc, 0), last(t.nb, -1);
+  for (int k = 0; k < nc; k++) {
+    int bodies[2] = {t.a[k], t.b[k]};
+    int seen = -1;
+    for (int j = 0; j < 2; j++) {
+      int b = bodies[j];
+      if (b < 0) continue;
+      int p = last[b];
+      if (p >= 0 && p != seen) { succ[p].push_back(k); npred[k]++; seen = p; }
+      last[b] = k;
+    }
+  }
+  // priority: longest chain to a sink
+  std::vector<int> prio(nc, 1);
+  for (int k = nc - 1; k >= 0; k--)
+    for (int s : succ[k]) prio[k] = std::max(prio[k], prio[s] + 1);
+  std::vector<int> left(npred);
+  std::vector<char> done(nc, 0);
+  std::vector<int> ready;
+  for (int k = 0; k < nc; k++) if (!left[k]) ready.push_back(k);
+  sched.clear(); rtype.clear();
+  int scheduled = 0;
+  while (scheduled < nc) {
+    // pick the class (0: pin/grip, 1: angle) holding the most urgent ready node
+    int best_cls = -1, best_p = -1, cnt_cls[2] = {0, 0};
+    for (int k : ready) {
+      int cls = t.type[k] == KX_ANGLE ? 1 : 0;
+      cnt_cls[cls]++;
+      if (prio[k] > best_p || (prio[k] == best_p && cls == 0)) { best_p = prio[k]; best_cls = cls; }
+    }
+    std::vector<int> cand;
+    for (int k : ready) if ((t.type[k] == KX_ANGLE ? 1 : 0) == best_cls) cand.push_back(k);
+    std::stable_sort(cand.begin(), cand.end(), [&](int x, int y) { return prio[x] > prio[y]; });
+    if ((int)cand.size() > G) cand.resize(G);
+    std::vector<int> row(G, -1);
+    for (int j = 0; j < (int)cand.size(); j++) row[j] = cand[j];
+    sched.insert(sched.end(), row.begin(), row.end());
+    rtype.push_back(best_cls);
+    for (int k : cand) done[k] = 1;
+    scheduled += (int)cand.size();
+    std::vector<int> next;
+    for (int k : ready) if (!done[k]) next.push_back(k);
+    for (int k : cand)
+      for (int s : succ[k]) if (--left[s] == 0) next.push_back(s);
+    ready.swap(next);
+  }
+  R = (int)rtype.size();
+}
+
+void kx_schedule_collisions(const TmplHost &t, int G, std::vector<int> &lists, std::vector<int> &counts, int &maxbl) {
+  std::vector<int> order(t.nb);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return t.pcnt[x] > t.pcnt[y]; });
+  std::vector<std::vector<int>> lane(G);
+  std::vector<int> load(G, 0);
+  for (int b : order) {
+    if (t.pcnt[b] == 0) continue;
+    int best = 0;
+    for (int l = 1; l < G; l++) if (load[l] < load[best]) best = l;
+    lane[best].push_back(b);
+    load[best] += t.pcnt[b];
+  }
+  maxbl = 1;
+  for (auto &l : lane) maxbl = std::max<int>(maxbl, (int)l.size());
+  lists.assign((size_t)G * maxbl, 0);
+  counts.assign(G, 0);
+  for (int l = 0; l < G; l++) {
+    counts[l] = (int)lane[l].size();
+    for (int j = 0; j < counts[l]; j++) lists[(size_t)l * maxbl + j] = lane[l][j];
+  }
+}
+
+static void put(std::vector<uint32_t> &pool, const std::vector<int> &v) {
+  for (int x : v) pool.push_back((uint32_t)x);
+}
+static void putf(std::vector<uint32_t> &pool, const std::vector<float> &v) {
+  for (float x : v) { uint32_t u; memcpy(&u, &x, 4); pool.push_back(u); }
+}
+
+// Serialise one template with its schedule for G lanes into the word pool.
+KxTmpl kx_pack_template(const TmplHost &t, int G, std::vector<uint32_t> &pool) {
+  KxTmpl T{};
+  T.nb = t.nb; T.nc = (int)t.type.size(); T.np = (int)t.plx.size();
+  T.narms = (int)t.arm_tip.size(); T.nmot = (int)t.mot_k.size(); T.ngrip = (int)t.grip_k.size();
+  std::vector<int> sched, rtype, lists, counts;
+  kx_schedule_rounds(t, G, sched, rtype, T.R);
+  kx_schedule_collisions(t, G, lists, counts, T.maxbl);
+  T.base = (int)pool.size();
+  auto at = [&]() { return (int)pool.size() - T.base; };
+  std::vector<int> cab(T.nc);
+  for (int k = 0; k < T.nc; k++) cab[k] = t.a[k] | ((t.b[k] < 0 ? 255 : t.b[k]) << 8) | (t.type[k] << 16);
+  T.o_cab = at(); put(pool, cab);
+  T.o_lax = at(); putf(pool, t.lax);
+  T.o_lay = at(); putf(pool, t.lay);
+  T.o_rest = at(); putf(pool, t.rest);
+  T.o_mn = at(); putf(pool, t.mn);
+  T.o_mx = at(); putf(pool, t.mx);
+  T.o_im = at(); putf(pool, t.im);
+  T.o_ii = at(); putf(pool, t.ii);
+  std::vector<int> bpo(t.nb);
+  for (int b = 0; b < t.nb; b++) bpo[b] = t.poff[b] | (t.pcnt[b] << 16);
+  T.o_bpo = at(); put(pool, bpo);
+  T.o_plx = at(); putf(pool, t.plx);
+  T.o_ply = at(); putf(pool, t.ply);
+  T.o_pr = at(); putf(pool, t.pr);
+  T.o_sched = at(); put(pool, sched);
+  T.o_rtype = at(); put(pool, rtype);
+  T.o_coll = at(); put(pool, lists);
+  T.o_ccnt = at(); put(pool, counts);
+  T.o_arm_tip = at(); put(pool, t.arm_tip);
+  T.o_arm_grip = at(); put(pool, t.arm_grip);
+  T.o_mot_k = at(); put(pool, t.mot_k);
+  T.o_mot_slot = at(); put(pool, t.mot_slot);
+  T.o_grip_k = at(); put(pool, t.grip_k);
+  T.o_grip_arm = at(); put(pool, t.grip_arm);
+  T.words = at();
+  return T;
+}
+
+std::string kx_template_key(const TmplHost &t) {
+  std::string s;
+  auto add = [&](const void *p, size_t n) { s.append((const char *)p, n); };
+  auto addv = [&](const auto &v) {
+    size_t n = v.size();
+    add(&n, sizeof n);
+    if (n) add(v.data(), n * sizeof(v[0]));
+  };
+  add(&t.nb, sizeof t.nb);
+  addv(t.type); addv(t.a); addv(t.b); addv(t.lax); addv(t.lay); addv(t.rest); addv(t.mn); addv(t.mx);
+  addv(t.im); addv(t.ii); addv(t.poff); addv(t.pcnt); addv(t.plx); addv(t.ply); addv(t.pr);
+  addv(t.arm_tip); addv(t.arm_grip); addv(t.mot_k); addv(t.mot_slot); addv(t.grip_k); addv(t.grip_arm);
+  return s;
+}

@@ -50,7 +50,7 @@ class Handle:
     """gpuworld handle `g`: owns the device world; keeps a reference to the
     caller's World (`g.cpu`) like gpuworld.create does."""
 
-    def __init__(self, cpu, path, creatures_per_block):
+    def __init__(self, cpu, path, lanes_per_creature, waves_per_block):
         self.cpu = cpu
         self.p = C.c_void_p()
         keep = []
@@ -60,7 +60,7 @@ class Handle:
         prm.substeps, prm.iterations, prm.h = s["substeps"], s["iterations"], s["h"]
         prm.lin_drag, prm.ang_drag = s["lin_drag"], s["ang_drag"]
         prm.max_speed, prm.max_ang_speed = s["max_speed"], s["max_ang_speed"]
-        prm.path, prm.creatures_per_block = path, creatures_per_block
+        prm.path, prm.lanes_per_creature, prm.waves_per_block = path, lanes_per_creature, waves_per_block
         has_cr = cpu.n_creatures > 0 and len(cpu.arm_tip) > 0
         K.check(K.lib().kx_create(C.byref(d), C.byref(c) if has_cr else None, C.byref(prm), C.byref(self.p)))
         nb, nc, npt, ncr = (C.c_int32() for _ in range(4))
@@ -107,10 +107,10 @@ def is_available():
     return True
 
 
-def create(cpu, path=K.KX_PATH_PERSISTENT, creatures_per_block=0):
+def create(cpu, path=K.KX_PATH_PERSISTENT, lanes_per_creature=0, waves_per_block=0):
     """gpuworld.create (gpuworld.js:34-171): colours the world, builds the
     device buffers; raises on implausible params or bad indices."""
-    return Handle(cpu, path, creatures_per_block)
+    return Handle(cpu, path, lanes_per_creature, waves_per_block)
 
 
 def _cons_state(g, cpu, with_dynamics):
@@ -270,9 +270,16 @@ def last_timing(g):
     return ms.value, n.value
 
 
-def set_path(g, path, creatures_per_block=0):
-    K.check(K.lib().kx_set_path(g.p, path, creatures_per_block))
+def set_path(g, path, lanes_per_creature=0, waves_per_block=0):
+    K.check(K.lib().kx_set_path(g.p, path, lanes_per_creature, waves_per_block))
     g.path = path
+
+
+def schedule_rounds(g):
+    """Rounds per iteration of every morphology template (persistent path)."""
+    buf = np.zeros(4096, dtype=np.int32)
+    n = K.check(K.lib().kx_schedule_info(g.p, K.ptr(buf, C.c_int32), len(buf)))
+    return buf[:n].tolist()
 
 
 def set_plain_sdf(g, use_grid):

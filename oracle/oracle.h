@@ -54,6 +54,7 @@ int      or_spawn_kraken(OrWorld *w, double x, double y, double angle, int arms,
 int      or_find_spawn(OrWorld *w, double clearance, uint32_t *rng_state, double *xy);
 double   or_rng_next(uint32_t *rng_state);     /* mulberry32 with explicit state */
 void     or_ensure_colored(OrWorld *w);
+void     or_permute_cons(OrWorld *w, const int *order);
 int      or_counts(const OrWorld *w, int *out4);            /* n, nc, np, ncreatures */
 void     or_color_info(const OrWorld *w, uint32_t *off16, uint32_t *cnt16);
 

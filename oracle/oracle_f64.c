@@ -357,6 +357,15 @@ void or_ensure_colored(OrWorld *w) {
   w->colored = 1;
 }
 
+/* reorder the constraint list: new position k holds old position order[k] */
+void or_permute_cons(OrWorld *w, const int *order) {
+  OrCons *tmp = (OrCons *)malloc(sizeof(OrCons) * (w->nc > 0 ? w->nc : 1));
+  for (int k = 0; k < w->nc; k++) tmp[k] = w->cons[order[k]];
+  memcpy(w->cons, tmp, sizeof(OrCons) * w->nc);
+  free(tmp);
+  for (int k = 0; k < w->nc; k++) w->pos_of_id[w->cons[k].id] = k;
+}
+
 int or_counts(const OrWorld *w, int *o) { o[0] = w->n; o[1] = w->nc; o[2] = w->np; o[3] = w->ncr; return w->n; }
 void or_color_info(const OrWorld *w, uint32_t *off, uint32_t *cnt) {
   for (int c = 0; c < 16; c++) { off[c] = w->colorOff[c]; cnt[c] = w->colorCnt[c]; }

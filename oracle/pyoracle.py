@@ -48,6 +48,7 @@ def lib():
             "or_find_spawn": (i, [P, d, C.POINTER(u32), dp]),
             "or_rng_next": (d, [C.POINTER(u32)]),
             "or_ensure_colored": (None, [P]),
+            "or_permute_cons": (None, [P, ip]),
             "or_counts": (i, [P, ip]),
             "or_color_info": (None, [P, C.POINTER(u32), C.POINTER(u32)]),
             "or_get_bodies": (None, [P, dp, dp, dp, dp, dp, dp, dp, dp]),
@@ -148,6 +149,10 @@ class World:
 
     def color(self):
         lib().or_ensure_colored(self.p)
+
+    def permute_cons(self, order):
+        o = np.ascontiguousarray(order, dtype=np.int32)
+        lib().or_permute_cons(self.p, _ip(o))
 
     def counts(self):
         o = np.zeros(4, dtype=np.int32)

@@ -176,7 +176,7 @@ def test_create_sanity_guards_raise(case):
     w = W.world_cpu64(2)
     keep = []
     d, c = w.desc(keep)
-    prm = K.Params(4, 8, 1 / 240, 3.0, 3.0, 360.0, 18.0, K.KX_PATH_PERSISTENT, 0)
+    prm = K.Params(4, 8, 1 / 240, 3.0, 3.0, 360.0, 18.0, K.KX_PATH_PERSISTENT, 0, 0)
     if case == "iterations":
         prm.iterations = 0
     elif case == "h":

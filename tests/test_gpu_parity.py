@@ -188,6 +188,26 @@ def test_graph_replay_matches_eager_launches():
     assert np.array_equal(w1.broken, w2.broken)
 
 
+def test_persistent_shapes_are_bit_identical():
+    """Any lanes-per-creature / waves-per-workgroup shape runs a topological
+    order of the same constraint dependencies: results must not depend on it."""
+    law = W.default_law(21)
+    ref = None
+    for shape in ((64, 1), (32, 2), (16, 1), (16, 4), (8, 1), (8, 2)):
+        w = W.world_uneven(40, seed=6)
+        g = G.create(w, path=K.KX_PATH_PERSISTENT, lanes_per_creature=shape[0], waves_per_block=shape[1])
+        G.upload(g)
+        say(f"[case] shape {shape}")
+        ev = G.rollout(g, law, 0, 30, events=True)
+        G.download(g)
+        cur = (w.px.copy(), w.py.copy(), w.ang.copy(), w.broken.copy(), ev)
+        if ref is None:
+            ref = cur
+        else:
+            assert all(np.array_equal(x, y) for x, y in zip(ref, cur)), shape
+        G.destroy(g)
+
+
 def test_plain_and_persistent_agree_uneven():
     """Ragged morphologies (4-8 arms x 5-9 segments, ring 6-10) through both paths."""
     from helpers import creature_of_bodies
