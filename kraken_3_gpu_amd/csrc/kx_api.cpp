@@ -229,6 +229,14 @@ static int build_grid(kx_handle *h, const float *rects, int nR, float rmax) {
       ids.insert(ids.end(), cells[c].begin(), cells[c].end());
     }
   }
+  std::vector<uint32_t> bits(((size_t)gnx * gny + 31) / 32, 0u);
+  for (size_t c = 0; c + 1 < off.size(); c++)
+    if (off[c + 1] > off[c]) bits[c >> 5] |= 1u << (c & 31);
+  uint32_t *dbits;
+  TRY(dalloc(h, &dbits, bits.size()));
+  TRY(dput(dbits, bits.data(), bits.size()));
+  h->d.cell_bits = dbits;
+  h->d.cell_words = (int)bits.size();
   int *doff, *dids;
   TRY(dalloc(h, &doff, off.size()));
   TRY(dalloc(h, &dids, ids.size()));
@@ -270,8 +278,12 @@ static int build_persist(kx_handle *h) {
       }
     }
   }
-  int stride = kx_persist_cr_words(NB, NC) | 1;
-  size_t lds = ((size_t)words + (size_t)WPB * CPW * stride) * 4;
+  // creature regions: 16-byte records, regions offset by 16 bytes modulo
+  // 128 so neighbouring creatures' same records fall in different banks
+  int stride = (kx_persist_cr_words(NB, NC) + 31) / 32 * 32 + 4;
+  int bmw = (h->d.cell_words + 3) / 4 * 4;
+  if ((size_t)bmw * 4 > 32 * 1024) bmw = 0;      // huge grids: no LDS bitmap
+  size_t lds = ((size_t)words + bmw + (size_t)WPB * CPW * stride) * 4;
   if (lds > 160 * 1024)
     return fail(KX_ENOTSUP, "persistent shape G=%d x %d waves needs %zu B of LDS (> 160 KiB)", G, WPB, lds);
   for (void *&p : h->pc_bufs) { if (p) (void)hipFree(p); p = nullptr; }
@@ -289,6 +301,7 @@ static int build_persist(kx_handle *h) {
   P.wave_cr = (const int *)h->pc_bufs[2];
   P.wave_tmpl = (const int *)h->pc_bufs[3];
   P.tmpl_words = words;
+  P.bm_words = bmw;
   P.cr_stride = stride;
   P.nwaves = (int)wave_tmpl.size();
   P.NB = NB; P.NC = NC;

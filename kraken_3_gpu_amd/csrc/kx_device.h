@@ -2,43 +2,68 @@
 //
 // Numerics, chosen to stay close to the f64 reference (src/physics/xpbd.js)
 // in binary32:
-//   * angle differences are formed in f32 (a correctly rounded a-b) and only
-//     the rare wrap past +-pi is folded in f64 with the exact 2*pi, so wraps
-//     carry no f32-TAU error (xpbd.js:128-143);
-//   * sin/cos use the accurate library sincosf (angles reach +-50 rad before
+//   * angle differences are folded with 2*pi split into two binary32 parts,
+//     so a wrap carries no f32-TAU error (xpbd.js:128-143);
+//   * sin/cos are accurate for the whole range angles reach (+-50 rad before
 //     the finalize wrap, xpbd.js:324);
-//   * the persistent path works in creature-local coordinates (integer
-//     origin), which keeps position ulps ~8x finer than global f32.
+//   * both paths work in creature-local coordinates (integer origin), which
+//     keeps position ulps several times finer than global f32.
 #pragma once
 #include "kx_internal.h"
 
 #define KX_SLACK_POS 2e-3f
 #define KX_SLACK_POS2 4e-6f
 #define KX_SLACK_ANG 2e-4f
-#define KX_PI_D 3.141592653589793
-#define KX_TAU_D 6.283185307179586
 // smallest binary32 above pi: |a-b| rounded below this is certainly < pi
 #define KX_PI_BELOW 3.14159250f
 
-__device__ __forceinline__ float kx_fold_d(double x) {
-  double r = x;
-  if (!(r > -KX_TAU_D && r < KX_TAU_D)) {
-    if (!isfinite(r)) return 0.0f;
-    r = fmod(r, KX_TAU_D);
-  }
-  if (r > KX_PI_D) r -= KX_TAU_D;
-  else if (r < -KX_PI_D) r += KX_TAU_D;
-  return (float)r;
+// Angle folding to [-pi, pi]: round-to-nearest multiple of 2*pi split in two
+// binary32 parts (hi exact in 24 bits, lo the remainder), so a fold adds no
+// more than an ulp of the result; branch-free for the common |d| <= pi.
+#define KX_TAU_HI 6.28318548202514648f
+#define KX_TAU_LO -1.74845553e-7f
+#define KX_INV_TAU 0.159154943091895336f
+
+__device__ __forceinline__ float kx_fold(float d) {
+  float q = rintf(d * KX_INV_TAU);
+  float r = fmaf(-q, KX_TAU_HI, d);
+  r = fmaf(-q, KX_TAU_LO, r);
+  return fabsf(d) <= KX_PI_BELOW ? d : (isfinite(d) ? r : 0.0f);
 }
 
-// angleDelta(a, b) (xpbd.js:137-143)
-__device__ __forceinline__ float kx_angle_delta(float a, float b) {
-  float d = a - b;
-  if (fabsf(d) <= KX_PI_BELOW) return d;
-  return kx_fold_d((double)a - (double)b);
-}
+// angleDelta(a, b) (xpbd.js:137-143): the difference folded into (-pi, pi]
+__device__ __forceinline__ float kx_angle_delta(float a, float b) { return kx_fold(a - b); }
 // wrapAngle (xpbd.js:128-134), only reached when |a| > 50
-__device__ __forceinline__ float kx_wrap_angle(float a) { return kx_fold_d((double)a); }
+__device__ __forceinline__ float kx_wrap_angle(float a) { return kx_fold(a); }
+
+// sin and cos of one argument for |x| up to a few hundred radians:
+// Cody-Waite reduction by pi/2 in three binary32 parts, then the cephes
+// single-precision minimax polynomials on [-pi/4, pi/4] (S. L. Moshier,
+// Cephes Math Library, sinf.c / cosf.c), about 1-2 ulp.
+__device__ __forceinline__ void kx_sincos(float x, float *s, float *c) {
+  float k = rintf(x * 0.636619772367581343f);
+  float r = fmaf(-k, 1.5703125f, x);
+  r = fmaf(-k, 4.837512969970703125e-4f, r);
+  r = fmaf(-k, 7.54978995489188216e-8f, r);
+  float r2 = r * r;
+  float ps = fmaf(fmaf(-1.9515295891e-4f, r2, 8.3321608736e-3f), r2, -1.6666654611e-1f);
+  float sn = fmaf(r * r2, ps, r);
+  float pc = fmaf(fmaf(2.443315711809948e-5f, r2, -1.388731625493765e-3f), r2, 4.166664568298827e-2f);
+  float cs = fmaf(r2 * r2, pc, fmaf(-0.5f, r2, 1.0f));
+  int q = (int)k;
+  float ss = (q & 1) ? cs : sn;
+  float cc = (q & 1) ? sn : cs;
+  if (q & 2) ss = -ss;
+  if ((q + 1) & 2) cc = -cc;
+  *s = ss;
+  *c = cc;
+}
+
+// 1/x: hardware reciprocal refined by one Newton step
+__device__ __forceinline__ float kx_rcp(float x) {
+  float r = __builtin_amdgcn_rcpf(x);
+  return r * fmaf(-x, r, 2.0f);
+}
 
 // ── SDF over the rect list (map.js:91-118), fold order and the best²
 //    exterior test preserved. (x, y) are relative to the origin (ox, oy).
@@ -98,80 +123,124 @@ __device__ __forceinline__ bool kx_sdf_grid(const KxDev &d, float x, float y, fl
   return true;
 }
 
+// Same fold, after a test of the cell's bit in an occupancy bitmap held in
+// LDS: queries in empty cells (no rect within the cull radius) return at once
+// without touching global memory.
+__device__ __forceinline__ bool kx_sdf_grid_bm(const KxDev &d, const uint32_t *bits, float x, float y, float ox,
+                                               float oy, KxHit &h) {
+  float gx = (x + ox - d.gx0) * d.inv_cs;
+  float gy = (y + oy - d.gy0) * d.inv_cs;
+  if (!(gx >= 0.0f && gy >= 0.0f && gx < (float)d.gnx && gy < (float)d.gny)) return false;
+  int ix = min((int)gx, d.gnx - 1), iy = min((int)gy, d.gny - 1);
+  int cell = iy * d.gnx + ix;
+  if (!((bits[cell >> 5] >> (cell & 31)) & 1u)) return false;
+  int b = d.cell_off[cell], e = d.cell_off[cell + 1];
+  float best = INFINITY, bnx = 0.0f, bny = 0.0f;
+  for (int k = b; k < e; k++) kx_fold_rect(x, y, d.rects[d.cell_ids[k]], ox, oy, best, bnx, bny);
+  if (!isfinite(best)) return false;
+  h.d = best; h.nx = bnx; h.ny = bny;
+  return true;
+}
+
 // ── constraint projections (xpbd.js:331-439) ─────────────────────────
-// Pin / grip: C = (pA + R(thA) la) - (pB + R(thB) lb) or the world anchor,
-// exact 2x2 effective mass.  P = float* to global or LDS arrays.
-template <class P, class Q>
-__device__ __forceinline__ void kx_solve_pin(P ppx, P ppy, P angP, Q im, Q ii, int A, int B,
-                                             float lax, float lay, float lbx, float lby,
-                                             float alpha, float &lamx, float &lamy) {
+// Pin / grip on values: C = (pA + R(thA) la) - (pB + R(thB) lb), or the
+// world anchor (lbx, lby) when !hasB, exact 2x2 effective mass. Returns true
+// and the projected body states when a correction applies.
+__device__ __forceinline__ bool kx_pin_core(float &xA, float &yA, float &tA, float wA, float wIA, float &xB,
+                                            float &yB, float &tB, float wB, float wIB, bool hasB, float lax,
+                                            float lay, float lbx, float lby, float alpha, float &lamx,
+                                            float &lamy) {
   float sa, ca;
-  sincosf(angP[A], &sa, &ca);
+  kx_sincos(tA, &sa, &ca);
   float rax = lax * ca - lay * sa, ray = lax * sa + lay * ca;
-  float gax = ppx[A] + rax, gay = ppy[A] + ray;
-  float gbx = lbx, gby = lby, rbx = 0.0f, rby = 0.0f, wB = 0.0f, wIB = 0.0f;
-  if (B >= 0) {
+  float gax = xA + rax, gay = yA + ray;
+  float gbx = lbx, gby = lby, rbx = 0.0f, rby = 0.0f;
+  if (hasB) {
     float sb, cb;
-    sincosf(angP[B], &sb, &cb);
+    kx_sincos(tB, &sb, &cb);
     rbx = lbx * cb - lby * sb;
     rby = lbx * sb + lby * cb;
-    gbx = ppx[B] + rbx; gby = ppy[B] + rby;
-    wB = im[B]; wIB = ii[B];
+    gbx = xB + rbx; gby = yB + rby;
+  } else {
+    wB = 0.0f; wIB = 0.0f;
   }
   float Cx = gax - gbx, Cy = gay - gby;
-  if (!isfinite(Cx) || !isfinite(Cy)) return;
-  if (Cx * Cx + Cy * Cy < KX_SLACK_POS2) return;
+  // a non-finite C makes dlx/dly non-finite and is rejected below
+  if (Cx * Cx + Cy * Cy < KX_SLACK_POS2) return false;
   float pax = -ray, pay = rax, pbx = -rby, pby = rbx;
-  float wA = im[A], wIA = ii[A];
   float wSum = wA + wB;
   float k00 = wSum + wIA * pax * pax + wIB * pbx * pbx;
   float k01 = wIA * pax * pay + wIB * pbx * pby;
   float k11 = wSum + wIA * pay * pay + wIB * pby * pby;
   float det = k00 * k11 - k01 * k01;
-  if (fabsf(det) < 1e-12f) return;
+  if (fabsf(det) < 1e-12f) return false;
   float bx = Cx + alpha * lamx;
   float by = Cy + alpha * lamy;
-  float inv = 1.0f / det;
+  float inv = kx_rcp(det);
   float dlx = -(k11 * inv * bx - k01 * inv * by);
   float dly = -(-k01 * inv * bx + k00 * inv * by);
-  if (!isfinite(dlx) || !isfinite(dly)) return;
+  if (!isfinite(dlx) || !isfinite(dly)) return false;
   lamx += dlx; lamy += dly;
-  ppx[A] += wA * dlx; ppy[A] += wA * dly;
-  angP[A] += wIA * (pax * dlx + pay * dly);
-  if (B >= 0) {
-    ppx[B] -= wB * dlx; ppy[B] -= wB * dly;
-    angP[B] -= wIB * (pbx * dlx + pby * dly);
-  }
+  xA += wA * dlx; yA += wA * dly;
+  tA += wIA * (pax * dlx + pay * dly);
+  xB -= wB * dlx; yB -= wB * dly;
+  tB -= wIB * (pbx * dlx + pby * dly);
+  return true;
 }
 
-// Angle motor: scalar XPBD on the deviation from rest, then the one-sided
-// joint limit (xpbd.js:392-439).
-template <class P, class Q>
-__device__ __forceinline__ void kx_solve_angle(P angP, Q ii, int A, int B, float rest, float target,
-                                               float mn, float mx, float alpha, float &lam) {
-  float wA = ii[A], wB = B >= 0 ? ii[B] : 0.0f;
+// Angle motor on values: scalar XPBD on the deviation from rest, then the
+// one-sided joint limit (xpbd.js:392-439). Returns true if an angle moved.
+__device__ __forceinline__ bool kx_angle_core(float &tA, float &tB, float wA, float wB, bool hasB, float rest,
+                                              float target, float mn, float mx, float alpha, float &lam) {
+  if (!hasB) wB = 0.0f;
   float wSum = wA + wB;
-  if (wSum <= 0.0f) return;
-  float rel = B >= 0 ? kx_angle_delta(angP[B], angP[A]) : angP[A];
+  if (wSum <= 0.0f) return false;
+  bool moved = false;
+  float rel = hasB ? kx_angle_delta(tB, tA) : tA;
   float dev = kx_angle_delta(rel, rest);
   float C = kx_angle_delta(dev, target);
   if (fabsf(C) >= KX_SLACK_ANG) {
-    float dl = -(C + alpha * lam) / (wSum + alpha);
-    if (!isfinite(dl)) return;
+    float dl = -(C + alpha * lam) * kx_rcp(wSum + alpha);
+    if (!isfinite(dl)) return false;
     lam += dl;
-    if (B >= 0) { angP[B] += wB * dl; angP[A] -= wA * dl; }
-    else angP[A] += wA * dl;
+    if (hasB) { tB += wB * dl; tA -= wA * dl; }
+    else tA += wA * dl;
+    moved = true;
   }
-  float rel2 = B >= 0 ? kx_angle_delta(angP[B], angP[A]) : angP[A];
+  float rel2 = hasB ? kx_angle_delta(tB, tA) : tA;
   float dev2 = kx_angle_delta(rel2, rest);
   float over = 0.0f;
   if (dev2 > mx) over = dev2 - mx;
   else if (dev2 < mn) over = dev2 - mn;
   if (fabsf(over) >= KX_SLACK_ANG) {
-    float dl2 = -over / wSum;
-    if (B >= 0) { angP[B] += wB * dl2; angP[A] -= wA * dl2; }
-    else angP[A] += wA * dl2;
+    float dl2 = -over * kx_rcp(wSum);
+    if (hasB) { tB += wB * dl2; tA -= wA * dl2; }
+    else tA += wA * dl2;
+    moved = true;
   }
+  return moved;
+}
+
+// SoA forms used by the plain path (P, Q = float* to global arrays).
+template <class P, class Q>
+__device__ __forceinline__ void kx_solve_pin(P ppx, P ppy, P angP, Q im, Q ii, int A, int B, float lax, float lay,
+                                             float lbx, float lby, float alpha, float &lamx, float &lamy) {
+  float xA = ppx[A], yA = ppy[A], tA = angP[A];
+  float xB = 0.0f, yB = 0.0f, tB = 0.0f, wB = 0.0f, wIB = 0.0f;
+  if (B >= 0) { xB = ppx[B]; yB = ppy[B]; tB = angP[B]; wB = im[B]; wIB = ii[B]; }
+  if (!kx_pin_core(xA, yA, tA, im[A], ii[A], xB, yB, tB, wB, wIB, B >= 0, lax, lay, lbx, lby, alpha, lamx, lamy))
+    return;
+  ppx[A] = xA; ppy[A] = yA; angP[A] = tA;
+  if (B >= 0) { ppx[B] = xB; ppy[B] = yB; angP[B] = tB; }
+}
+
+template <class P, class Q>
+__device__ __forceinline__ void kx_solve_angle(P angP, Q ii, int A, int B, float rest, float target, float mn,
+                                               float mx, float alpha, float &lam) {
+  float tA = angP[A], tB = B >= 0 ? angP[B] : 0.0f;
+  if (!kx_angle_core(tA, tB, ii[A], B >= 0 ? ii[B] : 0.0f, B >= 0, rest, target, mn, mx, alpha, lam)) return;
+  angP[A] = tA;
+  if (B >= 0) angP[B] = tB;
 }
 
 // Integration of one body (xpbd.js:262-287).

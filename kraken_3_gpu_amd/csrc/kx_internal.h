@@ -52,6 +52,8 @@ struct KxDev {
   int nr;
   const int *cell_off;           // [gnx*gny+1]
   const int *cell_ids;
+  const uint32_t *cell_bits;     // occupancy bitmap, 1 bit per cell
+  int cell_words;
   float gx0, gy0, inv_cs;
   int gnx, gny;
   // creatures / arms
@@ -90,6 +92,7 @@ struct KxPersistCfg {
   const KxTmpl *tmpl;
   const uint32_t *pool;
   int tmpl_words;           // LDS words reserved for the template
+  int bm_words;             // LDS words of the cell occupancy bitmap (after the template)
   int cr_stride;            // LDS words per creature (odd, avoids bank aliasing)
   int nwaves;
   int NB, NC;               // per-creature array sizes in LDS

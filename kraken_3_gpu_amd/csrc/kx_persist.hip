@@ -25,17 +25,16 @@
     __builtin_amdgcn_wave_barrier();                       \
   } while (0)
 
-// per-creature LDS layout (word offsets), arrays of NB bodies / NC constraints
+// per-creature LDS layout (word offsets, 16-byte records):
+//   pos[i] = {ppx, ppy, angP, -}   prv[i] = {px, py, ang, -}   vel[i] = {vx, vy, av, -}
+//   dyn[k] = {alpha, target, lbx, lby}   lam[k] = {lambda_x|lambda, lambda_y, breakForce, broken}
 struct KxCrLds {
-  int px, py, ang, vx, vy, av, ppx, ppy, angP;
-  int alpha, tgt, lbx, lby, bf, l0, l1, brk, ev, words;
+  int pos, prv, vel, dyn, lam, ev, words;
   __host__ __device__ KxCrLds(int NB, int NC) {
     int o = 0;
-    px = o; o += NB; py = o; o += NB; ang = o; o += NB; vx = o; o += NB; vy = o; o += NB; av = o; o += NB;
-    ppx = o; o += NB; ppy = o; o += NB; angP = o; o += NB;
-    alpha = o; o += NC; tgt = o; o += NC; lbx = o; o += NC; lby = o; o += NC; bf = o; o += NC;
-    l0 = o; o += NC; l1 = o; o += NC; brk = o; o += NC;
-    ev = o; o += 1;
+    pos = o; o += 4 * NB; prv = o; o += 4 * NB; vel = o; o += 4 * NB;
+    dyn = o; o += 4 * NC; lam = o; o += 4 * NC;
+    ev = o; o += 4;
     words = o;
   }
 };
@@ -47,27 +46,24 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
                                                              const int *step_base, int nsteps, int ev_row0,
                                                              int with_forces, KxPersistCfg P) {
   constexpr int CPW = KX_WAVE / G;
-  extern __shared__ uint32_t smem[];
+  extern __shared__ __attribute__((aligned(16))) uint32_t smem[];
   const int lane = threadIdx.x & (KX_WAVE - 1);
   const int q = lane / G, sub = lane % G;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / KX_WAVE);
   const int gw = blockIdx.x * WPB + w;
 
-  // ── template -> LDS (the only workgroup barrier) ──
+  // ── template + cell bitmap -> LDS (the only workgroup barrier) ──
   const int tid = P.wave_tmpl[blockIdx.x * WPB];
   const KxTmpl T = P.tmpl[tid];
   for (int i = threadIdx.x; i < T.words; i += blockDim.x) smem[i] = P.pool[T.base + i];
+  uint32_t *bits = smem + P.tmpl_words;
+  for (int i = threadIdx.x; i < P.bm_words; i += blockDim.x) bits[i] = d.cell_bits[i];
   __syncthreads();
   if (gw >= P.nwaves) return;
 
-  const int *t_cab = (const int *)(smem + T.o_cab);
-  const float *t_lax = (const float *)(smem + T.o_lax), *t_lay = (const float *)(smem + T.o_lay);
-  const float *t_rest = (const float *)(smem + T.o_rest), *t_mn = (const float *)(smem + T.o_mn);
-  const float *t_mx = (const float *)(smem + T.o_mx);
-  const float *t_im = (const float *)(smem + T.o_im), *t_ii = (const float *)(smem + T.o_ii);
-  const int *t_bpo = (const int *)(smem + T.o_bpo);
-  const float *t_plx = (const float *)(smem + T.o_plx), *t_ply = (const float *)(smem + T.o_ply);
-  const float *t_pr = (const float *)(smem + T.o_pr);
+  const float4 *t_crec = (const float4 *)(smem + T.o_crec);   // 2 per constraint
+  const float4 *t_brec = (const float4 *)(smem + T.o_brec);
+  const float4 *t_prec = (const float4 *)(smem + T.o_prec);
   const int *t_sched = (const int *)(smem + T.o_sched), *t_rtype = (const int *)(smem + T.o_rtype);
   const int *t_coll = (const int *)(smem + T.o_coll), *t_ccnt = (const int *)(smem + T.o_ccnt);
   const int *t_arm_tip = (const int *)(smem + T.o_arm_tip), *t_arm_grip = (const int *)(smem + T.o_arm_grip);
@@ -77,13 +73,10 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
   const int cid = P.wave_cr[gw * CPW + q];
   const bool act = cid >= 0;
   const KxCrLds Lo(P.NB, P.NC);
-  float *S = (float *)(smem + P.tmpl_words + (size_t)(w * CPW + q) * P.cr_stride);
-  float *px = S + Lo.px, *py = S + Lo.py, *ang = S + Lo.ang, *vx = S + Lo.vx, *vy = S + Lo.vy, *av = S + Lo.av;
-  float *ppx = S + Lo.ppx, *ppy = S + Lo.ppy, *angP = S + Lo.angP;
-  float *alpha = S + Lo.alpha, *tgt = S + Lo.tgt, *lbx = S + Lo.lbx, *lby = S + Lo.lby, *bf = S + Lo.bf;
-  float *l0 = S + Lo.l0, *l1 = S + Lo.l1;
-  int *brk = (int *)(S + Lo.brk);
-  uint32_t *evw = (uint32_t *)(S + Lo.ev);
+  uint32_t *S = smem + P.tmpl_words + P.bm_words + (size_t)(w * CPW + q) * P.cr_stride;
+  float4 *pos = (float4 *)(S + Lo.pos), *prv = (float4 *)(S + Lo.prv), *vel = (float4 *)(S + Lo.vel);
+  float4 *dyn = (float4 *)(S + Lo.dyn), *lamr = (float4 *)(S + Lo.lam);
+  uint32_t *evw = S + Lo.ev;
   const int nb = act ? T.nb : 0, nc = act ? T.nc : 0;
   const int b0 = act ? d.cr[cid].body_off : 0, c0 = act ? d.cr[cid].cons_off : 0;
 
@@ -93,17 +86,15 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
   // ── load the creature's dynamic state ──
   for (int i = sub; i < nb; i += G) {
     int g = b0 + i;
-    px[i] = d.px[g] - ox; py[i] = d.py[g] - oy; ang[i] = d.ang[g];
-    vx[i] = d.vx[g]; vy[i] = d.vy[g]; av[i] = d.av[g];
+    prv[i] = make_float4(d.px[g] - ox, d.py[g] - oy, d.ang[g], 0.0f);
+    vel[i] = make_float4(d.vx[g], d.vy[g], d.av[g], 0.0f);
   }
   for (int k = sub; k < nc; k += G) {
     int g = c0 + k;
-    bool world = ((t_cab[k] >> 8) & 255) == 255;
-    alpha[k] = d.comp[g] * d.invH2; tgt[k] = d.target[g];
-    lbx[k] = world ? d.lbx[g] - ox : d.lbx[g];
-    lby[k] = world ? d.lby[g] - oy : d.lby[g];
-    bf[k] = d.bf[g]; brk[k] = d.broken[g];
-    l0[k] = 0.0f; l1[k] = 0.0f;
+    bool world = ((__float_as_int(t_crec[2 * k].x) >> 8) & 255) == 255;
+    dyn[k] = make_float4(d.comp[g] * d.invH2, d.target[g], world ? d.lbx[g] - ox : d.lbx[g],
+                         world ? d.lby[g] - oy : d.lby[g]);
+    lamr[k] = make_float4(0.0f, 0.0f, d.bf[g], __int_as_float(d.broken[g]));
   }
   double my_cd = 0.0;
   if (act && sub < T.narms) my_cd = d.cd[cid * KX_MAX_ARMS + sub];
@@ -119,25 +110,30 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
       const float a_law = (float)L.comp * d.invH2;
       for (int j = sub; j < (act ? T.nmot : 0); j += G) {
         int k = t_mot_k[j];
-        tgt[k] = kx_law_target(L, (uint32_t)cid, (uint32_t)key, t_mot_slot[j]);
-        alpha[k] = a_law;
+        float4 dk = dyn[k];
+        dk.x = a_law;
+        dk.y = kx_law_target(L, (uint32_t)cid, (uint32_t)key, t_mot_slot[j]);
+        dyn[k] = dk;
       }
       // adhesion reflex (core.js:199-218), one lane per arm
       if (act && sub < T.narms) {
         double adh = kx_ctl_u(L.seed, (uint32_t)cid, (uint32_t)key, KX_SLOT_ADH + sub);
         int g = t_arm_grip[sub], t = t_arm_tip[sub];
-        int bk = brk[g];
-        float f_bf = bf[g], gx = lbx[g] + ox, gy = lby[g] + oy, lx = l0[g], ly = l1[g];
-        uint32_t e = kx_reflex(L, hS, adh, my_cd, bk, f_bf, gx, gy, lx, ly, vx[t], vy[t],
-                               (double)ox + (double)px[t], (double)oy + (double)py[t], ang[t], t_lax[g], t_lay[g]);
-        brk[g] = bk; bf[g] = f_bf; lbx[g] = gx - ox; lby[g] = gy - oy; l0[g] = lx; l1[g] = ly;
+        float4 dk = dyn[g], lk = lamr[g], tp = prv[t], tv = vel[t], cr0 = t_crec[2 * g];
+        int bk = __float_as_int(lk.w);
+        float f_bf = lk.z, gx = dk.z + ox, gy = dk.w + oy, lx = lk.x, ly = lk.y;
+        uint32_t e = kx_reflex(L, hS, adh, my_cd, bk, f_bf, gx, gy, lx, ly, tv.x, tv.y, (double)ox + (double)tp.x,
+                               (double)oy + (double)tp.y, tp.z, cr0.y, cr0.z);
+        dk.z = gx - ox; dk.w = gy - oy;
+        dyn[g] = dk;
+        lamr[g] = make_float4(lx, ly, f_bf, __int_as_float(bk));
         my_ev |= e << (3 * sub);
       }
       KX_WAVE_SYNC();
     }
     for (int s = 0; s < d.S; s++) {
       // resetLambda + integrate (xpbd.js:262-287)
-      for (int k = sub; k < nc; k += G) { l0[k] = 0.0f; l1[k] = 0.0f; }
+      for (int k = sub; k < nc; k += G) { float4 lk = lamr[k]; lk.x = 0.0f; lk.y = 0.0f; lamr[k] = lk; }
       const bool forces = with_forces && step == 0 && s == 0;
       for (int i = sub; i < nb; i += G) {
         float fx = 0.0f, fy = 0.0f, tq = 0.0f;
@@ -146,31 +142,43 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
           fx = d.fx[g]; fy = d.fy[g]; tq = d.tq[g];
           d.fx[g] = 0.0f; d.fy[g] = 0.0f; d.tq[g] = 0.0f;
         }
-        float a_vx = vx[i], a_vy = vy[i], a_av = av[i], q_x, q_y, q_a;
-        kx_integrate(d, a_vx, a_vy, a_av, px[i], py[i], ang[i], t_im[i], t_ii[i], fx, fy, tq, q_x, q_y, q_a);
-        vx[i] = a_vx; vy[i] = a_vy; av[i] = a_av; ppx[i] = q_x; ppy[i] = q_y; angP[i] = q_a;
+        float4 p = prv[i], v = vel[i], br = t_brec[i];
+        float q_x, q_y, q_a;
+        kx_integrate(d, v.x, v.y, v.z, p.x, p.y, p.z, br.x, br.y, fx, fy, tq, q_x, q_y, q_a);
+        vel[i] = v;
+        pos[i] = make_float4(q_x, q_y, q_a, 0.0f);
       }
       KX_WAVE_SYNC();
       for (int it = 0; it < d.I; it++) {
         // constraint rounds (type-homogeneous, wave-uniform branch)
         for (int r = 0; r < T.R; r++) {
           const int k = act ? t_sched[r * G + sub] : -1;
-          if (t_rtype[r]) {
-            if (k >= 0 && !brk[k]) {
-              int v = t_cab[k];
+          if (k >= 0) {
+            float4 lk = lamr[k];
+            if (__float_as_int(lk.w) == 0) {
+              float4 c0r = t_crec[2 * k];
+              int v = __float_as_int(c0r.x);
               int A = v & 255, B = (v >> 8) & 255;
-              float lam = l0[k];
-              kx_solve_angle(angP, t_ii, A, B == 255 ? -1 : B, t_rest[k], tgt[k], t_mn[k], t_mx[k], alpha[k], lam);
-              l0[k] = lam;
-            }
-          } else {
-            if (k >= 0 && !brk[k]) {
-              int v = t_cab[k];
-              int A = v & 255, B = (v >> 8) & 255;
-              float lx = l0[k], ly = l1[k];
-              kx_solve_pin(ppx, ppy, angP, t_im, t_ii, A, B == 255 ? -1 : B, t_lax[k], t_lay[k], lbx[k], lby[k],
-                           alpha[k], lx, ly);
-              l0[k] = lx; l1[k] = ly;
+              bool hasB = B != 255;
+              if (!hasB) B = A;
+              float4 pA = pos[A], pB = pos[B];
+              float4 bA = t_brec[A], bB = t_brec[B];
+              float4 dk = dyn[k];
+              if (t_rtype[r]) {
+                float4 c1r = t_crec[2 * k + 1];
+                if (kx_angle_core(pA.z, pB.z, bA.y, bB.y, hasB, c0r.w, dk.y, c1r.x, c1r.y, dk.x, lk.x)) {
+                  pos[A] = pA;
+                  if (hasB) pos[B] = pB;
+                  lamr[k] = lk;
+                }
+              } else {
+                if (kx_pin_core(pA.x, pA.y, pA.z, bA.x, bA.y, pB.x, pB.y, pB.z, bB.x, bB.y, hasB, c0r.y, c0r.z,
+                                dk.z, dk.w, dk.x, lk.x, lk.y)) {
+                  pos[A] = pA;
+                  if (hasB) pos[B] = pB;
+                  lamr[k] = lk;
+                }
+              }
             }
           }
           KX_WAVE_SYNC();
@@ -179,41 +187,47 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
         const int nbl = act ? t_ccnt[sub] : 0;
         for (int j = 0; j < nbl; j++) {
           int i = t_coll[sub * T.maxbl + j];
-          if (t_im[i] == 0.0f) continue;
-          int bp = t_bpo[i];
+          float4 br = t_brec[i];
+          if (br.x == 0.0f) continue;
+          int bp = __float_as_int(br.z);
           int o = bp & 0xffff, cnt = bp >> 16;
-          float x = ppx[i], y = ppy[i];
+          float4 p = pos[i];
+          float x = p.x, y = p.y;
           float sa, ca;
-          sincosf(angP[i], &sa, &ca);
+          kx_sincos(p.z, &sa, &ca);
           for (int pq = 0; pq < cnt; pq++) {
-            float lx = t_plx[o + pq], ly = t_ply[o + pq];
-            float wx = x + (lx * ca - ly * sa);
-            float wy = y + (lx * sa + ly * ca);
+            float4 pt = t_prec[o + pq];
+            float wx = x + (pt.x * ca - pt.y * sa);
+            float wy = y + (pt.x * sa + pt.y * ca);
             KxHit hh;
-            if (!kx_sdf_grid(d, wx, wy, ox, oy, hh)) continue;
-            float pen = t_pr[o + pq] - hh.d;
+            bool hit = P.bm_words ? kx_sdf_grid_bm(d, bits, wx, wy, ox, oy, hh) : kx_sdf_grid(d, wx, wy, ox, oy, hh);
+            if (!hit) continue;
+            float pen = pt.z - hh.d;
             if (pen > KX_SLACK_POS) { x += hh.nx * pen; y += hh.ny * pen; }
           }
-          ppx[i] = x; ppy[i] = y;
+          p.x = x; p.y = y;
+          pos[i] = p;
         }
         KX_WAVE_SYNC();
       }
       for (int i = sub; i < nb; i += G) {
-        float a_px = px[i], a_py = py[i], a_ang = ang[i], a_vx, a_vy, a_av, a_angP = angP[i];
-        kx_finalize(d, a_px, a_py, a_ang, a_vx, a_vy, a_av, ppx[i], ppy[i], a_angP);
-        px[i] = a_px; py[i] = a_py; ang[i] = a_ang; angP[i] = a_angP;
-        vx[i] = a_vx; vy[i] = a_vy; av[i] = a_av;
+        float4 p = prv[i], pp = pos[i];
+        float vx, vy, av;
+        kx_finalize(d, p.x, p.y, p.z, vx, vy, av, pp.x, pp.y, pp.z);
+        prv[i] = p;
+        pos[i] = pp;
+        vel[i] = make_float4(vx, vy, av, 0.0f);
       }
       KX_WAVE_SYNC();
     }
     // grip break (xpbd.js:496-504 / solver.wgsl:433-442)
     for (int j = sub; j < (act ? T.ngrip : 0); j += G) {
       int k = t_grip_k[j];
-      if (brk[k]) continue;
-      float lx = l0[k], ly = l1[k];
-      float f = sqrtf(lx * lx + ly * ly) * d.invH;
-      if (f > bf[k]) {
-        brk[k] = 1; l0[k] = 0.0f; l1[k] = 0.0f;
+      float4 lk = lamr[k];
+      if (__float_as_int(lk.w)) continue;
+      float f = sqrtf(lk.x * lk.x + lk.y * lk.y) * d.invH;
+      if (f > lk.z) {
+        lamr[k] = make_float4(0.0f, 0.0f, lk.z, __int_as_float(1));
         int arm = t_grip_arm[j];
         if (arm >= 0) my_ev |= 4u << (3 * arm);
       }
@@ -224,12 +238,21 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
       if (act && sub == 0) d.events[(size_t)(ev_row0 + step) * d.nCr + cid] = evw[0];
     }
     // re-centre the origin on the first body when it drifted
-    float sx = act ? rintf(px[0]) : 0.0f, sy = act ? rintf(py[0]) : 0.0f;
+    float4 p0 = act ? prv[0] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float sx = rintf(p0.x), sy = rintf(p0.y);
     KX_WAVE_SYNC();
     if (fabsf(sx) > 32.0f || fabsf(sy) > 32.0f) {
-      for (int i = sub; i < nb; i += G) { px[i] -= sx; py[i] -= sy; ppx[i] -= sx; ppy[i] -= sy; }
+      for (int i = sub; i < nb; i += G) {
+        float4 p = prv[i], pp = pos[i];
+        p.x -= sx; p.y -= sy; pp.x -= sx; pp.y -= sy;
+        prv[i] = p; pos[i] = pp;
+      }
       for (int k = sub; k < nc; k += G)
-        if (((t_cab[k] >> 8) & 255) == 255) { lbx[k] -= sx; lby[k] -= sy; }
+        if (((__float_as_int(t_crec[2 * k].x) >> 8) & 255) == 255) {
+          float4 dk = dyn[k];
+          dk.z -= sx; dk.w -= sy;
+          dyn[k] = dk;
+        }
       ox += sx; oy += sy;
     }
     KX_WAVE_SYNC();
@@ -238,19 +261,21 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
   // ── store ──
   for (int i = sub; i < nb; i += G) {
     int g = b0 + i;
-    d.px[g] = px[i] + ox; d.py[g] = py[i] + oy; d.ang[g] = ang[i];
-    d.vx[g] = vx[i]; d.vy[g] = vy[i]; d.av[g] = av[i];
-    d.ppx[g] = ppx[i] + ox; d.ppy[g] = ppy[i] + oy; d.angP[g] = angP[i];
+    float4 p = prv[i], pp = pos[i], v = vel[i];
+    d.px[g] = p.x + ox; d.py[g] = p.y + oy; d.ang[g] = p.z;
+    d.vx[g] = v.x; d.vy[g] = v.y; d.av[g] = v.z;
+    d.ppx[g] = pp.x + ox; d.ppy[g] = pp.y + oy; d.angP[g] = pp.z;
   }
   for (int k = sub; k < nc; k += G) {
     int g = c0 + k;
-    int v = t_cab[k];
-    d.broken[g] = brk[k];
-    if (((v >> 8) & 255) == 255) { d.lbx[g] = lbx[k] + ox; d.lby[g] = lby[k] + oy; }
-    d.bf[g] = bf[k]; d.target[g] = tgt[k];
+    int v = __float_as_int(t_crec[2 * k].x);
+    float4 dk = dyn[k], lk = lamr[k];
+    d.broken[g] = __float_as_int(lk.w);
+    if (((v >> 8) & 255) == 255) { d.lbx[g] = dk.z + ox; d.lby[g] = dk.w + oy; }
+    d.bf[g] = lk.z; d.target[g] = dk.y;
     if (has_law && (v >> 16) == KX_ANGLE) d.comp[g] = (float)L.comp;
-    if ((v >> 16) == KX_ANGLE) { d.lam[g] = l0[k]; d.lamx[g] = 0.0f; d.lamy[g] = 0.0f; }
-    else { d.lamx[g] = l0[k]; d.lamy[g] = l1[k]; d.lam[g] = 0.0f; }
+    if ((v >> 16) == KX_ANGLE) { d.lam[g] = lk.x; d.lamx[g] = 0.0f; d.lamy[g] = 0.0f; }
+    else { d.lamx[g] = lk.x; d.lamy[g] = lk.y; d.lam[g] = 0.0f; }
   }
   if (act && sub < T.narms) d.cd[cid * KX_MAX_ARMS + sub] = my_cd;
 }
@@ -268,7 +293,7 @@ template <int G, int WPB>
 static int launch_gw(const KxDev &d, const KxLaw *law, int step0, const int *step_base, int nsteps, int ev_row0,
                      int with_forces, const KxPersistCfg &P, hipStream_t st) {
   constexpr int CPW = KX_WAVE / G;
-  size_t lds = ((size_t)P.tmpl_words + (size_t)WPB * CPW * P.cr_stride) * 4;
+  size_t lds = ((size_t)P.tmpl_words + P.bm_words + (size_t)WPB * CPW * P.cr_stride) * 4;
   if (lds > 160 * 1024) return -22;
   static bool attr_set = false;
   if (!attr_set) {

@@ -120,7 +120,7 @@ __global__ __launch_bounds__(PLAIN_BLOCK) void kx_k_collide(KxDev d) {
   float ox = d.org[2 * c], oy = d.org[2 * c + 1];
   float x = d.ppx[i], y = d.ppy[i];
   float sa, ca;
-  sincosf(d.angP[i], &sa, &ca);
+  kx_sincos(d.angP[i], &sa, &ca);
   for (int q = 0; q < cnt; q++) {
     float lx = d.plx[o + q], ly = d.ply[o + q];
     float wx = x + (lx * ca - ly * sa);

@@ -97,11 +97,18 @@ void kx_schedule_collisions(const TmplHost &t, int G, std::vector<int> &lists, s
 static void put(std::vector<uint32_t> &pool, const std::vector<int> &v) {
   for (int x : v) pool.push_back((uint32_t)x);
 }
-static void putf(std::vector<uint32_t> &pool, const std::vector<float> &v) {
-  for (float x : v) { uint32_t u; memcpy(&u, &x, 4); pool.push_back(u); }
+static uint32_t fbits(float x) {
+  uint32_t u;
+  memcpy(&u, &x, 4);
+  return u;
 }
 
 // Serialise one template with its schedule for G lanes into the word pool.
+// Sections start on 16-byte boundaries; constraint, body and point records
+// are 16-byte vectors so the kernel reads each with one LDS instruction:
+//   crec[k] = {cab, lax, lay, rest} {min, max, -, -}   (2 x 16 B)
+//   brec[b] = {im, ii, point offset | count << 16, -}
+//   prec[q] = {lx, ly, r, -}
 KxTmpl kx_pack_template(const TmplHost &t, int G, std::vector<uint32_t> &pool) {
   KxTmpl T{};
   T.nb = t.nb; T.nc = (int)t.type.size(); T.np = (int)t.plx.size();
@@ -109,24 +116,24 @@ KxTmpl kx_pack_template(const TmplHost &t, int G, std::vector<uint32_t> &pool) {
   std::vector<int> sched, rtype, lists, counts;
   kx_schedule_rounds(t, G, sched, rtype, T.R);
   kx_schedule_collisions(t, G, lists, counts, T.maxbl);
+  while (pool.size() % 4) pool.push_back(0u);
   T.base = (int)pool.size();
-  auto at = [&]() { return (int)pool.size() - T.base; };
-  std::vector<int> cab(T.nc);
-  for (int k = 0; k < T.nc; k++) cab[k] = t.a[k] | ((t.b[k] < 0 ? 255 : t.b[k]) << 8) | (t.type[k] << 16);
-  T.o_cab = at(); put(pool, cab);
-  T.o_lax = at(); putf(pool, t.lax);
-  T.o_lay = at(); putf(pool, t.lay);
-  T.o_rest = at(); putf(pool, t.rest);
-  T.o_mn = at(); putf(pool, t.mn);
-  T.o_mx = at(); putf(pool, t.mx);
-  T.o_im = at(); putf(pool, t.im);
-  T.o_ii = at(); putf(pool, t.ii);
-  std::vector<int> bpo(t.nb);
-  for (int b = 0; b < t.nb; b++) bpo[b] = t.poff[b] | (t.pcnt[b] << 16);
-  T.o_bpo = at(); put(pool, bpo);
-  T.o_plx = at(); putf(pool, t.plx);
-  T.o_ply = at(); putf(pool, t.ply);
-  T.o_pr = at(); putf(pool, t.pr);
+  auto at = [&]() {
+    while ((pool.size() - T.base) % 4) pool.push_back(0u);
+    return (int)pool.size() - T.base;
+  };
+  T.o_crec = at();
+  for (int k = 0; k < T.nc; k++) {
+    int cab = t.a[k] | ((t.b[k] < 0 ? 255 : t.b[k]) << 8) | (t.type[k] << 16);
+    pool.insert(pool.end(), {(uint32_t)cab, fbits(t.lax[k]), fbits(t.lay[k]), fbits(t.rest[k]),
+                             fbits(t.mn[k]), fbits(t.mx[k]), 0u, 0u});
+  }
+  T.o_brec = at();
+  for (int b = 0; b < t.nb; b++)
+    pool.insert(pool.end(), {fbits(t.im[b]), fbits(t.ii[b]), (uint32_t)(t.poff[b] | (t.pcnt[b] << 16)), 0u});
+  T.o_prec = at();
+  for (size_t q = 0; q < t.plx.size(); q++)
+    pool.insert(pool.end(), {fbits(t.plx[q]), fbits(t.ply[q]), fbits(t.pr[q]), 0u});
   T.o_sched = at(); put(pool, sched);
   T.o_rtype = at(); put(pool, rtype);
   T.o_coll = at(); put(pool, lists);

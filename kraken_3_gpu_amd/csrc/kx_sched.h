@@ -10,7 +10,7 @@
 struct KxTmpl {
   int base, words;
   int nb, nc, np, narms, nmot, ngrip, R, maxbl;
-  int o_cab, o_lax, o_lay, o_rest, o_mn, o_mx, o_im, o_ii, o_bpo, o_plx, o_ply, o_pr;
+  int o_crec, o_brec, o_prec;     // 16-byte records (see kx_pack_template)
   int o_sched, o_rtype, o_coll, o_ccnt, o_arm_tip, o_arm_grip, o_mot_k, o_mot_slot, o_grip_k, o_grip_arm;
 };
 
