@@ -24,7 +24,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-FP32_PEAK_TFLOPS = 157.3       # MI355X FP32 vector peak
+FP32_PEAK_TFLOPS = 157.3       # MI355X published FP32 vector peak (packed FMA)
+FP32_UNPACKED_TFLOPS = 78.6    # the same without packed-FP32 instructions, which this build does not emit
 HBM_PEAK_GBS = 8000.0
 
 
@@ -126,6 +127,7 @@ def main():
     value = n_cr * args.steps / wall
     dev_ms = sum(m for m, _ in chunk_ms)
     st = G.download(g)
+    shape = G.persist_info(g)
 
     per_cr = dict(pins=42, motors=42, bodies=43, points=92, grips_total=6)
     F = flops_per_creature_step(w.S, w.I, per_cr["pins"], per_cr["motors"], per_cr["bodies"], per_cr["points"],
@@ -185,7 +187,9 @@ def main():
                      "frac": achieved_tflops / FP32_PEAK_TFLOPS, "traffic": None,
                      "flops_per_creature_step": F, "grips_attached_mean": g_att, "rects_within_16px_mean": r_near,
                      "kernel": "kx_k_persist", "launch_ms": launch_ms,
-                     "hbm_algorithmic_gbs": hbm_gbs, "hbm_peak_gbs": HBM_PEAK_GBS},
+                     "hbm_algorithmic_gbs": hbm_gbs, "hbm_peak_gbs": HBM_PEAK_GBS,
+                     "peak_unpacked": FP32_UNPACKED_TFLOPS, "frac_unpacked": achieved_tflops / FP32_UNPACKED_TFLOPS},
+        "launch_shape": dict(shape, lanes_per_creature=args.lanes or None, waves_per_block=args.wpb or None),
         "p50_rollout_ms": p50,
         "p99_rollout_ms": p99,
         "rollout_unit": f"{args.chunk} steps x {n_cr} creatures per launch",

@@ -75,8 +75,8 @@ typedef struct {
   double h;                          /* seconds per substep */
   double lin_drag, ang_drag, max_speed, max_ang_speed;
   int32_t path;                      /* KX_PATH_* */
-  int32_t lanes_per_creature;        /* persistent path: 8, 16, 32 or 64; 0 = default */
-  int32_t waves_per_block;           /* persistent path: 1, 2 or 4; 0 = default */
+  int32_t lanes_per_creature;        /* persistent path: 8, 16, 32 or 64; 0 = default (32) */
+  int32_t waves_per_block;           /* persistent path: 1, 2 or 4; 0 = default (2) */
 } kx_params;
 
 /* Body state in SoA (gpuworld.js:250-263 upload; 417-441 mirrorInto).
@@ -181,6 +181,13 @@ int kx_set_path(kx_handle *h, int32_t path, int32_t lanes_per_creature, int32_t 
 /* persistent path: rounds per iteration of each morphology template for the
  * current shape (out[n_templates]); returns the template count */
 int kx_schedule_info(const kx_handle *h, int32_t *rounds, int32_t max);
+/* persistent launch shape for the current (lanes, waves): info[0] LDS bytes
+ * per workgroup, [1] workgroups, [2] resident workgroups per CU, [3]
+ * workgroups per CU the launch needs, [4] LDS words of the staged map (0:
+ * map read from HBM), [5] cell-bitmap words, [6] template words, [7]
+ * per-creature LDS words. Returns the number of fields (8). No counterpart
+ * in the reference (its path has no persistent kernel). */
+int kx_persist_info(const kx_handle *h, int32_t *info, int32_t max);
 /* plain path only: 0 = linear scan over every rect per query (the shape of
  * src/gpu/solver.wgsl:114-147), 1 = the exact cull grid the persistent
  * path always uses */

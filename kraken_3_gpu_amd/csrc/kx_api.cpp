@@ -52,7 +52,8 @@ struct DevBuf {
 struct kx_handle {
   int nB = 0, nC = 0, nP = 0, nR = 0, nCr = 0, nArms = 0;
   kx_params prm{};
-  int path = KX_PATH_PERSISTENT, G = 16, wpb = 1;
+  int path = KX_PATH_PERSISTENT, G = 32, wpb = 2;
+  int n_cell_ids = 0;
   bool persist_ok = false, has_creatures = false, forces_pending = false;
   std::string persist_why;
   int maxb = 0, maxc = 0, maxp = 0;
@@ -83,6 +84,7 @@ struct kx_handle {
   std::vector<int> tmpl_rounds;
   KxPersistCfg pc{};
   void *pc_bufs[4] = {nullptr, nullptr, nullptr, nullptr};
+  int pinfo[8] = {};     // see kx_persist_info
   // scratch host staging
   std::vector<float> hf;
   std::vector<int> hi;
@@ -243,6 +245,7 @@ static int build_grid(kx_handle *h, const float *rects, int nR, float rmax) {
   TRY(dput(doff, off.data(), off.size()));
   TRY(dput(dids, ids.data(), ids.size()));
   h->d.cell_off = doff; h->d.cell_ids = dids;
+  h->n_cell_ids = (int)ids.size();
   h->d.gx0 = gx0; h->d.gy0 = gy0; h->d.inv_cs = 1.0f / cs; h->d.gnx = gnx; h->d.gny = gny;
   return 0;
 }
@@ -253,13 +256,13 @@ static int build_persist(kx_handle *h) {
   const int G = h->G, CPW = KX_WAVE / G, WPB = h->wpb;
   std::vector<uint32_t> pool;
   std::vector<KxTmpl> T;
-  int words = 0, NB = 1, NC = 1;
+  int words = 0, NB = 1, NC = 1, NP = 1;
   h->tmpl_rounds.clear();
   for (auto &t : h->tmpl_host) {
     KxTmpl k = kx_pack_template(t, G, pool);
     T.push_back(k);
     words = std::max(words, k.words);
-    NB = std::max(NB, k.nb); NC = std::max(NC, k.nc);
+    NB = std::max(NB, k.nb); NC = std::max(NC, k.nc); NP = std::max(NP, k.np);
     h->tmpl_rounds.push_back(k.R);
   }
   std::vector<std::vector<int>> members(T.size());
@@ -280,12 +283,29 @@ static int build_persist(kx_handle *h) {
   }
   // creature regions: 16-byte records, regions offset by 16 bytes modulo
   // 128 so neighbouring creatures' same records fall in different banks
-  int stride = (kx_persist_cr_words(NB, NC) + 31) / 32 * 32 + 4;
+  int stride = (kx_persist_cr_words(NB, NC, NP) + 31) / 32 * 32 + 4;
   int bmw = (h->d.cell_words + 3) / 4 * 4;
   if ((size_t)bmw * 4 > 32 * 1024) bmw = 0;      // huge grids: no LDS bitmap
-  size_t lds = ((size_t)words + bmw + (size_t)WPB * CPW * stride) * 4;
-  if (lds > 160 * 1024)
-    return fail(KX_ENOTSUP, "persistent shape G=%d x %d waves needs %zu B of LDS (> 160 KiB)", G, WPB, lds);
+  // Stage the whole map (cell offsets, ids, rects) in LDS when that costs no
+  // occupancy the launch can use: with nwg workgroups on nCU CUs, at most
+  // ceil(nwg / nCU) of them are ever resident on one CU.
+  int mcells = h->d.gnx * h->d.gny, mids = h->n_cell_ids;
+  int mapw = ((mcells + 1 + mids + 3) & ~3) + 4 * h->nR;
+  const size_t base = ((size_t)words + bmw + (size_t)WPB * CPW * stride) * 4;
+  if (base > 160 * 1024)
+    return fail(KX_ENOTSUP, "persistent shape G=%d x %d waves needs %zu B of LDS (> 160 KiB)", G, WPB, base);
+  const int nwg = ((int)wave_tmpl.size() + WPB - 1) / WPB;
+  int dev = 0, ncu = 1;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const int need = (nwg + std::max(ncu, 1) - 1) / std::max(ncu, 1);
+  const int occ0 = kx_persist_occupancy(G, WPB, base);
+  const size_t with_map = base + (size_t)mapw * 4;
+  const int occ1 = with_map <= 160 * 1024 ? kx_persist_occupancy(G, WPB, with_map) : 0;
+  if (std::min(occ1, need) < std::min(occ0, need)) mapw = 0;
+  const size_t lds = base + (size_t)mapw * 4;
+  h->pinfo[0] = (int)lds; h->pinfo[1] = nwg; h->pinfo[2] = mapw ? occ1 : occ0; h->pinfo[3] = need;
+  h->pinfo[4] = mapw; h->pinfo[5] = bmw; h->pinfo[6] = words; h->pinfo[7] = stride;
   for (void *&p : h->pc_bufs) { if (p) (void)hipFree(p); p = nullptr; }
   HIPCHK(hipMalloc(&h->pc_bufs[0], std::max<size_t>(pool.size(), 1) * 4));
   HIPCHK(hipMalloc(&h->pc_bufs[1], std::max<size_t>(T.size(), 1) * sizeof(KxTmpl)));
@@ -302,9 +322,12 @@ static int build_persist(kx_handle *h) {
   P.wave_tmpl = (const int *)h->pc_bufs[3];
   P.tmpl_words = words;
   P.bm_words = bmw;
+  P.map_words = mapw;
+  P.map_cells = mcells;
+  P.map_ids = mids;
   P.cr_stride = stride;
   P.nwaves = (int)wave_tmpl.size();
-  P.NB = NB; P.NC = NC;
+  P.NB = NB; P.NC = NC; P.NP = NP;
   return 0;
 }
 
@@ -395,7 +418,8 @@ static int create_impl(const kx_world_desc *w, const kx_creature_desc *cd, const
     int np = 0;
     for (int b = cmin[c]; b <= cmax[c]; b++) np += prange[nB + b];
     K.np = np;
-    if (K.nc > 65535 || K.np > 65535) { h->persist_ok = false; h->persist_why = "creature too large"; }
+    // round slots pack a constraint index in 8 bits, point ranges in 16
+    if (K.nc > 255 || K.np > 65535) { h->persist_ok = false; h->persist_why = "creature too large (> 255 constraints)"; }
   }
   h->h2i.resize(nC);
   std::vector<int> fill(ncomp, 0), i2h(nC);
@@ -918,6 +942,13 @@ extern "C" int kx_schedule_info(const kx_handle *h, int32_t *rounds, int32_t max
   int n = (int)h->tmpl_rounds.size();
   for (int i = 0; i < n && i < max; i++) rounds[i] = h->tmpl_rounds[i];
   return n;
+}
+
+extern "C" int kx_persist_info(const kx_handle *h, int32_t *info, int32_t max) {
+  if (!h || !info) return fail(KX_EINVAL, "null argument");
+  if (!h->persist_ok) return fail(KX_ENOTSUP, "persistent path unavailable: %s", h->persist_why.c_str());
+  for (int i = 0; i < 8 && i < max; i++) info[i] = h->pinfo[i];
+  return 8;
 }
 
 // plain-path SDF mode switch (0 linear WGSL scan, 1 cull grid) — exported for

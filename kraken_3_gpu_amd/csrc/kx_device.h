@@ -67,10 +67,17 @@ __device__ __forceinline__ float kx_rcp(float x) {
 
 // ── SDF over the rect list (map.js:91-118), fold order and the best²
 //    exterior test preserved. (x, y) are relative to the origin (ox, oy).
-struct KxHit { float d, nx, ny; };
+//    The fold keeps the winning offset (ux, uy) and whether it is exterior;
+//    the unit normal (offset / distance for an exterior hit, the face normal
+//    inside) is formed only where a correction uses it.
+struct KxHit { float d, ux, uy; bool ext; };
 
-__device__ __forceinline__ void kx_fold_rect(float x, float y, float4 r, float ox, float oy,
-                                             float &best, float &bnx, float &bny) {
+__device__ __forceinline__ void kx_hit_normal(const KxHit &h, float &nx, float &ny) {
+  if (h.ext) { nx = h.ux / h.d; ny = h.uy / h.d; }
+  else { nx = h.ux; ny = h.uy; }
+}
+
+__device__ __forceinline__ void kx_fold_rect(float x, float y, float4 r, float ox, float oy, KxHit &b) {
   float rx = r.x - ox, ry = r.y - oy;          // exact for an integer origin
   float rxw = rx + r.z, ryh = ry + r.w;
   float cx = fminf(fmaxf(x, rx), rxw);
@@ -78,27 +85,31 @@ __device__ __forceinline__ void kx_fold_rect(float x, float y, float4 r, float o
   float dx = x - cx, dy = y - cy;
   float d2 = dx * dx + dy * dy;
   if (d2 > 1e-12f) {
-    if (d2 < best * best) {
-      float d = sqrtf(d2);
-      best = d; bnx = dx / d; bny = dy / d;
-    }
+    if (d2 < b.d * b.d) { b.d = sqrtf(d2); b.ux = dx; b.uy = dy; b.ext = true; }
   } else {
     float dl = x - rx, dr = rxw - x, dt = y - ry, db = ryh - y;
     float m = dl, nx = -1.0f, ny = 0.0f;
     if (dr < m) { m = dr; nx = 1.0f; ny = 0.0f; }
     if (dt < m) { m = dt; nx = 0.0f; ny = -1.0f; }
     if (db < m) { m = db; nx = 0.0f; ny = 1.0f; }
-    if (-m < best) { best = -m; bnx = nx; bny = ny; }
+    if (-m < b.d) { b.d = -m; b.ux = nx; b.uy = ny; b.ext = false; }
   }
+}
+
+__device__ __forceinline__ void kx_hit_init(KxHit &h) { h.d = INFINITY; h.ux = 0.0f; h.uy = 0.0f; h.ext = false; }
+
+// World-frame offset of a body-local point: (x, y) + R(angle) (lx, ly).
+__device__ __forceinline__ void kx_world_point(float x, float y, float ca, float sa, float lx, float ly, float &wx,
+                                               float &wy) {
+  wx = x + (lx * ca - ly * sa);
+  wy = y + (lx * sa + ly * ca);
 }
 
 // Linear scan (the WGSL shape, solver.wgsl:114-147).
 __device__ __forceinline__ bool kx_sdf_linear(const KxDev &d, float x, float y, float ox, float oy, KxHit &h) {
-  float best = INFINITY, bnx = 0.0f, bny = 0.0f;
-  for (int i = 0; i < d.nr; i++) kx_fold_rect(x, y, d.rects[i], ox, oy, best, bnx, bny);
-  if (!isfinite(best)) return false;
-  h.d = best; h.nx = bnx; h.ny = bny;
-  return true;
+  kx_hit_init(h);
+  for (int i = 0; i < d.nr; i++) kx_fold_rect(x, y, d.rects[i], ox, oy, h);
+  return isfinite(h.d);
 }
 
 // Grid-culled fold: only rects registered in the query's cell, ascending.
@@ -116,11 +127,9 @@ __device__ __forceinline__ bool kx_sdf_grid(const KxDev &d, float x, float y, fl
   int cell = iy * d.gnx + ix;
   int b = d.cell_off[cell], e = d.cell_off[cell + 1];
   if (b == e) return false;
-  float best = INFINITY, bnx = 0.0f, bny = 0.0f;
-  for (int k = b; k < e; k++) kx_fold_rect(x, y, d.rects[d.cell_ids[k]], ox, oy, best, bnx, bny);
-  if (!isfinite(best)) return false;
-  h.d = best; h.nx = bnx; h.ny = bny;
-  return true;
+  kx_hit_init(h);
+  for (int k = b; k < e; k++) kx_fold_rect(x, y, d.rects[d.cell_ids[k]], ox, oy, h);
+  return isfinite(h.d);
 }
 
 // Same fold, after a test of the cell's bit in an occupancy bitmap held in
@@ -135,11 +144,24 @@ __device__ __forceinline__ bool kx_sdf_grid_bm(const KxDev &d, const uint32_t *b
   int cell = iy * d.gnx + ix;
   if (!((bits[cell >> 5] >> (cell & 31)) & 1u)) return false;
   int b = d.cell_off[cell], e = d.cell_off[cell + 1];
-  float best = INFINITY, bnx = 0.0f, bny = 0.0f;
-  for (int k = b; k < e; k++) kx_fold_rect(x, y, d.rects[d.cell_ids[k]], ox, oy, best, bnx, bny);
-  if (!isfinite(best)) return false;
-  h.d = best; h.nx = bnx; h.ny = bny;
-  return true;
+  kx_hit_init(h);
+  for (int k = b; k < e; k++) kx_fold_rect(x, y, d.rects[d.cell_ids[k]], ox, oy, h);
+  return isfinite(h.d);
+}
+
+// Same fold with the map staged in LDS: cell offsets, rect ids and rects.
+__device__ __forceinline__ bool kx_sdf_grid_lds(const KxDev &d, const int *coff, const int *cids, const float4 *rects,
+                                                float x, float y, float ox, float oy, KxHit &h) {
+  float gx = (x + ox - d.gx0) * d.inv_cs;
+  float gy = (y + oy - d.gy0) * d.inv_cs;
+  if (!(gx >= 0.0f && gy >= 0.0f && gx < (float)d.gnx && gy < (float)d.gny)) return false;
+  int ix = min((int)gx, d.gnx - 1), iy = min((int)gy, d.gny - 1);
+  int cell = iy * d.gnx + ix;
+  int b = coff[cell], e = coff[cell + 1];
+  if (b == e) return false;
+  kx_hit_init(h);
+  for (int k = b; k < e; k++) kx_fold_rect(x, y, rects[cids[k]], ox, oy, h);
+  return isfinite(h.d);
 }
 
 // ── constraint projections (xpbd.js:331-439) ─────────────────────────

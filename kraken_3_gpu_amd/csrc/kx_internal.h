@@ -93,11 +93,14 @@ struct KxPersistCfg {
   const uint32_t *pool;
   int tmpl_words;           // LDS words reserved for the template
   int bm_words;             // LDS words of the cell occupancy bitmap (after the template)
-  int cr_stride;            // LDS words per creature (odd, avoids bank aliasing)
+  int map_words;            // LDS words of the staged map (after the bitmap), 0 = map stays in HBM
+  int map_cells, map_ids;   // staged map: cell offsets [cells+1], then ids [ids], then rects
+  int cr_stride;            // LDS words per creature (16 bytes off a 128-byte multiple)
   int nwaves;
-  int NB, NC;               // per-creature array sizes in LDS
+  int NB, NC, NP;           // per-creature array sizes in LDS (bodies, constraints, points)
 };
 int kx_persist_launch(int G, int wpb, const KxDev &d, const KxLaw *law, int step0, const int *step_base,
                       int nsteps, int ev_row0, int with_forces, const KxPersistCfg &P, hipStream_t st);
-int kx_persist_cr_words(int NB, int NC);
+int kx_persist_cr_words(int NB, int NC, int NP);
+int kx_persist_occupancy(int G, int wpb, size_t lds);
 int kx_bump_counter(int *counter, int delta, hipStream_t st);

@@ -11,10 +11,11 @@
 // compiler fence between rounds is all the ordering they need. State is
 // read from and written to HBM (SoA) only at launch boundaries.
 //
-// Positions live in creature-local coordinates relative to an integer
-// origin (re-centred on the creature's first body between steps); since a
-// global binary32 coordinate is a multiple of its own ulp, the shift to and
-// from the origin is exact and the solver works with finer position ulps.
+// Within a step, positions are relative to an integer origin (rint of the
+// creature's first body, taken after the control law); the subtraction is
+// exact for coordinates of magnitude >= 1, so the solver works with finer
+// position ulps. Between steps the state is global binary32, exactly as the
+// plain path (kx_plain.hip) holds it, and the two paths agree bit for bit.
 #include "../../include/kxpbd.h"
 #include "kx_device.h"
 #include "kx_sched.h"
@@ -25,21 +26,54 @@
     __builtin_amdgcn_wave_barrier();                       \
   } while (0)
 
+// Forces every component of a just-loaded 16-byte record into registers at
+// this point, so the record is read by one full-width LDS load issued with
+// its neighbours rather than narrowed and re-issued inside later branches.
+#define KX_HOLD4(v) asm volatile("" : "+v"((v).x), "+v"((v).y), "+v"((v).z), "+v"((v).w))
+
 // per-creature LDS layout (word offsets, 16-byte records):
 //   pos[i] = {ppx, ppy, angP, -}   prv[i] = {px, py, ang, -}   vel[i] = {vx, vy, av, -}
 //   dyn[k] = {alpha, target, lbx, lby}   lam[k] = {lambda_x|lambda, lambda_y, breakForce, broken}
+//   flg    = one bit per collision point: "corrects at the pose after the rounds"
 struct KxCrLds {
-  int pos, prv, vel, dyn, lam, ev, words;
-  __host__ __device__ KxCrLds(int NB, int NC) {
+  int pos, prv, vel, dyn, lam, ev, flg, words;
+  __host__ __device__ KxCrLds(int NB, int NC, int NP) {
     int o = 0;
     pos = o; o += 4 * NB; prv = o; o += 4 * NB; vel = o; o += 4 * NB;
     dyn = o; o += 4 * NC; lam = o; o += 4 * NC;
     ev = o; o += 4;
+    flg = o; o += ((NP + 31) / 32 + 3) / 4 * 4;
     words = o;
   }
 };
 
-int kx_persist_cr_words(int NB, int NC) { return KxCrLds(NB, NC).words; }
+int kx_persist_cr_words(int NB, int NC, int NP) { return KxCrLds(NB, NC, NP).words; }
+
+// Adds (sx, sy) to a creature's body positions (start and predicted) and to
+// its world anchors; the same binary32 additions as the plain path's shift.
+__device__ __forceinline__ void kx_shift_creature(float4 *prv, float4 *pos, float4 *dyn, const float4 *t_crec, int nb,
+                                                  int nc, int sub, int G, float sx, float sy) {
+  for (int i = sub; i < nb; i += G) {
+    float4 p = prv[i], pp = pos[i];
+    p.x += sx; p.y += sy; pp.x += sx; pp.y += sy;
+    prv[i] = p; pos[i] = pp;
+  }
+  for (int k = sub; k < nc; k += G)
+    if (((__float_as_int(t_crec[2 * k].x) >> 8) & 255) == 255) {
+      float4 dk = dyn[k];
+      dk.z += sx; dk.w += sy;
+      dyn[k] = dk;
+    }
+}
+
+// Grid-culled SDF query against whichever copy of the map the launch uses.
+__device__ __forceinline__ bool kx_query(const KxDev &d, const KxPersistCfg &P, const uint32_t *bits,
+                                         const int *m_off, const int *m_ids, const float4 *m_rects, float wx,
+                                         float wy, float ox, float oy, KxHit &hh) {
+  return P.map_words ? kx_sdf_grid_lds(d, m_off, m_ids, m_rects, wx, wy, ox, oy, hh)
+         : P.bm_words ? kx_sdf_grid_bm(d, bits, wx, wy, ox, oy, hh)
+                      : kx_sdf_grid(d, wx, wy, ox, oy, hh);
+}
 
 template <int G, int WPB>
 __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, int has_law, int step0,
@@ -58,13 +92,22 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
   for (int i = threadIdx.x; i < T.words; i += blockDim.x) smem[i] = P.pool[T.base + i];
   uint32_t *bits = smem + P.tmpl_words;
   for (int i = threadIdx.x; i < P.bm_words; i += blockDim.x) bits[i] = d.cell_bits[i];
+  // the map (cell offsets, rect ids, rects) when it fits
+  int *m_off = (int *)(smem + P.tmpl_words + P.bm_words);
+  int *m_ids = m_off + P.map_cells + 1;
+  float4 *m_rects = (float4 *)(smem + P.tmpl_words + P.bm_words + ((P.map_cells + 1 + P.map_ids + 3) & ~3));
+  if (P.map_words) {
+    for (int i = threadIdx.x; i <= P.map_cells; i += blockDim.x) m_off[i] = d.cell_off[i];
+    for (int i = threadIdx.x; i < P.map_ids; i += blockDim.x) m_ids[i] = d.cell_ids[i];
+    for (int i = threadIdx.x; i < d.nr; i += blockDim.x) m_rects[i] = d.rects[i];
+  }
   __syncthreads();
   if (gw >= P.nwaves) return;
 
   const float4 *t_crec = (const float4 *)(smem + T.o_crec);   // 2 per constraint
   const float4 *t_brec = (const float4 *)(smem + T.o_brec);
   const float4 *t_prec = (const float4 *)(smem + T.o_prec);
-  const int *t_sched = (const int *)(smem + T.o_sched), *t_rtype = (const int *)(smem + T.o_rtype);
+  const int *t_sched = (const int *)(smem + T.o_sched), *t_rbits = (const int *)(smem + T.o_rtype);
   const int *t_coll = (const int *)(smem + T.o_coll), *t_ccnt = (const int *)(smem + T.o_ccnt);
   const int *t_arm_tip = (const int *)(smem + T.o_arm_tip), *t_arm_grip = (const int *)(smem + T.o_arm_grip);
   const int *t_mot_k = (const int *)(smem + T.o_mot_k), *t_mot_slot = (const int *)(smem + T.o_mot_slot);
@@ -72,28 +115,23 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
 
   const int cid = P.wave_cr[gw * CPW + q];
   const bool act = cid >= 0;
-  const KxCrLds Lo(P.NB, P.NC);
-  uint32_t *S = smem + P.tmpl_words + P.bm_words + (size_t)(w * CPW + q) * P.cr_stride;
+  const KxCrLds Lo(P.NB, P.NC, P.NP);
+  uint32_t *S = smem + P.tmpl_words + P.bm_words + P.map_words + (size_t)(w * CPW + q) * P.cr_stride;
   float4 *pos = (float4 *)(S + Lo.pos), *prv = (float4 *)(S + Lo.prv), *vel = (float4 *)(S + Lo.vel);
   float4 *dyn = (float4 *)(S + Lo.dyn), *lamr = (float4 *)(S + Lo.lam);
-  uint32_t *evw = S + Lo.ev;
+  uint32_t *evw = S + Lo.ev, *flg = S + Lo.flg;
   const int nb = act ? T.nb : 0, nc = act ? T.nc : 0;
   const int b0 = act ? d.cr[cid].body_off : 0, c0 = act ? d.cr[cid].cons_off : 0;
 
-  float ox = 0.0f, oy = 0.0f;
-  if (act) { ox = rintf(d.px[b0]); oy = rintf(d.py[b0]); }
-
-  // ── load the creature's dynamic state ──
+  // ── load the creature's dynamic state (global coordinates) ──
   for (int i = sub; i < nb; i += G) {
     int g = b0 + i;
-    prv[i] = make_float4(d.px[g] - ox, d.py[g] - oy, d.ang[g], 0.0f);
+    prv[i] = make_float4(d.px[g], d.py[g], d.ang[g], 0.0f);
     vel[i] = make_float4(d.vx[g], d.vy[g], d.av[g], 0.0f);
   }
   for (int k = sub; k < nc; k += G) {
     int g = c0 + k;
-    bool world = ((__float_as_int(t_crec[2 * k].x) >> 8) & 255) == 255;
-    dyn[k] = make_float4(d.comp[g] * d.invH2, d.target[g], world ? d.lbx[g] - ox : d.lbx[g],
-                         world ? d.lby[g] - oy : d.lby[g]);
+    dyn[k] = make_float4(d.comp[g] * d.invH2, d.target[g], d.lbx[g], d.lby[g]);
     lamr[k] = make_float4(0.0f, 0.0f, d.bf[g], __int_as_float(d.broken[g]));
   }
   double my_cd = 0.0;
@@ -121,16 +159,23 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
         int g = t_arm_grip[sub], t = t_arm_tip[sub];
         float4 dk = dyn[g], lk = lamr[g], tp = prv[t], tv = vel[t], cr0 = t_crec[2 * g];
         int bk = __float_as_int(lk.w);
-        float f_bf = lk.z, gx = dk.z + ox, gy = dk.w + oy, lx = lk.x, ly = lk.y;
-        uint32_t e = kx_reflex(L, hS, adh, my_cd, bk, f_bf, gx, gy, lx, ly, tv.x, tv.y, (double)ox + (double)tp.x,
-                               (double)oy + (double)tp.y, tp.z, cr0.y, cr0.z);
-        dk.z = gx - ox; dk.w = gy - oy;
+        float f_bf = lk.z, gx = dk.z, gy = dk.w, lx = lk.x, ly = lk.y;
+        uint32_t e = kx_reflex(L, hS, adh, my_cd, bk, f_bf, gx, gy, lx, ly, tv.x, tv.y, (double)tp.x, (double)tp.y,
+                               tp.z, cr0.y, cr0.z);
+        dk.z = gx; dk.w = gy;
         dyn[g] = dk;
         lamr[g] = make_float4(lx, ly, f_bf, __int_as_float(bk));
         my_ev |= e << (3 * sub);
       }
       KX_WAVE_SYNC();
     }
+    // Between steps the state is held in global binary32 coordinates, as the
+    // reference's buffers hold it; a step works relative to the integer
+    // origin rint(first body), subtracted before and added back after.
+    float ox = 0.0f, oy = 0.0f;
+    if (act) { float4 p0 = prv[0]; ox = rintf(p0.x); oy = rintf(p0.y); }
+    kx_shift_creature(prv, pos, dyn, t_crec, nb, nc, sub, G, -ox, -oy);
+    KX_WAVE_SYNC();
     for (int s = 0; s < d.S; s++) {
       // resetLambda + integrate (xpbd.js:262-287)
       for (int k = sub; k < nc; k += G) { float4 lk = lamr[k]; lk.x = 0.0f; lk.y = 0.0f; lamr[k] = lk; }
@@ -150,63 +195,113 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
       }
       KX_WAVE_SYNC();
       for (int it = 0; it < d.I; it++) {
-        // constraint rounds (type-homogeneous, wave-uniform branch)
+        // constraint rounds (type-homogeneous, wave-uniform branch); the slot
+        // descriptor of the next round is fetched while this one computes.
+        // Every LDS read of a slot is issued before any result is needed; a
+        // broken constraint is solved too and its result dropped.
+        int dsc = (act && T.R > 0) ? t_sched[sub] : -1;
+        uint32_t rbits = 0;
         for (int r = 0; r < T.R; r++) {
-          const int k = act ? t_sched[r * G + sub] : -1;
-          if (k >= 0) {
-            float4 lk = lamr[k];
-            if (__float_as_int(lk.w) == 0) {
-              float4 c0r = t_crec[2 * k];
-              int v = __float_as_int(c0r.x);
-              int A = v & 255, B = (v >> 8) & 255;
-              bool hasB = B != 255;
-              if (!hasB) B = A;
-              float4 pA = pos[A], pB = pos[B];
-              float4 bA = t_brec[A], bB = t_brec[B];
-              float4 dk = dyn[k];
-              if (t_rtype[r]) {
-                float4 c1r = t_crec[2 * k + 1];
-                if (kx_angle_core(pA.z, pB.z, bA.y, bB.y, hasB, c0r.w, dk.y, c1r.x, c1r.y, dk.x, lk.x)) {
-                  pos[A] = pA;
-                  if (hasB) pos[B] = pB;
-                  lamr[k] = lk;
-                }
-              } else {
-                if (kx_pin_core(pA.x, pA.y, pA.z, bA.x, bA.y, pB.x, pB.y, pB.z, bB.x, bB.y, hasB, c0r.y, c0r.z,
-                                dk.z, dk.w, dk.x, lk.x, lk.y)) {
-                  pos[A] = pA;
-                  if (hasB) pos[B] = pB;
-                  lamr[k] = lk;
-                }
+          if ((r & 31) == 0) rbits = (uint32_t)__builtin_amdgcn_readfirstlane(t_rbits[r >> 5]);
+          const bool is_ang = (rbits >> (r & 31)) & 1u;
+          const int dnext = (act && r + 1 < T.R) ? t_sched[(r + 1) * G + sub] : -1;
+          if (dsc >= 0) {
+            const int k = dsc & 255, A = (dsc >> 8) & 255, B0 = (dsc >> 16) & 255;
+            const bool hasB = B0 != 255;
+            const int B = hasB ? B0 : A;
+            float4 lk = lamr[k], c0r = t_crec[2 * k], c1r = t_crec[2 * k + 1], dk = dyn[k];
+            float4 pA = pos[A], pB = pos[B], bA = t_brec[A], bB = t_brec[B];
+            KX_HOLD4(lk); KX_HOLD4(c0r); KX_HOLD4(c1r); KX_HOLD4(dk);
+            KX_HOLD4(pA); KX_HOLD4(pB); KX_HOLD4(bA); KX_HOLD4(bB);
+            bool moved;
+            if (is_ang) {
+              moved = kx_angle_core(pA.z, pB.z, bA.y, bB.y, hasB, c0r.w, dk.y, c1r.x, c1r.y, dk.x, lk.x);
+            } else {
+              moved = kx_pin_core(pA.x, pA.y, pA.z, bA.x, bA.y, pB.x, pB.y, pB.z, bB.x, bB.y, hasB, c0r.y, c0r.z,
+                                  dk.z, dk.w, dk.x, lk.x, lk.y);
+            }
+            if (moved && __float_as_int(lk.w) == 0) {
+              pos[A] = pA;
+              if (hasB) pos[B] = pB;
+              lamr[k] = lk;
+            }
+          }
+          KX_WAVE_SYNC();
+          dsc = dnext;
+        }
+        // collisions (xpbd.js:441-470: per body, its points in order, each
+        // correction moving the body before the next point is tested).
+        // Phase A tests every point in parallel at its body's pose after the
+        // rounds and records which ones correct. Up to a body's first such
+        // point the sequential pass sees exactly that pose, so phase B (one
+        // lane per body) starts at that point and re-tests from there on in
+        // order; bodies with no flagged point are left untouched.
+        {
+          const int nchunk = (T.np + G - 1) / G;     // uniform: one template per workgroup
+          uint32_t facc = 0u;
+          for (int j = 0; j < nchunk; j++) {
+            const int p = j * G + sub;
+            bool f = false;
+            if (act && p < T.np) {
+              float4 pt = t_prec[p];
+              const int b = __float_as_int(pt.w);
+              float4 br = t_brec[b], pb = pos[b];
+              if (br.x != 0.0f) {
+                float sa, ca, wx, wy;
+                kx_sincos(pb.z, &sa, &ca);
+                kx_world_point(pb.x, pb.y, ca, sa, pt.x, pt.y, wx, wy);
+                KxHit hh;
+                f = kx_query(d, P, bits, m_off, m_ids, m_rects, wx, wy, ox, oy, hh) && pt.z - hh.d > KX_SLACK_POS;
+              }
+            }
+            const unsigned long long bal = __ballot(f);
+            if constexpr (G == 64) {
+              if (act && sub == 0) { flg[2 * j] = (uint32_t)bal; flg[2 * j + 1] = (uint32_t)(bal >> 32); }
+            } else {
+              const uint32_t mine = (uint32_t)(bal >> (q * G)) & (G == 32 ? 0xffffffffu : ((1u << G) - 1u));
+              facc |= mine << ((j * G) & 31);
+              if ((((j + 1) * G) & 31) == 0 || j == nchunk - 1) {
+                if (act && sub == 0) flg[(j * G) >> 5] = facc;
+                facc = 0u;
               }
             }
           }
           KX_WAVE_SYNC();
-        }
-        // collisions: this lane's bodies, each body's points in order
-        const int nbl = act ? t_ccnt[sub] : 0;
-        for (int j = 0; j < nbl; j++) {
-          int i = t_coll[sub * T.maxbl + j];
-          float4 br = t_brec[i];
-          if (br.x == 0.0f) continue;
-          int bp = __float_as_int(br.z);
-          int o = bp & 0xffff, cnt = bp >> 16;
-          float4 p = pos[i];
-          float x = p.x, y = p.y;
-          float sa, ca;
-          kx_sincos(p.z, &sa, &ca);
-          for (int pq = 0; pq < cnt; pq++) {
-            float4 pt = t_prec[o + pq];
-            float wx = x + (pt.x * ca - pt.y * sa);
-            float wy = y + (pt.x * sa + pt.y * ca);
-            KxHit hh;
-            bool hit = P.bm_words ? kx_sdf_grid_bm(d, bits, wx, wy, ox, oy, hh) : kx_sdf_grid(d, wx, wy, ox, oy, hh);
-            if (!hit) continue;
-            float pen = pt.z - hh.d;
-            if (pen > KX_SLACK_POS) { x += hh.nx * pen; y += hh.ny * pen; }
+          const int nbl = act ? t_ccnt[sub] : 0;
+          for (int j = 0; j < nbl; j++) {
+            const int i = t_coll[sub * T.maxbl + j];
+            float4 br = t_brec[i];
+            if (br.x == 0.0f) continue;
+            const int bp = __float_as_int(br.z);
+            const int o = bp & 0xffff, cnt = bp >> 16;
+            int pq = cnt;                              // first flagged point of the body
+            for (int wi = o >> 5; (wi << 5) < o + cnt; wi++) {
+              uint32_t wd = flg[wi];
+              const int lo = o - (wi << 5), hi = o + cnt - (wi << 5);
+              if (lo > 0) wd &= ~0u << lo;
+              if (hi < 32) wd &= (1u << hi) - 1u;
+              if (wd) { pq = (wi << 5) + __builtin_ctz(wd) - o; break; }
+            }
+            if (pq >= cnt) continue;
+            float4 pb = pos[i];
+            float x = pb.x, y = pb.y, sa, ca;
+            kx_sincos(pb.z, &sa, &ca);
+            for (; pq < cnt; pq++) {
+              float4 pt = t_prec[o + pq];
+              float wx, wy;
+              kx_world_point(x, y, ca, sa, pt.x, pt.y, wx, wy);
+              KxHit hh;
+              if (!kx_query(d, P, bits, m_off, m_ids, m_rects, wx, wy, ox, oy, hh)) continue;
+              const float pen = pt.z - hh.d;
+              if (pen > KX_SLACK_POS) {
+                float nx, ny;
+                kx_hit_normal(hh, nx, ny);
+                x += nx * pen; y += ny * pen;
+              }
+            }
+            pb.x = x; pb.y = y;
+            pos[i] = pb;
           }
-          p.x = x; p.y = y;
-          pos[i] = p;
         }
         KX_WAVE_SYNC();
       }
@@ -237,24 +332,7 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
       KX_WAVE_SYNC();
       if (act && sub == 0) d.events[(size_t)(ev_row0 + step) * d.nCr + cid] = evw[0];
     }
-    // re-centre the origin on the first body when it drifted
-    float4 p0 = act ? prv[0] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    float sx = rintf(p0.x), sy = rintf(p0.y);
-    KX_WAVE_SYNC();
-    if (fabsf(sx) > 32.0f || fabsf(sy) > 32.0f) {
-      for (int i = sub; i < nb; i += G) {
-        float4 p = prv[i], pp = pos[i];
-        p.x -= sx; p.y -= sy; pp.x -= sx; pp.y -= sy;
-        prv[i] = p; pos[i] = pp;
-      }
-      for (int k = sub; k < nc; k += G)
-        if (((__float_as_int(t_crec[2 * k].x) >> 8) & 255) == 255) {
-          float4 dk = dyn[k];
-          dk.z -= sx; dk.w -= sy;
-          dyn[k] = dk;
-        }
-      ox += sx; oy += sy;
-    }
+    kx_shift_creature(prv, pos, dyn, t_crec, nb, nc, sub, G, ox, oy);
     KX_WAVE_SYNC();
   }
 
@@ -262,16 +340,16 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
   for (int i = sub; i < nb; i += G) {
     int g = b0 + i;
     float4 p = prv[i], pp = pos[i], v = vel[i];
-    d.px[g] = p.x + ox; d.py[g] = p.y + oy; d.ang[g] = p.z;
+    d.px[g] = p.x; d.py[g] = p.y; d.ang[g] = p.z;
     d.vx[g] = v.x; d.vy[g] = v.y; d.av[g] = v.z;
-    d.ppx[g] = pp.x + ox; d.ppy[g] = pp.y + oy; d.angP[g] = pp.z;
+    d.ppx[g] = pp.x; d.ppy[g] = pp.y; d.angP[g] = pp.z;
   }
   for (int k = sub; k < nc; k += G) {
     int g = c0 + k;
     int v = __float_as_int(t_crec[2 * k].x);
     float4 dk = dyn[k], lk = lamr[k];
     d.broken[g] = __float_as_int(lk.w);
-    if (((v >> 8) & 255) == 255) { d.lbx[g] = dk.z + ox; d.lby[g] = dk.w + oy; }
+    if (((v >> 8) & 255) == 255) { d.lbx[g] = dk.z; d.lby[g] = dk.w; }
     d.bf[g] = lk.z; d.target[g] = dk.y;
     if (has_law && (v >> 16) == KX_ANGLE) d.comp[g] = (float)L.comp;
     if ((v >> 16) == KX_ANGLE) { d.lam[g] = lk.x; d.lamx[g] = 0.0f; d.lamy[g] = 0.0f; }
@@ -293,7 +371,7 @@ template <int G, int WPB>
 static int launch_gw(const KxDev &d, const KxLaw *law, int step0, const int *step_base, int nsteps, int ev_row0,
                      int with_forces, const KxPersistCfg &P, hipStream_t st) {
   constexpr int CPW = KX_WAVE / G;
-  size_t lds = ((size_t)P.tmpl_words + P.bm_words + (size_t)WPB * CPW * P.cr_stride) * 4;
+  size_t lds = ((size_t)P.tmpl_words + P.bm_words + P.map_words + (size_t)WPB * CPW * P.cr_stride) * 4;
   if (lds > 160 * 1024) return -22;
   static bool attr_set = false;
   if (!attr_set) {
@@ -306,6 +384,27 @@ static int launch_gw(const KxDev &d, const KxLaw *law, int step0, const int *ste
   hipLaunchKernelGGL((kx_k_persist<G, WPB>), dim3(blocks), dim3(KX_WAVE * WPB), lds, st, d, L, law ? 1 : 0, step0,
                      step_base, nsteps, ev_row0, with_forces, P);
   return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+template <int G, int WPB>
+static int occ_gw(size_t lds) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void *)kx_k_persist<G, WPB>, KX_WAVE * WPB, lds) !=
+      hipSuccess)
+    return 0;
+  return n;
+}
+
+// Resident workgroups per CU for shape (G, wpb) with `lds` bytes of dynamic
+// LDS each (0 when the shape is not instantiated or the query fails).
+int kx_persist_occupancy(int G, int wpb, size_t lds) {
+#define KX_OCC(g)                                   \
+  case g:                                           \
+    return wpb == 1 ? occ_gw<g, 1>(lds)             \
+         : wpb == 2 ? occ_gw<g, 2>(lds)             \
+         : wpb == 4 ? occ_gw<g, 4>(lds) : 0;
+  switch (G) { KX_OCC(8) KX_OCC(16) KX_OCC(32) KX_OCC(64) default: return 0; }
+#undef KX_OCC
 }
 
 template <int G>

@@ -122,14 +122,17 @@ __global__ __launch_bounds__(PLAIN_BLOCK) void kx_k_collide(KxDev d) {
   float sa, ca;
   kx_sincos(d.angP[i], &sa, &ca);
   for (int q = 0; q < cnt; q++) {
-    float lx = d.plx[o + q], ly = d.ply[o + q];
-    float wx = x + (lx * ca - ly * sa);
-    float wy = y + (lx * sa + ly * ca);
+    float wx, wy;
+    kx_world_point(x, y, ca, sa, d.plx[o + q], d.ply[o + q], wx, wy);
     KxHit h;
     bool ok = d.use_grid ? kx_sdf_grid(d, wx, wy, ox, oy, h) : kx_sdf_linear(d, wx, wy, ox, oy, h);
     if (!ok) continue;
     float pen = d.pr[o + q] - h.d;
-    if (pen > KX_SLACK_POS) { x += h.nx * pen; y += h.ny * pen; }
+    if (pen > KX_SLACK_POS) {
+      float nx, ny;
+      kx_hit_normal(h, nx, ny);
+      x += nx * pen; y += ny * pen;
+    }
   }
   d.ppx[i] = x; d.ppy[i] = y;
 }

@@ -108,7 +108,7 @@ static uint32_t fbits(float x) {
 // are 16-byte vectors so the kernel reads each with one LDS instruction:
 //   crec[k] = {cab, lax, lay, rest} {min, max, -, -}   (2 x 16 B)
 //   brec[b] = {im, ii, point offset | count << 16, -}
-//   prec[q] = {lx, ly, r, -}
+//   prec[q] = {lx, ly, r, body}
 KxTmpl kx_pack_template(const TmplHost &t, int G, std::vector<uint32_t> &pool) {
   KxTmpl T{};
   T.nb = t.nb; T.nc = (int)t.type.size(); T.np = (int)t.plx.size();
@@ -132,10 +132,24 @@ KxTmpl kx_pack_template(const TmplHost &t, int G, std::vector<uint32_t> &pool) {
   for (int b = 0; b < t.nb; b++)
     pool.insert(pool.end(), {fbits(t.im[b]), fbits(t.ii[b]), (uint32_t)(t.poff[b] | (t.pcnt[b] << 16)), 0u});
   T.o_prec = at();
+  std::vector<int> pbody(t.plx.size(), 0);
+  for (int b = 0; b < t.nb; b++)
+    for (int j = 0; j < t.pcnt[b]; j++) pbody[t.poff[b] + j] = b;
   for (size_t q = 0; q < t.plx.size(); q++)
-    pool.insert(pool.end(), {fbits(t.plx[q]), fbits(t.ply[q]), fbits(t.pr[q]), 0u});
-  T.o_sched = at(); put(pool, sched);
-  T.o_rtype = at(); put(pool, rtype);
+    pool.insert(pool.end(), {fbits(t.plx[q]), fbits(t.ply[q]), fbits(t.pr[q]), (uint32_t)pbody[q]});
+  // round slots as packed descriptors: k | A << 8 | B << 16 (B = 255: world
+  // anchor), -1 for an idle lane, so one LDS read yields every index
+  std::vector<int> desc(sched.size(), -1);
+  for (size_t i = 0; i < sched.size(); i++) {
+    int k = sched[i];
+    if (k >= 0) desc[i] = k | (t.a[k] << 8) | ((t.b[k] < 0 ? 255 : t.b[k]) << 16);
+  }
+  T.o_sched = at(); put(pool, desc);
+  // round types as a bitmask, 32 rounds per word (bit set = angle round)
+  std::vector<int> rbits((rtype.size() + 31) / 32 + 1, 0);
+  for (size_t r = 0; r < rtype.size(); r++)
+    if (rtype[r]) rbits[r >> 5] |= (int)(1u << (r & 31));
+  T.o_rtype = at(); put(pool, rbits);
   T.o_coll = at(); put(pool, lists);
   T.o_ccnt = at(); put(pool, counts);
   T.o_arm_tip = at(); put(pool, t.arm_tip);

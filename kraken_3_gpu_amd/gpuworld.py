@@ -282,6 +282,17 @@ def schedule_rounds(g):
     return buf[:n].tolist()
 
 
+PERSIST_INFO = ("lds_bytes", "workgroups", "resident_per_cu", "needed_per_cu", "map_words", "bitmap_words",
+                "template_words", "creature_words")
+
+
+def persist_info(g):
+    """Launch shape of the persistent path for the handle's (lanes, waves)."""
+    buf = np.zeros(8, dtype=np.int32)
+    K.check(K.lib().kx_persist_info(g.p, K.ptr(buf, C.c_int32), 8))
+    return dict(zip(PERSIST_INFO, buf.tolist()))
+
+
 def set_plain_sdf(g, use_grid):
     K.check(K.lib().kx_set_plain_sdf(g.p, 1 if use_grid else 0))
 

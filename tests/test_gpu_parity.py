@@ -231,6 +231,44 @@ def test_plain_and_persistent_agree_uneven():
         assert err <= TOL_PX and len(flips) <= 1
 
 
+@pytest.mark.parametrize("case", ["uniform4096", "uneven512"])
+def test_plain_and_persistent_bit_identical(case):
+    """Both paths hold global binary32 state between steps, run each step
+    relative to the same integer origin and solve the same constraints in
+    orders that differ only between constraints sharing no body; the
+    persistent path's two-phase collision pass reproduces the sequential one.
+    So a multi-step rollout under the control law must agree bit for bit:
+    positions, angles, velocities, constraint state and grip events."""
+    if case == "uniform4096":
+        make, steps, shapes = (lambda: W.world_uniform(4096, seed=1)), 24, ((32, 2),)
+    else:
+        make, steps, shapes = (lambda: W.world_uneven(512, seed=3)), 24, ((32, 2), (16, 1))
+    law = W.default_law(77)
+    wp = make()
+    gp = G.create(wp, path=K.KX_PATH_PLAIN)
+    G.upload(gp)
+    say(f"[case] bit-identity {case} plain {steps} steps")
+    evp = G.rollout(gp, law, 0, steps, events=True)
+    G.download(gp)
+    G.destroy(gp)
+    for lanes, wpb in shapes:
+        wq = make()
+        gq = G.create(wq, path=K.KX_PATH_PERSISTENT, lanes_per_creature=lanes, waves_per_block=wpb)
+        G.upload(gq)
+        say(f"[case] bit-identity {case} persistent lanes={lanes} wpb={wpb}")
+        evq = G.rollout(gq, law, 0, steps, events=True)
+        G.download(gq)
+        G.destroy(gq)
+        for key in ("px", "py", "ang", "vx", "vy", "av", "broken", "lambda_"):
+            a, b = getattr(wp, key), getattr(wq, key)
+            nd = int(np.count_nonzero(a != b))
+            if nd:
+                say(f"    {key}: {nd} differ, max {np.max(np.abs(a.astype(np.float64) - b)):.3e}")
+            assert nd == 0, key
+        assert np.array_equal(evp, evq)
+        say(f"    identical, {len(decode_events(evq))} events")
+
+
 def test_full_size_uniform_properties_and_subset_parity():
     """4096 creatures (the uniform batch): limits, NaN, and one-step parity of
     a 64-creature subset against the oracle."""
