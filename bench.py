@@ -71,6 +71,19 @@ def cpu_baseline(seconds=12.0):
             "sample": f"{threads} threads x 64 creatures x {steps} steps (uniform batch recipe, f64 oracle)"}
 
 
+def copy_probe(mib):
+    """Three plain device copies of a known size: the reference point that
+    scales the FETCH_SIZE / WRITE_SIZE counters to bytes moved."""
+    import torch
+    n = mib << 20
+    x = torch.ones(n, dtype=torch.uint8, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n, "copy_probe_bytes_written": n, "copies": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,7 +96,12 @@ def main():
     ap.add_argument("--wpb", type=int, default=0, help="wavefronts per workgroup (0 = default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 3 device-to-device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (creatures shard across "
                                    "GPUs with no per-step exchange, see DESIGN.md)"}))

@@ -165,82 +165,101 @@ __device__ __forceinline__ bool kx_sdf_grid_lds(const KxDev &d, const int *coff,
 }
 
 // ── constraint projections (xpbd.js:331-439) ─────────────────────────
-// Pin / grip on values: C = (pA + R(thA) la) - (pB + R(thB) lb), or the
-// world anchor (lbx, lby) when !hasB, exact 2x2 effective mass. Returns true
-// and the projected body states when a correction applies.
-__device__ __forceinline__ bool kx_pin_core(float &xA, float &yA, float &tA, float wA, float wIA, float &xB,
-                                            float &yB, float &tB, float wB, float wIB, bool hasB, float lax,
-                                            float lay, float lbx, float lby, float alpha, float &lamx,
-                                            float &lamy) {
-  float sa, ca;
-  kx_sincos(tA, &sa, &ca);
+// Pin / grip on values (xpbd.js:331-390): 2x2 XPBD solve on the world
+// anchor gap with generalized inverse masses, lambda accumulated per axis.
+// (sa, ca) and (sb, cb) are sin/cos of tA and tB. Branch-free: every lane
+// evaluates the whole solve and the result is committed only when the gap
+// exceeds the dead zone and the solve is regular and finite. Returns true
+// if the bodies moved.
+__device__ __forceinline__ bool kx_pin_core_sc(float &xA, float &yA, float &tA, float wA, float wIA, float &xB,
+                                               float &yB, float &tB, float wB, float wIB, bool hasB, float lax,
+                                               float lay, float lbx, float lby, float alpha, float &lamx,
+                                               float &lamy, float sa, float ca, float sb, float cb) {
   float rax = lax * ca - lay * sa, ray = lax * sa + lay * ca;
   float gax = xA + rax, gay = yA + ray;
-  float gbx = lbx, gby = lby, rbx = 0.0f, rby = 0.0f;
-  if (hasB) {
-    float sb, cb;
-    kx_sincos(tB, &sb, &cb);
-    rbx = lbx * cb - lby * sb;
-    rby = lbx * sb + lby * cb;
-    gbx = xB + rbx; gby = yB + rby;
-  } else {
-    wB = 0.0f; wIB = 0.0f;
-  }
+  float tbx = lbx * cb - lby * sb, tby = lbx * sb + lby * cb;
+  float rbx = hasB ? tbx : 0.0f, rby = hasB ? tby : 0.0f;
+  float gbx = hasB ? xB + tbx : lbx, gby = hasB ? yB + tby : lby;
+  if (!hasB) { wB = 0.0f; wIB = 0.0f; }
   float Cx = gax - gbx, Cy = gay - gby;
   // a non-finite C makes dlx/dly non-finite and is rejected below
-  if (Cx * Cx + Cy * Cy < KX_SLACK_POS2) return false;
+  bool ok = !(Cx * Cx + Cy * Cy < KX_SLACK_POS2);
   float pax = -ray, pay = rax, pbx = -rby, pby = rbx;
   float wSum = wA + wB;
   float k00 = wSum + wIA * pax * pax + wIB * pbx * pbx;
   float k01 = wIA * pax * pay + wIB * pbx * pby;
   float k11 = wSum + wIA * pay * pay + wIB * pby * pby;
   float det = k00 * k11 - k01 * k01;
-  if (fabsf(det) < 1e-12f) return false;
+  ok = ok && !(fabsf(det) < 1e-12f);
   float bx = Cx + alpha * lamx;
   float by = Cy + alpha * lamy;
   float inv = kx_rcp(det);
   float dlx = -(k11 * inv * bx - k01 * inv * by);
   float dly = -(-k01 * inv * bx + k00 * inv * by);
-  if (!isfinite(dlx) || !isfinite(dly)) return false;
-  lamx += dlx; lamy += dly;
-  xA += wA * dlx; yA += wA * dly;
-  tA += wIA * (pax * dlx + pay * dly);
-  xB -= wB * dlx; yB -= wB * dly;
-  tB -= wIB * (pbx * dlx + pby * dly);
-  return true;
+  ok = ok && isfinite(dlx) && isfinite(dly);
+  float nlx = lamx + dlx, nly = lamy + dly;
+  float nxA = xA + wA * dlx, nyA = yA + wA * dly;
+  float ntA = tA + wIA * (pax * dlx + pay * dly);
+  float nxB = xB - wB * dlx, nyB = yB - wB * dly;
+  float ntB = tB - wIB * (pbx * dlx + pby * dly);
+  lamx = ok ? nlx : lamx; lamy = ok ? nly : lamy;
+  xA = ok ? nxA : xA; yA = ok ? nyA : yA; tA = ok ? ntA : tA;
+  xB = ok ? nxB : xB; yB = ok ? nyB : yB; tB = ok ? ntB : tB;
+  return ok;
+}
+
+__device__ __forceinline__ bool kx_pin_core(float &xA, float &yA, float &tA, float wA, float wIA, float &xB,
+                                            float &yB, float &tB, float wB, float wIB, bool hasB, float lax,
+                                            float lay, float lbx, float lby, float alpha, float &lamx,
+                                            float &lamy) {
+  float sa, ca, sb, cb;
+  kx_sincos(tA, &sa, &ca);
+  kx_sincos(tB, &sb, &cb);
+  return kx_pin_core_sc(xA, yA, tA, wA, wIA, xB, yB, tB, wB, wIB, hasB, lax, lay, lbx, lby, alpha, lamx, lamy, sa,
+                        ca, sb, cb);
 }
 
 // Angle motor on values: scalar XPBD on the deviation from rest, then the
 // one-sided joint limit (xpbd.js:392-439). Returns true if an angle moved.
+// The angle differences are folded only when some lane of the wave needs it
+// (|d| > pi); otherwise fold(d) == d and the folds are skipped.
 __device__ __forceinline__ bool kx_angle_core(float &tA, float &tB, float wA, float wB, bool hasB, float rest,
                                               float target, float mn, float mx, float alpha, float &lam) {
   if (!hasB) wB = 0.0f;
   float wSum = wA + wB;
-  if (wSum <= 0.0f) return false;
-  bool moved = false;
-  float rel = hasB ? kx_angle_delta(tB, tA) : tA;
-  float dev = kx_angle_delta(rel, rest);
-  float C = kx_angle_delta(dev, target);
-  if (fabsf(C) >= KX_SLACK_ANG) {
-    float dl = -(C + alpha * lam) * kx_rcp(wSum + alpha);
-    if (!isfinite(dl)) return false;
-    lam += dl;
-    if (hasB) { tB += wB * dl; tA -= wA * dl; }
-    else tA += wA * dl;
-    moved = true;
+  float d1 = tB - tA;
+  float rel = hasB ? d1 : tA, dev = rel - rest, C = dev - target;
+  if (!__all((!hasB || fabsf(d1) <= KX_PI_BELOW) && fabsf(dev) <= KX_PI_BELOW && fabsf(C) <= KX_PI_BELOW)) {
+    rel = hasB ? kx_fold(d1) : tA;
+    dev = kx_angle_delta(rel, rest);
+    C = kx_angle_delta(dev, target);
   }
-  float rel2 = hasB ? kx_angle_delta(tB, tA) : tA;
-  float dev2 = kx_angle_delta(rel2, rest);
-  float over = 0.0f;
-  if (dev2 > mx) over = dev2 - mx;
-  else if (dev2 < mn) over = dev2 - mn;
-  if (fabsf(over) >= KX_SLACK_ANG) {
-    float dl2 = -over * kx_rcp(wSum);
-    if (hasB) { tB += wB * dl2; tA -= wA * dl2; }
-    else tA += wA * dl2;
-    moved = true;
+  const bool live = wSum > 0.0f;
+  const bool corr = fabsf(C) >= KX_SLACK_ANG;
+  float dl = -(C + alpha * lam) * kx_rcp(wSum + alpha);
+  // a non-finite correction rejects the whole constraint, limit included
+  const bool bad = corr && !isfinite(dl);
+  const bool ok1 = live && corr && !bad;
+  float lam1 = lam + dl;
+  float tA1 = hasB ? tA - wA * dl : tA + wA * dl;
+  float tB1 = tB + wB * dl;
+  lam = ok1 ? lam1 : lam;
+  tA = ok1 ? tA1 : tA;
+  tB = (ok1 && hasB) ? tB1 : tB;
+  float e1 = tB - tA;
+  float rel2 = hasB ? e1 : tA, dev2 = rel2 - rest;
+  if (!__all((!hasB || fabsf(e1) <= KX_PI_BELOW) && fabsf(dev2) <= KX_PI_BELOW)) {
+    rel2 = hasB ? kx_fold(e1) : tA;
+    dev2 = kx_angle_delta(rel2, rest);
   }
-  return moved;
+  float over = dev2 > mx ? dev2 - mx : (dev2 < mn ? dev2 - mn : 0.0f);
+  const bool ok2 = live && !bad && fabsf(over) >= KX_SLACK_ANG;
+  float dl2 = -over * kx_rcp(wSum);
+  float tA2 = hasB ? tA - wA * dl2 : tA + wA * dl2;
+  float tB2 = tB + wB * dl2;
+  tA = ok2 ? tA2 : tA;
+  tB = (ok2 && hasB) ? tB2 : tB;
+  return ok1 || ok2;
 }
 
 // SoA forms used by the plain path (P, Q = float* to global arrays).

@@ -66,6 +66,12 @@ __device__ __forceinline__ void kx_shift_creature(float4 *prv, float4 *pos, floa
     }
 }
 
+// Value of the neighbouring lane of an adjacent pair (lanes 2j <-> 2j+1),
+// one DPP move (quad_perm [1,0,3,2]).
+__device__ __forceinline__ float kx_swap_pair(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
+}
+
 // Grid-culled SDF query against whichever copy of the map the launch uses.
 __device__ __forceinline__ bool kx_query(const KxDev &d, const KxPersistCfg &P, const uint32_t *bits,
                                          const int *m_off, const int *m_ids, const float4 *m_rects, float wx,
@@ -217,8 +223,17 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
             if (is_ang) {
               moved = kx_angle_core(pA.z, pB.z, bA.y, bB.y, hasB, c0r.w, dk.y, c1r.x, c1r.y, dk.x, lk.x);
             } else {
-              moved = kx_pin_core(pA.x, pA.y, pA.z, bA.x, bA.y, pB.x, pB.y, pB.z, bB.x, bB.y, hasB, c0r.y, c0r.z,
-                                  dk.z, dk.w, dk.x, lk.x, lk.y);
+              // lanes 2j and 2j+1 hold the same pin: each evaluates sin/cos
+              // of one body's angle and the pair swaps results; the even
+              // lane commits
+              const bool odd = (sub & 1) != 0;
+              float s1, c1;
+              kx_sincos(odd ? pB.z : pA.z, &s1, &c1);
+              const float s2 = kx_swap_pair(s1), c2 = kx_swap_pair(c1);
+              moved = kx_pin_core_sc(pA.x, pA.y, pA.z, bA.x, bA.y, pB.x, pB.y, pB.z, bB.x, bB.y, hasB, c0r.y, c0r.z,
+                                     dk.z, dk.w, dk.x, lk.x, lk.y, odd ? s2 : s1, odd ? c2 : c1, odd ? s1 : s2,
+                                     odd ? c1 : c2) &&
+                      !odd;
             }
             if (moved && __float_as_int(lk.w) == 0) {
               pos[A] = pA;

@@ -19,6 +19,10 @@
 #include "../../include/kxpbd.h"
 #include "kx_sched.h"
 
+// Pin and grip rounds hold at most G/2 constraints: each one is solved by a
+// pair of adjacent lanes (one sin/cos per lane), see kx_pack_template.
+int kx_pin_capacity(int G) { return G >= 2 ? G / 2 : 1; }
+
 void kx_schedule_rounds(const TmplHost &t, int G, std::vector<int> &sched, std::vector<int> &rtype, int &R) {
   const int nc = (int)t.type.size();
   std::vector<std::vector<int>> succ(nc);
@@ -55,7 +59,8 @@ void kx_schedule_rounds(const TmplHost &t, int G, std::vector<int> &sched, std::
     std::vector<int> cand;
     for (int k : ready) if ((t.type[k] == KX_ANGLE ? 1 : 0) == best_cls) cand.push_back(k);
     std::stable_sort(cand.begin(), cand.end(), [&](int x, int y) { return prio[x] > prio[y]; });
-    if ((int)cand.size() > G) cand.resize(G);
+    const int cap = best_cls == 0 ? kx_pin_capacity(G) : G;
+    if ((int)cand.size() > cap) cand.resize(cap);
     std::vector<int> row(G, -1);
     for (int j = 0; j < (int)cand.size(); j++) row[j] = cand[j];
     sched.insert(sched.end(), row.begin(), row.end());
@@ -138,12 +143,17 @@ KxTmpl kx_pack_template(const TmplHost &t, int G, std::vector<uint32_t> &pool) {
   for (size_t q = 0; q < t.plx.size(); q++)
     pool.insert(pool.end(), {fbits(t.plx[q]), fbits(t.ply[q]), fbits(t.pr[q]), (uint32_t)pbody[q]});
   // round slots as packed descriptors: k | A << 8 | B << 16 (B = 255: world
-  // anchor), -1 for an idle lane, so one LDS read yields every index
+  // anchor), -1 for an idle lane, so one LDS read yields every index; a pin
+  // round's j-th constraint occupies lanes 2j and 2j+1
   std::vector<int> desc(sched.size(), -1);
-  for (size_t i = 0; i < sched.size(); i++) {
-    int k = sched[i];
-    if (k >= 0) desc[i] = k | (t.a[k] << 8) | ((t.b[k] < 0 ? 255 : t.b[k]) << 16);
-  }
+  for (int r = 0; r < T.R; r++)
+    for (int j = 0; j < G; j++) {
+      int k = sched[(size_t)r * G + j];
+      if (k < 0) continue;
+      int v = k | (t.a[k] << 8) | ((t.b[k] < 0 ? 255 : t.b[k]) << 16);
+      if (rtype[r]) desc[(size_t)r * G + j] = v;
+      else { desc[(size_t)r * G + 2 * j] = v; desc[(size_t)r * G + 2 * j + 1] = v; }
+    }
   T.o_sched = at(); put(pool, desc);
   // round types as a bitmask, 32 rounds per word (bit set = angle round)
   std::vector<int> rbits((rtype.size() + 31) / 32 + 1, 0);

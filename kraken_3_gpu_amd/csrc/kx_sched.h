@@ -30,6 +30,7 @@ struct TmplHost {
   std::vector<int> grip_k, grip_arm;      // every grip, its arm (-1 if none)
 };
 
+int kx_pin_capacity(int G);
 void kx_schedule_rounds(const TmplHost &t, int G, std::vector<int> &sched, std::vector<int> &rtype, int &R);
 void kx_schedule_collisions(const TmplHost &t, int G, std::vector<int> &lists, std::vector<int> &counts, int &maxbl);
 KxTmpl kx_pack_template(const TmplHost &t, int G, std::vector<uint32_t> &pool);
