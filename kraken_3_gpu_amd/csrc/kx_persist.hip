@@ -34,15 +34,18 @@
 // per-creature LDS layout (word offsets, 16-byte records):
 //   pos[i] = {ppx, ppy, angP, -}   prv[i] = {px, py, ang, -}   vel[i] = {vx, vy, av, -}
 //   dyn[k] = {alpha, target, lbx, lby}   lam[k] = {lambda_x|lambda, lambda_y, breakForce, broken}
+//   ev     = {step events, collision candidate count, -, -}
 //   flg    = one bit per collision point: "corrects at the pose after the rounds"
+//   cand   = collision candidates: 16-bit point indices
 struct KxCrLds {
-  int pos, prv, vel, dyn, lam, ev, flg, words;
+  int pos, prv, vel, dyn, lam, ev, flg, cand, words;
   __host__ __device__ KxCrLds(int NB, int NC, int NP) {
     int o = 0;
     pos = o; o += 4 * NB; prv = o; o += 4 * NB; vel = o; o += 4 * NB;
     dyn = o; o += 4 * NC; lam = o; o += 4 * NC;
     ev = o; o += 4;
     flg = o; o += ((NP + 31) / 32 + 3) / 4 * 4;
+    cand = o; o += ((NP + 1) / 2 + 3) / 4 * 4;
     words = o;
   }
 };
@@ -70,6 +73,37 @@ __device__ __forceinline__ void kx_shift_creature(float4 *prv, float4 *pos, floa
 // one DPP move (quad_perm [1,0,3,2]).
 __device__ __forceinline__ float kx_swap_pair(float v) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
+}
+
+// True if the square [x - r, x + r] x [y - r, y + r] (creature-local, origin
+// (ox, oy)) overlaps a grid cell holding a rect, or its position is not
+// finite. Cell indices come from the same monotone binary32 expressions as
+// the queries (kx_sdf_grid*), so a point inside the square never lands in a
+// cell outside the tested range; cells outside the grid never hit. A square
+// narrower than a cell spans at most 2 x 2 cells, read with four independent
+// loads; wider squares take the general loop.
+__device__ __forceinline__ bool kx_box_touches_cells(const KxDev &d, const uint32_t *cbits, float x, float y, float r,
+                                                     float ox, float oy) {
+  const float ax = (x - r + ox - d.gx0) * d.inv_cs, bx = (x + r + ox - d.gx0) * d.inv_cs;
+  const float ay = (y - r + oy - d.gy0) * d.inv_cs, by = (y + r + oy - d.gy0) * d.inv_cs;
+  if (!(isfinite(ax) && isfinite(bx) && isfinite(ay) && isfinite(by))) return true;
+  const float nx = (float)d.gnx, ny = (float)d.gny;
+  const int ix0 = (int)fminf(fmaxf(floorf(ax), 0.0f), nx), ix1 = (int)fmaxf(fminf(floorf(bx), nx - 1.0f), -1.0f);
+  const int iy0 = (int)fminf(fmaxf(floorf(ay), 0.0f), ny), iy1 = (int)fmaxf(fminf(floorf(by), ny - 1.0f), -1.0f);
+  if (ix0 > ix1 || iy0 > iy1) return false;
+  if (ix1 - ix0 <= 1 && iy1 - iy0 <= 1) {
+    const int c00 = iy0 * d.gnx + ix0, c01 = c00 + (ix1 - ix0), c10 = c00 + (iy1 - iy0) * d.gnx,
+              c11 = c10 + (ix1 - ix0);
+    const uint32_t w00 = cbits[c00 >> 5], w01 = cbits[c01 >> 5], w10 = cbits[c10 >> 5], w11 = cbits[c11 >> 5];
+    return ((w00 >> (c00 & 31)) | (w01 >> (c01 & 31)) | (w10 >> (c10 & 31)) | (w11 >> (c11 & 31))) & 1u;
+  }
+  bool hit = false;
+  for (int iy = iy0; iy <= iy1; iy++)
+    for (int ix = ix0; ix <= ix1; ix++) {
+      const int c = iy * d.gnx + ix;
+      hit = hit || ((cbits[c >> 5] >> (c & 31)) & 1u);
+    }
+  return hit;
 }
 
 // Grid-culled SDF query against whichever copy of the map the launch uses.
@@ -126,6 +160,7 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
   float4 *pos = (float4 *)(S + Lo.pos), *prv = (float4 *)(S + Lo.prv), *vel = (float4 *)(S + Lo.vel);
   float4 *dyn = (float4 *)(S + Lo.dyn), *lamr = (float4 *)(S + Lo.lam);
   uint32_t *evw = S + Lo.ev, *flg = S + Lo.flg;
+  uint16_t *cand = (uint16_t *)(S + Lo.cand);
   const int nb = act ? T.nb : 0, nc = act ? T.nc : 0;
   const int b0 = act ? d.cr[cid].body_off : 0, c0 = act ? d.cr[cid].cons_off : 0;
 
@@ -246,76 +281,81 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
         }
         // collisions (xpbd.js:441-470: per body, its points in order, each
         // correction moving the body before the next point is tested).
-        // Phase A tests every point in parallel at its body's pose after the
-        // rounds and records which ones correct. Up to a body's first such
-        // point the sequential pass sees exactly that pose, so phase B (one
-        // lane per body) starts at that point and re-tests from there on in
-        // order; bodies with no flagged point are left untouched.
+        //
+        // Broad phase: a query in a grid cell with no registered rect never
+        // hits, and every point of a body lies within its reach of the body
+        // centre, so only bodies whose reach box touches an occupied cell can
+        // collide. Lane `sub` owns bodies sub, sub + G, ...; the points of its
+        // near bodies are appended to the creature's candidate list.
+        // Phase A tests the candidates in parallel at their body's pose after
+        // the rounds and flags those that correct. Up to a body's first
+        // flagged point the sequential pass sees exactly that pose, so phase B
+        // (the owning lane) starts there and re-tests the rest in order. The
+        // order of the candidate list does not affect any result.
         {
-          const int nchunk = (T.np + G - 1) / G;     // uniform: one template per workgroup
-          uint32_t facc = 0u;
-          for (int j = 0; j < nchunk; j++) {
-            const int p = j * G + sub;
-            bool f = false;
-            if (act && p < T.np) {
-              float4 pt = t_prec[p];
-              const int b = __float_as_int(pt.w);
-              float4 br = t_brec[b], pb = pos[b];
-              if (br.x != 0.0f) {
-                float sa, ca, wx, wy;
-                kx_sincos(pb.z, &sa, &ca);
-                kx_world_point(pb.x, pb.y, ca, sa, pt.x, pt.y, wx, wy);
-                KxHit hh;
-                f = kx_query(d, P, bits, m_off, m_ids, m_rects, wx, wy, ox, oy, hh) && pt.z - hh.d > KX_SLACK_POS;
-              }
-            }
-            const unsigned long long bal = __ballot(f);
-            if constexpr (G == 64) {
-              if (act && sub == 0) { flg[2 * j] = (uint32_t)bal; flg[2 * j + 1] = (uint32_t)(bal >> 32); }
-            } else {
-              const uint32_t mine = (uint32_t)(bal >> (q * G)) & (G == 32 ? 0xffffffffu : ((1u << G) - 1u));
-              facc |= mine << ((j * G) & 31);
-              if ((((j + 1) * G) & 31) == 0 || j == nchunk - 1) {
-                if (act && sub == 0) flg[(j * G) >> 5] = facc;
-                facc = 0u;
-              }
+          const int nfw = (T.np + 31) >> 5;
+          if (act && sub == 0) evw[1] = 0u;
+          for (int wi = sub; wi < (act ? nfw : 0); wi += G) flg[wi] = 0u;
+          KX_WAVE_SYNC();
+          uint32_t mine = 0u;
+          const uint32_t *cb = P.bm_words ? bits : d.cell_bits;
+          for (int k = 0, i = sub; i < nb; k++, i += G) {
+            const float4 br = t_brec[i], pb = pos[i];
+            const int bp = __float_as_int(br.z), cnt = bp >> 16;
+            if (br.x != 0.0f && cnt > 0 && kx_box_touches_cells(d, cb, pb.x, pb.y, br.w + 0.01f, ox, oy)) {
+              mine |= 1u << k;
+              const int at = (int)atomicAdd(&evw[1], (uint32_t)cnt);
+              for (int t = 0; t < cnt; t++) cand[at + t] = (uint16_t)((bp & 0xffff) + t);
             }
           }
-          KX_WAVE_SYNC();
-          const int nbl = act ? t_ccnt[sub] : 0;
-          for (int j = 0; j < nbl; j++) {
-            const int i = t_coll[sub * T.maxbl + j];
-            float4 br = t_brec[i];
-            if (br.x == 0.0f) continue;
-            const int bp = __float_as_int(br.z);
-            const int o = bp & 0xffff, cnt = bp >> 16;
-            int pq = cnt;                              // first flagged point of the body
-            for (int wi = o >> 5; (wi << 5) < o + cnt; wi++) {
-              uint32_t wd = flg[wi];
-              const int lo = o - (wi << 5), hi = o + cnt - (wi << 5);
-              if (lo > 0) wd &= ~0u << lo;
-              if (hi < 32) wd &= (1u << hi) - 1u;
-              if (wd) { pq = (wi << 5) + __builtin_ctz(wd) - o; break; }
-            }
-            if (pq >= cnt) continue;
-            float4 pb = pos[i];
-            float x = pb.x, y = pb.y, sa, ca;
-            kx_sincos(pb.z, &sa, &ca);
-            for (; pq < cnt; pq++) {
-              float4 pt = t_prec[o + pq];
-              float wx, wy;
-              kx_world_point(x, y, ca, sa, pt.x, pt.y, wx, wy);
+          if (__any(mine != 0u)) {
+            KX_WAVE_SYNC();
+            const int ncand = act ? (int)evw[1] : 0;
+            for (int j = sub; j < ncand; j += G) {
+              const int p = cand[j];
+              const float4 pt = t_prec[p];
+              const float4 pb = pos[__float_as_int(pt.w)];
+              float sa, ca, wx, wy;
+              kx_sincos(pb.z, &sa, &ca);
+              kx_world_point(pb.x, pb.y, ca, sa, pt.x, pt.y, wx, wy);
               KxHit hh;
-              if (!kx_query(d, P, bits, m_off, m_ids, m_rects, wx, wy, ox, oy, hh)) continue;
-              const float pen = pt.z - hh.d;
-              if (pen > KX_SLACK_POS) {
-                float nx, ny;
-                kx_hit_normal(hh, nx, ny);
-                x += nx * pen; y += ny * pen;
-              }
+              if (kx_query(d, P, bits, m_off, m_ids, m_rects, wx, wy, ox, oy, hh) && pt.z - hh.d > KX_SLACK_POS)
+                atomicOr(&flg[p >> 5], 1u << (p & 31));
             }
-            pb.x = x; pb.y = y;
-            pos[i] = pb;
+            KX_WAVE_SYNC();
+            for (uint32_t m = mine; m; m &= m - 1u) {
+              const int i = sub + __builtin_ctz(m) * G;
+              float4 br = t_brec[i];
+              const int bp = __float_as_int(br.z);
+              const int o = bp & 0xffff, cnt = bp >> 16;
+              int pq = cnt;                              // first flagged point of the body
+              for (int wi = o >> 5; (wi << 5) < o + cnt; wi++) {
+                uint32_t wd = flg[wi];
+                const int lo = o - (wi << 5), hi = o + cnt - (wi << 5);
+                if (lo > 0) wd &= ~0u << lo;
+                if (hi < 32) wd &= (1u << hi) - 1u;
+                if (wd) { pq = (wi << 5) + __builtin_ctz(wd) - o; break; }
+              }
+              if (pq >= cnt) continue;
+              float4 pb = pos[i];
+              float x = pb.x, y = pb.y, sa, ca;
+              kx_sincos(pb.z, &sa, &ca);
+              for (; pq < cnt; pq++) {
+                float4 pt = t_prec[o + pq];
+                float wx, wy;
+                kx_world_point(x, y, ca, sa, pt.x, pt.y, wx, wy);
+                KxHit hh;
+                if (!kx_query(d, P, bits, m_off, m_ids, m_rects, wx, wy, ox, oy, hh)) continue;
+                const float pen = pt.z - hh.d;
+                if (pen > KX_SLACK_POS) {
+                  float nx, ny;
+                  kx_hit_normal(hh, nx, ny);
+                  x += nx * pen; y += ny * pen;
+                }
+              }
+              pb.x = x; pb.y = y;
+              pos[i] = pb;
+            }
           }
         }
         KX_WAVE_SYNC();

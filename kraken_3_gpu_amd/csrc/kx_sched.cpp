@@ -10,6 +10,7 @@
 // G constraints, all pins/grips or all angle motors (no divergent paths),
 // filled by critical-path list scheduling.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <numeric>
@@ -112,7 +113,7 @@ static uint32_t fbits(float x) {
 // Sections start on 16-byte boundaries; constraint, body and point records
 // are 16-byte vectors so the kernel reads each with one LDS instruction:
 //   crec[k] = {cab, lax, lay, rest} {min, max, -, -}   (2 x 16 B)
-//   brec[b] = {im, ii, point offset | count << 16, -}
+//   brec[b] = {im, ii, point offset | count << 16, reach of its points}
 //   prec[q] = {lx, ly, r, body}
 KxTmpl kx_pack_template(const TmplHost &t, int G, std::vector<uint32_t> &pool) {
   KxTmpl T{};
@@ -134,8 +135,15 @@ KxTmpl kx_pack_template(const TmplHost &t, int G, std::vector<uint32_t> &pool) {
                              fbits(t.mn[k]), fbits(t.mx[k]), 0u, 0u});
   }
   T.o_brec = at();
-  for (int b = 0; b < t.nb; b++)
-    pool.insert(pool.end(), {fbits(t.im[b]), fbits(t.ii[b]), (uint32_t)(t.poff[b] | (t.pcnt[b] << 16)), 0u});
+  for (int b = 0; b < t.nb; b++) {
+    // reach: largest distance of the body's collision points from its centre
+    float reach = 0.0f;
+    for (int j = 0; j < t.pcnt[b]; j++) {
+      int q = t.poff[b] + j;
+      reach = std::max(reach, (float)std::sqrt((double)t.plx[q] * t.plx[q] + (double)t.ply[q] * t.ply[q]));
+    }
+    pool.insert(pool.end(), {fbits(t.im[b]), fbits(t.ii[b]), (uint32_t)(t.poff[b] | (t.pcnt[b] << 16)), fbits(reach)});
+  }
   T.o_prec = at();
   std::vector<int> pbody(t.plx.size(), 0);
   for (int b = 0; b < t.nb; b++)
