@@ -147,7 +147,8 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
   const float4 *t_crec = (const float4 *)(smem + T.o_crec);   // 2 per constraint
   const float4 *t_brec = (const float4 *)(smem + T.o_brec);
   const float4 *t_prec = (const float4 *)(smem + T.o_prec);
-  const int *t_sched = (const int *)(smem + T.o_sched), *t_rbits = (const int *)(smem + T.o_rtype);
+  const int4 *t_sched4 = (const int4 *)(smem + T.o_sched);
+  const int *t_rbits = (const int *)(smem + T.o_rtype);
   const int *t_coll = (const int *)(smem + T.o_coll), *t_ccnt = (const int *)(smem + T.o_ccnt);
   const int *t_arm_tip = (const int *)(smem + T.o_arm_tip), *t_arm_grip = (const int *)(smem + T.o_arm_grip);
   const int *t_mot_k = (const int *)(smem + T.o_mot_k), *t_mot_slot = (const int *)(smem + T.o_mot_slot);
@@ -236,48 +237,52 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
       }
       KX_WAVE_SYNC();
       for (int it = 0; it < d.I; it++) {
-        // constraint rounds (type-homogeneous, wave-uniform branch); the slot
-        // descriptor of the next round is fetched while this one computes.
+        // constraint rounds (type-homogeneous, wave-uniform branch), four per
+        // group: a lane reads the group's four slot descriptors with one load.
         // Every LDS read of a slot is issued before any result is needed; a
         // broken constraint is solved too and its result dropped.
-        int dsc = (act && T.R > 0) ? t_sched[sub] : -1;
         uint32_t rbits = 0;
-        for (int r = 0; r < T.R; r++) {
-          if ((r & 31) == 0) rbits = (uint32_t)__builtin_amdgcn_readfirstlane(t_rbits[r >> 5]);
-          const bool is_ang = (rbits >> (r & 31)) & 1u;
-          const int dnext = (act && r + 1 < T.R) ? t_sched[(r + 1) * G + sub] : -1;
-          if (dsc >= 0) {
-            const int k = dsc & 255, A = (dsc >> 8) & 255, B0 = (dsc >> 16) & 255;
-            const bool hasB = B0 != 255;
-            const int B = hasB ? B0 : A;
-            float4 lk = lamr[k], c0r = t_crec[2 * k], c1r = t_crec[2 * k + 1], dk = dyn[k];
-            float4 pA = pos[A], pB = pos[B], bA = t_brec[A], bB = t_brec[B];
-            KX_HOLD4(lk); KX_HOLD4(c0r); KX_HOLD4(c1r); KX_HOLD4(dk);
-            KX_HOLD4(pA); KX_HOLD4(pB); KX_HOLD4(bA); KX_HOLD4(bB);
-            bool moved;
-            if (is_ang) {
-              moved = kx_angle_core(pA.z, pB.z, bA.y, bB.y, hasB, c0r.w, dk.y, c1r.x, c1r.y, dk.x, lk.x);
-            } else {
-              // lanes 2j and 2j+1 hold the same pin: each evaluates sin/cos
-              // of one body's angle and the pair swaps results; the even
-              // lane commits
-              const bool odd = (sub & 1) != 0;
-              float s1, c1;
-              kx_sincos(odd ? pB.z : pA.z, &s1, &c1);
-              const float s2 = kx_swap_pair(s1), c2 = kx_swap_pair(c1);
-              moved = kx_pin_core_sc(pA.x, pA.y, pA.z, bA.x, bA.y, pB.x, pB.y, pB.z, bB.x, bB.y, hasB, c0r.y, c0r.z,
-                                     dk.z, dk.w, dk.x, lk.x, lk.y, odd ? s2 : s1, odd ? c2 : c1, odd ? s1 : s2,
-                                     odd ? c1 : c2) &&
-                      !odd;
+        for (int r0 = 0; r0 < T.R; r0 += 4) {
+          if ((r0 & 31) == 0) rbits = (uint32_t)__builtin_amdgcn_readfirstlane(t_rbits[r0 >> 5]);
+          const int4 d4 = act ? t_sched4[(r0 >> 2) * G + sub] : make_int4(-1, -1, -1, -1);
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const int r = r0 + u;
+            if (r >= T.R) break;
+            const bool is_ang = (rbits >> (r & 31)) & 1u;
+            const int dsc = u == 0 ? d4.x : u == 1 ? d4.y : u == 2 ? d4.z : d4.w;
+            if (dsc >= 0) {
+              const int k = dsc & 255, A = (dsc >> 8) & 255, B0 = (dsc >> 16) & 255;
+              const bool hasB = B0 != 255;
+              const int B = hasB ? B0 : A;
+              float4 lk = lamr[k], c0r = t_crec[2 * k], c1r = t_crec[2 * k + 1], dk = dyn[k];
+              float4 pA = pos[A], pB = pos[B], bA = t_brec[A], bB = t_brec[B];
+              KX_HOLD4(lk); KX_HOLD4(c0r); KX_HOLD4(c1r); KX_HOLD4(dk);
+              KX_HOLD4(pA); KX_HOLD4(pB); KX_HOLD4(bA); KX_HOLD4(bB);
+              bool moved;
+              if (is_ang) {
+                moved = kx_angle_core(pA.z, pB.z, bA.y, bB.y, hasB, c0r.w, dk.y, c1r.x, c1r.y, dk.x, lk.x);
+              } else {
+                // lanes 2j and 2j+1 hold the same pin: each evaluates sin/cos
+                // of one body's angle and the pair swaps results; the even
+                // lane commits
+                const bool odd = (sub & 1) != 0;
+                float s1, c1;
+                kx_sincos(odd ? pB.z : pA.z, &s1, &c1);
+                const float s2 = kx_swap_pair(s1), c2 = kx_swap_pair(c1);
+                moved = kx_pin_core_sc(pA.x, pA.y, pA.z, bA.x, bA.y, pB.x, pB.y, pB.z, bB.x, bB.y, hasB, c0r.y,
+                                       c0r.z, dk.z, dk.w, dk.x, lk.x, lk.y, odd ? s2 : s1, odd ? c2 : c1,
+                                       odd ? s1 : s2, odd ? c1 : c2) &&
+                        !odd;
+              }
+              if (moved && __float_as_int(lk.w) == 0) {
+                pos[A] = pA;
+                if (hasB) pos[B] = pB;
+                lamr[k] = lk;
+              }
             }
-            if (moved && __float_as_int(lk.w) == 0) {
-              pos[A] = pA;
-              if (hasB) pos[B] = pB;
-              lamr[k] = lk;
-            }
+            KX_WAVE_SYNC();
           }
-          KX_WAVE_SYNC();
-          dsc = dnext;
         }
         // collisions (xpbd.js:441-470: per body, its points in order, each
         // correction moving the body before the next point is tested).

@@ -152,15 +152,19 @@ KxTmpl kx_pack_template(const TmplHost &t, int G, std::vector<uint32_t> &pool) {
     pool.insert(pool.end(), {fbits(t.plx[q]), fbits(t.ply[q]), fbits(t.pr[q]), (uint32_t)pbody[q]});
   // round slots as packed descriptors: k | A << 8 | B << 16 (B = 255: world
   // anchor), -1 for an idle lane, so one LDS read yields every index; a pin
-  // round's j-th constraint occupies lanes 2j and 2j+1
-  std::vector<int> desc(sched.size(), -1);
+  // round's j-th constraint occupies lanes 2j and 2j+1. Stored by groups of
+  // four rounds, lane-major within a group: desc4[(grp * G + lane) * 4 + r % 4]
+  // so a lane reads a group's four slots with one 16-byte load.
+  const int R4 = (T.R + 3) / 4;
+  std::vector<int> desc((size_t)R4 * G * 4, -1);
+  auto put_slot = [&](int r, int lane, int v) { desc[((size_t)(r >> 2) * G + lane) * 4 + (r & 3)] = v; };
   for (int r = 0; r < T.R; r++)
     for (int j = 0; j < G; j++) {
       int k = sched[(size_t)r * G + j];
       if (k < 0) continue;
       int v = k | (t.a[k] << 8) | ((t.b[k] < 0 ? 255 : t.b[k]) << 16);
-      if (rtype[r]) desc[(size_t)r * G + j] = v;
-      else { desc[(size_t)r * G + 2 * j] = v; desc[(size_t)r * G + 2 * j + 1] = v; }
+      if (rtype[r]) put_slot(r, j, v);
+      else { put_slot(r, 2 * j, v); put_slot(r, 2 * j + 1, v); }
     }
   T.o_sched = at(); put(pool, desc);
   // round types as a bitmask, 32 rounds per word (bit set = angle round)

@@ -11,7 +11,8 @@ struct KxTmpl {
   int base, words;
   int nb, nc, np, narms, nmot, ngrip, R, maxbl;
   int o_crec, o_brec, o_prec;     // 16-byte records (see kx_pack_template)
-  // o_sched: per round, G packed slots k | A << 8 | B << 16 (-1 idle);
+  // o_sched: groups of four rounds, per lane four packed slots
+  //          k | A << 8 | B << 16 (-1 idle);
   // o_rtype: round-type bitmask words (bit r set = angle round)
   int o_sched, o_rtype, o_coll, o_ccnt, o_arm_tip, o_arm_grip, o_mot_k, o_mot_slot, o_grip_k, o_grip_arm;
 };
