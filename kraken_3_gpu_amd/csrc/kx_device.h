@@ -168,9 +168,9 @@ __device__ __forceinline__ bool kx_sdf_grid_lds(const KxDev &d, const int *coff,
 // Pin / grip on values (xpbd.js:331-390): 2x2 XPBD solve on the world
 // anchor gap with generalized inverse masses, lambda accumulated per axis.
 // (sa, ca) and (sb, cb) are sin/cos of tA and tB. Branch-free: every lane
-// evaluates the whole solve and the result is committed only when the gap
-// exceeds the dead zone and the solve is regular and finite. Returns true
-// if the bodies moved.
+// evaluates the whole solve and writes the projected states; the return value
+// says whether they apply (gap beyond the dead zone, regular and finite
+// solve) and callers store them only then.
 __device__ __forceinline__ bool kx_pin_core_sc(float &xA, float &yA, float &tA, float wA, float wIA, float &xB,
                                                float &yB, float &tB, float wB, float wIB, bool hasB, float lax,
                                                float lay, float lbx, float lby, float alpha, float &lamx,
@@ -197,14 +197,11 @@ __device__ __forceinline__ bool kx_pin_core_sc(float &xA, float &yA, float &tA, 
   float dlx = -(k11 * inv * bx - k01 * inv * by);
   float dly = -(-k01 * inv * bx + k00 * inv * by);
   ok = ok && isfinite(dlx) && isfinite(dly);
-  float nlx = lamx + dlx, nly = lamy + dly;
-  float nxA = xA + wA * dlx, nyA = yA + wA * dly;
-  float ntA = tA + wIA * (pax * dlx + pay * dly);
-  float nxB = xB - wB * dlx, nyB = yB - wB * dly;
-  float ntB = tB - wIB * (pbx * dlx + pby * dly);
-  lamx = ok ? nlx : lamx; lamy = ok ? nly : lamy;
-  xA = ok ? nxA : xA; yA = ok ? nyA : yA; tA = ok ? ntA : tA;
-  xB = ok ? nxB : xB; yB = ok ? nyB : yB; tB = ok ? ntB : tB;
+  lamx += dlx; lamy += dly;
+  xA += wA * dlx; yA += wA * dly;
+  tA += wIA * (pax * dlx + pay * dly);
+  xB -= wB * dlx; yB -= wB * dly;
+  tB -= wIB * (pbx * dlx + pby * dly);
   return ok;
 }
 
@@ -269,8 +266,10 @@ __device__ __forceinline__ void kx_solve_pin(P ppx, P ppy, P angP, Q im, Q ii, i
   float xA = ppx[A], yA = ppy[A], tA = angP[A];
   float xB = 0.0f, yB = 0.0f, tB = 0.0f, wB = 0.0f, wIB = 0.0f;
   if (B >= 0) { xB = ppx[B]; yB = ppy[B]; tB = angP[B]; wB = im[B]; wIB = ii[B]; }
-  if (!kx_pin_core(xA, yA, tA, im[A], ii[A], xB, yB, tB, wB, wIB, B >= 0, lax, lay, lbx, lby, alpha, lamx, lamy))
+  float nlx = lamx, nly = lamy;
+  if (!kx_pin_core(xA, yA, tA, im[A], ii[A], xB, yB, tB, wB, wIB, B >= 0, lax, lay, lbx, lby, alpha, nlx, nly))
     return;
+  lamx = nlx; lamy = nly;
   ppx[A] = xA; ppy[A] = yA; angP[A] = tA;
   if (B >= 0) { ppx[B] = xB; ppy[B] = yB; angP[B] = tB; }
 }

@@ -29,7 +29,15 @@
 // Forces every component of a just-loaded 16-byte record into registers at
 // this point, so the record is read by one full-width LDS load issued with
 // its neighbours rather than narrowed and re-issued inside later branches.
-#define KX_HOLD4(v) asm volatile("" : "+v"((v).x), "+v"((v).y), "+v"((v).z), "+v"((v).w))
+typedef float kx_f4v __attribute__((ext_vector_type(4)));
+typedef float kx_f3v __attribute__((ext_vector_type(3)));   // 16-byte aligned: one 96-bit LDS access
+typedef float kx_f2v __attribute__((ext_vector_type(2)));
+#define KX_HOLD4(v)                          \
+  do {                                       \
+    kx_f4v t_ = {(v).x, (v).y, (v).z, (v).w}; \
+    asm volatile("" : "+v"(t_));             \
+    (v) = make_float4(t_.x, t_.y, t_.z, t_.w); \
+  } while (0)
 
 // per-creature LDS layout (word offsets, 16-byte records):
 //   pos[i] = {ppx, ppy, angP, -}   prv[i] = {px, py, ang, -}   vel[i] = {vx, vy, av, -}
@@ -62,7 +70,7 @@ __device__ __forceinline__ void kx_shift_creature(float4 *prv, float4 *pos, floa
     prv[i] = p; pos[i] = pp;
   }
   for (int k = sub; k < nc; k += G)
-    if (((__float_as_int(t_crec[2 * k].x) >> 8) & 255) == 255) {
+    if (((__float_as_int(t_crec[2 * k + 1].x) >> 8) & 255) == 255) {
       float4 dk = dyn[k];
       dk.z += sx; dk.w += sy;
       dyn[k] = dk;
@@ -199,11 +207,11 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
       if (act && sub < T.narms) {
         double adh = kx_ctl_u(L.seed, (uint32_t)cid, (uint32_t)key, KX_SLOT_ADH + sub);
         int g = t_arm_grip[sub], t = t_arm_tip[sub];
-        float4 dk = dyn[g], lk = lamr[g], tp = prv[t], tv = vel[t], cr0 = t_crec[2 * g];
+        float4 dk = dyn[g], lk = lamr[g], tp = prv[t], tv = vel[t], cr0 = t_crec[2 * g];   // {lax, lay, -, -}
         int bk = __float_as_int(lk.w);
         float f_bf = lk.z, gx = dk.z, gy = dk.w, lx = lk.x, ly = lk.y;
         uint32_t e = kx_reflex(L, hS, adh, my_cd, bk, f_bf, gx, gy, lx, ly, tv.x, tv.y, (double)tp.x, (double)tp.y,
-                               tp.z, cr0.y, cr0.z);
+                               tp.z, cr0.x, cr0.y);
         dk.z = gx; dk.w = gy;
         dyn[g] = dk;
         lamr[g] = make_float4(lx, ly, f_bf, __int_as_float(bk));
@@ -255,14 +263,31 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
               const int k = dsc & 255, A = (dsc >> 8) & 255, B0 = (dsc >> 16) & 255;
               const bool hasB = B0 != 255;
               const int B = hasB ? B0 : A;
-              float4 lk = lamr[k], c0r = t_crec[2 * k], c1r = t_crec[2 * k + 1], dk = dyn[k];
-              float4 pA = pos[A], pB = pos[B], bA = t_brec[A], bB = t_brec[B];
-              KX_HOLD4(lk); KX_HOLD4(c0r); KX_HOLD4(c1r); KX_HOLD4(dk);
-              KX_HOLD4(pA); KX_HOLD4(pB); KX_HOLD4(bA); KX_HOLD4(bB);
+              // each round type reads only the fields it uses
+              float4 lk = lamr[k];
               bool moved;
               if (is_ang) {
-                moved = kx_angle_core(pA.z, pB.z, bA.y, bB.y, hasB, c0r.w, dk.y, c1r.x, c1r.y, dk.x, lk.x);
+                // crec {rest, min, max, -}, dyn {alpha, target}, angles, inverse inertias
+                kx_f3v cr = *(const kx_f3v *)&t_crec[2 * k];
+                kx_f2v dk = *(const kx_f2v *)&dyn[k];
+                float tA = ((const float *)&pos[A])[2], tB = ((const float *)&pos[B])[2];
+                float iA = ((const float *)&t_brec[A])[1], iB = ((const float *)&t_brec[B])[1];
+                KX_HOLD4(lk);
+                asm volatile("" : "+v"(cr), "+v"(dk), "+v"(tA), "+v"(tB), "+v"(iA), "+v"(iB));
+                moved = kx_angle_core(tA, tB, iA, iB, hasB, cr.x, dk.y, cr.y, cr.z, dk.x, lk.x);
+                if (moved && __float_as_int(lk.w) == 0) {
+                  if (hasB) ((float *)&pos[B])[2] = tB;
+                  ((float *)&pos[A])[2] = tA;
+                  ((float *)&lamr[k])[0] = lk.x;
+                }
               } else {
+                // crec {lax, lay}, dyn {alpha, -, lbx, lby}, positions, {im, ii}
+                kx_f2v cr = *(const kx_f2v *)&t_crec[2 * k];
+                float4 dk = dyn[k];
+                kx_f3v pA = *(const kx_f3v *)&pos[A], pB = *(const kx_f3v *)&pos[B];
+                kx_f2v bA = *(const kx_f2v *)&t_brec[A], bB = *(const kx_f2v *)&t_brec[B];
+                KX_HOLD4(lk); KX_HOLD4(dk);
+                asm volatile("" : "+v"(cr), "+v"(pA), "+v"(pB), "+v"(bA), "+v"(bB));
                 // lanes 2j and 2j+1 hold the same pin: each evaluates sin/cos
                 // of one body's angle and the pair swaps results; the even
                 // lane commits
@@ -270,15 +295,19 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
                 float s1, c1;
                 kx_sincos(odd ? pB.z : pA.z, &s1, &c1);
                 const float s2 = kx_swap_pair(s1), c2 = kx_swap_pair(c1);
-                moved = kx_pin_core_sc(pA.x, pA.y, pA.z, bA.x, bA.y, pB.x, pB.y, pB.z, bB.x, bB.y, hasB, c0r.y,
-                                       c0r.z, dk.z, dk.w, dk.x, lk.x, lk.y, odd ? s2 : s1, odd ? c2 : c1,
-                                       odd ? s1 : s2, odd ? c1 : c2) &&
+                float xA = pA.x, yA = pA.y, tA = pA.z, xB = pB.x, yB = pB.y, tB = pB.z;
+                moved = kx_pin_core_sc(xA, yA, tA, bA.x, bA.y, xB, yB, tB, bB.x, bB.y, hasB, cr.x, cr.y, dk.z, dk.w,
+                                       dk.x, lk.x, lk.y, odd ? s2 : s1, odd ? c2 : c1, odd ? s1 : s2,
+                                       odd ? c1 : c2) &&
                         !odd;
-              }
-              if (moved && __float_as_int(lk.w) == 0) {
-                pos[A] = pA;
-                if (hasB) pos[B] = pB;
-                lamr[k] = lk;
+                pA = kx_f3v{xA, yA, tA};
+                pB = kx_f3v{xB, yB, tB};
+                if (moved && __float_as_int(lk.w) == 0) {
+                  // B first: for a world anchor B == A and pA must win
+                  *(kx_f3v *)&pos[B] = pB;
+                  *(kx_f3v *)&pos[A] = pA;
+                  *(kx_f2v *)&lamr[k] = kx_f2v{lk.x, lk.y};
+                }
               }
             }
             KX_WAVE_SYNC();
@@ -406,7 +435,7 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
   }
   for (int k = sub; k < nc; k += G) {
     int g = c0 + k;
-    int v = __float_as_int(t_crec[2 * k].x);
+    int v = __float_as_int(t_crec[2 * k + 1].x);
     float4 dk = dyn[k], lk = lamr[k];
     d.broken[g] = __float_as_int(lk.w);
     if (((v >> 8) & 255) == 255) { d.lbx[g] = dk.z; d.lby[g] = dk.w; }

@@ -112,7 +112,8 @@ static uint32_t fbits(float x) {
 // Serialise one template with its schedule for G lanes into the word pool.
 // Sections start on 16-byte boundaries; constraint, body and point records
 // are 16-byte vectors so the kernel reads each with one LDS instruction:
-//   crec[k] = {cab, lax, lay, rest} {min, max, -, -}   (2 x 16 B)
+//   crec[k] = {lax, lay, -, -} (pin, grip) or {rest, min, max, -} (angle),
+//             then {a | b << 8 | type << 16, -, -, -}          (2 x 16 B)
 //   brec[b] = {im, ii, point offset | count << 16, reach of its points}
 //   prec[q] = {lx, ly, r, body}
 KxTmpl kx_pack_template(const TmplHost &t, int G, std::vector<uint32_t> &pool) {
@@ -131,8 +132,10 @@ KxTmpl kx_pack_template(const TmplHost &t, int G, std::vector<uint32_t> &pool) {
   T.o_crec = at();
   for (int k = 0; k < T.nc; k++) {
     int cab = t.a[k] | ((t.b[k] < 0 ? 255 : t.b[k]) << 8) | (t.type[k] << 16);
-    pool.insert(pool.end(), {(uint32_t)cab, fbits(t.lax[k]), fbits(t.lay[k]), fbits(t.rest[k]),
-                             fbits(t.mn[k]), fbits(t.mx[k]), 0u, 0u});
+    if (t.type[k] == KX_ANGLE)
+      pool.insert(pool.end(), {fbits(t.rest[k]), fbits(t.mn[k]), fbits(t.mx[k]), 0u, (uint32_t)cab, 0u, 0u, 0u});
+    else
+      pool.insert(pool.end(), {fbits(t.lax[k]), fbits(t.lay[k]), 0u, 0u, (uint32_t)cab, 0u, 0u, 0u});
   }
   T.o_brec = at();
   for (int b = 0; b < t.nb; b++) {
