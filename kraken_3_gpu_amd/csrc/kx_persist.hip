@@ -187,6 +187,9 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
   double my_cd = 0.0;
   if (act && sub < T.narms) my_cd = d.cd[cid * KX_MAX_ARMS + sub];
   const double hS = d.hd * d.S;
+  // round types of the first 64 rounds (bit r set = angle round)
+  const uint32_t rb0 = (uint32_t)__builtin_amdgcn_readfirstlane(t_rbits[0]);
+  const uint32_t rb1 = (uint32_t)__builtin_amdgcn_readfirstlane(t_rbits[1]);
   KX_WAVE_SYNC();
 
   for (int step = 0; step < nsteps; step++) {
@@ -228,7 +231,7 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
     KX_WAVE_SYNC();
     for (int s = 0; s < d.S; s++) {
       // resetLambda + integrate (xpbd.js:262-287)
-      for (int k = sub; k < nc; k += G) { float4 lk = lamr[k]; lk.x = 0.0f; lk.y = 0.0f; lamr[k] = lk; }
+      for (int k = sub; k < nc; k += G) *(kx_f2v *)&lamr[k] = kx_f2v{0.0f, 0.0f};
       const bool forces = with_forces && step == 0 && s == 0;
       for (int i = sub; i < nb; i += G) {
         float fx = 0.0f, fy = 0.0f, tq = 0.0f;
@@ -249,10 +252,12 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
         // group: a lane reads the group's four slot descriptors with one load.
         // Every LDS read of a slot is issued before any result is needed; a
         // broken constraint is solved too and its result dropped.
-        uint32_t rbits = 0;
+        // the next group's descriptors are fetched while this group runs
+        int4 d4n = act ? t_sched4[sub] : make_int4(-1, -1, -1, -1);
         for (int r0 = 0; r0 < T.R; r0 += 4) {
-          if ((r0 & 31) == 0) rbits = (uint32_t)__builtin_amdgcn_readfirstlane(t_rbits[r0 >> 5]);
-          const int4 d4 = act ? t_sched4[(r0 >> 2) * G + sub] : make_int4(-1, -1, -1, -1);
+          const uint32_t rbits = r0 < 32 ? rb0 : r0 < 64 ? rb1 : (uint32_t)__builtin_amdgcn_readfirstlane(t_rbits[r0 >> 5]);
+          const int4 d4 = d4n;
+          if (act && r0 + 4 < T.R) d4n = t_sched4[((r0 >> 2) + 1) * G + sub];
 #pragma unroll
           for (int u = 0; u < 4; u++) {
             const int r = r0 + u;

@@ -171,7 +171,7 @@ KxTmpl kx_pack_template(const TmplHost &t, int G, std::vector<uint32_t> &pool) {
     }
   T.o_sched = at(); put(pool, desc);
   // round types as a bitmask, 32 rounds per word (bit set = angle round)
-  std::vector<int> rbits((rtype.size() + 31) / 32 + 1, 0);
+  std::vector<int> rbits(std::max<size_t>((rtype.size() + 31) / 32, 2), 0);   // at least two words
   for (size_t r = 0; r < rtype.size(); r++)
     if (rtype[r]) rbits[r >> 5] |= (int)(1u << (r & 31));
   T.o_rtype = at(); put(pool, rbits);
