@@ -290,7 +290,8 @@ static int build_persist(kx_handle *h) {
   // occupancy the launch can use: with nwg workgroups on nCU CUs, at most
   // ceil(nwg / nCU) of them are ever resident on one CU.
   int mcells = h->d.gnx * h->d.gny, mids = h->n_cell_ids;
-  int mapw = ((mcells + 1 + mids + 3) & ~3) + 4 * h->nR;
+  int mapw = 4 * mids + ((mcells + 2) / 2 + 3) / 4 * 4;
+  if (mids > 65535) mapw = 0;                 // offsets are staged as 16-bit
   const size_t base = ((size_t)words + bmw + (size_t)WPB * CPW * stride) * 4;
   if (base > 160 * 1024)
     return fail(KX_ENOTSUP, "persistent shape G=%d x %d waves needs %zu B of LDS (> 160 KiB)", G, WPB, base);
@@ -302,7 +303,7 @@ static int build_persist(kx_handle *h) {
   const int occ0 = kx_persist_occupancy(G, WPB, base);
   const size_t with_map = base + (size_t)mapw * 4;
   const int occ1 = with_map <= 160 * 1024 ? kx_persist_occupancy(G, WPB, with_map) : 0;
-  if (std::min(occ1, need) < std::min(occ0, need)) mapw = 0;
+  if (!mapw || std::min(occ1, need) < std::min(occ0, need)) mapw = 0;
   const size_t lds = base + (size_t)mapw * 4;
   h->pinfo[0] = (int)lds; h->pinfo[1] = nwg; h->pinfo[2] = mapw ? occ1 : occ0; h->pinfo[3] = need;
   h->pinfo[4] = mapw; h->pinfo[5] = bmw; h->pinfo[6] = words; h->pinfo[7] = stride;

@@ -94,7 +94,8 @@ struct KxPersistCfg {
   int tmpl_words;           // LDS words reserved for the template
   int bm_words;             // LDS words of the cell occupancy bitmap (after the template)
   int map_words;            // LDS words of the staged map (after the bitmap), 0 = map stays in HBM
-  int map_cells, map_ids;   // staged map: cell offsets [cells+1], then ids [ids], then rects
+  int map_cells, map_ids;   // staged map: the rect of each of the map_ids cell entries (16 B each),
+                            // then map_cells + 1 16-bit cell offsets
   int cr_stride;            // LDS words per creature (16 bytes off a 128-byte multiple)
   int nwaves;
   int NB, NC, NP;           // per-creature array sizes in LDS (bodies, constraints, points)

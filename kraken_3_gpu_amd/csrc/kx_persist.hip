@@ -44,7 +44,7 @@ typedef float kx_f2v __attribute__((ext_vector_type(2)));
 //   dyn[k] = {alpha, target, lbx, lby}   lam[k] = {lambda_x|lambda, lambda_y, breakForce, broken}
 //   ev     = {step events, collision candidate count, -, -}
 //   flg    = one bit per collision point: "corrects at the pose after the rounds"
-//   cand   = collision candidates: 16-bit point indices
+//   cand   = collision candidates: point | body << 16
 struct KxCrLds {
   int pos, prv, vel, dyn, lam, ev, flg, cand, words;
   __host__ __device__ KxCrLds(int NB, int NC, int NP) {
@@ -53,7 +53,7 @@ struct KxCrLds {
     dyn = o; o += 4 * NC; lam = o; o += 4 * NC;
     ev = o; o += 4;
     flg = o; o += ((NP + 31) / 32 + 3) / 4 * 4;
-    cand = o; o += ((NP + 1) / 2 + 3) / 4 * 4;
+    cand = o; o += (NP + 3) / 4 * 4;
     words = o;
   }
 };
@@ -114,11 +114,31 @@ __device__ __forceinline__ bool kx_box_touches_cells(const KxDev &d, const uint3
   return hit;
 }
 
+// The map staged in LDS: per cell entry the rect itself (ascending rect
+// index within a cell, the fold order), then 16-bit cell offsets. Two entries
+// are read at once; a cell rarely holds more.
+__device__ __forceinline__ bool kx_sdf_cells_lds(const KxDev &d, const uint16_t *coff, const float4 *crect, float x,
+                                                 float y, float ox, float oy, KxHit &h) {
+  const float gx = (x + ox - d.gx0) * d.inv_cs;
+  const float gy = (y + oy - d.gy0) * d.inv_cs;
+  if (!(gx >= 0.0f && gy >= 0.0f && gx < (float)d.gnx && gy < (float)d.gny)) return false;
+  const int ix = min((int)gx, d.gnx - 1), iy = min((int)gy, d.gny - 1);
+  const int cell = iy * d.gnx + ix;
+  const int b = coff[cell], e = coff[cell + 1];
+  if (b == e) return false;
+  const float4 r0 = crect[b], r1 = crect[b + 1 < e ? b + 1 : b];
+  kx_hit_init(h);
+  kx_fold_rect(x, y, r0, ox, oy, h);
+  if (b + 1 < e) kx_fold_rect(x, y, r1, ox, oy, h);
+  for (int k = b + 2; k < e; k++) kx_fold_rect(x, y, crect[k], ox, oy, h);
+  return isfinite(h.d);
+}
+
 // Grid-culled SDF query against whichever copy of the map the launch uses.
 __device__ __forceinline__ bool kx_query(const KxDev &d, const KxPersistCfg &P, const uint32_t *bits,
-                                         const int *m_off, const int *m_ids, const float4 *m_rects, float wx,
-                                         float wy, float ox, float oy, KxHit &hh) {
-  return P.map_words ? kx_sdf_grid_lds(d, m_off, m_ids, m_rects, wx, wy, ox, oy, hh)
+                                         const uint16_t *m_off, const float4 *m_crect, float wx, float wy,
+                                         float ox, float oy, KxHit &hh) {
+  return P.map_words ? kx_sdf_cells_lds(d, m_off, m_crect, wx, wy, ox, oy, hh)
          : P.bm_words ? kx_sdf_grid_bm(d, bits, wx, wy, ox, oy, hh)
                       : kx_sdf_grid(d, wx, wy, ox, oy, hh);
 }
@@ -140,14 +160,12 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
   for (int i = threadIdx.x; i < T.words; i += blockDim.x) smem[i] = P.pool[T.base + i];
   uint32_t *bits = smem + P.tmpl_words;
   for (int i = threadIdx.x; i < P.bm_words; i += blockDim.x) bits[i] = d.cell_bits[i];
-  // the map (cell offsets, rect ids, rects) when it fits
-  int *m_off = (int *)(smem + P.tmpl_words + P.bm_words);
-  int *m_ids = m_off + P.map_cells + 1;
-  float4 *m_rects = (float4 *)(smem + P.tmpl_words + P.bm_words + ((P.map_cells + 1 + P.map_ids + 3) & ~3));
+  // the map when it fits: the rect of every cell entry, then cell offsets
+  float4 *m_crect = (float4 *)(smem + P.tmpl_words + P.bm_words);
+  uint16_t *m_off = (uint16_t *)(smem + P.tmpl_words + P.bm_words + 4 * P.map_ids);
   if (P.map_words) {
-    for (int i = threadIdx.x; i <= P.map_cells; i += blockDim.x) m_off[i] = d.cell_off[i];
-    for (int i = threadIdx.x; i < P.map_ids; i += blockDim.x) m_ids[i] = d.cell_ids[i];
-    for (int i = threadIdx.x; i < d.nr; i += blockDim.x) m_rects[i] = d.rects[i];
+    for (int i = threadIdx.x; i < P.map_ids; i += blockDim.x) m_crect[i] = d.rects[d.cell_ids[i]];
+    for (int i = threadIdx.x; i <= P.map_cells; i += blockDim.x) m_off[i] = (uint16_t)d.cell_off[i];
   }
   __syncthreads();
   if (gw >= P.nwaves) return;
@@ -169,7 +187,7 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
   float4 *pos = (float4 *)(S + Lo.pos), *prv = (float4 *)(S + Lo.prv), *vel = (float4 *)(S + Lo.vel);
   float4 *dyn = (float4 *)(S + Lo.dyn), *lamr = (float4 *)(S + Lo.lam);
   uint32_t *evw = S + Lo.ev, *flg = S + Lo.flg;
-  uint16_t *cand = (uint16_t *)(S + Lo.cand);
+  uint32_t *cand = S + Lo.cand;
   const int nb = act ? T.nb : 0, nc = act ? T.nc : 0;
   const int b0 = act ? d.cr[cid].body_off : 0, c0 = act ? d.cr[cid].cons_off : 0;
 
@@ -338,27 +356,47 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
           KX_WAVE_SYNC();
           uint32_t mine = 0u;
           const uint32_t *cb = P.bm_words ? bits : d.cell_bits;
-          for (int k = 0, i = sub; i < nb; k++, i += G) {
+          // a lane's first four bodies: every load issued before any test
+          constexpr int NBL = 4;
+          float4 brs[NBL], pbs[NBL];
+#pragma unroll
+          for (int k = 0; k < NBL; k++) {
+            const int i = sub + k * G;
+            brs[k] = t_brec[i < nb ? i : 0];
+            pbs[k] = pos[i < nb ? i : 0];
+          }
+#pragma unroll
+          for (int k = 0; k < NBL; k++) {
+            const int i = sub + k * G;
+            const int bp = __float_as_int(brs[k].z), cnt = bp >> 16;
+            if (i < nb && brs[k].x != 0.0f && cnt > 0 &&
+                kx_box_touches_cells(d, cb, pbs[k].x, pbs[k].y, brs[k].w + 0.01f, ox, oy)) {
+              mine |= 1u << k;
+              const int at = (int)atomicAdd(&evw[1], (uint32_t)cnt);
+              for (int t = 0; t < cnt; t++) cand[at + t] = (uint32_t)((bp & 0xffff) + t) | ((uint32_t)i << 16);
+            }
+          }
+          for (int k = NBL, i = sub + NBL * G; i < nb; k++, i += G) {
             const float4 br = t_brec[i], pb = pos[i];
             const int bp = __float_as_int(br.z), cnt = bp >> 16;
             if (br.x != 0.0f && cnt > 0 && kx_box_touches_cells(d, cb, pb.x, pb.y, br.w + 0.01f, ox, oy)) {
               mine |= 1u << k;
               const int at = (int)atomicAdd(&evw[1], (uint32_t)cnt);
-              for (int t = 0; t < cnt; t++) cand[at + t] = (uint16_t)((bp & 0xffff) + t);
+              for (int t = 0; t < cnt; t++) cand[at + t] = (uint32_t)((bp & 0xffff) + t) | ((uint32_t)i << 16);
             }
           }
           if (__any(mine != 0u)) {
             KX_WAVE_SYNC();
             const int ncand = act ? (int)evw[1] : 0;
             for (int j = sub; j < ncand; j += G) {
-              const int p = cand[j];
-              const float4 pt = t_prec[p];
-              const float4 pb = pos[__float_as_int(pt.w)];
+              const uint32_t e = cand[j];
+              const int p = (int)(e & 0xffffu);
+              const float4 pt = t_prec[p], pb = pos[e >> 16];
               float sa, ca, wx, wy;
               kx_sincos(pb.z, &sa, &ca);
               kx_world_point(pb.x, pb.y, ca, sa, pt.x, pt.y, wx, wy);
               KxHit hh;
-              if (kx_query(d, P, bits, m_off, m_ids, m_rects, wx, wy, ox, oy, hh) && pt.z - hh.d > KX_SLACK_POS)
+              if (kx_query(d, P, bits, m_off, m_crect, wx, wy, ox, oy, hh) && pt.z - hh.d > KX_SLACK_POS)
                 atomicOr(&flg[p >> 5], 1u << (p & 31));
             }
             KX_WAVE_SYNC();
@@ -384,7 +422,7 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
                 float wx, wy;
                 kx_world_point(x, y, ca, sa, pt.x, pt.y, wx, wy);
                 KxHit hh;
-                if (!kx_query(d, P, bits, m_off, m_ids, m_rects, wx, wy, ox, oy, hh)) continue;
+                if (!kx_query(d, P, bits, m_off, m_crect, wx, wy, ox, oy, hh)) continue;
                 const float pen = pt.z - hh.d;
                 if (pen > KX_SLACK_POS) {
                   float nx, ny;
