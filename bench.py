@@ -52,6 +52,40 @@ def rects_within(w, radius=16.0):
     return float(cnt.mean())
 
 
+def algorithmic_launch_bytes(w, nsteps):
+    """HBM bytes one persistent launch must move: the state it reads at entry
+    and writes at exit (the LDS holds everything in between) plus one event
+    word per creature and step.
+    in:  bodies px, py, ang, vx, vy, av (6 f32); constraints compliance,
+         target, lbx, lby, breakForce (5 f32) + broken (u8); 8 f64 reflex
+         cooldowns per creature
+    out: bodies 9 f32 (the 6 above + predicted ppx, ppy, angP); constraints
+         broken, breakForce, target, lambda, lambda_x, lambda_y (1 B + 5 f32),
+         compliance of motors, world anchors of world-anchored constraints;
+         cooldowns; events"""
+    from kraken_3_gpu_amd import _lib as K
+    nb, nc, ncr = w.n, w.nc, w.n_creatures
+    n_ang = int((w.type == K.KX_ANGLE).sum())
+    n_world = int((w.b < 0).sum())
+    bin_ = nb * 6 * 4 + nc * (5 * 4 + 1) + ncr * 8 * 8
+    bout = nb * 9 * 4 + nc * (1 + 5 * 4) + n_ang * 4 + n_world * 8 + ncr * 8 * 8 + ncr * 4 * nsteps
+    return bin_ + bout
+
+
+def measured_traffic(n_cr, chunk):
+    """HBM bytes per launch from the committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE, copy-calibrated), when it covers this workload."""
+    path = os.path.join(ROOT, "profiles", "kx_k_persist_uniform4096_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("creatures") != n_cr or t.get("steps_per_launch") != chunk:
+        return None
+    return t
+
+
 def cpu_baseline(seconds=12.0):
     """The oracle (C restatement of xpbd.js, f64) stepping 64-creature worlds
     under the same control law on the host cores, one world per thread."""
@@ -153,11 +187,9 @@ def main():
     full = [m for m, n in chunk_ms if n == args.chunk] or [chunk_ms[0][0]]
     launch_ms = float(np.mean(full))
     achieved_tflops = F * n_cr * args.chunk / (launch_ms * 1e-3) / 1e12
-    # step-boundary HBM traffic of one launch: body state in+out (6 f32 x2),
-    # statics (im, ii, point table, constraint table) and the dynamic
-    # constraint fields, per creature
-    bytes_per_creature_launch = 43 * (6 * 4 * 2 + 9 * 4) + 90 * (16 * 4 + 9 * 4) + 92 * 3 * 4 + 8 * 8 * 2
-    hbm_gbs = bytes_per_creature_launch * n_cr / (launch_ms * 1e-3) / 1e9
+    launch_bytes = algorithmic_launch_bytes(w, args.chunk)
+    hbm_gbs = launch_bytes / (launch_ms * 1e-3) / 1e9
+    traffic = measured_traffic(n_cr, args.chunk)
     lat = sorted(m for m, n in chunk_ms if n == args.chunk) or [launch_ms]
     p50 = float(np.percentile(lat, 50))
     p99 = float(np.percentile(lat, 99))
@@ -202,7 +234,10 @@ def main():
                    "creatures": n_cr, "sim_steps": args.steps, "steps_per_launch": args.chunk,
                    "path": "persistent (one wavefront per creature, LDS-resident)"},
         "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved_tflops / FP32_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved_tflops / FP32_PEAK_TFLOPS,
+                     "traffic": traffic["traffic_bytes"] if traffic else None,
+                     "traffic_source": "profiles/kx_k_persist_uniform4096_traffic.json" if traffic else None,
+                     "algorithmic_bytes_per_launch": launch_bytes,
                      "flops_per_creature_step": F, "grips_attached_mean": g_att, "rects_within_16px_mean": r_near,
                      "kernel": "kx_k_persist", "launch_ms": launch_ms,
                      "hbm_algorithmic_gbs": hbm_gbs, "hbm_peak_gbs": HBM_PEAK_GBS,

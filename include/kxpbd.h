@@ -75,8 +75,8 @@ typedef struct {
   double h;                          /* seconds per substep */
   double lin_drag, ang_drag, max_speed, max_ang_speed;
   int32_t path;                      /* KX_PATH_* */
-  int32_t lanes_per_creature;        /* persistent path: 8, 16, 32 or 64; 0 = default (32) */
-  int32_t waves_per_block;           /* persistent path: 1, 2 or 4; 0 = default (2) */
+  int32_t lanes_per_creature;        /* persistent path: 8, 16, 32 or 64; 0 = default (16) */
+  int32_t waves_per_block;           /* persistent path: 1, 2 or 4; 0 = default (1) */
 } kx_params;
 
 /* Body state in SoA (gpuworld.js:250-263 upload; 417-441 mirrorInto).

@@ -52,7 +52,7 @@ struct DevBuf {
 struct kx_handle {
   int nB = 0, nC = 0, nP = 0, nR = 0, nCr = 0, nArms = 0;
   kx_params prm{};
-  int path = KX_PATH_PERSISTENT, G = 32, wpb = 2;
+  int path = KX_PATH_PERSISTENT, G = 16, wpb = 1;
   int n_cell_ids = 0;
   bool persist_ok = false, has_creatures = false, forces_pending = false;
   std::string persist_why;

@@ -1,0 +1,122 @@
+"""Summarise rocprofv3 CSV output of a bench.py run into the short text kept
+under profiles/ and the per-launch HBM traffic bench.py reports.
+
+HBM traffic of a kernel launch comes from two counter passes (FETCH_SIZE, then
+WRITE_SIZE, KiB per dispatch). Each is scaled by what the same counter reads
+for a device copy of known size (bench.py --copy-probe): on gfx950 a 256 MiB
+copy reads back FETCH_SIZE = 128 MiB and WRITE_SIZE = 256 MiB, so the fetch
+counter is scaled by 2.0 and the write counter by 1.0.
+
+  python -m kraken_3_gpu_amd.rocprof_summary --kernel kx_k_persist \\
+      --stats DIR/kt_kernel_stats.csv --trace DIR/kt_kernel_trace.csv \\
+      --fetch F.csv --write W.csv --copy-fetch CF.csv --copy-write CW.csv \\
+      [--sq S1.csv S2.csv] --workload "..." --creatures N --steps-per-launch K --command "..." \\
+      --out profiles/NAME.md --json profiles/NAME_traffic.json
+"""
+import argparse
+import collections
+import csv
+import json
+
+
+def _rows(path):
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def counter_per_dispatch(path, kernel, counter):
+    """{dispatch id: summed counter value} for dispatches of `kernel`."""
+    out = collections.defaultdict(float)
+    for r in _rows(path):
+        if kernel in r["Kernel_Name"] and r["Counter_Name"] == counter:
+            out[int(r["Dispatch_Id"])] += float(r["Counter_Value"])
+    return dict(out)
+
+
+def copy_factor(path, counter, bytes_moved):
+    """Bytes per counter KiB, from the largest copyBuffer dispatch of a copy probe."""
+    vals = counter_per_dispatch(path, "copyBuffer", counter)
+    if not vals:
+        return None
+    kib = max(vals.values())
+    return bytes_moved / (kib * 1024.0) if kib > 0 else None
+
+
+def trace_durations(path, kernel):
+    out = []
+    for r in _rows(path):
+        if kernel in r["Kernel_Name"]:
+            out.append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-6)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--kernel", required=True)
+    ap.add_argument("--stats", required=True)
+    ap.add_argument("--trace")
+    ap.add_argument("--fetch", required=True)
+    ap.add_argument("--write", required=True)
+    ap.add_argument("--copy-fetch", required=True)
+    ap.add_argument("--copy-write", required=True)
+    ap.add_argument("--copy-bytes", type=int, default=256 << 20)
+    ap.add_argument("--sq", nargs="*", default=[])
+    ap.add_argument("--workload", required=True)
+    ap.add_argument("--creatures", type=int, required=True)
+    ap.add_argument("--steps-per-launch", type=int, required=True)
+    ap.add_argument("--command", action="append", default=[])
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--json", required=True)
+    a = ap.parse_args()
+
+    stats = _rows(a.stats)
+    fetch = counter_per_dispatch(a.fetch, a.kernel, "FETCH_SIZE")
+    write = counter_per_dispatch(a.write, a.kernel, "WRITE_SIZE")
+    ff = copy_factor(a.copy_fetch, "FETCH_SIZE", a.copy_bytes)
+    fw = copy_factor(a.copy_write, "WRITE_SIZE", a.copy_bytes)
+    fetch_kib = max(fetch.values()) if fetch else None
+    write_kib = max(write.values()) if write else None
+    fetch_b = fetch_kib * 1024 * ff if fetch_kib is not None and ff else None
+    write_b = write_kib * 1024 * fw if write_kib is not None and fw else None
+
+    lines = [f"# {a.kernel}: {a.workload}", "", "Commands (run from the repository root, `cd /tmp && export TMPDIR=/tmp` first):", ""]
+    lines += [f"    {c}" for c in a.command]
+    lines += ["", "## Kernel trace (rocprofv3 --kernel-trace --stats)", "",
+              "| kernel | calls | average ms | share % |", "|---|---|---|---|"]
+    for r in stats[:12]:
+        lines.append(f"| `{r['Name'][:70]}` | {r['Calls']} | {float(r['AverageNs']) * 1e-6:.3f} | {float(r['Percentage']):.2f} |")
+    if a.trace:
+        d = trace_durations(a.trace, a.kernel)
+        if d:
+            lines += ["", f"{a.kernel} dispatch durations (ms): " + ", ".join(f"{x:.3f}" for x in d)]
+    lines += ["", "## HBM traffic per launch (rocprofv3 --pmc, one counter per run)", "",
+              "| counter | KiB per dispatch | copy-probe scale | bytes per launch |", "|---|---|---|---|",
+              f"| FETCH_SIZE | {fetch_kib} | {ff} | {fetch_b} |",
+              f"| WRITE_SIZE | {write_kib} | {fw} | {write_b} |"]
+    sq = {}
+    for p in a.sq:
+        for r in _rows(p):
+            if a.kernel in r["Kernel_Name"]:
+                sq[r["Counter_Name"]] = sq.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    if sq:
+        waves = sq.get("SQ_WAVES", 0.0)
+        wc = sq.get("SQ_WAVE_CYCLES", 0.0)
+        lines += ["", "## SQ counters (one launch)", "", "| counter | total | per wave | / SQ_WAVE_CYCLES |", "|---|---|---|---|"]
+        for k in sorted(sq):
+            v = sq[k]
+            lines.append(f"| {k} | {v:.4e} | {v / waves if waves else 0:.4e} | {v / wc if wc else 0:.4f} |")
+        if sq.get("SQ_INSTS_VALU") and "SQ_THREAD_CYCLES_VALU" in sq:
+            lines.append("")
+            lines.append(f"Active lanes per VALU instruction: {sq['SQ_THREAD_CYCLES_VALU'] / (sq['SQ_INSTS_VALU'] * 64):.3f}")
+    with open(a.out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    with open(a.json, "w") as f:
+        json.dump({"kernel": a.kernel, "workload": a.workload, "creatures": a.creatures,
+                   "steps_per_launch": a.steps_per_launch, "fetch_kib": fetch_kib, "write_kib": write_kib,
+                   "fetch_scale": ff, "write_scale": fw, "fetch_bytes": fetch_b, "write_bytes": write_b,
+                   "traffic_bytes": (fetch_b + write_b) if fetch_b is not None and write_b is not None else None,
+                   "commands": a.command}, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()
