@@ -86,3 +86,50 @@ def load_state_subset(o, w, creatures, bidx, cidx):
                        compliance=f32(w.compliance[cidx]), target=f32(w.target[cidx]),
                        breakForce=f32(w.breakForce[cidx]))
     o.set_cd(np.ascontiguousarray(w.cd[np.asarray(creatures)]))
+
+
+def _js_num(x):
+    x = float(x)
+    if x != x:
+        return "NaN"
+    if x == float("inf"):
+        return "Infinity"
+    if x == float("-inf"):
+        return "-Infinity"
+    return x
+
+
+def world_to_js(w, colored=False):
+    """An XpbdWorld-shaped dict (xpbd.js:188-240 field names) of a World, for
+    the Node gpuworld surface; +-Infinity / NaN as strings."""
+    cons = []
+    for k in range(w.nc):
+        cons.append(dict(type=int(w.type[k]), a=int(w.a[k]), b=int(w.b[k]), owner=int(w.owner[k]),
+                         lax=_js_num(w.lax[k]), lay=_js_num(w.lay[k]), lbx=_js_num(w.lbx[k]), lby=_js_num(w.lby[k]),
+                         compliance=_js_num(w.compliance[k]), rest=_js_num(w.rest[k]),
+                         minAngle=_js_num(w.minA[k]), maxAngle=_js_num(w.maxA[k]), target=_js_num(w.target[k]),
+                         breakForce=_js_num(w.breakForce[k]), broken=bool(w.broken[k]),
+                         **{"lambda": _js_num(w.lambda_[k])}, lambdaX=_js_num(w.lambdaX[k]),
+                         lambdaY=_js_num(w.lambdaY[k]), force=_js_num(w.force[k]),
+                         color=int(w.color[k])))
+    s = w.solver
+    out = dict(n=int(w.n), substeps=int(s["substeps"]), iterations=int(s["iterations"]), h=float(s["h"]),
+               linDrag=float(s["lin_drag"]), angDrag=float(s["ang_drag"]), maxSpeed=float(s["max_speed"]),
+               maxAngSpeed=float(s["max_ang_speed"]),
+               px=[float(v) for v in w.px], py=[float(v) for v in w.py], vx=[float(v) for v in w.vx],
+               vy=[float(v) for v in w.vy], ang=[float(v) for v in w.ang], av=[float(v) for v in w.av],
+               im=[float(v) for v in w.im], ii=[float(v) for v in w.ii], constraints=cons,
+               collPts=[dict(body=int(b), lx=float(x), ly=float(y), r=float(r))
+                        for b, x, y, r in zip(w.p_body, w.p_lx, w.p_ly, w.p_r)],
+               rects=[dict(x=float(r[0]), y=float(r[1]), w=float(r[2]), h=float(r[3])) for r in w.rects])
+    if colored and w.colorOff is not None:
+        out["colorOff"], out["colorCnt"] = [int(v) for v in w.colorOff], [int(v) for v in w.colorCnt]
+    return out
+
+
+def creatures_to_js(w):
+    """kx_creature_desc arm tables of a World (desc indices)."""
+    return dict(nCreatures=int(w.n_creatures), nArms=int(len(w.arm_tip)),
+                creatureArmOff=[int(v) for v in w.arm_off], armTip=[int(v) for v in w.arm_tip],
+                armGrip=[int(v) for v in w.arm_grip], armMotorOff=[int(v) for v in w.motor_off],
+                motors=[int(v) for v in w.motors])
