@@ -118,12 +118,115 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes_read": n, "copy_probe_bytes_written": n, "copies": 3}), flush=True)
 
 
+def law_for(workload, seed):
+    """The control law of a workload: the reference's reflex constants, or for
+    the stress run every tip adhesive with a per-tip break limit drawn once
+    from U[50, 200] N."""
+    from kraken_3_gpu_amd import _lib as K
+    from kraken_3_gpu_amd import world as W
+    if workload == "stress":
+        law = W.default_law(seed, grip_min=50.0, grip_max=200.0)
+        law.attach_min, law.release_below = -1.0, -1.0
+        law.flags = K.KX_LAW_FIXED_GRIP
+        return law
+    return W.default_law(seed)
+
+
+def flops_per_step(w, g_att_total, r_near):
+    """Algorithmic FLOPs of one step of the whole world (the per-creature
+    formula summed over its creatures, any morphology)."""
+    from kraken_3_gpu_amd import _lib as K
+    pins = int((w.type == K.KX_PIN).sum())
+    motors = int((w.type == K.KX_ANGLE).sum())
+    grips = int((w.type == K.KX_GRIP).sum())
+    it = 85 * pins + 42 * motors + 66 * g_att_total + len(w.p_body) * (11 + 12 * r_near)
+    return w.S * (w.I * it + 45 * w.n) + 5 * grips
+
+
+def persistent_run(g, law, steps, warmup, chunk, step0=0):
+    """Warm up, then time `steps` simulation steps in launches of `chunk`
+    steps (state resident in HBM). Returns timing and the next step key."""
+    from kraken_3_gpu_amd import gpuworld as G
+    step = step0
+    while step < step0 + warmup:
+        n = min(chunk, step0 + warmup - step)
+        G.rollout(g, law, step, n)
+        step += n
+    G.sync(g)
+    chunk_ms = []
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        n = min(chunk, steps - done)
+        G.rollout(g, law, step, n)
+        ms, _ = G.last_timing(g)
+        chunk_ms.append((ms, n))
+        step += n
+        done += n
+    G.sync(g)
+    wall = time.perf_counter() - t0
+    full = [m for m, n in chunk_ms if n == chunk] or [chunk_ms[0][0]]
+    lat = sorted(full)
+    return dict(wall=wall, chunk_ms=chunk_ms, launch_ms=float(np.mean(full)), step=step,
+                p50=float(np.percentile(lat, 50)), p99=float(np.percentile(lat, 99)),
+                dev_ms=sum(m for m, _ in chunk_ms))
+
+
+def warm_accounting(g, w):
+    """Attached grips and rects near the collision points, from the device state."""
+    from kraken_3_gpu_amd import _lib as K
+    from kraken_3_gpu_amd import gpuworld as G
+    G.download(g)
+    grips = w.type == K.KX_GRIP
+    return float((w.broken[grips] == 0).sum()), rects_within(w)
+
+
+def bench_stress(args):
+    """65 536 creatures on makeMap(4096, 4096, 32 x 32 corridors), every tip
+    adhesive with a per-tip break limit from U[50, 200] N."""
+    from kraken_3_gpu_amd import _lib as K
+    from kraken_3_gpu_amd import gpuworld as G
+    from kraken_3_gpu_amd import world as W
+    steps = args.steps or 2000
+    law = law_for("stress", 2024)
+    w = W.world_stress(args.creatures or 65536, seed=1)
+    n_cr = w.n_creatures
+    g = G.create(w, path=K.KX_PATH_PERSISTENT, lanes_per_creature=args.lanes, waves_per_block=args.wpb)
+    G.upload(g)
+    G.rollout(g, law, 0, args.warmup)
+    G.sync(g)
+    g_att, r_near = warm_accounting(g, w)
+    r = persistent_run(g, law, steps, 0, args.chunk, step0=args.warmup)
+    st = G.download(g)
+    F = flops_per_step(w, g_att, min(r_near, 64.0))
+    ach = F * args.chunk / (r["launch_ms"] * 1e-3) / 1e12
+    lb = algorithmic_launch_bytes(w, args.chunk)
+    out = {
+        "metric": "creature-steps/s, stress run (65536 creatures, 4096x4096 map, 32x32 corridors)",
+        "value": n_cr * steps / r["wall"], "unit": "creature-steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": r["wall"] * 1e3 / steps, "higher_is_better": True,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"stress: {n_cr} creatures (6 arms x 7 segments) in the 32x32 corridor cells of "
+                               f"makeMap(4096, 4096), all tips adhesive, per-tip break limit U[50,200] N",
+                   "creatures": n_cr, "rects": int(len(w.rects)), "steps_per_launch": args.chunk},
+        "roofline": {"bound": "valu", "achieved": ach, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": ach / FP32_PEAK_TFLOPS, "launch_ms": r["launch_ms"],
+                     "hbm_algorithmic_gbs": lb / (r["launch_ms"] * 1e-3) / 1e9, "hbm_peak_gbs": HBM_PEAK_GBS,
+                     "grips_attached": g_att, "rects_within_16px_mean": r_near},
+        "launch_shape": G.persist_info(g),
+        "p50_rollout_ms": r["p50"], "p99_rollout_ms": r["p99"], "final_stats": st,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--workload", choices=["uniform", "stress"], default="uniform",
+                    help="uniform batch (the headline line) or the stress run")
+    ap.add_argument("--steps", type=int, default=0, help="timed steps (default 1000; stress 2000)")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--creatures", type=int, default=4096)
+    ap.add_argument("--creatures", type=int, default=0, help="creatures (default 4096; stress 65536)")
     ap.add_argument("--chunk", type=int, default=50, help="simulation steps per persistent launch")
     ap.add_argument("--plain-steps", type=int, default=100)
     ap.add_argument("--lanes", type=int, default=0, help="lanes per creature (0 = default)")
@@ -140,6 +243,12 @@ def main():
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (creatures shard across "
                                    "GPUs with no per-step exchange, see DESIGN.md)"}))
         sys.exit(2)
+
+    if args.workload == "stress":
+        bench_stress(args)
+        return
+    args.steps = args.steps or 1000
+    args.creatures = args.creatures or 4096
 
     from kraken_3_gpu_amd import _lib as K
     from kraken_3_gpu_amd import gpuworld as G

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define KXPBD_ABI_VERSION 1
+#define KXPBD_ABI_VERSION 2
 
 /* error codes */
 #define KX_OK 0
@@ -112,7 +112,12 @@ typedef struct {
   double tone, shoulder_limit, joint_limit;
   double stiff_min, stiff_max, grip_min, grip_max;
   double slip_max, cooldown, attach_min, release_below;
+  uint32_t creature_base;            /* creature c of this handle draws as creature creature_base + c */
+  uint32_t flags;                    /* KX_LAW_* */
 } kx_law;
+/* each tip's break force is drawn once (per creature and arm) instead of
+ * following that step's adhesion */
+#define KX_LAW_FIXED_GRIP 1u
 
 typedef struct kx_handle kx_handle;
 

@@ -11,6 +11,7 @@ LIB_PATH = os.path.join(HERE, "libkxpbd.so")
 
 KX_OK, KX_EINVAL, KX_ENOMEM, KX_EDEVICE, KX_ENOTSUP = 0, -22, -12, -5, -95
 KX_PIN, KX_ANGLE, KX_GRIP = 0, 1, 2
+KX_LAW_FIXED_GRIP = 1
 KX_PATH_PLAIN, KX_PATH_PERSISTENT = 0, 1
 
 i32, u32, f64, f32p, i32p, u8p, f64p, u32p = (C.c_int32, C.c_uint32, C.c_double, C.POINTER(C.c_float),
@@ -56,7 +57,8 @@ class Stats(C.Structure):
 class Law(C.Structure):
     _fields_ = [("seed", u32), ("tone", f64), ("shoulder_limit", f64), ("joint_limit", f64),
                 ("stiff_min", f64), ("stiff_max", f64), ("grip_min", f64), ("grip_max", f64),
-                ("slip_max", f64), ("cooldown", f64), ("attach_min", f64), ("release_below", f64)]
+                ("slip_max", f64), ("cooldown", f64), ("attach_min", f64), ("release_below", f64),
+                ("creature_base", u32), ("flags", u32)]
 
 
 EXPORTS = {

@@ -800,6 +800,7 @@ static KxLaw to_law(const kx_law *l) {
   L.shoulder_limit = l->shoulder_limit; L.joint_limit = l->joint_limit;
   L.grip_min = l->grip_min; L.grip_max = l->grip_max; L.slip_max = l->slip_max;
   L.cooldown = l->cooldown; L.attach_min = l->attach_min; L.release_below = l->release_below;
+  L.cbase = l->creature_base; L.flags = l->flags;
   return L;
 }
 

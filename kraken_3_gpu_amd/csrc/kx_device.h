@@ -334,13 +334,14 @@ __device__ __forceinline__ float kx_law_target(const KxLaw &L, uint32_t cid, uin
 // velocity and pose (tip position in global coordinates as f64), updates
 // the grip fields and the f64 cooldown. Returns event bits (attach 1,
 // release 2) for this arm.
-__device__ __forceinline__ uint32_t kx_reflex(const KxLaw &L, double hS, double adh_u, double &cd,
+__device__ __forceinline__ uint32_t kx_reflex(const KxLaw &L, double hS, double adh_u, double grip_u, double &cd,
                                               int &broken, float &bf, float &lbx_g, float &lby_g,
                                               float &lamx, float &lamy, float tvx, float tvy,
                                               double tpx, double tpy, float tang, float glax, float glay) {
   double adh = kx_jsmax(0.0, kx_jsmin(1.0, adh_u));
   if (cd > 0.0) cd = __dsub_rn(cd, hS);
-  double bfd = __dadd_rn(L.grip_min, __dmul_rn(__dsub_rn(L.grip_max, L.grip_min), adh));
+  const double ga = (L.flags & KX_LAW_FIXED_GRIP) ? kx_jsmax(0.0, kx_jsmin(1.0, grip_u)) : adh;
+  double bfd = __dadd_rn(L.grip_min, __dmul_rn(__dsub_rn(L.grip_max, L.grip_min), ga));
   if (broken) {
     if (adh > L.attach_min && cd <= 0.0) {
       double vx = tvx, vy = tvy;

@@ -10,12 +10,14 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/kxpbd.h"
 #include "kx_sched.h"
 
 #define KX_MAX_COLORS 16
 #define KX_MAX_ARMS 8
 #define KX_SLOTS_PER_ARM 16
 #define KX_SLOT_ADH 256
+#define KX_GRIP_DRAW_KEY 0xFFFFFFFFu   // step key of the once-per-tip grip limit draw
 #define KX_WAVE 64
 
 // Per-creature metadata (internal indices).
@@ -79,6 +81,7 @@ struct KxLaw {
   double comp;                   // 1/(stiff_min + tone*(stiff_max-stiff_min))
   double shoulder_limit, joint_limit;
   double grip_min, grip_max, slip_max, cooldown, attach_min, release_below;
+  uint32_t cbase, flags;         // creature key offset, KX_LAW_* flags
 };
 
 // Launchers (kx_plain.hip / kx_persist.hip). All asynchronous on `st`.

@@ -221,17 +221,19 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
         int k = t_mot_k[j];
         float4 dk = dyn[k];
         dk.x = a_law;
-        dk.y = kx_law_target(L, (uint32_t)cid, (uint32_t)key, t_mot_slot[j]);
+        dk.y = kx_law_target(L, (uint32_t)cid + L.cbase, (uint32_t)key, t_mot_slot[j]);
         dyn[k] = dk;
       }
       // adhesion reflex (core.js:199-218), one lane per arm
       if (act && sub < T.narms) {
-        double adh = kx_ctl_u(L.seed, (uint32_t)cid, (uint32_t)key, KX_SLOT_ADH + sub);
+        const uint32_t ck = (uint32_t)cid + L.cbase;
+        double adh = kx_ctl_u(L.seed, ck, (uint32_t)key, KX_SLOT_ADH + sub);
+        double gu = (L.flags & KX_LAW_FIXED_GRIP) ? kx_ctl_u(L.seed, ck, KX_GRIP_DRAW_KEY, KX_SLOT_ADH + sub) : 0.0;
         int g = t_arm_grip[sub], t = t_arm_tip[sub];
         float4 dk = dyn[g], lk = lamr[g], tp = prv[t], tv = vel[t], cr0 = t_crec[2 * g];   // {lax, lay, -, -}
         int bk = __float_as_int(lk.w);
         float f_bf = lk.z, gx = dk.z, gy = dk.w, lx = lk.x, ly = lk.y;
-        uint32_t e = kx_reflex(L, hS, adh, my_cd, bk, f_bf, gx, gy, lx, ly, tv.x, tv.y, (double)tp.x, (double)tp.y,
+        uint32_t e = kx_reflex(L, hS, adh, gu, my_cd, bk, f_bf, gx, gy, lx, ly, tv.x, tv.y, (double)tp.x, (double)tp.y,
                                tp.z, cr0.x, cr0.y);
         dk.z = gx; dk.w = gy;
         dyn[g] = dk;

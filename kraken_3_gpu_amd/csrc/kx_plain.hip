@@ -25,11 +25,13 @@ __global__ __launch_bounds__(PLAIN_BLOCK) void kx_k_arms(KxDev d, KxLaw L, int s
   }
   int c = lo, arm = a - d.cr[c].arm_off;
   int tip = d.arm_tip[a], g = d.arm_grip[a];
-  double adh = kx_ctl_u(L.seed, (uint32_t)c, (uint32_t)step_key, KX_SLOT_ADH + arm);
+  const uint32_t ck = (uint32_t)c + L.cbase;
+  double adh = kx_ctl_u(L.seed, ck, (uint32_t)step_key, KX_SLOT_ADH + arm);
+  double gu = (L.flags & KX_LAW_FIXED_GRIP) ? kx_ctl_u(L.seed, ck, KX_GRIP_DRAW_KEY, KX_SLOT_ADH + arm) : 0.0;
   double &cd = d.cd[c * KX_MAX_ARMS + arm];
   int broken = d.broken[g];
   float bf = d.bf[g], lbx = d.lbx[g], lby = d.lby[g], lx = d.lamx[g], ly = d.lamy[g];
-  uint32_t ev = kx_reflex(L, d.hd * d.S, adh, cd, broken, bf, lbx, lby, lx, ly, d.vx[tip], d.vy[tip],
+  uint32_t ev = kx_reflex(L, d.hd * d.S, adh, gu, cd, broken, bf, lbx, lby, lx, ly, d.vx[tip], d.vy[tip],
                           (double)d.px[tip], (double)d.py[tip], d.ang[tip], d.lax[g], d.lay[g]);
   d.broken[g] = broken; d.bf[g] = bf; d.lbx[g] = lbx; d.lby[g] = lby; d.lamx[g] = lx; d.lamy[g] = ly;
   if (ev && d.events) atomicOr(&d.events[(size_t)ev_row * d.nCr + c], ev << (3 * arm));
@@ -42,7 +44,7 @@ __global__ __launch_bounds__(PLAIN_BLOCK) void kx_k_motors(KxDev d, KxLaw L, int
   if (step_base) step_key += step_base[0];
   int slot = d.cslot[k];
   if (slot < 0) return;
-  d.target[k] = kx_law_target(L, (uint32_t)d.ccr[k], (uint32_t)step_key, slot);
+  d.target[k] = kx_law_target(L, (uint32_t)d.ccr[k] + L.cbase, (uint32_t)step_key, slot);
   d.comp[k] = (float)L.comp;
 }
 

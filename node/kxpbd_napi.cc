@@ -480,6 +480,11 @@ napi_value Rollout(napi_env env, napi_callback_info info) {
   get_f64(env, a[1], "cooldown", &w->law.cooldown, 0.05);
   get_f64(env, a[1], "attachMin", &w->law.attach_min, 0.08);
   get_f64(env, a[1], "releaseBelow", &w->law.release_below, 0.04);
+  int32_t cbase = 0, flags = 0;
+  get_i32(env, a[1], "creatureBase", &cbase, 0);
+  get_i32(env, a[1], "flags", &flags, 0);
+  w->law.creature_base = (uint32_t)cbase;
+  w->law.flags = (uint32_t)flags;
   napi_get_value_int32(env, a[2], &w->step0);
   napi_get_value_int32(env, a[3], &w->nsteps);
   std::vector<napi_value> keep{a[0]};

@@ -89,6 +89,8 @@ void or_apply_control(OrWorld *w, int ci, double tone, const double *shoulder,
  * Appends reflex events to the event log. */
 void or_apply_law(OrWorld *w, uint32_t seed, uint32_t step_index);
 void or_apply_law_one(OrWorld *w, uint32_t seed, uint32_t step_index, int creature, uint32_t key);
+/* the same law with p5 = {grip_min, grip_max, attach_min, release_below, fixed_grip} */
+void or_apply_law_ex(OrWorld *w, uint32_t seed, uint32_t step_index, int creature, uint32_t key, const double *p5);
 
 /* step */
 void or_step(OrWorld *w);                /* S substeps + grip break (f64 or f32 emulation) */

@@ -459,27 +459,36 @@ static void log_event(OrWorld *w, int ci, int a, int kind) {
   int *e = w->ev + 4 * w->nev++;
   e[0] = w->step_counter; e[1] = ci; e[2] = a; e[3] = kind;
 }
-/* adhesion reflex for one arm (core.js:199-218) */
-static void reflex(OrWorld *w, int ci, int a, double adhesion) {
+/* adhesion reflex for one arm (core.js:199-218); p carries the grip range
+ * and thresholds (the reference's constants unless a test law overrides
+ * them) and grip_u the once-per-tip draw used when p->fixed_grip is set */
+typedef struct { double grip_min, grip_max, attach_min, release_below; int fixed_grip; } OrReflexParams;
+static const OrReflexParams REF_REFLEX = {B_GRIP_MIN, B_GRIP_MAX, 0.08, 0.04, 0};
+
+static void reflex_p(OrWorld *w, int ci, int a, double adhesion, double grip_u, const OrReflexParams *p) {
   OrArm *arm = &w->cr[ci].arm[a];
   arm->adhesion = clamp01(adhesion);
   if (arm->cd > 0) arm->cd -= w->h * w->S;
   OrCons *g = CONS_ID(w, arm->grip);
+  double ga = p->fixed_grip ? clamp01(grip_u) : arm->adhesion;
+  double bf = p->grip_min + (p->grip_max - p->grip_min) * ga;
   if (g->broken) {
-    if (arm->adhesion > 0.08 && arm->cd <= 0) {
+    if (arm->adhesion > p->attach_min && arm->cd <= 0) {
       double slip = or_js_hypot(VX[arm->tip], VY[arm->tip]);
       if (slip < B_SLIP_MAX) {
-        or_attach_grip(w, ci, a, adhesion_to_break_force(arm->adhesion));
+        or_attach_grip(w, ci, a, bf);
         log_event(w, ci, a, 0);
       } else {
         arm->cd = B_COOLDOWN;
       }
     }
   } else {
-    g->breakForce = adhesion_to_break_force(arm->adhesion);
-    if (arm->adhesion < 0.04) { g->broken = 1; g->force = 0; log_event(w, ci, a, 1); }
+    g->breakForce = bf;
+    if (arm->adhesion < p->release_below) { g->broken = 1; g->force = 0; log_event(w, ci, a, 1); }
   }
 }
+static void reflex(OrWorld *w, int ci, int a, double adhesion) { reflex_p(w, ci, a, adhesion, 0.0, &REF_REFLEX); }
+
 void or_apply_control(OrWorld *w, int ci, double tone, const double *shoulder, const double *curl,
                       const double *adhesion) {
   OrCreature *K = &w->cr[ci];
@@ -494,8 +503,17 @@ void or_apply_control(OrWorld *w, int ci, double tone, const double *shoulder, c
   }
 }
 /* the control law for creature ci, keyed in the PRNG as creature `key`
- * (the batch index when ci lives in a sub-world) */
+ * (the batch index when ci lives in a sub-world); p = {grip_min, grip_max,
+ * attach_min, release_below, fixed_grip} */
+static void apply_law_p(OrWorld *w, uint32_t seed, uint32_t step, int ci, uint32_t key, const OrReflexParams *p);
+void or_apply_law_ex(OrWorld *w, uint32_t seed, uint32_t step, int ci, uint32_t key, const double *p5) {
+  OrReflexParams p = {p5[0], p5[1], p5[2], p5[3], p5[4] != 0.0};
+  apply_law_p(w, seed, step, ci, key, &p);
+}
 void or_apply_law_one(OrWorld *w, uint32_t seed, uint32_t step, int ci, uint32_t key) {
+  apply_law_p(w, seed, step, ci, key, &REF_REFLEX);
+}
+static void apply_law_p(OrWorld *w, uint32_t seed, uint32_t step, int ci, uint32_t key, const OrReflexParams *p) {
   double comp = tone_to_compliance(0.5);
   OrCreature *K = &w->cr[ci];
   for (int a = 0; a < K->arms; a++) {
@@ -508,7 +526,8 @@ void or_apply_law_one(OrWorld *w, uint32_t seed, uint32_t step, int ci, uint32_t
       c->compliance = comp;
       c->target = or_js_max(-lim, or_js_min(lim, t));
     }
-    reflex(w, ci, a, or_ctl_u(seed, key, step, OR_SLOT_ADH + a));
+    reflex_p(w, ci, a, or_ctl_u(seed, key, step, OR_SLOT_ADH + a),
+             p->fixed_grip ? or_ctl_u(seed, key, 0xFFFFFFFFu, OR_SLOT_ADH + a) : 0.0, p);
   }
 }
 void or_apply_law(OrWorld *w, uint32_t seed, uint32_t step) {

@@ -67,6 +67,7 @@ def lib():
             "or_apply_control": (None, [P, i, d, dp, dp, dp]),
             "or_apply_law": (None, [P, u32, u32]),
             "or_apply_law_one": (None, [P, u32, u32, i, u32]),
+            "or_apply_law_ex": (None, [P, u32, u32, i, u32, C.POINTER(C.c_double)]),
             "or_step": (None, [P]),
             "or_substep": (None, [P]),
             "or_solve_collisions": (None, [P]),
@@ -253,6 +254,11 @@ class World:
 
     def apply_law(self, seed, step):
         lib().or_apply_law(self.p, seed & 0xFFFFFFFF, step)
+
+    def apply_law_ex(self, seed, step, creature, key, grip_min, grip_max, attach_min, release_below, fixed_grip):
+        """The control law with a test grip range / thresholds for one creature."""
+        p = (C.c_double * 5)(grip_min, grip_max, attach_min, release_below, 1.0 if fixed_grip else 0.0)
+        lib().or_apply_law_ex(self.p, seed & 0xFFFFFFFF, step, creature, key & 0xFFFFFFFF, p)
 
     def step(self):
         lib().or_step(self.p)

@@ -308,6 +308,58 @@ def test_full_size_uniform_properties_and_subset_parity():
     assert err[ok].max() <= TOL_PX
 
 
+def test_stress_map_full_size_properties_and_subset_parity():
+    """65 536 creatures on the 4096 x 4096 corridor map (1 461 rects, the map
+    read from HBM, not LDS), every tip adhesive with a once-drawn break limit
+    in [50, 200] N: limits and NaN over a rollout, then one step of a
+    64-creature subset against the oracle under the same law."""
+    from helpers import creature_of_bodies
+    seed = 41
+    w = W.world_stress(65536, seed=1)
+    law = W.default_law(seed, grip_min=50.0, grip_max=200.0)
+    law.attach_min, law.release_below = -1.0, -1.0
+    law.flags = K.KX_LAW_FIXED_GRIP
+    g = G.create(w, path=K.KX_PATH_PERSISTENT)
+    G.upload(g)
+    info = G.persist_info(g)
+    say(f"[case] stress 65536 rollout 30, launch shape {info}")
+    assert info["map_words"] == 0                  # this map does not fit in LDS
+    ev = G.rollout(g, law, 0, 30, events=True)
+    st = G.download(g)
+    assert st["nan"] == 0
+    assert st["maxSpeed"] <= 360 * 1.001 and st["maxAngSpeed"] <= 18 * 1.001
+    attaches = int(((ev & 0o111111) != 0).sum())
+    say(f"    stress: {attaches} creature-steps with attach events, grips held "
+        f"{int((w.broken[w.type == K.KX_GRIP] == 0).sum())}")
+    assert attaches > 0
+    _pull_state(g)
+    grips = w.type == K.KX_GRIP
+    held = w.breakForce[grips & (w.broken == 0)]
+    assert held.size and held.min() >= 50.0 and held.max() <= 200.0
+    sub = list(range(0, 65536, 1024))
+    o, bidx, cidx = oracle_subset(w, sub)
+    load_state_subset(o, w, sub, bidx, cidx)
+    o.set_step_counter(30)
+    o.clear_events()
+    for local, key in enumerate(sub):
+        o.apply_law_ex(seed, 30, local, key, 50.0, 200.0, -1.0, -1.0, True)
+    o.step()
+    say("[case] stress 65536 one more step")
+    ev1 = G.rollout(g, law, 30, 1, events=True)
+    G.download(g)
+    ev_gpu = {(0, sub.index(c), a, k) for (_, c, a, k) in decode_events(ev1) if c in sub}
+    ev_or = {(0, c, a, k) for (_, c, a, k) in o.events()}
+    flips = {e[1] for e in ev_gpu ^ ev_or}
+    b = o.bodies()
+    err = np.maximum(np.abs(w.px[bidx] - b["px"]), np.abs(w.py[bidx] - b["py"]))
+    crb = np.repeat(np.arange(len(sub)), 43)
+    ok = ~np.isin(crb, list(flips))
+    say(f"    stress subset max|dpos|={err[ok].max():.3e} p99={np.percentile(err[ok], 99):.3e} flips={sorted(flips)}")
+    assert len(flips) <= 1
+    assert err[ok].max() <= TOL_PX
+    G.destroy(g)
+
+
 def _apply_law_keyed(o, seed, step, keys):
     """or_apply_law keys the PRNG by the oracle's own creature index; a
     subset needs the batch index, so remap through a seed-preserving loop."""
