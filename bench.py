@@ -219,15 +219,83 @@ def bench_stress(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_uneven(args):
+    """8192 creatures of seeded morphology (4-8 arms x 5-9 segments, mantle
+    ring 6-10) in one persistent launch: lanes per creature tuned per
+    morphology bucket, waves per workgroup and steps per launch tuned on the
+    whole batch, then 1500 steps replayed from a captured hipGraph."""
+    from kraken_3_gpu_amd import _lib as K
+    from kraken_3_gpu_amd import autotune as A
+    from kraken_3_gpu_amd import gpuworld as G
+    from kraken_3_gpu_amd import world as W
+    steps = args.steps or 1500
+    law = law_for("uneven", 2024)
+    w = W.world_uneven(args.creatures or 8192, seed=1)
+    n_cr = w.n_creatures
+    g, rep = A.tune_buckets(w, law, tiles=(25, 50, 100) if not args.chunk_set else (args.chunk,))
+    tile = rep["steps_per_launch"]
+    steps = -(-steps // tile) * tile
+    G.upload(g)
+    G.rollout_graph(g, law, 0, tile, max(1, args.warmup // tile))     # capture + warm
+    G.sync(g)
+    g_att, r_near = warm_accounting(g, w)
+    warm = max(1, args.warmup // tile) * tile
+    G.sync(g)
+    t0 = time.perf_counter()
+    G.rollout_graph(g, law, warm, tile, steps // tile)
+    G.sync(g)
+    wall = time.perf_counter() - t0
+    dev_ms, launches = G.last_timing(g)
+    st = G.download(g)
+    launch_ms = dev_ms / launches
+    F = flops_per_step(w, g_att, min(r_near, 64.0))
+    ach = F * tile / (launch_ms * 1e-3) / 1e12
+    lb = algorithmic_launch_bytes(w, tile)
+    value = n_cr * steps / wall
+    plain = None
+    if not args.no_plain:
+        gp = G.create(w, path=K.KX_PATH_PLAIN)
+        G.set_plain_sdf(gp, True)
+        G.upload(gp)
+        G.rollout(gp, law, 0, 2)
+        G.sync(gp)
+        t1 = time.perf_counter()
+        G.rollout(gp, law, 2, args.plain_steps)
+        G.sync(gp)
+        pwall = time.perf_counter() - t1
+        plain = {"value": n_cr * args.plain_steps / pwall, "steps": args.plain_steps, "sdf": "grid"}
+        G.destroy(gp)
+    sizes = np.array([a * s for a, s, _ in w.morph])
+    out = {
+        "metric": "creature-steps/s, uneven batch in one hipGraph-captured persistent launch",
+        "value": value, "unit": "creature-steps/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"uneven batch: {n_cr} creatures, seeded morphology 4-8 arms x 5-9 segments, "
+                               f"mantle ring 6-10, makeMap 1200x800 seed 1, {steps} steps from a hipGraph",
+                   "creatures": n_cr, "bodies": int(w.n), "constraints": int(len(w.type)),
+                   "arm_segments_range": [int(sizes.min()), int(sizes.max())], "steps_per_launch": tile},
+        "roofline": {"bound": "valu", "achieved": ach, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": ach / FP32_PEAK_TFLOPS, "launch_ms": launch_ms,
+                     "hbm_algorithmic_gbs": lb / (launch_ms * 1e-3) / 1e9, "hbm_peak_gbs": HBM_PEAK_GBS},
+        "autotune": rep, "launch_shape": G.persist_info(g), "final_stats": st,
+    }
+    if plain:
+        out["plain"] = plain
+        out["speedup_vs_plain"] = value / plain["value"]
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["uniform", "stress"], default="uniform",
-                    help="uniform batch (the headline line) or the stress run")
+    ap.add_argument("--workload", choices=["uniform", "uneven", "stress"], default="uniform",
+                    help="uniform batch (the headline line), the uneven batch or the stress run")
     ap.add_argument("--steps", type=int, default=0, help="timed steps (default 1000; stress 2000)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--creatures", type=int, default=0, help="creatures (default 4096; stress 65536)")
-    ap.add_argument("--chunk", type=int, default=50, help="simulation steps per persistent launch")
+    ap.add_argument("--chunk", type=int, default=0,
+                    help="simulation steps per persistent launch (default 50; uneven: tuned)")
     ap.add_argument("--plain-steps", type=int, default=100)
     ap.add_argument("--lanes", type=int, default=0, help="lanes per creature (0 = default)")
     ap.add_argument("--wpb", type=int, default=0, help="wavefronts per workgroup (0 = default)")
@@ -236,6 +304,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
+    args.chunk_set = args.chunk > 0
+    args.chunk = args.chunk or 50
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
@@ -246,6 +316,9 @@ def main():
 
     if args.workload == "stress":
         bench_stress(args)
+        return
+    if args.workload == "uneven":
+        bench_uneven(args)
         return
     args.steps = args.steps or 1000
     args.creatures = args.creatures or 4096

@@ -190,9 +190,20 @@ int kx_schedule_info(const kx_handle *h, int32_t *rounds, int32_t max);
  * per workgroup, [1] workgroups, [2] resident workgroups per CU, [3]
  * workgroups per CU the launch needs, [4] LDS words of the staged map (0:
  * map read from HBM), [5] cell-bitmap words, [6] template words, [7]
- * per-creature LDS words. Returns the number of fields (8). No counterpart
- * in the reference (its path has no persistent kernel). */
+ * per-creature LDS words, [8] 1 if lane counts differ between workgroups.
+ * Returns the number of fields (9). No counterpart in the reference (its path
+ * has no persistent kernel). */
 int kx_persist_info(const kx_handle *h, int32_t *info, int32_t max);
+/* Morphology templates (creatures with identical structure share one):
+ * t < 0 returns the template count; else out6 = {bodies, constraints, points,
+ * creatures, rounds per iteration, lanes per creature} of template t. */
+int kx_template_info(const kx_handle *h, int32_t t, int32_t *out6);
+/* Template index of every creature (creature order of the description). */
+int kx_creature_templates(const kx_handle *h, int32_t *tmpl_of_creature);
+/* Lanes per creature per template (0 = the handle's lanes_per_creature), so
+ * one persistent launch runs every morphology bucket at its own shape; results
+ * do not depend on it. n must equal the template count. */
+int kx_set_template_lanes(kx_handle *h, int32_t n, const int32_t *lanes);
 /* plain path only: 0 = linear scan over every rect per query (the shape of
  * src/gpu/solver.wgsl:114-147), 1 = the exact cull grid the persistent
  * path always uses */

@@ -85,6 +85,9 @@ EXPORTS = {
     "kx_set_path": (i32, [C.c_void_p, i32, i32, i32]),
     "kx_schedule_info": (i32, [C.c_void_p, i32p, i32]),
     "kx_persist_info": (i32, [C.c_void_p, i32p, i32]),
+    "kx_template_info": (i32, [C.c_void_p, i32, i32p]),
+    "kx_set_template_lanes": (i32, [C.c_void_p, i32, i32p]),
+    "kx_creature_templates": (i32, [C.c_void_p, i32p]),
     "kx_set_plain_sdf": (i32, [C.c_void_p, i32]),
     "kx_builder_new": (C.c_void_p, []),
     "kx_builder_free": (None, [C.c_void_p]),

@@ -84,7 +84,9 @@ struct kx_handle {
   std::vector<int> tmpl_rounds;
   KxPersistCfg pc{};
   void *pc_bufs[4] = {nullptr, nullptr, nullptr, nullptr};
-  int pinfo[8] = {};     // see kx_persist_info
+  int pinfo[9] = {};     // see kx_persist_info
+  std::vector<int> tmpl_lanes;   // per template lanes per creature, 0 = the handle's G
+  int launch_G = 16;             // lane count the uniform kernel is launched with
   // scratch host staging
   std::vector<float> hf;
   std::vector<int> hi;
@@ -250,16 +252,24 @@ static int build_grid(kx_handle *h, const float *rects, int nR, float rmax) {
   return 0;
 }
 
-// Pack every template's schedule for G lanes, map creatures onto waves
-// (64/G creatures of one template per wave, workgroups template-uniform).
+// Lanes per creature of template t: its own setting, else the handle's.
+static int tmpl_lanes_of(const kx_handle *h, size_t t) {
+  return t < h->tmpl_lanes.size() && h->tmpl_lanes[t] ? h->tmpl_lanes[t] : h->G;
+}
+
+// Pack every template's schedule for its lane count G_t and map creatures
+// onto workgroups: WPB wavefronts of 64/G_t creatures of one template each.
 static int build_persist(kx_handle *h) {
-  const int G = h->G, CPW = KX_WAVE / G, WPB = h->wpb;
+  const int WPB = h->wpb;
   std::vector<uint32_t> pool;
   std::vector<KxTmpl> T;
   int words = 0, NB = 1, NC = 1, NP = 1;
   h->tmpl_rounds.clear();
-  for (auto &t : h->tmpl_host) {
-    KxTmpl k = kx_pack_template(t, G, pool);
+  for (size_t t = 0; t < h->tmpl_host.size(); t++) {
+    KxTmpl k = kx_pack_template(h->tmpl_host[t], tmpl_lanes_of(h, t), pool);
+    // creature regions: 16-byte records, regions offset by 16 bytes modulo
+    // 128 so neighbouring creatures' same records fall in different banks
+    k.cr_stride = (kx_persist_cr_words(k.nb, k.nc, k.np) + 31) / 32 * 32 + 4;
     T.push_back(k);
     words = std::max(words, k.words);
     NB = std::max(NB, k.nb); NC = std::max(NC, k.nc); NP = std::max(NP, k.np);
@@ -267,68 +277,98 @@ static int build_persist(kx_handle *h) {
   }
   std::vector<std::vector<int>> members(T.size());
   for (int c = 0; c < h->nCr; c++) members[h->cr_tmpl[c]].push_back(c);
-  std::vector<int> wave_cr, wave_tmpl;
-  for (size_t t = 0; t < T.size(); t++) {
+  std::vector<int> wave_cr, wg(0);
+  std::vector<int> wg_tmpl, wg_lanes, wg_off;
+  int max_cpw = 1, lanes0 = -1, cr_words = 0;
+  bool mixed = false;
+  // Workgroups are issued longest first (per-wave cost ~ rounds per
+  // iteration plus collision bodies per lane), so the dispatcher's dynamic
+  // assignment ends on short workgroups rather than on the largest
+  // morphologies; consecutive workgroups of similar cost also spread evenly
+  // over the XCDs, which take blockIdx round-robin.
+  std::vector<int> order(T.size());
+  for (size_t t = 0; t < T.size(); t++) order[t] = (int)t;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    return 8 * T[x].R + 4 * T[x].maxbl > 8 * T[y].R + 4 * T[y].maxbl;
+  });
+  for (int t : order) {
     auto &m = members[t];
     if (m.empty()) continue;
-    int nw = (int)((m.size() + CPW - 1) / CPW);
-    nw = (nw + WPB - 1) / WPB * WPB;
-    for (int wv = 0; wv < nw; wv++) {
-      wave_tmpl.push_back((int)t);
-      for (int j = 0; j < CPW; j++) {
-        size_t idx = (size_t)wv * CPW + j;
+    const int Gt = tmpl_lanes_of(h, t), CPW = KX_WAVE / Gt;
+    if (lanes0 < 0) lanes0 = Gt;
+    mixed = mixed || Gt != lanes0;
+    max_cpw = std::max(max_cpw, CPW);
+    const int per_wg = WPB * CPW;
+    cr_words = std::max(cr_words, per_wg * T[t].cr_stride);
+    const int nwg_t = (int)((m.size() + per_wg - 1) / per_wg);
+    for (int b = 0; b < nwg_t; b++) {
+      wg_tmpl.push_back((int)t);
+      wg_lanes.push_back(Gt);
+      wg_off.push_back((int)wave_cr.size());
+      for (int j = 0; j < per_wg; j++) {
+        size_t idx = (size_t)b * per_wg + j;
         wave_cr.push_back(idx < m.size() ? m[idx] : -1);
       }
     }
   }
-  // creature regions: 16-byte records, regions offset by 16 bytes modulo
-  // 128 so neighbouring creatures' same records fall in different banks
-  int stride = (kx_persist_cr_words(NB, NC, NP) + 31) / 32 * 32 + 4;
+  const int nwg = (int)wg_tmpl.size();
+  const int Gk = mixed ? 0 : (lanes0 > 0 ? lanes0 : h->G);    // kernel: 0 = mixed-lane kernel
+  const int stride = (kx_persist_cr_words(NB, NC, NP) + 31) / 32 * 32 + 4;   // largest region (reported)
   int bmw = (h->d.cell_words + 3) / 4 * 4;
   if ((size_t)bmw * 4 > 32 * 1024) bmw = 0;      // huge grids: no LDS bitmap
-  // Stage the whole map (cell offsets, ids, rects) in LDS when that costs no
-  // occupancy the launch can use: with nwg workgroups on nCU CUs, at most
-  // ceil(nwg / nCU) of them are ever resident on one CU.
+  // Stage the whole map (rects of the cell entries, 16-bit cell offsets) in
+  // LDS when that costs no occupancy the launch can use: with nwg workgroups
+  // on nCU CUs, at most ceil(nwg / nCU) of them are ever resident on one CU.
   int mcells = h->d.gnx * h->d.gny, mids = h->n_cell_ids;
   int mapw = 4 * mids + ((mcells + 2) / 2 + 3) / 4 * 4;
   if (mids > 65535) mapw = 0;                 // offsets are staged as 16-bit
-  const size_t base = ((size_t)words + bmw + (size_t)WPB * CPW * stride) * 4;
+  const size_t base = ((size_t)words + bmw + (size_t)cr_words) * 4;
   if (base > 160 * 1024)
-    return fail(KX_ENOTSUP, "persistent shape G=%d x %d waves needs %zu B of LDS (> 160 KiB)", G, WPB, base);
-  const int nwg = ((int)wave_tmpl.size() + WPB - 1) / WPB;
+    return fail(KX_ENOTSUP, "persistent shape (%d creatures per wave) x %d waves needs %zu B of LDS (> 160 KiB)",
+                max_cpw, WPB, base);
   int dev = 0, ncu = 1;
   HIPCHK(hipGetDevice(&dev));
   HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  const int need = (nwg + std::max(ncu, 1) - 1) / std::max(ncu, 1);
-  const int occ0 = kx_persist_occupancy(G, WPB, base);
+  const int need = (std::max(nwg, 1) + std::max(ncu, 1) - 1) / std::max(ncu, 1);
+  const int occ0 = kx_persist_occupancy(Gk, WPB, base);
   const size_t with_map = base + (size_t)mapw * 4;
-  const int occ1 = with_map <= 160 * 1024 ? kx_persist_occupancy(G, WPB, with_map) : 0;
+  const int occ1 = with_map <= 160 * 1024 ? kx_persist_occupancy(Gk, WPB, with_map) : 0;
   if (!mapw || std::min(occ1, need) < std::min(occ0, need)) mapw = 0;
   const size_t lds = base + (size_t)mapw * 4;
   h->pinfo[0] = (int)lds; h->pinfo[1] = nwg; h->pinfo[2] = mapw ? occ1 : occ0; h->pinfo[3] = need;
   h->pinfo[4] = mapw; h->pinfo[5] = bmw; h->pinfo[6] = words; h->pinfo[7] = stride;
+  h->pinfo[8] = mixed ? 1 : 0;
+  std::vector<int> tables;
+  tables.insert(tables.end(), wg_tmpl.begin(), wg_tmpl.end());
+  tables.insert(tables.end(), wg_lanes.begin(), wg_lanes.end());
+  tables.insert(tables.end(), wg_off.begin(), wg_off.end());
   for (void *&p : h->pc_bufs) { if (p) (void)hipFree(p); p = nullptr; }
   HIPCHK(hipMalloc(&h->pc_bufs[0], std::max<size_t>(pool.size(), 1) * 4));
   HIPCHK(hipMalloc(&h->pc_bufs[1], std::max<size_t>(T.size(), 1) * sizeof(KxTmpl)));
   HIPCHK(hipMalloc(&h->pc_bufs[2], std::max<size_t>(wave_cr.size(), 1) * 4));
-  HIPCHK(hipMalloc(&h->pc_bufs[3], std::max<size_t>(wave_tmpl.size(), 1) * 4));
+  HIPCHK(hipMalloc(&h->pc_bufs[3], std::max<size_t>(tables.size(), 1) * 4));
   HIPCHK(hipMemcpy(h->pc_bufs[0], pool.data(), pool.size() * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(h->pc_bufs[1], T.data(), T.size() * sizeof(KxTmpl), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(h->pc_bufs[2], wave_cr.data(), wave_cr.size() * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(h->pc_bufs[3], wave_tmpl.data(), wave_tmpl.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->pc_bufs[3], tables.data(), tables.size() * 4, hipMemcpyHostToDevice));
   KxPersistCfg &P = h->pc;
   P.pool = (const uint32_t *)h->pc_bufs[0];
   P.tmpl = (const KxTmpl *)h->pc_bufs[1];
   P.wave_cr = (const int *)h->pc_bufs[2];
-  P.wave_tmpl = (const int *)h->pc_bufs[3];
+  P.wg_tmpl = (const int *)h->pc_bufs[3];
+  P.wg_lanes = P.wg_tmpl + nwg;
+  P.wg_cr_off = P.wg_tmpl + 2 * nwg;
+  P.nwg = nwg;
+  P.max_cpw = max_cpw;
+  P.mixed = mixed ? 1 : 0;
   P.tmpl_words = words;
   P.bm_words = bmw;
   P.map_words = mapw;
   P.map_cells = mcells;
   P.map_ids = mids;
-  P.cr_stride = stride;
-  P.nwaves = (int)wave_tmpl.size();
+  P.cr_words = cr_words;
   P.NB = NB; P.NC = NC; P.NP = NP;
+  h->launch_G = mixed ? h->G : (lanes0 > 0 ? lanes0 : h->G);
   return 0;
 }
 
@@ -811,7 +851,7 @@ static int run_steps(kx_handle *h, const KxLaw *law, int step0, const int *step_
   int with_forces = h->forces_pending ? 1 : 0;
   h->forces_pending = false;
   if (h->path == KX_PATH_PERSISTENT) {
-    int r = kx_persist_launch(h->G, h->wpb, d, law, step0, step_base, nsteps, 0, with_forces, h->pc, h->st);
+    int r = kx_persist_launch(h->launch_G, h->wpb, d, law, step0, step_base, nsteps, 0, with_forces, h->pc, h->st);
     if (r) return fail(KX_EDEVICE, "persistent launch failed: %s", hipGetErrorString(hipGetLastError()));
     launches = 1;
   } else {
@@ -879,7 +919,7 @@ extern "C" int kx_rollout_graph(kx_handle *h, const kx_law *law, int32_t step0, 
     if (h->path == KX_PATH_PERSISTENT) {
       KxDev d = h->d;
       d.events = nullptr;
-      r = kx_persist_launch(h->G, h->wpb, d, &L, 0, h->step_base, nsteps, 0, 0, h->pc, h->st);
+      r = kx_persist_launch(h->launch_G, h->wpb, d, &L, 0, h->step_base, nsteps, 0, 0, h->pc, h->st);
     } else {
       KxDev d = h->d;
       d.events = nullptr;
@@ -949,8 +989,51 @@ extern "C" int kx_schedule_info(const kx_handle *h, int32_t *rounds, int32_t max
 extern "C" int kx_persist_info(const kx_handle *h, int32_t *info, int32_t max) {
   if (!h || !info) return fail(KX_EINVAL, "null argument");
   if (!h->persist_ok) return fail(KX_ENOTSUP, "persistent path unavailable: %s", h->persist_why.c_str());
-  for (int i = 0; i < 8 && i < max; i++) info[i] = h->pinfo[i];
-  return 8;
+  for (int i = 0; i < 9 && i < max; i++) info[i] = h->pinfo[i];
+  return 9;
+}
+
+extern "C" int kx_template_info(const kx_handle *h, int32_t t, int32_t *out6) {
+  if (!h) return fail(KX_EINVAL, "null handle");
+  const int n = (int)h->tmpl_host.size();
+  if (t < 0) return n;
+  if (t >= n || !out6) return fail(KX_EINVAL, "template %d of %d", t, n);
+  const TmplHost &T = h->tmpl_host[t];
+  int members = 0;
+  for (int c = 0; c < h->nCr; c++) members += h->cr_tmpl[c] == t;
+  out6[0] = T.nb; out6[1] = (int)T.type.size(); out6[2] = (int)T.plx.size(); out6[3] = members;
+  out6[4] = t < (int)h->tmpl_rounds.size() ? h->tmpl_rounds[t] : 0;
+  out6[5] = tmpl_lanes_of(h, t);
+  return n;
+}
+
+extern "C" int kx_creature_templates(const kx_handle *h, int32_t *tmpl_of_creature) {
+  if (!h || !tmpl_of_creature) return fail(KX_EINVAL, "null argument");
+  for (int c = 0; c < h->nCr; c++) tmpl_of_creature[c] = h->cr_tmpl[c];
+  return h->nCr;
+}
+
+extern "C" int kx_set_template_lanes(kx_handle *h, int32_t n, const int32_t *lanes) {
+  if (!h) return fail(KX_EINVAL, "null handle");
+  if (!h->persist_ok) return fail(KX_ENOTSUP, "persistent path unavailable: %s", h->persist_why.c_str());
+  if (n != (int)h->tmpl_host.size() || (n && !lanes)) return fail(KX_EINVAL, "%d lane counts for %d templates", n,
+                                                                  (int)h->tmpl_host.size());
+  for (int t = 0; t < n; t++)
+    if (lanes[t] != 0 && lanes[t] != 8 && lanes[t] != 16 && lanes[t] != 32 && lanes[t] != 64)
+      return fail(KX_EINVAL, "template %d: lanes_per_creature %d", t, lanes[t]);
+  HIPCHK(hipStreamSynchronize(h->st));
+  std::vector<int> old = h->tmpl_lanes;
+  h->tmpl_lanes.assign(lanes, lanes + n);
+  if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; h->g_nsteps = -1; }
+  int r = build_persist(h);
+  if (r) {
+    std::string keep = g_err;
+    h->tmpl_lanes = old;
+    build_persist(h);
+    g_err = keep;
+    return r;
+  }
+  return 0;
 }
 
 // plain-path SDF mode switch (0 linear WGSL scan, 1 cull grid) — exported for

@@ -90,8 +90,12 @@ int kx_plain_step(const KxDev &d, const KxLaw *law, int step_key, const int *ste
                   int ev_row, hipStream_t st, int *launches);
 // Persistent-path launch description (see kx_persist.hip).
 struct KxPersistCfg {
-  const int *wave_cr;       // [nwaves * 64/G] creature ids, -1 = empty slot
-  const int *wave_tmpl;     // [nwaves] template of each wave (uniform per workgroup)
+  const int *wave_cr;       // creature ids of every wavefront slot, -1 = empty slot
+  const int *wg_tmpl;       // [nwg] template of each workgroup
+  const int *wg_lanes;      // [nwg] lanes per creature of each workgroup
+  const int *wg_cr_off;     // [nwg] first wave_cr slot of each workgroup (WPB * 64/lanes slots)
+  int nwg, max_cpw;         // workgroups; most creatures per wavefront of any workgroup
+  int mixed;                // lane counts differ between workgroups: launch kx_k_persist_mix
   const KxTmpl *tmpl;
   const uint32_t *pool;
   int tmpl_words;           // LDS words reserved for the template
@@ -99,12 +103,11 @@ struct KxPersistCfg {
   int map_words;            // LDS words of the staged map (after the bitmap), 0 = map stays in HBM
   int map_cells, map_ids;   // staged map: the rect of each of the map_ids cell entries (16 B each),
                             // then map_cells + 1 16-bit cell offsets
-  int cr_stride;            // LDS words per creature (16 bytes off a 128-byte multiple)
-  int nwaves;
+  int cr_words;             // LDS words of the creature regions of the largest workgroup
   int NB, NC, NP;           // per-creature array sizes in LDS (bodies, constraints, points)
 };
 int kx_persist_launch(int G, int wpb, const KxDev &d, const KxLaw *law, int step0, const int *step_base,
                       int nsteps, int ev_row0, int with_forces, const KxPersistCfg &P, hipStream_t st);
 int kx_persist_cr_words(int NB, int NC, int NP);
-int kx_persist_occupancy(int G, int wpb, size_t lds);
+int kx_persist_occupancy(int G, int wpb, size_t lds);   // G = 0: the mixed-lane kernel
 int kx_bump_counter(int *counter, int delta, hipStream_t st);

@@ -143,19 +143,20 @@ __device__ __forceinline__ bool kx_query(const KxDev &d, const KxPersistCfg &P, 
                       : kx_sdf_grid(d, wx, wy, ox, oy, hh);
 }
 
+// The body of one workgroup `bid`: WPB wavefronts of 64/G creatures of one
+// template (workgroup tables P.wg_*).
 template <int G, int WPB>
-__global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, int has_law, int step0,
-                                                             const int *step_base, int nsteps, int ev_row0,
-                                                             int with_forces, KxPersistCfg P) {
+__device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, int has_law, int step0,
+                                                const int *step_base, int nsteps, int ev_row0, int with_forces,
+                                                const KxPersistCfg &P, int bid) {
   constexpr int CPW = KX_WAVE / G;
   extern __shared__ __attribute__((aligned(16))) uint32_t smem[];
   const int lane = threadIdx.x & (KX_WAVE - 1);
   const int q = lane / G, sub = lane % G;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / KX_WAVE);
-  const int gw = blockIdx.x * WPB + w;
 
   // ── template + cell bitmap -> LDS (the only workgroup barrier) ──
-  const int tid = P.wave_tmpl[blockIdx.x * WPB];
+  const int tid = P.wg_tmpl[bid];
   const KxTmpl T = P.tmpl[tid];
   for (int i = threadIdx.x; i < T.words; i += blockDim.x) smem[i] = P.pool[T.base + i];
   uint32_t *bits = smem + P.tmpl_words;
@@ -168,7 +169,6 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
     for (int i = threadIdx.x; i <= P.map_cells; i += blockDim.x) m_off[i] = (uint16_t)d.cell_off[i];
   }
   __syncthreads();
-  if (gw >= P.nwaves) return;
 
   const float4 *t_crec = (const float4 *)(smem + T.o_crec);   // 2 per constraint
   const float4 *t_brec = (const float4 *)(smem + T.o_brec);
@@ -180,10 +180,10 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
   const int *t_mot_k = (const int *)(smem + T.o_mot_k), *t_mot_slot = (const int *)(smem + T.o_mot_slot);
   const int *t_grip_k = (const int *)(smem + T.o_grip_k), *t_grip_arm = (const int *)(smem + T.o_grip_arm);
 
-  const int cid = P.wave_cr[gw * CPW + q];
+  const int cid = P.wave_cr[P.wg_cr_off[bid] + w * CPW + q];
   const bool act = cid >= 0;
-  const KxCrLds Lo(P.NB, P.NC, P.NP);
-  uint32_t *S = smem + P.tmpl_words + P.bm_words + P.map_words + (size_t)(w * CPW + q) * P.cr_stride;
+  const KxCrLds Lo(T.nb, T.nc, T.np);
+  uint32_t *S = smem + P.tmpl_words + P.bm_words + P.map_words + (size_t)(w * CPW + q) * T.cr_stride;
   float4 *pos = (float4 *)(S + Lo.pos), *prv = (float4 *)(S + Lo.prv), *vel = (float4 *)(S + Lo.vel);
   float4 *dyn = (float4 *)(S + Lo.dyn), *lamr = (float4 *)(S + Lo.lam);
   uint32_t *evw = S + Lo.ev, *flg = S + Lo.flg;
@@ -492,6 +492,29 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
   if (act && sub < T.narms) d.cd[cid * KX_MAX_ARMS + sub] = my_cd;
 }
 
+// Every workgroup with the same lane count G.
+template <int G, int WPB>
+__global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, int has_law, int step0,
+                                                             const int *step_base, int nsteps, int ev_row0,
+                                                             int with_forces, KxPersistCfg P) {
+  kx_persist_body<G, WPB>(d, L, has_law, step0, step_base, nsteps, ev_row0, with_forces, P, blockIdx.x);
+}
+
+// Lane count per workgroup (per morphology bucket): a wave-uniform switch to
+// the body instantiated for that count.
+template <int WPB>
+__global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist_mix(KxDev d, KxLaw L, int has_law, int step0,
+                                                                 const int *step_base, int nsteps, int ev_row0,
+                                                                 int with_forces, KxPersistCfg P) {
+  const int bid = blockIdx.x;
+  switch (P.wg_lanes[bid]) {
+    case 8: kx_persist_body<8, WPB>(d, L, has_law, step0, step_base, nsteps, ev_row0, with_forces, P, bid); break;
+    case 16: kx_persist_body<16, WPB>(d, L, has_law, step0, step_base, nsteps, ev_row0, with_forces, P, bid); break;
+    case 32: kx_persist_body<32, WPB>(d, L, has_law, step0, step_base, nsteps, ev_row0, with_forces, P, bid); break;
+    default: kx_persist_body<64, WPB>(d, L, has_law, step0, step_base, nsteps, ev_row0, with_forces, P, bid); break;
+  }
+}
+
 __global__ void kx_k_bump(int *counter, int delta) {
   if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] += delta;
 }
@@ -504,8 +527,7 @@ int kx_bump_counter(int *counter, int delta, hipStream_t st) {
 template <int G, int WPB>
 static int launch_gw(const KxDev &d, const KxLaw *law, int step0, const int *step_base, int nsteps, int ev_row0,
                      int with_forces, const KxPersistCfg &P, hipStream_t st) {
-  constexpr int CPW = KX_WAVE / G;
-  size_t lds = ((size_t)P.tmpl_words + P.bm_words + P.map_words + (size_t)WPB * CPW * P.cr_stride) * 4;
+  size_t lds = ((size_t)P.tmpl_words + P.bm_words + P.map_words + (size_t)P.cr_words) * 4;
   if (lds > 160 * 1024) return -22;
   static bool attr_set = false;
   if (!attr_set) {
@@ -514,10 +536,35 @@ static int launch_gw(const KxDev &d, const KxLaw *law, int step0, const int *ste
     attr_set = true;
   }
   KxLaw L = law ? *law : KxLaw{};
-  int blocks = (P.nwaves + WPB - 1) / WPB;
-  hipLaunchKernelGGL((kx_k_persist<G, WPB>), dim3(blocks), dim3(KX_WAVE * WPB), lds, st, d, L, law ? 1 : 0, step0,
+  hipLaunchKernelGGL((kx_k_persist<G, WPB>), dim3(P.nwg), dim3(KX_WAVE * WPB), lds, st, d, L, law ? 1 : 0, step0,
                      step_base, nsteps, ev_row0, with_forces, P);
   return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+template <int WPB>
+static int launch_mix(const KxDev &d, const KxLaw *law, int step0, const int *step_base, int nsteps, int ev_row0,
+                      int with_forces, const KxPersistCfg &P, hipStream_t st) {
+  size_t lds = ((size_t)P.tmpl_words + P.bm_words + P.map_words + (size_t)P.cr_words) * 4;
+  if (lds > 160 * 1024) return -22;
+  static bool attr_set = false;
+  if (!attr_set) {
+    (void)hipFuncSetAttribute((const void *)kx_k_persist_mix<WPB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024);
+    attr_set = true;
+  }
+  KxLaw L = law ? *law : KxLaw{};
+  hipLaunchKernelGGL((kx_k_persist_mix<WPB>), dim3(P.nwg), dim3(KX_WAVE * WPB), lds, st, d, L, law ? 1 : 0, step0,
+                     step_base, nsteps, ev_row0, with_forces, P);
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+template <int WPB>
+static int occ_mix(size_t lds) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void *)kx_k_persist_mix<WPB>, KX_WAVE * WPB, lds) !=
+      hipSuccess)
+    return 0;
+  return n;
 }
 
 template <int G, int WPB>
@@ -532,6 +579,7 @@ static int occ_gw(size_t lds) {
 // Resident workgroups per CU for shape (G, wpb) with `lds` bytes of dynamic
 // LDS each (0 when the shape is not instantiated or the query fails).
 int kx_persist_occupancy(int G, int wpb, size_t lds) {
+  if (G == 0) return wpb == 1 ? occ_mix<1>(lds) : wpb == 2 ? occ_mix<2>(lds) : wpb == 4 ? occ_mix<4>(lds) : 0;
 #define KX_OCC(g)                                   \
   case g:                                           \
     return wpb == 1 ? occ_gw<g, 1>(lds)             \
@@ -554,6 +602,14 @@ static int launch_g(int wpb, const KxDev &d, const KxLaw *law, int step0, const 
 
 int kx_persist_launch(int G, int wpb, const KxDev &d, const KxLaw *law, int step0, const int *step_base,
                       int nsteps, int ev_row0, int with_forces, const KxPersistCfg &P, hipStream_t st) {
+  if (P.mixed) {
+    switch (wpb) {
+      case 1: return launch_mix<1>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
+      case 2: return launch_mix<2>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
+      case 4: return launch_mix<4>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
+      default: return -22;
+    }
+  }
   switch (G) {
     case 8: return launch_g<8>(wpb, d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
     case 16: return launch_g<16>(wpb, d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);

@@ -9,6 +9,7 @@
 // template pool. `words` words starting at pool[base] are copied to LDS.
 struct KxTmpl {
   int base, words;
+  int cr_stride;                  // LDS words per creature region of this template (set by the host)
   int nb, nc, np, narms, nmot, ngrip, R, maxbl;
   int o_crec, o_brec, o_prec;     // 16-byte records (see kx_pack_template)
   // o_sched: groups of four rounds, per lane four packed slots

@@ -283,14 +283,41 @@ def schedule_rounds(g):
 
 
 PERSIST_INFO = ("lds_bytes", "workgroups", "resident_per_cu", "needed_per_cu", "map_words", "bitmap_words",
-                "template_words", "creature_words")
+                "template_words", "creature_words", "mixed_lanes")
 
 
 def persist_info(g):
     """Launch shape of the persistent path for the handle's (lanes, waves)."""
-    buf = np.zeros(8, dtype=np.int32)
-    K.check(K.lib().kx_persist_info(g.p, K.ptr(buf, C.c_int32), 8))
+    buf = np.zeros(9, dtype=np.int32)
+    K.check(K.lib().kx_persist_info(g.p, K.ptr(buf, C.c_int32), 9))
     return dict(zip(PERSIST_INFO, buf.tolist()))
+
+
+TEMPLATE_INFO = ("bodies", "constraints", "points", "creatures", "rounds", "lanes")
+
+
+def template_info(g):
+    """Per morphology template: size, population, rounds and lane count."""
+    n = K.check(K.lib().kx_template_info(g.p, -1, None))
+    out = []
+    buf = np.zeros(6, dtype=np.int32)
+    for t in range(n):
+        K.check(K.lib().kx_template_info(g.p, t, K.ptr(buf, C.c_int32)))
+        out.append(dict(zip(TEMPLATE_INFO, buf.tolist())))
+    return out
+
+
+def creature_templates(g):
+    """Template index of every creature."""
+    out = np.zeros(g.nCr, dtype=np.int32)
+    K.check(K.lib().kx_creature_templates(g.p, K.ptr(out, C.c_int32)))
+    return out
+
+
+def set_template_lanes(g, lanes):
+    """Lanes per creature of every template (0 = the handle's default)."""
+    a = np.ascontiguousarray(lanes, dtype=np.int32)
+    K.check(K.lib().kx_set_template_lanes(g.p, len(a), K.ptr(a, C.c_int32)))
 
 
 def set_plain_sdf(g, use_grid):

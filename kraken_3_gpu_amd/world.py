@@ -233,6 +233,16 @@ def world_uneven(n=8192, seed=1, solver=None, sort_by_size=True):
     return w
 
 
+def world_from_spawns(spawns, map_args, solver=None):
+    """A world of the given spawn tuples (x, y, angle, arms, segs, ring, curl)
+    on the map built from map_args (width, height, seed, cols, rows)."""
+    b = Builder()
+    b.map(*map_args)
+    for s in spawns:
+        b.spawn(*s)
+    return b.world(solver)
+
+
 def world_stress(n=65536, seed=1, solver=None):
     """Config 4: makeMap(4096,4096,{32x32 corridors}); creatures start in the
     corridor cells (cell centre +-20 px) so arms reach the walls."""

@@ -208,6 +208,33 @@ def test_persistent_shapes_are_bit_identical():
         G.destroy(g)
 
 
+def test_mixed_lanes_per_template_are_bit_identical():
+    """One launch with a different lane count per morphology template (the
+    per-bucket shapes the autotuner picks) gives exactly the uniform-shape
+    result, events included."""
+    law = W.default_law(23)
+    ref = None
+    for mode in ("uniform", "mixed"):
+        w = W.world_uneven(256, seed=7)
+        g = G.create(w, path=K.KX_PATH_PERSISTENT, lanes_per_creature=16, waves_per_block=1)
+        tinfo = G.template_info(g)
+        assert len(tinfo) > 4 and sum(t["creatures"] for t in tinfo) == 256
+        if mode == "mixed":
+            G.set_template_lanes(g, [(8, 16, 32, 64)[t % 4] for t in range(len(tinfo))])
+            assert G.persist_info(g)["mixed_lanes"] == 1
+            assert [t["lanes"] for t in G.template_info(g)] == [(8, 16, 32, 64)[t % 4] for t in range(len(tinfo))]
+        G.upload(g)
+        say(f"[case] per-template lanes {mode}: {len(tinfo)} templates")
+        ev = G.rollout(g, law, 0, 25, events=True)
+        G.download(g)
+        cur = (w.px.copy(), w.py.copy(), w.ang.copy(), w.broken.copy(), ev)
+        if ref is None:
+            ref = cur
+        else:
+            assert all(np.array_equal(x, y) for x, y in zip(ref, cur))
+        G.destroy(g)
+
+
 def test_plain_and_persistent_agree_uneven():
     """Ragged morphologies (4-8 arms x 5-9 segments, ring 6-10) through both paths."""
     from helpers import creature_of_bodies
