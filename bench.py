@@ -286,11 +286,50 @@ def bench_uneven(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_serve(args):
+    """ES rollout requests through the continuous-batching server: a burst of
+    requests first measures the serving capacity (creature-steps/s with the
+    queue never empty), then a Poisson stream at 70 % of that capacity gives
+    p50 / p99 request latency (arrival -> result) and the aggregate rate."""
+    from kraken_3_gpu_amd import serve as S
+    chunk = args.chunk if args.chunk_set else 50
+    opts = S.default_opts(chunk_steps=chunk)
+    n_probe, n_req = 24, args.requests
+    with S.Server(opts) as srv:
+        t, reqs = S.es_stream(n_probe, rate=0.0, seed=101)
+        _, probe = S.run_stream(srv, t, reqs)
+    cap = probe["creature_steps_per_s"]
+    t, reqs = S.es_stream(n_req, rate=1.0, seed=7)
+    mean_cs = float(np.mean([2 * r["pairs"] * r["steps"] for r in reqs]))
+    rate = args.load * cap / mean_cs
+    t = t / rate                                   # unit-rate gaps scaled to the target rate
+    with S.Server(opts) as srv:
+        res, summ = S.run_stream(srv, t, reqs)
+    out = {
+        "metric": "creature-steps/s of ES rollout requests served by continuous batching; p50/p99 request ms",
+        "value": summ["creature_steps_per_s"], "unit": "creature-steps/s", "n_gpus": 1, "steps": None,
+        "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"serving: {n_req} ES requests, Poisson arrivals at {rate:.2f}/s "
+                               f"({args.load:.0%} of the measured capacity), 16-512 antithetic pairs, "
+                               f"300-1500 steps, morphology 4-8 arms x 5-9 segments, ring 6-10, "
+                               f"controls from a 64-32-48 tanh policy on the host",
+                   "requests": n_req, "arrival_rate_per_s": rate, "offered_load": args.load,
+                   "mean_request_creature_steps": mean_cs, "steps_per_launch": chunk},
+        "p50_request_ms": summ["p50_ms"], "p99_request_ms": summ["p99_ms"], "max_request_ms": summ["max_ms"],
+        "p50_admit_wait_ms": summ["p50_admit_wait_ms"], "p99_admit_wait_ms": summ["p99_admit_wait_ms"],
+        "capacity_creature_steps_per_s": cap, "capacity_probe": probe, "stream": summ,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["uniform", "uneven", "stress"], default="uniform",
-                    help="uniform batch (the headline line), the uneven batch or the stress run")
+    ap.add_argument("--workload", choices=["uniform", "uneven", "serve", "stress"], default="uniform",
+                    help="uniform batch (the headline line), the uneven batch, the serving loop or the stress run")
+    ap.add_argument("--requests", type=int, default=120, help="serve: requests in the Poisson stream")
+    ap.add_argument("--load", type=float, default=0.7, help="serve: offered load as a fraction of capacity")
     ap.add_argument("--steps", type=int, default=0, help="timed steps (default 1000; stress 2000)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--creatures", type=int, default=0, help="creatures (default 4096; stress 65536)")
@@ -319,6 +358,9 @@ def main():
         return
     if args.workload == "uneven":
         bench_uneven(args)
+        return
+    if args.workload == "serve":
+        bench_serve(args)
         return
     args.steps = args.steps or 1000
     args.creatures = args.creatures or 4096

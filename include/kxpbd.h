@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define KXPBD_ABI_VERSION 2
+#define KXPBD_ABI_VERSION 3
 
 /* error codes */
 #define KX_OK 0
@@ -118,6 +118,15 @@ typedef struct {
 /* each tip's break force is drawn once (per creature and arm) instead of
  * following that step's adhesion */
 #define KX_LAW_FIXED_GRIP 1u
+/* controls come from the handle's control tape (kx_upload_tape) instead of
+ * the seeded draws: row r drives the r-th step of a rollout, and applies
+ * applyControl (src/sim/core.js:183-220) to its record. seed and tone are
+ * unused; limits, stiffness range and the reflex constants still apply. */
+#define KX_LAW_TAPE 2u
+/* one control-tape record per creature and step, the control struct that
+ * decode (src/brain/net.js:114-124) fills: [0] tone, then for arm a < 8
+ * [1+3a] shoulder, [2+3a] curl, [3+3a] adhesion */
+#define KX_TAPE_WORDS 25
 
 typedef struct kx_handle kx_handle;
 
@@ -173,6 +182,13 @@ int kx_step(kx_handle *h, int32_t nsteps);
 int kx_rollout(kx_handle *h, const kx_law *law, int32_t step0, int32_t nsteps,
                uint32_t *events_host);
 int kx_sync(kx_handle *h);
+/* Control tape for rollouts under KX_LAW_TAPE: n_rows rows of n_creatures
+ * records of KX_TAPE_WORDS floats (row-major). Ordered after the handle's
+ * earlier work and copied before return, so the caller may reuse `tape`.
+ * A tape rollout of nsteps needs nsteps <= n_rows. Replaces the host side of
+ * the reference's per-step control upload (gputrain.js applyControl +
+ * gpuworld.js uploadControls 312-315) for K steps at once. */
+int kx_upload_tape(kx_handle *h, const float *tape, int32_t n_rows);
 /* capture `nsteps` rollout steps into a hipGraph once and replay it `reps`
  * times (step keys advance by nsteps per replay) */
 int kx_rollout_graph(kx_handle *h, const kx_law *law, int32_t step0, int32_t nsteps, int32_t reps);
@@ -227,6 +243,83 @@ int  kx_builder_state(const kx_builder *b, double *px, double *py, double *ang);
  * break force, broken (spawnKraken: grips broken with anchor at spawn) */
 int  kx_builder_cons_init(const kx_builder *b, double *lbx, double *lby, double *compliance,
                           double *target, double *break_force, uint8_t *broken);
+
+/* ── serving: continuous batching of ES rollout requests (the request loop
+ *    around the generation loop of src/sim/gputrain.js:72-160, with
+ *    the episode of every population member run to its end on the device).
+ *    No counterpart in the reference as an API: it runs one population at a
+ *    time from one host loop. ─────────────────────────────────────────── */
+
+/* One ES evaluation request: 2*pairs creatures of one morphology
+ * (spawnKraken with arms x segs and a mantle ring of `ring` points) at one
+ * spawn pose on the server's map, run for `steps` steps. Creature 2p is
+ * driven by the policy theta + sigma*eps_p, creature 2p+1 by theta -
+ * sigma*eps_p (mirrored sampling as genomeAt does, src/brain/es.js:60-80); theta is
+ * drawn from policy_seed and eps_p from (noise_seed, p). The policy is a
+ * 64 -> 32 -> 48 tanh MLP of open-loop phase features of the step time,
+ * decoded as src/brain/net.js:114-124 decode does (out[0] tone, out[1+3a],
+ * out[2+3a], out[3+3a] shoulder, curl, adhesion of arm a); kx_policy_tape
+ * gives its records. Outputs in caller memory, written before the
+ * request's completion is reported: fitness[2*pairs] (x displacement of each
+ * creature's first body over the episode; may be NULL) and final_state
+ * (px, py, ang, vx, vy, av blocks of n_bodies floats each; may be NULL or
+ * hold final_cap floats, at least 6*n_bodies). */
+typedef struct {
+  int32_t arms, segs, ring;
+  int32_t pairs, steps;
+  double x, y, angle;
+  uint32_t policy_seed, noise_seed;
+  double sigma;
+  int64_t tag;
+  float *fitness;
+  float *final_state;
+  int64_t final_cap;
+} kx_request;
+
+typedef struct {
+  int64_t tag;
+  int32_t status;                    /* 0, or a KX_E* code if the request failed */
+  int32_t n_creatures, n_bodies;
+  double t_submit, t_admit, t_done;  /* seconds on the server clock: arrival, first launch, result ready */
+  int64_t device_bytes;              /* device memory the request held (handle arrays and tapes) */
+} kx_request_done;
+
+typedef struct {
+  int32_t chunk_steps;               /* steps per batched launch (0: 50) */
+  int32_t max_active;                /* requests resident on the device at once (0: 64) */
+  int32_t host_threads;              /* policy workers (0: hardware threads, at most 16) */
+  int64_t hbm_reserve;               /* bytes of free HBM admission leaves untouched */
+  kx_params params;                  /* solver parameters (path and shape: persistent, 16 lanes, 1 wave) */
+  kx_law law;                        /* limits, stiffness range, reflex constants; KX_LAW_TAPE is implied */
+  double map_width, map_height, map_density, map_wall;  /* makeMap (src/world/map.js:22-71) */
+  int32_t map_cols, map_rows;
+  uint32_t map_seed;
+} kx_server_opts;
+
+typedef struct {
+  int32_t queued, active, done;      /* waiting for admission, on the device, completed */
+  int64_t launches, creature_steps;  /* batched launches so far, creature-steps they ran */
+  double now, busy;                  /* server clock; device seconds inside batched launches */
+  int64_t peak_active, admit_waits;  /* most requests on the device at once; admissions held back by HBM */
+} kx_server_info;
+
+typedef struct kx_server kx_server;
+/* starts the server's admission and scheduling threads on the current device */
+int  kx_server_create(const kx_server_opts *opts, kx_server **out);
+/* queues a request (thread-safe, returns at once); admission is FIFO */
+int  kx_server_submit(kx_server *s, const kx_request *req);
+/* up to cap completions, waiting at most wait_s seconds for the first */
+int  kx_server_poll(kx_server *s, kx_request_done *done, int32_t cap, int32_t *n, double wait_s);
+int  kx_server_info_get(kx_server *s, kx_server_info *info);
+/* finishes the requests on the device, drops the queued ones, frees all */
+void kx_server_destroy(kx_server *s);
+/* device bytes admission reserves for a request (state, tables, tapes) */
+int64_t kx_request_bytes(const kx_request *req, int32_t chunk_steps);
+/* free and total memory of the current device (hipMemGetInfo) */
+int  kx_mem_info(int64_t *free_bytes, int64_t *total_bytes);
+/* host only: the control-tape records the server feeds request `req` for
+ * steps step0 .. step0+nsteps-1, tape[nsteps][2*pairs][KX_TAPE_WORDS] */
+int  kx_policy_tape(const kx_request *req, int32_t step0, int32_t nsteps, double step_seconds, float *tape);
 
 #ifdef __cplusplus
 }

@@ -12,6 +12,8 @@ LIB_PATH = os.path.join(HERE, "libkxpbd.so")
 KX_OK, KX_EINVAL, KX_ENOMEM, KX_EDEVICE, KX_ENOTSUP = 0, -22, -12, -5, -95
 KX_PIN, KX_ANGLE, KX_GRIP = 0, 1, 2
 KX_LAW_FIXED_GRIP = 1
+KX_LAW_TAPE = 2
+KX_TAPE_WORDS = 25
 KX_PATH_PLAIN, KX_PATH_PERSISTENT = 0, 1
 
 i32, u32, f64, f32p, i32p, u8p, f64p, u32p = (C.c_int32, C.c_uint32, C.c_double, C.POINTER(C.c_float),
@@ -61,6 +63,31 @@ class Law(C.Structure):
                 ("creature_base", u32), ("flags", u32)]
 
 
+class Request(C.Structure):
+    _fields_ = [("arms", i32), ("segs", i32), ("ring", i32), ("pairs", i32), ("steps", i32),
+                ("x", f64), ("y", f64), ("angle", f64), ("policy_seed", u32), ("noise_seed", u32),
+                ("sigma", f64), ("tag", C.c_int64), ("fitness", f32p), ("final_state", f32p),
+                ("final_cap", C.c_int64)]
+
+
+class RequestDone(C.Structure):
+    _fields_ = [("tag", C.c_int64), ("status", i32), ("n_creatures", i32), ("n_bodies", i32),
+                ("t_submit", f64), ("t_admit", f64), ("t_done", f64), ("device_bytes", C.c_int64)]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("chunk_steps", i32), ("max_active", i32), ("host_threads", i32), ("hbm_reserve", C.c_int64),
+                ("params", Params), ("law", Law), ("map_width", f64), ("map_height", f64),
+                ("map_density", f64), ("map_wall", f64), ("map_cols", i32), ("map_rows", i32),
+                ("map_seed", u32)]
+
+
+class ServerInfo(C.Structure):
+    _fields_ = [("queued", i32), ("active", i32), ("done", i32), ("launches", C.c_int64),
+                ("creature_steps", C.c_int64), ("now", f64), ("busy", f64), ("peak_active", C.c_int64),
+                ("admit_waits", C.c_int64)]
+
+
 EXPORTS = {
     "kx_create": (i32, [C.POINTER(WorldDesc), C.POINTER(CreatureDesc), C.POINTER(Params), C.POINTER(C.c_void_p)]),
     "kx_destroy": (None, [C.c_void_p]),
@@ -80,6 +107,15 @@ EXPORTS = {
     "kx_step": (i32, [C.c_void_p, i32]),
     "kx_rollout": (i32, [C.c_void_p, C.POINTER(Law), i32, i32, u32p]),
     "kx_sync": (i32, [C.c_void_p]),
+    "kx_upload_tape": (i32, [C.c_void_p, C.POINTER(C.c_float), i32]),
+    "kx_server_create": (i32, [C.POINTER(ServerOpts), C.POINTER(C.c_void_p)]),
+    "kx_server_submit": (i32, [C.c_void_p, C.POINTER(Request)]),
+    "kx_server_poll": (i32, [C.c_void_p, C.POINTER(RequestDone), i32, C.POINTER(i32), f64]),
+    "kx_server_info_get": (i32, [C.c_void_p, C.POINTER(ServerInfo)]),
+    "kx_server_destroy": (None, [C.c_void_p]),
+    "kx_policy_tape": (i32, [C.POINTER(Request), i32, i32, f64, C.POINTER(C.c_float)]),
+    "kx_request_bytes": (C.c_int64, [C.POINTER(Request), i32]),
+    "kx_mem_info": (i32, [C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "kx_rollout_graph": (i32, [C.c_void_p, C.POINTER(Law), i32, i32, i32]),
     "kx_last_timing": (i32, [C.c_void_p, f64p, i32p]),
     "kx_set_path": (i32, [C.c_void_p, i32, i32, i32]),

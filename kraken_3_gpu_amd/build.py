@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libkxpbd.so")
-SOURCES = ["kx_api.cpp", "kx_builder.cpp", "kx_sched.cpp", "kx_plain.hip", "kx_persist.hip"]
+SOURCES = ["kx_api.cpp", "kx_builder.cpp", "kx_sched.cpp", "kx_serve.cpp", "kx_plain.hip", "kx_persist.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -36,6 +36,11 @@ static int fail(int code, const char *fmt, ...) {
 }
 
 extern "C" const char *kx_last_error(void) { return g_err.c_str(); }
+
+int kx_error(int code, const char *msg) {
+  g_err = msg;
+  return code;
+}
 extern "C" int kx_abi_version(void) { return KXPBD_ABI_VERSION; }
 
 #define HIPCHK(expr)                                                              \
@@ -70,6 +75,9 @@ struct kx_handle {
   int *step_base = nullptr;
   uint32_t *ev_dev = nullptr;
   size_t ev_cap = 0;
+  float *tape_dev = nullptr;     // control tape (KX_LAW_TAPE), tape_rows rows valid
+  size_t tape_cap = 0;
+  int tape_rows = 0;
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int last_launches = 0;
@@ -379,6 +387,7 @@ static void destroy_impl(kx_handle *h) {
   for (auto &b : h->bufs) (void)hipFree(b.p);
   for (void *p : h->pc_bufs) if (p) (void)hipFree(p);
   if (h->ev_dev) (void)hipFree(h->ev_dev);
+  if (h->tape_dev) (void)hipFree(h->tape_dev);
   if (h->e0) (void)hipEventDestroy(h->e0);
   if (h->e1) (void)hipEventDestroy(h->e1);
   if (h->st) (void)hipStreamDestroy(h->st);
@@ -840,6 +849,7 @@ static KxLaw to_law(const kx_law *l) {
   L.shoulder_limit = l->shoulder_limit; L.joint_limit = l->joint_limit;
   L.grip_min = l->grip_min; L.grip_max = l->grip_max; L.slip_max = l->slip_max;
   L.cooldown = l->cooldown; L.attach_min = l->attach_min; L.release_below = l->release_below;
+  L.stiff_min = l->stiff_min; L.stiff_max = l->stiff_max;
   L.cbase = l->creature_base; L.flags = l->flags;
   return L;
 }
@@ -856,8 +866,8 @@ static int run_steps(kx_handle *h, const KxLaw *law, int step0, const int *step_
     launches = 1;
   } else {
     for (int s = 0; s < nsteps; s++) {
-      int r = step_base ? kx_plain_step(d, law, s, step_base, with_forces && s == 0, s, h->st, &launches)
-                        : kx_plain_step(d, law, step0 + s, nullptr, with_forces && s == 0, s, h->st, &launches);
+      int r = step_base ? kx_plain_step(d, law, s, step_base, with_forces && s == 0, s, s, h->st, &launches)
+                        : kx_plain_step(d, law, step0 + s, nullptr, with_forces && s == 0, s, s, h->st, &launches);
       if (r) return fail(KX_EDEVICE, "plain launch failed: %s", hipGetErrorString(hipGetLastError()));
     }
   }
@@ -882,6 +892,8 @@ extern "C" int kx_rollout(kx_handle *h, const kx_law *law, int32_t step0, int32_
   if (nsteps < 0 || step0 < 0) return fail(KX_EINVAL, "bad step range");
   if (nsteps == 0) return 0;
   KxLaw L = to_law(law);
+  if ((L.flags & KX_LAW_TAPE) && nsteps > h->tape_rows)
+    return fail(KX_EINVAL, "tape rollout of %d steps with %d tape rows", nsteps, h->tape_rows);
   size_t need = (size_t)nsteps * h->nCr;
   if (events_host) {
     if (need > h->ev_cap) {
@@ -903,11 +915,34 @@ extern "C" int kx_rollout(kx_handle *h, const kx_law *law, int32_t step0, int32_
   return 0;
 }
 
+extern "C" int kx_upload_tape(kx_handle *h, const float *tape, int32_t n_rows) {
+  if (!h || (n_rows > 0 && !tape)) return fail(KX_EINVAL, "null argument");
+  if (n_rows < 0) return fail(KX_EINVAL, "negative row count");
+  const size_t n = (size_t)n_rows * h->nCr * KX_TAPE_WORDS;
+  HIPCHK(hipStreamSynchronize(h->st));
+  if (n > h->tape_cap) {
+    if (h->tape_dev) HIPCHK(hipFree(h->tape_dev));
+    h->tape_dev = nullptr;
+    h->tape_cap = 0;
+    HIPCHK(hipMalloc(&h->tape_dev, n * 4));
+    h->tape_cap = n;
+    h->d.tape = h->tape_dev;
+    // a captured rollout holds the old tape address
+    if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; h->g_nsteps = -1; }
+  }
+  if (n) HIPCHK(hipMemcpyAsync(h->tape_dev, tape, n * 4, hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  h->tape_rows = n_rows;
+  return 0;
+}
+
 extern "C" int kx_rollout_graph(kx_handle *h, const kx_law *law, int32_t step0, int32_t nsteps, int32_t reps) {
   if (!h || !law) return fail(KX_EINVAL, "null argument");
   if (!h->has_creatures) return fail(KX_ENOTSUP, "rollout needs a creature description at create");
   if (nsteps <= 0 || reps <= 0 || step0 < 0) return fail(KX_EINVAL, "bad step range");
   KxLaw L = to_law(law);
+  if ((L.flags & KX_LAW_TAPE) && nsteps > h->tape_rows)
+    return fail(KX_EINVAL, "tape rollout of %d steps with %d tape rows", nsteps, h->tape_rows);
   HIPCHK(hipStreamSynchronize(h->st));
   bool rebuild = !h->gexec || h->g_nsteps != nsteps || h->g_path != h->path || memcmp(&h->g_law, &L, sizeof L) != 0;
   // (kx_set_path drops the cached graph when the persistent shape changes)
@@ -924,7 +959,7 @@ extern "C" int kx_rollout_graph(kx_handle *h, const kx_law *law, int32_t step0, 
       KxDev d = h->d;
       d.events = nullptr;
       int launches = 0;
-      for (int s = 0; s < nsteps && !r; s++) r = kx_plain_step(d, &L, s, h->step_base, 0, 0, h->st, &launches);
+      for (int s = 0; s < nsteps && !r; s++) r = kx_plain_step(d, &L, s, h->step_base, 0, 0, s, h->st, &launches);
     }
     if (!r) r = kx_bump_counter(h->step_base, nsteps, h->st);
     hipError_t ec = hipStreamEndCapture(h->st, &g);
@@ -1033,6 +1068,20 @@ extern "C" int kx_set_template_lanes(kx_handle *h, int32_t n, const int32_t *lan
     g_err = keep;
     return r;
   }
+  return 0;
+}
+
+int kx_handle_view(kx_handle *h, KxHandleView *v) {
+  if (!h || !v) return fail(KX_EINVAL, "null argument");
+  if (h->path != KX_PATH_PERSISTENT || !h->persist_ok)
+    return fail(KX_ENOTSUP, "batched launches need the persistent path");
+  v->d = h->d;
+  v->d.events = nullptr;
+  v->P = h->pc;
+  v->wpb = h->wpb;
+  v->st = h->st;
+  v->bytes = 0;
+  for (auto &b : h->bufs) v->bytes += b.bytes;
   return 0;
 }
 

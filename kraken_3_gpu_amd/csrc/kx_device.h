@@ -330,6 +330,26 @@ __device__ __forceinline__ float kx_law_target(const KxLaw &L, uint32_t cid, uin
   return (float)kx_jsmax(-lim, kx_jsmin(lim, t));
 }
 
+// Control tape: per step and creature the inputs of applyControl
+// (core.js:183-198) as decode leaves them (src/brain/net.js:114-124):
+// rec[0] tone, rec[1+3a] shoulder, rec[2+3a] curl, rec[3+3a] adhesion of arm a.
+__device__ __forceinline__ const float *kx_tape_rec(const KxDev &d, int row, int c) {
+  return d.tape + ((size_t)row * d.nCr + c) * KX_TAPE_WORDS;
+}
+// toneToCompliance of the clamped tone (core.js:185-187)
+__device__ __forceinline__ double kx_tape_comp(const KxLaw &L, float tone) {
+  double t = kx_jsmax(0.0, kx_jsmin(1.0, (double)tone));
+  return __ddiv_rn(1.0, __dadd_rn(L.stiff_min, __dmul_rn(t, __dsub_rn(L.stiff_max, L.stiff_min))));
+}
+// shoulder target clamped to the shoulder limit, every joint of the arm the
+// curl clamped to the joint limit (core.js:193-197)
+__device__ __forceinline__ float kx_tape_target(const KxLaw &L, const float *rec, int slot) {
+  const int arm = slot / KX_SLOTS_PER_ARM, m = slot % KX_SLOTS_PER_ARM;
+  const double lim = m == 0 ? L.shoulder_limit : L.joint_limit;
+  const double u = (double)rec[(m == 0 ? 1 : 2) + 3 * arm];
+  return (float)kx_jsmax(-lim, kx_jsmin(lim, u));
+}
+
 // Adhesion reflex of one arm (core.js:199-218). Reads the tip body's f32
 // velocity and pose (tip position in global coordinates as f64), updates
 // the grip fields and the f64 cooldown. Returns event bits (attach 1,

@@ -66,6 +66,7 @@ struct KxDev {
   const int *arm_grip;           // internal constraint
   double *cd;                    // [nCr * KX_MAX_ARMS]
   uint32_t *events;              // [nsteps * nCr] or null
+  const float *tape;             // [rows][nCr][KX_TAPE_WORDS] controls under KX_LAW_TAPE, or null
   // constants
   float h, invH, invH2, linDamp, angDamp, maxSpeed, maxAngSpeed;
   double hd;                     // h in f64 (reflex cooldown arithmetic)
@@ -81,13 +82,15 @@ struct KxLaw {
   double comp;                   // 1/(stiff_min + tone*(stiff_max-stiff_min))
   double shoulder_limit, joint_limit;
   double grip_min, grip_max, slip_max, cooldown, attach_min, release_below;
+  double stiff_min, stiff_max;   // tone -> compliance of tape controls
   uint32_t cbase, flags;         // creature key offset, KX_LAW_* flags
 };
 
 // Launchers (kx_plain.hip / kx_persist.hip). All asynchronous on `st`.
 // step key = (step_base ? *step_base : 0) + step_key
+// tape_row: the control-tape row of this step (KX_LAW_TAPE)
 int kx_plain_step(const KxDev &d, const KxLaw *law, int step_key, const int *step_base, int with_forces,
-                  int ev_row, hipStream_t st, int *launches);
+                  int ev_row, int tape_row, hipStream_t st, int *launches);
 // Persistent-path launch description (see kx_persist.hip).
 struct KxPersistCfg {
   const int *wave_cr;       // creature ids of every wavefront slot, -1 = empty slot
@@ -111,3 +114,33 @@ int kx_persist_launch(int G, int wpb, const KxDev &d, const KxLaw *law, int step
 int kx_persist_cr_words(int NB, int NC, int NP);
 int kx_persist_occupancy(int G, int wpb, size_t lds);   // G = 0: the mixed-lane kernel
 int kx_bump_counter(int *counter, int delta, hipStream_t st);
+
+// One handle's share of a batched persistent launch: its device state and
+// launch tables, its control law (KxLaw) and step range. Workgroup b of the
+// batched grid runs local workgroup wgmap[b].y of entry wgmap[b].x.
+struct KxBatchEnt {
+  KxDev d;
+  KxLaw L;
+  KxPersistCfg P;
+  int has_law, step0, nsteps, pad;
+};
+// LDS bytes a handle's persistent workgroups need
+inline size_t kx_persist_lds_bytes(const KxPersistCfg &P) {
+  return ((size_t)P.tmpl_words + P.bm_words + P.map_words + (size_t)P.cr_words) * 4;
+}
+int kx_persist_launch_batch(int wpb, const KxBatchEnt *ents_dev, const int2 *wgmap_dev, int nwg, size_t lds,
+                            hipStream_t st);
+
+// What a batched launch needs of a handle (kx_api.cpp); fails unless the
+// handle runs the persistent path.
+struct kx_handle;
+struct KxHandleView {
+  KxDev d;
+  KxPersistCfg P;
+  int wpb;
+  hipStream_t st;
+  size_t bytes;             // device memory the handle's state arrays hold
+};
+int kx_handle_view(kx_handle *h, KxHandleView *v);
+// sets the thread's kx_last_error message and returns code
+int kx_error(int code, const char *msg);

@@ -175,7 +175,6 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
   const float4 *t_prec = (const float4 *)(smem + T.o_prec);
   const int4 *t_sched4 = (const int4 *)(smem + T.o_sched);
   const int *t_rbits = (const int *)(smem + T.o_rtype);
-  const int *t_coll = (const int *)(smem + T.o_coll), *t_ccnt = (const int *)(smem + T.o_ccnt);
   const int *t_arm_tip = (const int *)(smem + T.o_arm_tip), *t_arm_grip = (const int *)(smem + T.o_arm_grip);
   const int *t_mot_k = (const int *)(smem + T.o_mot_k), *t_mot_slot = (const int *)(smem + T.o_mot_slot);
   const int *t_grip_k = (const int *)(smem + T.o_grip_k), *t_grip_arm = (const int *)(smem + T.o_grip_arm);
@@ -205,6 +204,7 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
   double my_cd = 0.0;
   if (act && sub < T.narms) my_cd = d.cd[cid * KX_MAX_ARMS + sub];
   const double hS = d.hd * d.S;
+  double comp_last = L.comp;      // motor compliance of the latest control step
   // round types of the first 64 rounds (bit r set = angle round)
   const uint32_t rb0 = (uint32_t)__builtin_amdgcn_readfirstlane(t_rbits[0]);
   const uint32_t rb1 = (uint32_t)__builtin_amdgcn_readfirstlane(t_rbits[1]);
@@ -215,19 +215,24 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
     if (act && sub == 0) evw[0] = 0u;
     uint32_t my_ev = 0u;
     if (has_law) {
-      // motor targets + compliance (core.js:190-198)
-      const float a_law = (float)L.comp * d.invH2;
+      // motor targets + compliance (core.js:190-198), from the seeded law or
+      // from this step's tape record
+      const bool tape = (L.flags & KX_LAW_TAPE) != 0;
+      const float *rec = tape && act ? kx_tape_rec(d, step, cid) : nullptr;
+      if (rec) comp_last = kx_tape_comp(L, rec[0]);
+      const float a_law = (float)comp_last * d.invH2;
       for (int j = sub; j < (act ? T.nmot : 0); j += G) {
         int k = t_mot_k[j];
         float4 dk = dyn[k];
         dk.x = a_law;
-        dk.y = kx_law_target(L, (uint32_t)cid + L.cbase, (uint32_t)key, t_mot_slot[j]);
+        dk.y = rec ? kx_tape_target(L, rec, t_mot_slot[j])
+                   : kx_law_target(L, (uint32_t)cid + L.cbase, (uint32_t)key, t_mot_slot[j]);
         dyn[k] = dk;
       }
       // adhesion reflex (core.js:199-218), one lane per arm
       if (act && sub < T.narms) {
         const uint32_t ck = (uint32_t)cid + L.cbase;
-        double adh = kx_ctl_u(L.seed, ck, (uint32_t)key, KX_SLOT_ADH + sub);
+        double adh = rec ? (double)rec[3 + 3 * sub] : kx_ctl_u(L.seed, ck, (uint32_t)key, KX_SLOT_ADH + sub);
         double gu = (L.flags & KX_LAW_FIXED_GRIP) ? kx_ctl_u(L.seed, ck, KX_GRIP_DRAW_KEY, KX_SLOT_ADH + sub) : 0.0;
         int g = t_arm_grip[sub], t = t_arm_tip[sub];
         float4 dk = dyn[g], lk = lamr[g], tp = prv[t], tv = vel[t], cr0 = t_crec[2 * g];   // {lax, lay, -, -}
@@ -485,7 +490,7 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
     d.broken[g] = __float_as_int(lk.w);
     if (((v >> 8) & 255) == 255) { d.lbx[g] = dk.z; d.lby[g] = dk.w; }
     d.bf[g] = lk.z; d.target[g] = dk.y;
-    if (has_law && (v >> 16) == KX_ANGLE) d.comp[g] = (float)L.comp;
+    if (has_law && (v >> 16) == KX_ANGLE) d.comp[g] = (float)comp_last;
     if ((v >> 16) == KX_ANGLE) { d.lam[g] = lk.x; d.lamx[g] = 0.0f; d.lamy[g] = 0.0f; }
     else { d.lamx[g] = lk.x; d.lamy[g] = lk.y; d.lam[g] = 0.0f; }
   }
@@ -512,6 +517,46 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist_mix(KxDev d, KxLaw 
     case 16: kx_persist_body<16, WPB>(d, L, has_law, step0, step_base, nsteps, ev_row0, with_forces, P, bid); break;
     case 32: kx_persist_body<32, WPB>(d, L, has_law, step0, step_base, nsteps, ev_row0, with_forces, P, bid); break;
     default: kx_persist_body<64, WPB>(d, L, has_law, step0, step_base, nsteps, ev_row0, with_forces, P, bid); break;
+  }
+}
+
+// Several handles in one grid (the serving loop's continuous batch): each
+// workgroup runs its entry's body with that entry's state, tables, law and
+// step count.
+template <int WPB>
+__global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist_batch(const KxBatchEnt *__restrict__ E,
+                                                                   const int2 *__restrict__ wgmap) {
+  const int2 m = wgmap[blockIdx.x];
+  const KxBatchEnt &e = E[m.x];
+  const int bid = m.y;
+  switch (e.P.wg_lanes[bid]) {
+    case 8: kx_persist_body<8, WPB>(e.d, e.L, e.has_law, e.step0, nullptr, e.nsteps, 0, 0, e.P, bid); break;
+    case 16: kx_persist_body<16, WPB>(e.d, e.L, e.has_law, e.step0, nullptr, e.nsteps, 0, 0, e.P, bid); break;
+    case 32: kx_persist_body<32, WPB>(e.d, e.L, e.has_law, e.step0, nullptr, e.nsteps, 0, 0, e.P, bid); break;
+    default: kx_persist_body<64, WPB>(e.d, e.L, e.has_law, e.step0, nullptr, e.nsteps, 0, 0, e.P, bid); break;
+  }
+}
+
+template <int WPB>
+static int launch_batch(const KxBatchEnt *E, const int2 *wgmap, int nwg, size_t lds, hipStream_t st) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    (void)hipFuncSetAttribute((const void *)kx_k_persist_batch<WPB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024);
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((kx_k_persist_batch<WPB>), dim3(nwg), dim3(KX_WAVE * WPB), lds, st, E, wgmap);
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+int kx_persist_launch_batch(int wpb, const KxBatchEnt *ents_dev, const int2 *wgmap_dev, int nwg, size_t lds,
+                            hipStream_t st) {
+  if (nwg <= 0) return 0;
+  if (lds > 160 * 1024) return -22;
+  switch (wpb) {
+    case 1: return launch_batch<1>(ents_dev, wgmap_dev, nwg, lds, st);
+    case 2: return launch_batch<2>(ents_dev, wgmap_dev, nwg, lds, st);
+    default: return -22;
   }
 }
 

@@ -13,7 +13,7 @@ static inline int nblk(int n) { return (n + PLAIN_BLOCK - 1) / PLAIN_BLOCK; }
 
 // Control law + reflex, one thread per arm (core.js:183-220).
 __global__ __launch_bounds__(PLAIN_BLOCK) void kx_k_arms(KxDev d, KxLaw L, int step_key, const int *step_base,
-                                                         int ev_row) {
+                                                         int ev_row, int tape_row) {
   int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= d.nArms) return;
   if (step_base) step_key += step_base[0];
@@ -26,7 +26,8 @@ __global__ __launch_bounds__(PLAIN_BLOCK) void kx_k_arms(KxDev d, KxLaw L, int s
   int c = lo, arm = a - d.cr[c].arm_off;
   int tip = d.arm_tip[a], g = d.arm_grip[a];
   const uint32_t ck = (uint32_t)c + L.cbase;
-  double adh = kx_ctl_u(L.seed, ck, (uint32_t)step_key, KX_SLOT_ADH + arm);
+  double adh = (L.flags & KX_LAW_TAPE) ? (double)kx_tape_rec(d, tape_row, c)[3 + 3 * arm]
+                                       : kx_ctl_u(L.seed, ck, (uint32_t)step_key, KX_SLOT_ADH + arm);
   double gu = (L.flags & KX_LAW_FIXED_GRIP) ? kx_ctl_u(L.seed, ck, KX_GRIP_DRAW_KEY, KX_SLOT_ADH + arm) : 0.0;
   double &cd = d.cd[c * KX_MAX_ARMS + arm];
   int broken = d.broken[g];
@@ -37,13 +38,20 @@ __global__ __launch_bounds__(PLAIN_BLOCK) void kx_k_arms(KxDev d, KxLaw L, int s
   if (ev && d.events) atomicOr(&d.events[(size_t)ev_row * d.nCr + c], ev << (3 * arm));
 }
 
-// Motor targets + compliance under the law, one thread per constraint.
-__global__ __launch_bounds__(PLAIN_BLOCK) void kx_k_motors(KxDev d, KxLaw L, int step_key, const int *step_base) {
+// Motor targets + compliance under the law or the tape, one thread per constraint.
+__global__ __launch_bounds__(PLAIN_BLOCK) void kx_k_motors(KxDev d, KxLaw L, int step_key, const int *step_base,
+                                                           int tape_row) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= d.nC) return;
   if (step_base) step_key += step_base[0];
   int slot = d.cslot[k];
   if (slot < 0) return;
+  if (L.flags & KX_LAW_TAPE) {
+    const float *rec = kx_tape_rec(d, tape_row, d.ccr[k]);
+    d.target[k] = kx_tape_target(L, rec, slot);
+    d.comp[k] = (float)kx_tape_comp(L, rec[0]);
+    return;
+  }
   d.target[k] = kx_law_target(L, (uint32_t)d.ccr[k] + L.cbase, (uint32_t)step_key, slot);
   d.comp[k] = (float)L.comp;
 }
@@ -164,12 +172,14 @@ __global__ __launch_bounds__(PLAIN_BLOCK) void kx_k_grip_break(KxDev d, int ev_r
 }
 
 int kx_plain_step(const KxDev &d, const KxLaw *law, int step_key, const int *step_base, int with_forces,
-                  int ev_row, hipStream_t st, int *launches) {
+                  int ev_row, int tape_row, hipStream_t st, int *launches) {
   int n = 0;
   if (law) {
-    hipLaunchKernelGGL(kx_k_motors, dim3(nblk(d.nC)), dim3(PLAIN_BLOCK), 0, st, d, *law, step_key, step_base); n++;
+    hipLaunchKernelGGL(kx_k_motors, dim3(nblk(d.nC)), dim3(PLAIN_BLOCK), 0, st, d, *law, step_key, step_base,
+                       tape_row); n++;
     if (d.nArms > 0) {
-      hipLaunchKernelGGL(kx_k_arms, dim3(nblk(d.nArms)), dim3(PLAIN_BLOCK), 0, st, d, *law, step_key, step_base, ev_row); n++;
+      hipLaunchKernelGGL(kx_k_arms, dim3(nblk(d.nArms)), dim3(PLAIN_BLOCK), 0, st, d, *law, step_key, step_base,
+                         ev_row, tape_row); n++;
     }
   }
   int nbc = d.nB > d.nC ? d.nB : d.nC;

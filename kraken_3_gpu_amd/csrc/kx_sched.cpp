@@ -175,8 +175,9 @@ KxTmpl kx_pack_template(const TmplHost &t, int G, std::vector<uint32_t> &pool) {
   for (size_t r = 0; r < rtype.size(); r++)
     if (rtype[r]) rbits[r >> 5] |= (int)(1u << (r & 31));
   T.o_rtype = at(); put(pool, rbits);
-  T.o_coll = at(); put(pool, lists);
-  T.o_ccnt = at(); put(pool, counts);
+  // (the per-lane collision lists only set maxbl, the workgroup cost order;
+  // the kernel finds its collision candidates itself)
+  T.o_coll = T.o_ccnt = 0;
   T.o_arm_tip = at(); put(pool, t.arm_tip);
   T.o_arm_grip = at(); put(pool, t.arm_grip);
   T.o_mot_k = at(); put(pool, t.mot_k);

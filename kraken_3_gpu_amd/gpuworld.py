@@ -255,6 +255,15 @@ def rollout(g, law, step0, nsteps, events=False):
     return ev
 
 
+def upload_tape(g, tape):
+    """Control tape for KX_LAW_TAPE rollouts: float32 [rows, n_creatures,
+    KX_TAPE_WORDS] (tone, then shoulder, curl, adhesion per arm)."""
+    t = np.ascontiguousarray(tape, dtype=np.float32)
+    if t.ndim != 3 or t.shape[1] != g.nCr or t.shape[2] != K.KX_TAPE_WORDS:
+        raise ValueError(f"tape shape {t.shape}, want (rows, {g.nCr}, {K.KX_TAPE_WORDS})")
+    K.check(K.lib().kx_upload_tape(g.p, K.ptr(t, C.c_float), t.shape[0]))
+
+
 def rollout_graph(g, law, step0, nsteps, reps):
     K.check(K.lib().kx_rollout_graph(g.p, C.byref(law), step0, nsteps, reps))
 

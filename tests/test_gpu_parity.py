@@ -258,6 +258,121 @@ def test_plain_and_persistent_agree_uneven():
         assert err <= TOL_PX and len(flips) <= 1
 
 
+def _random_tape(rng, rows, ncr, kind="clamps"):
+    """Tape records. "clamps": beyond every clamp (tone outside [0, 1],
+    shoulder and curl past their limits, adhesion below the release and above
+    the attach thresholds). "moderate": inside the limits, tone 0.3-0.7,
+    adhesion 0-1 (the seeded law's ranges)."""
+    t = np.zeros((rows, ncr, K.KX_TAPE_WORDS), np.float32)
+    wide = kind == "clamps"
+    t[:, :, 0] = rng.uniform(-0.2, 1.2, (rows, ncr)) if wide else rng.uniform(0.3, 0.7, (rows, ncr))
+    for a in range(8):
+        t[:, :, 1 + 3 * a] = rng.uniform(-3.0, 3.0, (rows, ncr)) if wide else rng.uniform(-1.0, 1.0, (rows, ncr))
+        t[:, :, 2 + 3 * a] = rng.uniform(-2.0, 2.0, (rows, ncr)) if wide else rng.uniform(-0.8, 0.8, (rows, ncr))
+        t[:, :, 3 + 3 * a] = rng.uniform(-0.3, 1.1, (rows, ncr)) if wide else rng.uniform(0.0, 1.0, (rows, ncr))
+    return t
+
+
+def _tape_law():
+    law = W.default_law(0)
+    law.flags = K.KX_LAW_TAPE
+    return law
+
+
+# Records past the clamps drive joints onto their limits at random tone; a
+# pin or motor dead-zone comparison (2e-3 px, 2e-4 rad) that binary32 and
+# binary64 decide differently then moves a body by up to the dead zone (the
+# oracle's own binary32 emulation of the reference shows the same, 1.0-1.6e-3
+# px on these worlds), so that case bounds positions by the dead zone plus
+# arithmetic error and pins the control mapping itself bit for bit.
+TOL_PX_CLAMPS = 2.5e-3
+
+
+@pytest.mark.parametrize("kind", ["moderate", "clamps"])
+@pytest.mark.parametrize("path", ["plain", "persistent"])
+def test_tape_step_parity(path, kind):
+    """KX_LAW_TAPE: applyControl of each creature's tape record (core.js:183-220)
+    then one step, against the oracle's applyControl from identical state;
+    4-8 arms per creature, grips attached and free. Targets, compliances and
+    break forces must equal the oracle's rounded to binary32; events must
+    match; positions within 1e-3 px (moderate records) or the dead-zone
+    bound (records past the clamps)."""
+    from helpers import creature_of_bodies
+    w, o, g = _setup(lambda: W.world_uneven(24, seed=3, sort_by_size=False), path)
+    say(f"[case] tape warm-up path={path} kind={kind}")
+    G.rollout(g, W.default_law(11), 0, 12)
+    _pull_state(g)
+    load_state(o, w)
+    o.set_step_counter(12)
+    o.clear_events()
+    tape = _random_tape(np.random.default_rng(5), 1, w.n_creatures, kind)
+    for ci in range(w.n_creatures):
+        arms = int(w.arm_off[ci + 1] - w.arm_off[ci])
+        r = tape[0, ci].astype(np.float64)
+        o.apply_control(ci, r[0], r[1::3][:arms], r[2::3][:arms], r[3::3][:arms])
+    o.step()
+    G.upload_tape(g, tape)
+    say(f"[case] tape one step path={path} kind={kind}")
+    ev_gpu = [(12 + s, c, a, k) for (s, c, a, k) in decode_events(G.rollout(g, _tape_law(), 12, 1, events=True))]
+    G.download(g)
+    ev_or = _oracle_events(o)
+    flips = _creatures_with_event_diff(ev_or, ev_gpu)
+    err, _ = _pos_err(w, o, creature_of_bodies(w), flips)
+    say(f"    tape path={path} kind={kind} max|dpos|={err:.3e} px events={len(ev_or)} flips={sorted(flips)}")
+    assert len(ev_or) > 0
+    assert len(flips) <= 1
+    assert err <= (TOL_PX if kind == "moderate" else TOL_PX_CLAMPS)
+    ctl, oc = G.download_controls(g), o.cons()
+    mot = w.type == K.KX_ANGLE
+    for key in ("compliance", "target"):
+        assert np.array_equal(ctl[key][mot], oc[key][mot].astype(np.float32)), key
+    grip = (w.type == K.KX_GRIP) & (w.broken == oc["broken"])
+    assert np.array_equal(ctl["breakForce"][grip], oc["breakForce"][grip].astype(np.float32))
+    G.destroy(g)
+
+
+def test_tape_rollout_paths_and_graph_bit_identical():
+    """A 16-row tape drives 16 steps identically on the plain and persistent
+    paths; a captured rollout replays rows 0..7 on every replay, exactly as
+    eager rollouts over the same rows; too few rows is an error."""
+    make = lambda: W.world_uneven(256, seed=9)
+    tape = None
+    states = []
+    for path in ("plain", "persistent"):
+        w = make()
+        if tape is None:
+            tape = _random_tape(np.random.default_rng(3), 16, w.n_creatures)
+        g = G.create(w, path=PATHS[path])
+        G.upload(g)
+        G.upload_tape(g, tape)
+        say(f"[case] tape 16 steps path={path}")
+        ev = G.rollout(g, _tape_law(), 0, 16, events=True)
+        G.download(g)
+        states.append((w.px.copy(), w.py.copy(), w.ang.copy(), w.broken.copy(), ev))
+        G.destroy(g)
+    assert all(np.array_equal(a, b) for a, b in zip(*states))
+    assert states[0][4].any()
+    runs = []
+    for mode in ("eager", "graph"):
+        w = make()
+        g = G.create(w, path=K.KX_PATH_PERSISTENT)
+        G.upload(g)
+        G.upload_tape(g, tape[:8])
+        with pytest.raises(K.KxError):
+            G.rollout(g, _tape_law(), 0, 9)
+        say(f"[case] tape rows 0..7 twice, {mode}")
+        if mode == "eager":
+            G.rollout(g, _tape_law(), 0, 8)
+            G.rollout(g, _tape_law(), 8, 8)
+        else:
+            G.rollout_graph(g, _tape_law(), 0, 8, 2)
+        G.sync(g)
+        G.download(g)
+        runs.append((w.px.copy(), w.py.copy(), w.ang.copy(), w.broken.copy()))
+        G.destroy(g)
+    assert all(np.array_equal(a, b) for a, b in zip(*runs))
+
+
 @pytest.mark.parametrize("case", ["uniform4096", "uneven512"])
 def test_plain_and_persistent_bit_identical(case):
     """Both paths hold global binary32 state between steps, run each step
