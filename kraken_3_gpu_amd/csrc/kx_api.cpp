@@ -205,7 +205,12 @@ static int uf_find(std::vector<int> &p, int x) {
 }
 
 // ── exact cull grid over the rect list ──────────────────────────────
-static int build_grid(kx_handle *h, const float *rects, int nR, float rmax) {
+// Also two cell bitmaps, each padded to a multiple of four words and stored
+// back to back: occupancy (cells with a registered rect) and occupancy
+// dilated by ceil(reach / cs) cells, where reach bounds every collision
+// point's distance from its body centre: a body whose centre cell is clear in
+// the dilated map has no point near any registered rect.
+static int build_grid(kx_handle *h, const float *rects, int nR, float rmax, float reach) {
   float half = 0.0f;
   float x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
   for (int i = 0; i < nR; i++) {
@@ -241,14 +246,26 @@ static int build_grid(kx_handle *h, const float *rects, int nR, float rmax) {
       ids.insert(ids.end(), cells[c].begin(), cells[c].end());
     }
   }
-  std::vector<uint32_t> bits(((size_t)gnx * gny + 31) / 32, 0u);
+  const size_t W4 = (((size_t)gnx * gny + 31) / 32 + 3) / 4 * 4;
+  std::vector<uint32_t> bits(2 * W4, 0u);
   for (size_t c = 0; c + 1 < off.size(); c++)
     if (off[c + 1] > off[c]) bits[c >> 5] |= 1u << (c & 31);
+  const int kd = std::max(1, (int)std::ceil((reach + 0.01f) / cs));
+  for (int iy = 0; iy < gny; iy++)
+    for (int ix = 0; ix < gnx; ix++) {
+      const size_t c = (size_t)iy * gnx + ix;
+      if (!((bits[c >> 5] >> (c & 31)) & 1u)) continue;
+      for (int jy = std::max(0, iy - kd); jy <= std::min(gny - 1, iy + kd); jy++)
+        for (int jx = std::max(0, ix - kd); jx <= std::min(gnx - 1, ix + kd); jx++) {
+          const size_t e = (size_t)jy * gnx + jx;
+          bits[W4 + (e >> 5)] |= 1u << (e & 31);
+        }
+    }
   uint32_t *dbits;
   TRY(dalloc(h, &dbits, bits.size()));
   TRY(dput(dbits, bits.data(), bits.size()));
   h->d.cell_bits = dbits;
-  h->d.cell_words = (int)bits.size();
+  h->d.cell_words = (int)W4;
   int *doff, *dids;
   TRY(dalloc(h, &doff, off.size()));
   TRY(dalloc(h, &dids, ids.size()));
@@ -322,8 +339,8 @@ static int build_persist(kx_handle *h) {
   const int nwg = (int)wg_tmpl.size();
   const int Gk = mixed ? 0 : (lanes0 > 0 ? lanes0 : h->G);    // kernel: 0 = mixed-lane kernel
   const int stride = (kx_persist_cr_words(NB, NC, NP) + 31) / 32 * 32 + 4;   // largest region (reported)
-  int bmw = (h->d.cell_words + 3) / 4 * 4;
-  if ((size_t)bmw * 4 > 32 * 1024) bmw = 0;      // huge grids: no LDS bitmap
+  int bmw = 2 * h->d.cell_words;                 // occupancy + dilated bitmaps
+  if ((size_t)bmw * 4 > 32 * 1024) bmw = 0;      // huge grids: no LDS bitmaps
   // Stage the whole map (rects of the cell entries, 16-bit cell offsets) in
   // LDS when that costs no occupancy the launch can use: with nwg workgroups
   // on nCU CUs, at most ceil(nwg / nCU) of them are ever resident on one CU.
@@ -570,11 +587,12 @@ static int create_impl(const kx_world_desc *w, const kx_creature_desc *cd, const
   TRY(dalloc(h, &ip, nC)); TRY(dput(ip, h->h2i.data(), nC)); d.col_list = ip;
   // points
   std::vector<float> plx(nP), ply(nP), pr(nP);
-  float rmax = 0.0f;
+  float rmax = 0.0f, reach = 0.0f;
   for (int q = 0; q < nP; q++) {
     int s = porder[q];
     plx[q] = w->p_lx[s]; ply[q] = w->p_ly[s]; pr[q] = w->p_r[s];
     rmax = std::max(rmax, std::fabs(w->p_r[s]));
+    reach = std::max(reach, std::sqrt(w->p_lx[s] * w->p_lx[s] + w->p_ly[s] * w->p_ly[s]));
   }
   TRY(dalloc(h, &f, nP)); TRY(dput(f, plx.data(), nP)); d.plx = f;
   TRY(dalloc(h, &f, nP)); TRY(dput(f, ply.data(), nP)); d.ply = f;
@@ -586,7 +604,7 @@ static int create_impl(const kx_world_desc *w, const kx_creature_desc *cd, const
   TRY(dalloc(h, &r4, nR));
   TRY(dput((float *)r4, w->rects, 4 * (size_t)nR));
   d.rects = r4; d.nr = nR;
-  TRY(build_grid(h, w->rects, nR, rmax));
+  TRY(build_grid(h, w->rects, nR, rmax, reach));
   // morphology templates (creature-local static structure, deduplicated)
   if (h->persist_ok) {
     std::map<std::string, int> seen;

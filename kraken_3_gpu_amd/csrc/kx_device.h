@@ -26,10 +26,12 @@
 #define KX_INV_TAU 0.159154943091895336f
 
 __device__ __forceinline__ float kx_fold(float d) {
+  // for |d| <= pi, d/(2 pi) rounds to 0 and r == d exactly, so no separate
+  // in-range test is needed
   float q = rintf(d * KX_INV_TAU);
   float r = fmaf(-q, KX_TAU_HI, d);
   r = fmaf(-q, KX_TAU_LO, r);
-  return fabsf(d) <= KX_PI_BELOW ? d : (isfinite(d) ? r : 0.0f);
+  return isfinite(d) ? r : 0.0f;
 }
 
 // angleDelta(a, b) (xpbd.js:137-143): the difference folded into (-pi, pi]

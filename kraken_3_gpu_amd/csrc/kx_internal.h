@@ -54,8 +54,8 @@ struct KxDev {
   int nr;
   const int *cell_off;           // [gnx*gny+1]
   const int *cell_ids;
-  const uint32_t *cell_bits;     // occupancy bitmap, 1 bit per cell
-  int cell_words;
+  const uint32_t *cell_bits;     // occupancy bitmap, 1 bit per cell, then the dilated one
+  int cell_words;                // words per bitmap (a multiple of 4)
   float gx0, gy0, inv_cs;
   int gnx, gny;
   // creatures / arms

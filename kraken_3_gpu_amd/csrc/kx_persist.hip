@@ -83,37 +83,6 @@ __device__ __forceinline__ float kx_swap_pair(float v) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
 }
 
-// True if the square [x - r, x + r] x [y - r, y + r] (creature-local, origin
-// (ox, oy)) overlaps a grid cell holding a rect, or its position is not
-// finite. Cell indices come from the same monotone binary32 expressions as
-// the queries (kx_sdf_grid*), so a point inside the square never lands in a
-// cell outside the tested range; cells outside the grid never hit. A square
-// narrower than a cell spans at most 2 x 2 cells, read with four independent
-// loads; wider squares take the general loop.
-__device__ __forceinline__ bool kx_box_touches_cells(const KxDev &d, const uint32_t *cbits, float x, float y, float r,
-                                                     float ox, float oy) {
-  const float ax = (x - r + ox - d.gx0) * d.inv_cs, bx = (x + r + ox - d.gx0) * d.inv_cs;
-  const float ay = (y - r + oy - d.gy0) * d.inv_cs, by = (y + r + oy - d.gy0) * d.inv_cs;
-  if (!(isfinite(ax) && isfinite(bx) && isfinite(ay) && isfinite(by))) return true;
-  const float nx = (float)d.gnx, ny = (float)d.gny;
-  const int ix0 = (int)fminf(fmaxf(floorf(ax), 0.0f), nx), ix1 = (int)fmaxf(fminf(floorf(bx), nx - 1.0f), -1.0f);
-  const int iy0 = (int)fminf(fmaxf(floorf(ay), 0.0f), ny), iy1 = (int)fmaxf(fminf(floorf(by), ny - 1.0f), -1.0f);
-  if (ix0 > ix1 || iy0 > iy1) return false;
-  if (ix1 - ix0 <= 1 && iy1 - iy0 <= 1) {
-    const int c00 = iy0 * d.gnx + ix0, c01 = c00 + (ix1 - ix0), c10 = c00 + (iy1 - iy0) * d.gnx,
-              c11 = c10 + (ix1 - ix0);
-    const uint32_t w00 = cbits[c00 >> 5], w01 = cbits[c01 >> 5], w10 = cbits[c10 >> 5], w11 = cbits[c11 >> 5];
-    return ((w00 >> (c00 & 31)) | (w01 >> (c01 & 31)) | (w10 >> (c10 & 31)) | (w11 >> (c11 & 31))) & 1u;
-  }
-  bool hit = false;
-  for (int iy = iy0; iy <= iy1; iy++)
-    for (int ix = ix0; ix <= ix1; ix++) {
-      const int c = iy * d.gnx + ix;
-      hit = hit || ((cbits[c >> 5] >> (c & 31)) & 1u);
-    }
-  return hit;
-}
-
 // The map staged in LDS: per cell entry the rect itself (ascending rect
 // index within a cell, the fold order), then 16-bit cell offsets. Two entries
 // are read at once; a cell rarely holds more.
@@ -361,35 +330,68 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
           if (act && sub == 0) evw[1] = 0u;
           for (int wi = sub; wi < (act ? nfw : 0); wi += G) flg[wi] = 0u;
           KX_WAVE_SYNC();
-          uint32_t mine = 0u;
-          const uint32_t *cb = P.bm_words ? bits : d.cell_bits;
-          // a lane's first four bodies: every load issued before any test
+          // A body is a candidate when its centre cell is set in the dilated
+          // bitmap (occupancy grown by kd = ceil((reach + 0.01) / cs) cells,
+          // reach bounding every point's distance from its body centre): one
+          // LDS read per body. It never drops a point the query could hit:
+          // the centre and the point go through the same monotone binary32
+          // cell expression as the query, whose rounding (~1e-5 cell) stays
+          // inside the 0.01 px margin, so their cells differ by at most kd;
+          // clamping the centre into the grid only moves it toward in-grid
+          // points, and a query outside the grid never hits.
+          // The bitmaps are read from LDS only; a map whose bitmaps do not
+          // fit there gets no broad phase (every body is a candidate, which
+          // is exact). A pointer that may point to LDS or to HBM would compile
+          // to flat loads, which wait on both the LDS and the memory counters.
+          const bool use_bm = P.bm_words != 0;
+          const uint32_t *dil = bits + (P.bm_words >> 1);
+          auto near = [&](float x, float y) {
+            const float gx = (x + ox - d.gx0) * d.inv_cs, gy = (y + oy - d.gy0) * d.inv_cs;
+            if (!use_bm || !(isfinite(gx) && isfinite(gy))) return true;
+            const int ix = (int)fminf(fmaxf(floorf(gx), 0.0f), (float)(d.gnx - 1));
+            const int iy = (int)fminf(fmaxf(floorf(gy), 0.0f), (float)(d.gny - 1));
+            const int c = iy * d.gnx + ix;
+            return ((dil[c >> 5] >> (c & 31)) & 1u) != 0u;
+          };
+          // a lane's bodies sub, sub + G, ...: flags and point counts first
+          // (every load issued before any test), then one reservation in the
+          // creature's candidate list for all of the lane's points
           constexpr int NBL = 4;
-          float4 brs[NBL], pbs[NBL];
+          uint32_t mine = 0u;
+          int ntot = 0;
+          {
+            float4 brs[NBL], pbs[NBL];
 #pragma unroll
-          for (int k = 0; k < NBL; k++) {
-            const int i = sub + k * G;
-            brs[k] = t_brec[i < nb ? i : 0];
-            pbs[k] = pos[i < nb ? i : 0];
-          }
+            for (int k = 0; k < NBL; k++) {
+              const int i = sub + k * G;
+              brs[k] = t_brec[i < nb ? i : 0];
+              pbs[k] = pos[i < nb ? i : 0];
+            }
 #pragma unroll
-          for (int k = 0; k < NBL; k++) {
-            const int i = sub + k * G;
-            const int bp = __float_as_int(brs[k].z), cnt = bp >> 16;
-            if (i < nb && brs[k].x != 0.0f && cnt > 0 &&
-                kx_box_touches_cells(d, cb, pbs[k].x, pbs[k].y, brs[k].w + 0.01f, ox, oy)) {
-              mine |= 1u << k;
-              const int at = (int)atomicAdd(&evw[1], (uint32_t)cnt);
-              for (int t = 0; t < cnt; t++) cand[at + t] = (uint32_t)((bp & 0xffff) + t) | ((uint32_t)i << 16);
+            for (int k = 0; k < NBL; k++) {
+              const int i = sub + k * G;
+              const int cnt = __float_as_int(brs[k].z) >> 16;
+              if (i < nb && brs[k].x != 0.0f && cnt > 0 && near(pbs[k].x, pbs[k].y)) {
+                mine |= 1u << k;
+                ntot += cnt;
+              }
             }
           }
           for (int k = NBL, i = sub + NBL * G; i < nb; k++, i += G) {
             const float4 br = t_brec[i], pb = pos[i];
-            const int bp = __float_as_int(br.z), cnt = bp >> 16;
-            if (br.x != 0.0f && cnt > 0 && kx_box_touches_cells(d, cb, pb.x, pb.y, br.w + 0.01f, ox, oy)) {
+            const int cnt = __float_as_int(br.z) >> 16;
+            if (br.x != 0.0f && cnt > 0 && near(pb.x, pb.y)) {
               mine |= 1u << k;
-              const int at = (int)atomicAdd(&evw[1], (uint32_t)cnt);
+              ntot += cnt;
+            }
+          }
+          if (ntot) {
+            int at = (int)atomicAdd(&evw[1], (uint32_t)ntot);
+            for (uint32_t m = mine; m; m &= m - 1u) {
+              const int i = sub + __builtin_ctz(m) * G;
+              const int bp = __float_as_int(t_brec[i].z), cnt = bp >> 16;
               for (int t = 0; t < cnt; t++) cand[at + t] = (uint32_t)((bp & 0xffff) + t) | ((uint32_t)i << 16);
+              at += cnt;
             }
           }
           if (__any(mine != 0u)) {
