@@ -55,10 +55,10 @@ def main():
     ap.add_argument("--kernel", required=True)
     ap.add_argument("--stats", required=True)
     ap.add_argument("--trace")
-    ap.add_argument("--fetch", required=True)
-    ap.add_argument("--write", required=True)
-    ap.add_argument("--copy-fetch", required=True)
-    ap.add_argument("--copy-write", required=True)
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--copy-fetch")
+    ap.add_argument("--copy-write")
     ap.add_argument("--copy-bytes", type=int, default=256 << 20)
     ap.add_argument("--sq", nargs="*", default=[])
     ap.add_argument("--workload", required=True)
@@ -66,14 +66,16 @@ def main():
     ap.add_argument("--steps-per-launch", type=int, required=True)
     ap.add_argument("--command", action="append", default=[])
     ap.add_argument("--out", required=True)
-    ap.add_argument("--json", required=True)
+    ap.add_argument("--json")
+    ap.add_argument("--note", action="append", default=[], help="free text lines appended to the summary")
     a = ap.parse_args()
 
     stats = _rows(a.stats)
-    fetch = counter_per_dispatch(a.fetch, a.kernel, "FETCH_SIZE")
-    write = counter_per_dispatch(a.write, a.kernel, "WRITE_SIZE")
-    ff = copy_factor(a.copy_fetch, "FETCH_SIZE", a.copy_bytes)
-    fw = copy_factor(a.copy_write, "WRITE_SIZE", a.copy_bytes)
+    traffic = bool(a.fetch and a.write and a.copy_fetch and a.copy_write)
+    fetch = counter_per_dispatch(a.fetch, a.kernel, "FETCH_SIZE") if traffic else {}
+    write = counter_per_dispatch(a.write, a.kernel, "WRITE_SIZE") if traffic else {}
+    ff = copy_factor(a.copy_fetch, "FETCH_SIZE", a.copy_bytes) if traffic else None
+    fw = copy_factor(a.copy_write, "WRITE_SIZE", a.copy_bytes) if traffic else None
     fetch_kib = max(fetch.values()) if fetch else None
     write_kib = max(write.values()) if write else None
     fetch_b = fetch_kib * 1024 * ff if fetch_kib is not None and ff else None
@@ -88,11 +90,16 @@ def main():
     if a.trace:
         d = trace_durations(a.trace, a.kernel)
         if d:
-            lines += ["", f"{a.kernel} dispatch durations (ms): " + ", ".join(f"{x:.3f}" for x in d)]
-    lines += ["", "## HBM traffic per launch (rocprofv3 --pmc, one counter per run)", "",
-              "| counter | KiB per dispatch | copy-probe scale | bytes per launch |", "|---|---|---|---|",
-              f"| FETCH_SIZE | {fetch_kib} | {ff} | {fetch_b} |",
-              f"| WRITE_SIZE | {write_kib} | {fw} | {write_b} |"]
+            s = sorted(d)
+            lines += ["", f"{a.kernel} dispatches: {len(d)}, mean {sum(d) / len(d):.3f} ms, median "
+                          f"{s[len(s) // 2]:.3f} ms, min {s[0]:.3f} ms, max {s[-1]:.3f} ms"]
+            if len(d) <= 12:
+                lines += [f"{a.kernel} dispatch durations (ms): " + ", ".join(f"{x:.3f}" for x in d)]
+    if traffic:
+        lines += ["", "## HBM traffic per launch (rocprofv3 --pmc, one counter per run)", "",
+                  "| counter | KiB per dispatch | copy-probe scale | bytes per launch |", "|---|---|---|---|",
+                  f"| FETCH_SIZE | {fetch_kib} | {ff} | {fetch_b} |",
+                  f"| WRITE_SIZE | {write_kib} | {fw} | {write_b} |"]
     sq = {}
     for p in a.sq:
         for r in _rows(p):
@@ -108,8 +115,12 @@ def main():
         if sq.get("SQ_INSTS_VALU") and "SQ_THREAD_CYCLES_VALU" in sq:
             lines.append("")
             lines.append(f"Active lanes per VALU instruction: {sq['SQ_THREAD_CYCLES_VALU'] / (sq['SQ_INSTS_VALU'] * 64):.3f}")
+    if a.note:
+        lines += [""] + a.note
     with open(a.out, "w") as f:
         f.write("\n".join(lines) + "\n")
+    if not a.json:
+        return
     with open(a.json, "w") as f:
         json.dump({"kernel": a.kernel, "workload": a.workload, "creatures": a.creatures,
                    "steps_per_launch": a.steps_per_launch, "fetch_kib": fetch_kib, "write_kib": write_kib,
