@@ -288,16 +288,16 @@ static int build_persist(kx_handle *h) {
   const int WPB = h->wpb;
   std::vector<uint32_t> pool;
   std::vector<KxTmpl> T;
-  int words = 0, NB = 1, NC = 1, NP = 1;
+  int words = 0, NB = 1, NC = 1, NP = 1, NG = 0;
   h->tmpl_rounds.clear();
   for (size_t t = 0; t < h->tmpl_host.size(); t++) {
     KxTmpl k = kx_pack_template(h->tmpl_host[t], tmpl_lanes_of(h, t), pool);
     // creature regions: 16-byte records, regions offset by 16 bytes modulo
     // 128 so neighbouring creatures' same records fall in different banks
-    k.cr_stride = (kx_persist_cr_words(k.nb, k.nc, k.np) + 31) / 32 * 32 + 4;
+    k.cr_stride = (kx_persist_cr_words(k.nb, k.nc, k.np, k.ngrip) + 31) / 32 * 32 + 4;
     T.push_back(k);
     words = std::max(words, k.words);
-    NB = std::max(NB, k.nb); NC = std::max(NC, k.nc); NP = std::max(NP, k.np);
+    NB = std::max(NB, k.nb); NC = std::max(NC, k.nc); NP = std::max(NP, k.np); NG = std::max(NG, k.ngrip);
     h->tmpl_rounds.push_back(k.R);
   }
   std::vector<std::vector<int>> members(T.size());
@@ -338,7 +338,7 @@ static int build_persist(kx_handle *h) {
   }
   const int nwg = (int)wg_tmpl.size();
   const int Gk = mixed ? 0 : (lanes0 > 0 ? lanes0 : h->G);    // kernel: 0 = mixed-lane kernel
-  const int stride = (kx_persist_cr_words(NB, NC, NP) + 31) / 32 * 32 + 4;   // largest region (reported)
+  const int stride = (kx_persist_cr_words(NB, NC, NP, NG) + 31) / 32 * 32 + 4;   // largest region (reported)
   int bmw = 2 * h->d.cell_words;                 // occupancy + dilated bitmaps
   if ((size_t)bmw * 4 > 32 * 1024) bmw = 0;      // huge grids: no LDS bitmaps
   // Stage the whole map (rects of the cell entries, 16-bit cell offsets) in

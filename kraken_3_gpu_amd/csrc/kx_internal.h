@@ -111,7 +111,7 @@ struct KxPersistCfg {
 };
 int kx_persist_launch(int G, int wpb, const KxDev &d, const KxLaw *law, int step0, const int *step_base,
                       int nsteps, int ev_row0, int with_forces, const KxPersistCfg &P, hipStream_t st);
-int kx_persist_cr_words(int NB, int NC, int NP);
+int kx_persist_cr_words(int NB, int NC, int NP, int NG);
 int kx_persist_occupancy(int G, int wpb, size_t lds);   // G = 0: the mixed-lane kernel
 int kx_bump_counter(int *counter, int delta, hipStream_t st);
 

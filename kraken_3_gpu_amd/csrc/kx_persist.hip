@@ -39,18 +39,22 @@ typedef float kx_f2v __attribute__((ext_vector_type(2)));
     (v) = make_float4(t_.x, t_.y, t_.z, t_.w); \
   } while (0)
 
-// per-creature LDS layout (word offsets, 16-byte records):
+// per-creature LDS layout (word offsets; records of 16 or 8 bytes):
 //   pos[i] = {ppx, ppy, angP, -}   prv[i] = {px, py, ang, -}   vel[i] = {vx, vy, av, -}
-//   dyn[k] = {alpha, target, lbx, lby}   lam[k] = {lambda_x|lambda, lambda_y, breakForce, broken}
+//   dyn[k] = {alpha, broken, lbx, lby} (pin, grip) or {alpha, target, broken, -} (angle):
+//            each type keeps its broken flag in the word it does not use
+//   lam[k] = {lambda_x, lambda_y} (pin, grip) or {lambda, -} (angle)
+//   gbf[j] = break force of the j-th grip (the only constraints that break)
 //   ev     = {step events, collision candidate count, -, -}
 //   flg    = one bit per collision point: "corrects at the pose after the rounds"
 //   cand   = collision candidates: point | body << 16
 struct KxCrLds {
-  int pos, prv, vel, dyn, lam, ev, flg, cand, words;
-  __host__ __device__ KxCrLds(int NB, int NC, int NP) {
+  int pos, prv, vel, dyn, lam, gbf, ev, flg, cand, words;
+  __host__ __device__ KxCrLds(int NB, int NC, int NP, int NG) {
     int o = 0;
     pos = o; o += 4 * NB; prv = o; o += 4 * NB; vel = o; o += 4 * NB;
-    dyn = o; o += 4 * NC; lam = o; o += 4 * NC;
+    dyn = o; o += 4 * NC; lam = o; o += (2 * NC + 3) / 4 * 4;
+    gbf = o; o += (NG + 3) / 4 * 4;
     ev = o; o += 4;
     flg = o; o += ((NP + 31) / 32 + 3) / 4 * 4;
     cand = o; o += (NP + 3) / 4 * 4;
@@ -58,10 +62,11 @@ struct KxCrLds {
   }
 };
 
-int kx_persist_cr_words(int NB, int NC, int NP) { return KxCrLds(NB, NC, NP).words; }
+int kx_persist_cr_words(int NB, int NC, int NP, int NG) { return KxCrLds(NB, NC, NP, NG).words; }
 
 // Adds (sx, sy) to a creature's body positions (start and predicted) and to
-// its world anchors; the same binary32 additions as the plain path's shift.
+// the world anchors of its world-anchored pins and grips; the same binary32
+// additions as the plain path's shift (an angle constraint has no anchor).
 __device__ __forceinline__ void kx_shift_creature(float4 *prv, float4 *pos, float4 *dyn, const float4 *t_crec, int nb,
                                                   int nc, int sub, int G, float sx, float sy) {
   for (int i = sub; i < nb; i += G) {
@@ -69,12 +74,14 @@ __device__ __forceinline__ void kx_shift_creature(float4 *prv, float4 *pos, floa
     p.x += sx; p.y += sy; pp.x += sx; pp.y += sy;
     prv[i] = p; pos[i] = pp;
   }
-  for (int k = sub; k < nc; k += G)
-    if (((__float_as_int(t_crec[2 * k + 1].x) >> 8) & 255) == 255) {
+  for (int k = sub; k < nc; k += G) {
+    const int v = __float_as_int(t_crec[k].w);
+    if (((v >> 8) & 255) == 255 && (v >> 16) != KX_ANGLE) {
       float4 dk = dyn[k];
       dk.z += sx; dk.w += sy;
       dyn[k] = dk;
     }
+  }
 }
 
 // Value of the neighbouring lane of an adjacent pair (lanes 2j <-> 2j+1),
@@ -139,7 +146,7 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
   }
   __syncthreads();
 
-  const float4 *t_crec = (const float4 *)(smem + T.o_crec);   // 2 per constraint
+  const float4 *t_crec = (const float4 *)(smem + T.o_crec);   // one per constraint
   const float4 *t_brec = (const float4 *)(smem + T.o_brec);
   const float4 *t_prec = (const float4 *)(smem + T.o_prec);
   const int4 *t_sched4 = (const int4 *)(smem + T.o_sched);
@@ -147,13 +154,16 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
   const int *t_arm_tip = (const int *)(smem + T.o_arm_tip), *t_arm_grip = (const int *)(smem + T.o_arm_grip);
   const int *t_mot_k = (const int *)(smem + T.o_mot_k), *t_mot_slot = (const int *)(smem + T.o_mot_slot);
   const int *t_grip_k = (const int *)(smem + T.o_grip_k), *t_grip_arm = (const int *)(smem + T.o_grip_arm);
+  const int *t_arm_gslot = (const int *)(smem + T.o_arm_gslot);
 
   const int cid = P.wave_cr[P.wg_cr_off[bid] + w * CPW + q];
   const bool act = cid >= 0;
-  const KxCrLds Lo(T.nb, T.nc, T.np);
+  const KxCrLds Lo(T.nb, T.nc, T.np, T.ngrip);
   uint32_t *S = smem + P.tmpl_words + P.bm_words + P.map_words + (size_t)(w * CPW + q) * T.cr_stride;
   float4 *pos = (float4 *)(S + Lo.pos), *prv = (float4 *)(S + Lo.prv), *vel = (float4 *)(S + Lo.vel);
-  float4 *dyn = (float4 *)(S + Lo.dyn), *lamr = (float4 *)(S + Lo.lam);
+  float4 *dyn = (float4 *)(S + Lo.dyn);
+  kx_f2v *lam2 = (kx_f2v *)(S + Lo.lam);
+  float *gbf = (float *)(S + Lo.gbf);
   uint32_t *evw = S + Lo.ev, *flg = S + Lo.flg;
   uint32_t *cand = S + Lo.cand;
   const int nb = act ? T.nb : 0, nc = act ? T.nc : 0;
@@ -166,10 +176,13 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
     vel[i] = make_float4(d.vx[g], d.vy[g], d.av[g], 0.0f);
   }
   for (int k = sub; k < nc; k += G) {
-    int g = c0 + k;
-    dyn[k] = make_float4(d.comp[g] * d.invH2, d.target[g], d.lbx[g], d.lby[g]);
-    lamr[k] = make_float4(0.0f, 0.0f, d.bf[g], __int_as_float(d.broken[g]));
+    const int g = c0 + k;
+    const float alpha = d.comp[g] * d.invH2, brk = __int_as_float(d.broken[g]);
+    dyn[k] = (__float_as_int(t_crec[k].w) >> 16) == KX_ANGLE ? make_float4(alpha, d.target[g], brk, 0.0f)
+                                                              : make_float4(alpha, brk, d.lbx[g], d.lby[g]);
+    lam2[k] = kx_f2v{0.0f, 0.0f};
   }
+  for (int j = sub; j < (act ? T.ngrip : 0); j += G) gbf[j] = d.bf[c0 + t_grip_k[j]];
   double my_cd = 0.0;
   if (act && sub < T.narms) my_cd = d.cd[cid * KX_MAX_ARMS + sub];
   const double hS = d.hd * d.S;
@@ -203,15 +216,16 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
         const uint32_t ck = (uint32_t)cid + L.cbase;
         double adh = rec ? (double)rec[3 + 3 * sub] : kx_ctl_u(L.seed, ck, (uint32_t)key, KX_SLOT_ADH + sub);
         double gu = (L.flags & KX_LAW_FIXED_GRIP) ? kx_ctl_u(L.seed, ck, KX_GRIP_DRAW_KEY, KX_SLOT_ADH + sub) : 0.0;
-        int g = t_arm_grip[sub], t = t_arm_tip[sub];
-        float4 dk = dyn[g], lk = lamr[g], tp = prv[t], tv = vel[t], cr0 = t_crec[2 * g];   // {lax, lay, -, -}
-        int bk = __float_as_int(lk.w);
-        float f_bf = lk.z, gx = dk.z, gy = dk.w, lx = lk.x, ly = lk.y;
+        const int g = t_arm_grip[sub], t = t_arm_tip[sub], j = t_arm_gslot[sub];
+        float4 dk = dyn[g], tp = prv[t], tv = vel[t], cr0 = t_crec[g];   // {lax, lay, -, cab}
+        const kx_f2v lk = lam2[g];
+        int bk = __float_as_int(dk.y);
+        float f_bf = gbf[j], gx = dk.z, gy = dk.w, lx = lk.x, ly = lk.y;
         uint32_t e = kx_reflex(L, hS, adh, gu, my_cd, bk, f_bf, gx, gy, lx, ly, tv.x, tv.y, (double)tp.x, (double)tp.y,
                                tp.z, cr0.x, cr0.y);
-        dk.z = gx; dk.w = gy;
-        dyn[g] = dk;
-        lamr[g] = make_float4(lx, ly, f_bf, __int_as_float(bk));
+        dyn[g] = make_float4(dk.x, __int_as_float(bk), gx, gy);
+        lam2[g] = kx_f2v{lx, ly};
+        gbf[j] = f_bf;
         my_ev |= e << (3 * sub);
       }
       KX_WAVE_SYNC();
@@ -225,7 +239,7 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
     KX_WAVE_SYNC();
     for (int s = 0; s < d.S; s++) {
       // resetLambda + integrate (xpbd.js:262-287)
-      for (int k = sub; k < nc; k += G) *(kx_f2v *)&lamr[k] = kx_f2v{0.0f, 0.0f};
+      for (int k = sub; k < nc; k += G) lam2[k] = kx_f2v{0.0f, 0.0f};
       const bool forces = with_forces && step == 0 && s == 0;
       for (int i = sub; i < nb; i += G) {
         float fx = 0.0f, fy = 0.0f, tq = 0.0f;
@@ -262,31 +276,35 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
               const int k = dsc & 255, A = (dsc >> 8) & 255, B0 = (dsc >> 16) & 255;
               const bool hasB = B0 != 255;
               const int B = hasB ? B0 : A;
-              // each round type reads only the fields it uses
-              float4 lk = lamr[k];
+              // the constraint's own records (dyn, lambda) are read ahead of
+              // the round-type branch; otherwise each type reads only the
+              // fields it uses
+              float4 dk = dyn[k];
+              kx_f2v lk = lam2[k];
               bool moved;
               if (is_ang) {
-                // crec {rest, min, max, -}, dyn {alpha, target}, angles, inverse inertias
-                kx_f3v cr = *(const kx_f3v *)&t_crec[2 * k];
-                kx_f2v dk = *(const kx_f2v *)&dyn[k];
+                // crec {rest, min, max}, dyn {alpha, target, broken}, lambda,
+                // angles, inverse inertias
+                kx_f3v cr = *(const kx_f3v *)&t_crec[k];
                 float tA = ((const float *)&pos[A])[2], tB = ((const float *)&pos[B])[2];
                 float iA = ((const float *)&t_brec[A])[1], iB = ((const float *)&t_brec[B])[1];
-                KX_HOLD4(lk);
-                asm volatile("" : "+v"(cr), "+v"(dk), "+v"(tA), "+v"(tB), "+v"(iA), "+v"(iB));
-                moved = kx_angle_core(tA, tB, iA, iB, hasB, cr.x, dk.y, cr.y, cr.z, dk.x, lk.x);
-                if (moved && __float_as_int(lk.w) == 0) {
+                KX_HOLD4(dk);
+                asm volatile("" : "+v"(cr), "+v"(lk), "+v"(tA), "+v"(tB), "+v"(iA), "+v"(iB));
+                float lam = lk.x;
+                moved = kx_angle_core(tA, tB, iA, iB, hasB, cr.x, dk.y, cr.y, cr.z, dk.x, lam);
+                if (moved && __float_as_int(dk.z) == 0) {
                   if (hasB) ((float *)&pos[B])[2] = tB;
                   ((float *)&pos[A])[2] = tA;
-                  ((float *)&lamr[k])[0] = lk.x;
+                  ((float *)&lam2[k])[0] = lam;
                 }
               } else {
-                // crec {lax, lay}, dyn {alpha, -, lbx, lby}, positions, {im, ii}
-                kx_f2v cr = *(const kx_f2v *)&t_crec[2 * k];
-                float4 dk = dyn[k];
+                // crec {lax, lay}, dyn {alpha, broken, lbx, lby}, lambda,
+                // positions, {im, ii}
+                kx_f2v cr = *(const kx_f2v *)&t_crec[k];
                 kx_f3v pA = *(const kx_f3v *)&pos[A], pB = *(const kx_f3v *)&pos[B];
                 kx_f2v bA = *(const kx_f2v *)&t_brec[A], bB = *(const kx_f2v *)&t_brec[B];
-                KX_HOLD4(lk); KX_HOLD4(dk);
-                asm volatile("" : "+v"(cr), "+v"(pA), "+v"(pB), "+v"(bA), "+v"(bB));
+                KX_HOLD4(dk);
+                asm volatile("" : "+v"(cr), "+v"(lk), "+v"(pA), "+v"(pB), "+v"(bA), "+v"(bB));
                 // lanes 2j and 2j+1 hold the same pin: each evaluates sin/cos
                 // of one body's angle and the pair swaps results; the even
                 // lane commits
@@ -295,17 +313,18 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
                 kx_sincos(odd ? pB.z : pA.z, &s1, &c1);
                 const float s2 = kx_swap_pair(s1), c2 = kx_swap_pair(c1);
                 float xA = pA.x, yA = pA.y, tA = pA.z, xB = pB.x, yB = pB.y, tB = pB.z;
+                float lx = lk.x, ly = lk.y;
                 moved = kx_pin_core_sc(xA, yA, tA, bA.x, bA.y, xB, yB, tB, bB.x, bB.y, hasB, cr.x, cr.y, dk.z, dk.w,
-                                       dk.x, lk.x, lk.y, odd ? s2 : s1, odd ? c2 : c1, odd ? s1 : s2,
+                                       dk.x, lx, ly, odd ? s2 : s1, odd ? c2 : c1, odd ? s1 : s2,
                                        odd ? c1 : c2) &&
                         !odd;
                 pA = kx_f3v{xA, yA, tA};
                 pB = kx_f3v{xB, yB, tB};
-                if (moved && __float_as_int(lk.w) == 0) {
+                if (moved && __float_as_int(dk.y) == 0) {
                   // B first: for a world anchor B == A and pA must win
                   *(kx_f3v *)&pos[B] = pB;
                   *(kx_f3v *)&pos[A] = pA;
-                  *(kx_f2v *)&lamr[k] = kx_f2v{lk.x, lk.y};
+                  lam2[k] = kx_f2v{lx, ly};
                 }
               }
             }
@@ -458,12 +477,13 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
     }
     // grip break (xpbd.js:496-504 / solver.wgsl:433-442)
     for (int j = sub; j < (act ? T.ngrip : 0); j += G) {
-      int k = t_grip_k[j];
-      float4 lk = lamr[k];
-      if (__float_as_int(lk.w)) continue;
+      const int k = t_grip_k[j];
+      if (__float_as_int(((const float *)&dyn[k])[1])) continue;
+      const kx_f2v lk = lam2[k];
       float f = sqrtf(lk.x * lk.x + lk.y * lk.y) * d.invH;
-      if (f > lk.z) {
-        lamr[k] = make_float4(0.0f, 0.0f, lk.z, __int_as_float(1));
+      if (f > gbf[j]) {
+        lam2[k] = kx_f2v{0.0f, 0.0f};
+        ((float *)&dyn[k])[1] = __int_as_float(1);
         int arm = t_grip_arm[j];
         if (arm >= 0) my_ev |= 4u << (3 * arm);
       }
@@ -486,16 +506,22 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
     d.ppx[g] = pp.x; d.ppy[g] = pp.y; d.angP[g] = pp.z;
   }
   for (int k = sub; k < nc; k += G) {
-    int g = c0 + k;
-    int v = __float_as_int(t_crec[2 * k + 1].x);
-    float4 dk = dyn[k], lk = lamr[k];
-    d.broken[g] = __float_as_int(lk.w);
-    if (((v >> 8) & 255) == 255) { d.lbx[g] = dk.z; d.lby[g] = dk.w; }
-    d.bf[g] = lk.z; d.target[g] = dk.y;
-    if (has_law && (v >> 16) == KX_ANGLE) d.comp[g] = (float)comp_last;
-    if ((v >> 16) == KX_ANGLE) { d.lam[g] = lk.x; d.lamx[g] = 0.0f; d.lamy[g] = 0.0f; }
-    else { d.lamx[g] = lk.x; d.lamy[g] = lk.y; d.lam[g] = 0.0f; }
+    const int g = c0 + k;
+    const int v = __float_as_int(t_crec[k].w);
+    const float4 dk = dyn[k];
+    const kx_f2v lk = lam2[k];
+    if ((v >> 16) == KX_ANGLE) {
+      d.broken[g] = __float_as_int(dk.z);
+      d.target[g] = dk.y;
+      if (has_law) d.comp[g] = (float)comp_last;
+      d.lam[g] = lk.x; d.lamx[g] = 0.0f; d.lamy[g] = 0.0f;
+    } else {
+      d.broken[g] = __float_as_int(dk.y);
+      if (((v >> 8) & 255) == 255) { d.lbx[g] = dk.z; d.lby[g] = dk.w; }
+      d.lamx[g] = lk.x; d.lamy[g] = lk.y; d.lam[g] = 0.0f;
+    }
   }
+  for (int j = sub; j < (act ? T.ngrip : 0); j += G) d.bf[c0 + t_grip_k[j]] = gbf[j];
   if (act && sub < T.narms) d.cd[cid * KX_MAX_ARMS + sub] = my_cd;
 }
 

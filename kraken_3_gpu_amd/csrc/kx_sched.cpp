@@ -112,8 +112,8 @@ static uint32_t fbits(float x) {
 // Serialise one template with its schedule for G lanes into the word pool.
 // Sections start on 16-byte boundaries; constraint, body and point records
 // are 16-byte vectors so the kernel reads each with one LDS instruction:
-//   crec[k] = {lax, lay, -, -} (pin, grip) or {rest, min, max, -} (angle),
-//             then {a | b << 8 | type << 16, -, -, -}          (2 x 16 B)
+//   crec[k] = {lax, lay, -, cab} (pin, grip) or {rest, min, max, cab}
+//             (angle), cab = a | b << 8 | type << 16            (16 B)
 //   brec[b] = {im, ii, point offset | count << 16, reach of its points}
 //   prec[q] = {lx, ly, r, body}
 KxTmpl kx_pack_template(const TmplHost &t, int G, std::vector<uint32_t> &pool) {
@@ -133,9 +133,9 @@ KxTmpl kx_pack_template(const TmplHost &t, int G, std::vector<uint32_t> &pool) {
   for (int k = 0; k < T.nc; k++) {
     int cab = t.a[k] | ((t.b[k] < 0 ? 255 : t.b[k]) << 8) | (t.type[k] << 16);
     if (t.type[k] == KX_ANGLE)
-      pool.insert(pool.end(), {fbits(t.rest[k]), fbits(t.mn[k]), fbits(t.mx[k]), 0u, (uint32_t)cab, 0u, 0u, 0u});
+      pool.insert(pool.end(), {fbits(t.rest[k]), fbits(t.mn[k]), fbits(t.mx[k]), (uint32_t)cab});
     else
-      pool.insert(pool.end(), {fbits(t.lax[k]), fbits(t.lay[k]), 0u, 0u, (uint32_t)cab, 0u, 0u, 0u});
+      pool.insert(pool.end(), {fbits(t.lax[k]), fbits(t.lay[k]), 0u, (uint32_t)cab});
   }
   T.o_brec = at();
   for (int b = 0; b < t.nb; b++) {
@@ -184,6 +184,11 @@ KxTmpl kx_pack_template(const TmplHost &t, int G, std::vector<uint32_t> &pool) {
   T.o_mot_slot = at(); put(pool, t.mot_slot);
   T.o_grip_k = at(); put(pool, t.grip_k);
   T.o_grip_arm = at(); put(pool, t.grip_arm);
+  std::vector<int> arm_gslot(t.arm_grip.size(), -1);
+  for (size_t a = 0; a < t.arm_grip.size(); a++)
+    for (size_t j = 0; j < t.grip_k.size(); j++)
+      if (t.grip_k[j] == t.arm_grip[a]) arm_gslot[a] = (int)j;
+  T.o_arm_gslot = at(); put(pool, arm_gslot);
   T.words = at();
   return T;
 }

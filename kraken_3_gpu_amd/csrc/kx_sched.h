@@ -16,6 +16,7 @@ struct KxTmpl {
   //          k | A << 8 | B << 16 (-1 idle);
   // o_rtype: round-type bitmask words (bit r set = angle round)
   int o_sched, o_rtype, o_coll, o_ccnt, o_arm_tip, o_arm_grip, o_mot_k, o_mot_slot, o_grip_k, o_grip_arm;
+  int o_arm_gslot;                // per arm: the slot of its grip in the grip list
 };
 
 // Host form of a template, in creature-local indices (constraints in the
