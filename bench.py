@@ -287,17 +287,18 @@ def bench_uneven(args):
 
 
 def bench_serve(args):
-    """ES rollout requests through the continuous-batching server: a burst of
-    requests first measures the serving capacity (creature-steps/s with the
-    queue never empty), then a Poisson stream at 70 % of that capacity gives
-    p50 / p99 request latency (arrival -> result) and the aggregate rate."""
+    """ES rollout requests through the continuous-batching server: a closed
+    loop with 48 requests always outstanding first measures the serving
+    capacity (creature-steps/s executed with the queue never empty), then a
+    Poisson stream at 70 % of that capacity gives p50 / p99 request latency
+    (arrival -> result) and the aggregate rate."""
     from kraken_3_gpu_amd import serve as S
     chunk = args.chunk if args.chunk_set else 50
     opts = S.default_opts(chunk_steps=chunk)
-    n_probe, n_req = 24, args.requests
+    n_req = args.requests
     with S.Server(opts) as srv:
-        t, reqs = S.es_stream(n_probe, rate=0.0, seed=101)
-        _, probe = S.run_stream(srv, t, reqs)
+        _, reqs = S.es_stream(64, rate=0.0, seed=101)
+        probe = S.saturate(srv, reqs)
     cap = probe["creature_steps_per_s"]
     t, reqs = S.es_stream(n_req, rate=1.0, seed=7)
     mean_cs = float(np.mean([2 * r["pairs"] * r["steps"] for r in reqs]))

@@ -6,6 +6,9 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libkxpbd.so")
+# sources with no device code: compiled for the host only (they use x86
+# builtins such as the run-time AVX2 check)
+HOST_ONLY = ("kx_serve.cpp",)
 SOURCES = ["kx_api.cpp", "kx_builder.cpp", "kx_sched.cpp", "kx_serve.cpp", "kx_plain.hip", "kx_persist.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -24,6 +27,8 @@ def build(verbose=False):
         obj = os.path.join(CSRC, "_build", s + ".o")
         os.makedirs(os.path.dirname(obj), exist_ok=True)
         lang = ["-x", "hip"] if s.endswith(".hip") else []
+        if s in HOST_ONLY:
+            lang = ["--offload-host-only"]
         cmd = [HIPCC] + FLAGS + lang + ["-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)

@@ -548,20 +548,47 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist_mix(KxDev d, KxLaw 
   }
 }
 
+// A pointer read from memory is generic (flat) to the compiler; taking it as
+// a global pointer lets every access through it compile to a global load or
+// store, which waits on the memory counter only (a flat access also holds
+// back the LDS counter). The cast changes no bits; the empty asm keeps the
+// compiler from folding the cast pair away.
+template <class T>
+__device__ __forceinline__ T *kx_glob(T *p) {
+  __attribute__((address_space(1))) T *g = (__attribute__((address_space(1))) T *)p;
+  asm volatile("" : "+s"(g));
+  return (T *)g;
+}
+
 // Several handles in one grid (the serving loop's continuous batch): each
 // workgroup runs its entry's body with that entry's state, tables, law and
-// step count.
+// step count. The entry's HBM pointers are taken as global pointers.
 template <int WPB>
 __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist_batch(const KxBatchEnt *__restrict__ E,
                                                                    const int2 *__restrict__ wgmap) {
   const int2 m = wgmap[blockIdx.x];
   const KxBatchEnt &e = E[m.x];
   const int bid = m.y;
-  switch (e.P.wg_lanes[bid]) {
-    case 8: kx_persist_body<8, WPB>(e.d, e.L, e.has_law, e.step0, nullptr, e.nsteps, 0, 0, e.P, bid); break;
-    case 16: kx_persist_body<16, WPB>(e.d, e.L, e.has_law, e.step0, nullptr, e.nsteps, 0, 0, e.P, bid); break;
-    case 32: kx_persist_body<32, WPB>(e.d, e.L, e.has_law, e.step0, nullptr, e.nsteps, 0, 0, e.P, bid); break;
-    default: kx_persist_body<64, WPB>(e.d, e.L, e.has_law, e.step0, nullptr, e.nsteps, 0, 0, e.P, bid); break;
+  KxDev d = e.d;
+  d.px = kx_glob(d.px); d.py = kx_glob(d.py); d.ang = kx_glob(d.ang);
+  d.vx = kx_glob(d.vx); d.vy = kx_glob(d.vy); d.av = kx_glob(d.av);
+  d.ppx = kx_glob(d.ppx); d.ppy = kx_glob(d.ppy); d.angP = kx_glob(d.angP);
+  d.fx = kx_glob(d.fx); d.fy = kx_glob(d.fy); d.tq = kx_glob(d.tq);
+  d.comp = kx_glob(d.comp); d.target = kx_glob(d.target); d.lbx = kx_glob(d.lbx); d.lby = kx_glob(d.lby);
+  d.bf = kx_glob(d.bf); d.broken = kx_glob(d.broken);
+  d.lam = kx_glob(d.lam); d.lamx = kx_glob(d.lamx); d.lamy = kx_glob(d.lamy);
+  d.rects = kx_glob(d.rects); d.cell_off = kx_glob(d.cell_off); d.cell_ids = kx_glob(d.cell_ids);
+  d.cell_bits = kx_glob(d.cell_bits); d.cr = kx_glob(d.cr); d.cd = kx_glob(d.cd); d.tape = kx_glob(d.tape);
+  d.events = nullptr;
+  KxPersistCfg P = e.P;
+  P.wave_cr = kx_glob(P.wave_cr); P.wg_tmpl = kx_glob(P.wg_tmpl); P.wg_lanes = kx_glob(P.wg_lanes);
+  P.wg_cr_off = kx_glob(P.wg_cr_off); P.tmpl = kx_glob(P.tmpl); P.pool = kx_glob(P.pool);
+  const KxLaw L = e.L;
+  switch (P.wg_lanes[bid]) {
+    case 8: kx_persist_body<8, WPB>(d, L, e.has_law, e.step0, nullptr, e.nsteps, 0, 0, P, bid); break;
+    case 16: kx_persist_body<16, WPB>(d, L, e.has_law, e.step0, nullptr, e.nsteps, 0, 0, P, bid); break;
+    case 32: kx_persist_body<32, WPB>(d, L, e.has_law, e.step0, nullptr, e.nsteps, 0, 0, P, bid); break;
+    default: kx_persist_body<64, WPB>(d, L, e.has_law, e.step0, nullptr, e.nsteps, 0, 0, P, bid); break;
   }
 }
 

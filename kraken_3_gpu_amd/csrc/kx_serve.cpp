@@ -69,7 +69,7 @@ uint32_t pair_seed(uint32_t noise_seed, uint32_t p) {
 // tanh in binary32, the same in every loop and thread: odd Taylor series
 // below 1/16, else 1 - 2/(exp(2|x|) + 1) with exp from a Cody-Waite range
 // reduction by ln 2 and its degree-6 Taylor polynomial; |x| >= 9 gives +-1.
-inline float tanh32(float x) {
+__attribute__((always_inline)) inline float tanh32(float x) {
   const float ax = std::fabs(x);
   const float x2 = x * x;
   const float small = x * (1.0f + x2 * (-1.0f / 3.0f + x2 * (2.0f / 15.0f + x2 * (-17.0f / 315.0f))));
@@ -159,7 +159,8 @@ void build_policy(const kx_request &q, Policy &P) {
 
 // Tape records of pairs [p0, p1) for steps step0 .. step0+n-1 into
 // tape[(row * ncr + creature) * KX_TAPE_WORDS]; `A` holds W1 x + b1 per row.
-void policy_rows(const Policy &P, int p0, int p1, int step0, int n, double step_s, float *tape) {
+__attribute__((always_inline)) inline void policy_rows_body(const Policy &P, int p0, int p1, int step0, int n,
+                                                            double step_s, float *tape) {
   const int ncr = 2 * P.pairs;
   alignas(64) float x[NI], A[NH], D[NH], h[2][NH], out[NO];
   for (int r = 0; r < n; r++) {
@@ -204,6 +205,26 @@ void policy_rows(const Policy &P, int p0, int p1, int step0, int n, double step_
       }
     }
   }
+}
+
+// The same loops compiled twice: for AVX2 (8-wide, and vector round-to-
+// nearest for tanh's range reduction) and for the x86-64 baseline, chosen at
+// run time. No FMA in either: every product and sum is rounded on its own, so
+// both give the same bits.
+__attribute__((target("avx2"))) void policy_rows_avx2(const Policy &P, int p0, int p1, int step0, int n,
+                                                      double step_s, float *tape) {
+  policy_rows_body(P, p0, p1, step0, n, step_s, tape);
+}
+void policy_rows_base(const Policy &P, int p0, int p1, int step0, int n, double step_s, float *tape) {
+  policy_rows_body(P, p0, p1, step0, n, step_s, tape);
+}
+void policy_rows(const Policy &P, int p0, int p1, int step0, int n, double step_s, float *tape) {
+  static const bool avx2 = [] {
+    __builtin_cpu_init();
+    return __builtin_cpu_supports("avx2") != 0;
+  }();
+  if (avx2) policy_rows_avx2(P, p0, p1, step0, n, step_s, tape);
+  else policy_rows_base(P, p0, p1, step0, n, step_s, tape);
 }
 
 // ── worker pool ────────────────────────────────────────────────────────
@@ -294,6 +315,7 @@ struct Req {
   std::vector<float> px0;           // first body x of every creature at spawn
   int status = 0;
   size_t dev_bytes = 0;
+  int cost = 0;                     // relative time of one of its workgroups per step
 };
 
 struct Chunk {
@@ -323,7 +345,8 @@ struct kx_server {
   std::deque<Req *> ready;          // admitted (handle built), not yet scheduled
   std::vector<Req *> active;        // scheduled (scheduler thread only)
   std::deque<kx_request_done> done;
-  int n_admitted_live = 0;          // ready + active + retiring
+  int n_admitted_live = 0;          // building + ready + active
+  int64_t pending_bytes = 0;        // admission estimates of requests still being built
   bool stop = false, sched_exit = false;
   std::atomic<int64_t> launches{0}, creature_steps{0}, peak_active{0}, admit_waits{0};
   std::atomic<int> n_done{0};
@@ -339,10 +362,16 @@ struct kx_server {
   float *dl_host = nullptr;
   size_t dl_cap = 0;
 
-  std::thread th_admit, th_sched;
+  std::vector<std::thread> th_admit;  // parallel builders, taking requests in arrival order
+  std::thread th_sched, th_reap;
+  std::deque<Req *> reap;            // finished requests whose memory is still held
+  std::condition_variable cv_reap;
+  bool reap_stop = false;
+  std::atomic<int> n_reaped{0};
 
   void admit_loop();
   void sched_loop();
+  void reap_loop();
   int build(Req *r);
   void release(Req *r);
   int issue(int64_t id, Chunk &c);
@@ -420,6 +449,12 @@ int kx_server::build(Req *r) {
       return server_fail(KX_ENOMEM, "device tape allocation failed");
   }
   r->dev_bytes = r->v.bytes + 2 * tw * 4;
+  {
+    // one morphology per request: rounds per iteration and collision points
+    // per lane of its template set the time of each of its workgroups
+    int32_t ti[6] = {0, 0, 0, 0, 0, 0};
+    if (kx_template_info(r->h, 0, ti) >= 0) r->cost = 8 * ti[4] + 4 * ((ti[2] + ti[5] - 1) / std::max(ti[5], 1));
+  }
   build_policy(q, r->pol);
   return 0;
 }
@@ -445,16 +480,37 @@ void kx_server::finish(Req *r, int status) {
   d.t_admit = r->t_admit;
   d.t_done = secs_since(t0);
   d.device_bytes = (int64_t)r->dev_bytes;
-  release(r);
   {
     std::lock_guard<std::mutex> g(m);
     done.push_back(d);
-    n_admitted_live--;
+    reap.push_back(r);
+    n_admitted_live--;               // the slot is free; the memory is until reaped
   }
   n_done++;
   cv_done.notify_all();
+  cv_reap.notify_one();
   cv_admit.notify_all();
-  delete r;
+}
+
+// Frees finished requests off the scheduler's path: freeing device memory
+// can wait for the device, which would drain the chunks in flight. A
+// request's memory returns to admission when it has been freed here.
+void kx_server::reap_loop() {
+  if (hipSetDevice(device) != hipSuccess) return;
+  for (;;) {
+    Req *r = nullptr;
+    {
+      std::unique_lock<std::mutex> l(m);
+      cv_reap.wait(l, [&] { return reap_stop || !reap.empty(); });
+      if (reap.empty()) return;
+      r = reap.front();
+      reap.pop_front();
+    }
+    release(r);
+    delete r;
+    n_reaped++;
+    cv_admit.notify_all();
+  }
 }
 
 // Device bytes of a request before its handle exists: the handle's arrays
@@ -474,17 +530,18 @@ void kx_server::admit_loop() {
   if (hipSetDevice(device) != hipSuccess) return;
   for (;;) {
     Req *r = nullptr;
+    int64_t est = 0;
     {
       std::unique_lock<std::mutex> l(m);
       cv_admit.wait(l, [&] { return stop || (!queued.empty() && n_admitted_live < o.max_active); });
       if (stop) return;
       r = queued.front();
+      // free HBM less what the builds in flight are about to allocate
       size_t free_b = 0, total_b = 0;
-      l.unlock();
+      est = (int64_t)estimate_bytes(r->q, K);
       const bool mem_ok = hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
-                          (int64_t)free_b - (int64_t)estimate_bytes(r->q, K) >= o.hbm_reserve;
-      l.lock();
-      if (!mem_ok && n_admitted_live == 0) {
+                          (int64_t)free_b - pending_bytes - est >= o.hbm_reserve;
+      if (!mem_ok && n_admitted_live == 0 && reap.empty() && n_reaped.load() == n_done.load()) {
         // nothing resident will ever return memory: the request cannot fit
         queued.pop_front();
         n_admitted_live++;
@@ -496,22 +553,24 @@ void kx_server::admit_loop() {
       if (!mem_ok) {
         // wait for a retirement to return memory (or for stop)
         admit_waits++;
-        const int seen = n_done.load();
-        cv_admit.wait_for(l, std::chrono::milliseconds(20), [&] { return stop || n_done.load() != seen; });
+        const int seen = n_reaped.load();
+        cv_admit.wait_for(l, std::chrono::milliseconds(20), [&] { return stop || n_reaped.load() != seen; });
         continue;
       }
       queued.pop_front();
       n_admitted_live++;
+      pending_bytes += est;
     }
     const int rc = build(r);
+    {
+      std::lock_guard<std::mutex> g(m);
+      pending_bytes -= est;
+      if (rc >= 0) ready.push_back(r);
+    }
     if (rc < 0) {
       r->status = rc;
       finish(r, rc);
       continue;
-    }
-    {
-      std::lock_guard<std::mutex> g(m);
-      ready.push_back(r);
     }
     cv_sched.notify_all();
   }
@@ -570,7 +629,16 @@ int kx_server::issue(int64_t id, Chunk &c) {
     L.stiff_min = l.stiff_min; L.stiff_max = l.stiff_max;
     L.cbase = 0; L.flags = (l.flags & KX_LAW_FIXED_GRIP) | KX_LAW_TAPE;
   }
+  // workgroups longest first (cost per step x steps this chunk), so the
+  // chunk ends on short workgroups rather than on the largest morphologies
+  std::vector<int> order(ne);
+  for (size_t i = 0; i < ne; i++) order[i] = (int)i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+    return (int64_t)c.members[a]->cost * c.nsteps[a] > (int64_t)c.members[b]->cost * c.nsteps[b];
+  });
   size_t w = 0;
+  for (int i : order)
+    for (int b = 0; b < c.members[i]->v.P.nwg; b++) M[w++] = make_int2(i, b);
   for (size_t i = 0; i < ne; i++) {
     Req *r = c.members[i];
     E[i].d = r->v.d;
@@ -581,7 +649,6 @@ int kx_server::issue(int64_t id, Chunk &c) {
     E[i].step0 = r->launched;
     E[i].nsteps = c.nsteps[i];
     E[i].pad = 0;
-    for (int b = 0; b < r->v.P.nwg; b++) M[w++] = make_int2((int)i, b);
   }
   if (hipMemcpyAsync(ent_dev[slot], E, ne * sizeof(KxBatchEnt), hipMemcpyHostToDevice, comp) != hipSuccess ||
       hipMemcpyAsync(map_dev[slot], M, nwg * sizeof(int2), hipMemcpyHostToDevice, comp) != hipSuccess ||
@@ -719,7 +786,8 @@ extern "C" int kx_server_create(const kx_server_opts *opts, kx_server **out) {
     }
   }
   s->pool.reset(new Pool(nt - 1));
-  s->th_admit = std::thread([s] { s->admit_loop(); });
+  for (int k = 0; k < 4; k++) s->th_admit.emplace_back([s] { s->admit_loop(); });
+  s->th_reap = std::thread([s] { s->reap_loop(); });
   s->th_sched = std::thread([s] { s->sched_loop(); });
   *out = s;
   return 0;
@@ -787,8 +855,15 @@ extern "C" void kx_server_destroy(kx_server *s) {
   }
   s->cv_admit.notify_all();
   s->cv_sched.notify_all();
-  if (s->th_admit.joinable()) s->th_admit.join();
+  for (auto &t : s->th_admit)
+    if (t.joinable()) t.join();
   if (s->th_sched.joinable()) s->th_sched.join();
+  {
+    std::lock_guard<std::mutex> g(s->m);
+    s->reap_stop = true;
+  }
+  s->cv_reap.notify_all();
+  if (s->th_reap.joinable()) s->th_reap.join();
   for (Req *r : s->ready) {
     s->release(r);
     delete r;

@@ -151,6 +151,40 @@ def es_stream(n, rate, seed=1, pairs=(16, 512), steps=(300, 1500), map_=None):
     return t, reqs
 
 
+def saturate(server, reqs, outstanding=48, warm_s=2.0, measure_s=4.0):
+    """Serving capacity with the queue never empty: keeps `outstanding`
+    requests submitted (a new one as each completes, cycling through reqs)
+    and returns the creature-steps/s the server executed over `measure_s`
+    seconds after `warm_s` seconds of warm-up, with the device-busy fraction
+    of that window."""
+    it = iter(reqs * 1000)
+    tag = [0]
+    live = [0]
+
+    def top_up():
+        while live[0] < outstanding:
+            tag[0] += 1
+            server.submit(dict(next(it), tag=1_000_000 + tag[0]))
+            live[0] += 1
+    top_up()
+    t0 = time.perf_counter()
+    marks = []
+    while True:
+        live[0] -= len(server.poll(wait=0.02))
+        top_up()
+        el = time.perf_counter() - t0
+        if not marks and el >= warm_s:
+            marks.append(server.info())
+        if marks and el >= warm_s + measure_s:
+            marks.append(server.info())
+            break
+    a, b = marks
+    dt = b["now"] - a["now"]
+    return dict(creature_steps_per_s=(b["creature_steps"] - a["creature_steps"]) / dt,
+                device_busy_frac=(b["busy"] - a["busy"]) / dt, window_s=dt, outstanding=outstanding,
+                launches=b["launches"] - a["launches"])
+
+
 def run_stream(server, arrivals, reqs, want_state=False, timeout=600.0):
     """Submits reqs at their arrival offsets (relative to now) and collects
     every completion. Returns (results sorted by tag, summary)."""
