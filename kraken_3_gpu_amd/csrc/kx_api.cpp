@@ -206,10 +206,19 @@ static int uf_find(std::vector<int> &p, int x) {
 
 // ── exact cull grid over the rect list ──────────────────────────────
 // Also two cell bitmaps, each padded to a multiple of four words and stored
-// back to back: occupancy (cells with a registered rect) and occupancy
-// dilated by ceil(reach / cs) cells, where reach bounds every collision
-// point's distance from its body centre: a body whose centre cell is clear in
-// the dilated map has no point near any registered rect.
+// back to back: occupancy (cull cells with a registered rect) and the near
+// bitmap, on a grid with the cull grid's origin and cells of cs / m (m = 2
+// when that costs at most 256 words, else 1). A near cell's bit is set when
+// the distance from the cell to some rect's box is below reach + rmax plus a
+// margin, where reach bounds every collision point's distance from its body
+// centre and rmax every point's radius. A point p of a body centred at c in a
+// clear cell is then farther than rmax from every rect, so no query of p
+// corrects it (a correction needs the fold's distance below the point's
+// radius, and the fold's nearest point lies in the rect's box whatever the
+// signs of its width and height). The margin (0.05 px + fs / 1000) covers the
+// binary32 rounding of the device's cell expression and of the world point; a
+// centre outside the grid clamps to the edge cell, whose distance to every
+// rect (all inside the grid) is no larger than the centre's.
 static int build_grid(kx_handle *h, const float *rects, int nR, float rmax, float reach) {
   float half = 0.0f;
   float x0 = INFINITY, y0 = INFINITY, x1 = -INFINITY, y1 = -INFINITY;
@@ -246,26 +255,51 @@ static int build_grid(kx_handle *h, const float *rects, int nR, float rmax, floa
       ids.insert(ids.end(), cells[c].begin(), cells[c].end());
     }
   }
-  const size_t W4 = (((size_t)gnx * gny + 31) / 32 + 3) / 4 * 4;
-  std::vector<uint32_t> bits(2 * W4, 0u);
+  auto words4 = [](size_t cells) { return ((cells + 31) / 32 + 3) / 4 * 4; };
+  const size_t W4 = words4((size_t)gnx * gny);
+  const int m = words4((size_t)4 * gnx * gny) <= 256 ? 2 : 1;
+  const int nnx = m * gnx, nny = m * gny;
+  const size_t N4 = words4((size_t)nnx * nny);
+  std::vector<uint32_t> bits(W4 + N4, 0u);
   for (size_t c = 0; c + 1 < off.size(); c++)
     if (off[c + 1] > off[c]) bits[c >> 5] |= 1u << (c & 31);
-  const int kd = std::max(1, (int)std::ceil((reach + 0.01f) / cs));
-  for (int iy = 0; iy < gny; iy++)
-    for (int ix = 0; ix < gnx; ix++) {
-      const size_t c = (size_t)iy * gnx + ix;
-      if (!((bits[c >> 5] >> (c & 31)) & 1u)) continue;
-      for (int jy = std::max(0, iy - kd); jy <= std::min(gny - 1, iy + kd); jy++)
-        for (int jx = std::max(0, ix - kd); jx <= std::min(gnx - 1, ix + kd); jx++) {
-          const size_t e = (size_t)jy * gnx + jx;
-          bits[W4 + (e >> 5)] |= 1u << (e & 31);
+  {
+    const double fs = (double)cs / m;
+    const double D = (double)reach + (double)rmax + 0.05 + 1e-3 * fs;
+    uint32_t *nb = bits.data() + W4;
+    for (int i = 0; i < nR; i++) {
+      const float *r = rects + 4 * i;
+      const double bx0 = std::min((double)r[0], (double)r[0] + r[2]), bx1 = std::max((double)r[0], (double)r[0] + r[2]);
+      const double by0 = std::min((double)r[1], (double)r[1] + r[3]), by1 = std::max((double)r[1], (double)r[1] + r[3]);
+      if (!(std::isfinite(bx0) && std::isfinite(bx1) && std::isfinite(by0) && std::isfinite(by1))) {
+        for (size_t k = 0; k < N4; k++) nb[k] = ~0u;      // unknown geometry: every body is a candidate
+        break;
+      }
+      const int ix0 = std::max(0, (int)std::floor((bx0 - D - gx0) / fs) - 1);
+      const int ix1 = std::min(nnx - 1, (int)std::floor((bx1 + D - gx0) / fs) + 1);
+      const int iy0 = std::max(0, (int)std::floor((by0 - D - gy0) / fs) - 1);
+      const int iy1 = std::min(nny - 1, (int)std::floor((by1 + D - gy0) / fs) + 1);
+      for (int iy = iy0; iy <= iy1; iy++)
+        for (int ix = ix0; ix <= ix1; ix++) {
+          const double cx0 = gx0 + ix * fs, cy0 = gy0 + iy * fs;
+          const double dx = std::max(0.0, std::max(bx0 - (cx0 + fs), cx0 - bx1));
+          const double dy = std::max(0.0, std::max(by0 - (cy0 + fs), cy0 - by1));
+          if (dx * dx + dy * dy < D * D) {
+            const size_t e = (size_t)iy * nnx + ix;
+            nb[e >> 5] |= 1u << (e & 31);
+          }
         }
     }
+  }
   uint32_t *dbits;
   TRY(dalloc(h, &dbits, bits.size()));
   TRY(dput(dbits, bits.data(), bits.size()));
   h->d.cell_bits = dbits;
   h->d.cell_words = (int)W4;
+  h->d.near_words = (int)N4;
+  h->d.inv_ns = (float)m / cs;
+  h->d.nnx = nnx;
+  h->d.nny = nny;
   int *doff, *dids;
   TRY(dalloc(h, &doff, off.size()));
   TRY(dalloc(h, &dids, ids.size()));
@@ -339,7 +373,7 @@ static int build_persist(kx_handle *h) {
   const int nwg = (int)wg_tmpl.size();
   const int Gk = mixed ? 0 : (lanes0 > 0 ? lanes0 : h->G);    // kernel: 0 = mixed-lane kernel
   const int stride = (kx_persist_cr_words(NB, NC, NP, NG) + 31) / 32 * 32 + 4;   // largest region (reported)
-  int bmw = 2 * h->d.cell_words;                 // occupancy + dilated bitmaps
+  int bmw = h->d.cell_words + h->d.near_words;   // occupancy + near bitmaps
   if ((size_t)bmw * 4 > 32 * 1024) bmw = 0;      // huge grids: no LDS bitmaps
   // Stage the whole map (rects of the cell entries, 16-bit cell offsets) in
   // LDS when that costs no occupancy the launch can use: with nwg workgroups

@@ -54,10 +54,12 @@ struct KxDev {
   int nr;
   const int *cell_off;           // [gnx*gny+1]
   const int *cell_ids;
-  const uint32_t *cell_bits;     // occupancy bitmap, 1 bit per cell, then the dilated one
-  int cell_words;                // words per bitmap (a multiple of 4)
+  const uint32_t *cell_bits;     // occupancy bitmap (1 bit per cull cell), then the near bitmap
+  int cell_words, near_words;    // words of each bitmap (multiples of 4)
   float gx0, gy0, inv_cs;
   int gnx, gny;
+  float inv_ns;                  // near-bitmap cells per px (a power-of-two multiple of inv_cs)
+  int nnx, nny;                  // near-bitmap grid, same origin as the cull grid
   // creatures / arms
   const KxCreature *cr;
   const int *bcr;                // creature of every body

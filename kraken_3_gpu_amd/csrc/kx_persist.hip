@@ -349,57 +349,73 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
           if (act && sub == 0) evw[1] = 0u;
           for (int wi = sub; wi < (act ? nfw : 0); wi += G) flg[wi] = 0u;
           KX_WAVE_SYNC();
-          // A body is a candidate when its centre cell is set in the dilated
-          // bitmap (occupancy grown by kd = ceil((reach + 0.01) / cs) cells,
-          // reach bounding every point's distance from its body centre): one
-          // LDS read per body. It never drops a point the query could hit:
-          // the centre and the point go through the same monotone binary32
-          // cell expression as the query, whose rounding (~1e-5 cell) stays
-          // inside the 0.01 px margin, so their cells differ by at most kd;
-          // clamping the centre into the grid only moves it toward in-grid
-          // points, and a query outside the grid never hits.
+          // A body is a candidate when its centre's cell is set in the near
+          // bitmap: one LDS read per body. A clear cell lies farther than
+          // reach + rmax (+ a margin for the binary32 cell expression) from
+          // every rect's box, reach bounding every point's distance from its
+          // body centre and rmax every point's radius, so no point of such a
+          // body corrects; a centre outside the grid clamps to the edge cell,
+          // which is no farther from any rect than the centre.
           // The bitmaps are read from LDS only; a map whose bitmaps do not
           // fit there gets no broad phase (every body is a candidate, which
           // is exact). A pointer that may point to LDS or to HBM would compile
           // to flat loads, which wait on both the LDS and the memory counters.
           const bool use_bm = P.bm_words != 0;
-          const uint32_t *dil = bits + (P.bm_words >> 1);
-          auto near = [&](float x, float y) {
-            const float gx = (x + ox - d.gx0) * d.inv_cs, gy = (y + oy - d.gy0) * d.inv_cs;
-            if (!use_bm || !(isfinite(gx) && isfinite(gy))) return true;
-            const int ix = (int)fminf(fmaxf(floorf(gx), 0.0f), (float)(d.gnx - 1));
-            const int iy = (int)fminf(fmaxf(floorf(gy), 0.0f), (float)(d.gny - 1));
-            const int c = iy * d.gnx + ix;
-            return ((dil[c >> 5] >> (c & 31)) & 1u) != 0u;
+          const uint32_t *nearb = bits + (use_bm ? d.cell_words : 0);   // word 0: any word of the LDS allocation
+          // the near-bitmap cell of a body centre: its word (always a valid
+          // index: NaN clamps to 0, +-inf to the edges; word 0 without
+          // bitmaps), its bit, and whether the test may drop the body at all
+          // (bitmaps present and a finite cell coordinate)
+          auto cell = [&](float x, float y, int &wi, int &bi, bool &testable) {
+            const float gx = (x + ox - d.gx0) * d.inv_ns, gy = (y + oy - d.gy0) * d.inv_ns;
+            testable = use_bm && isfinite(gx) && isfinite(gy);
+            const int ix = (int)fminf(fmaxf(floorf(gx), 0.0f), (float)(d.nnx - 1));
+            const int iy = (int)fminf(fmaxf(floorf(gy), 0.0f), (float)(d.nny - 1));
+            const int c = iy * d.nnx + ix;
+            wi = use_bm ? c >> 5 : 0;
+            bi = c & 31;
           };
-          // a lane's bodies sub, sub + G, ...: flags and point counts first
-          // (every load issued before any test), then one reservation in the
-          // creature's candidate list for all of the lane's points
+          // a lane's bodies sub, sub + G, ...: records, poses and bitmap
+          // words all read before any test (no branch per body), then one
+          // reservation in the creature's candidate list for all of the
+          // lane's points
           constexpr int NBL = 4;
           uint32_t mine = 0u;
           int ntot = 0;
           {
-            float4 brs[NBL], pbs[NBL];
+            kx_f2v brs[NBL], pbs[NBL];   // {im, points}, {x, y}
 #pragma unroll
             for (int k = 0; k < NBL; k++) {
-              const int i = sub + k * G;
-              brs[k] = t_brec[i < nb ? i : 0];
-              pbs[k] = pos[i < nb ? i : 0];
+              const int i = sub + k * G, ic = i < nb ? i : 0;
+              brs[k] = kx_f2v{((const float *)&t_brec[ic])[0], ((const float *)&t_brec[ic])[2]};
+              pbs[k] = *(const kx_f2v *)&pos[ic];
+            }
+            uint32_t wd[NBL];
+            int bi[NBL];
+            bool testable[NBL];
+#pragma unroll
+            for (int k = 0; k < NBL; k++) {
+              int wi;
+              cell(pbs[k].x, pbs[k].y, wi, bi[k], testable[k]);
+              wd[k] = nearb[wi];
             }
 #pragma unroll
             for (int k = 0; k < NBL; k++) {
               const int i = sub + k * G;
-              const int cnt = __float_as_int(brs[k].z) >> 16;
-              if (i < nb && brs[k].x != 0.0f && cnt > 0 && near(pbs[k].x, pbs[k].y)) {
-                mine |= 1u << k;
-                ntot += cnt;
-              }
+              const int cnt = __float_as_int(brs[k].y) >> 16;
+              const bool nr = !testable[k] || ((wd[k] >> bi[k]) & 1u) != 0u;
+              const bool c = i < nb && brs[k].x != 0.0f && cnt > 0 && nr;
+              mine |= c ? 1u << k : 0u;
+              ntot += c ? cnt : 0;
             }
           }
           for (int k = NBL, i = sub + NBL * G; i < nb; k++, i += G) {
             const float4 br = t_brec[i], pb = pos[i];
             const int cnt = __float_as_int(br.z) >> 16;
-            if (br.x != 0.0f && cnt > 0 && near(pb.x, pb.y)) {
+            int wi, bi;
+            bool testable;
+            cell(pb.x, pb.y, wi, bi, testable);
+            if (br.x != 0.0f && cnt > 0 && (!testable || ((nearb[wi] >> bi) & 1u) != 0u)) {
               mine |= 1u << k;
               ntot += cnt;
             }
