@@ -39,6 +39,37 @@ typedef float kx_f2v __attribute__((ext_vector_type(2)));
     (v) = make_float4(t_.x, t_.y, t_.z, t_.w); \
   } while (0)
 
+// A round slot's records other than the body poses: dyn, lambda, the
+// constraint record (3 words: both round types' fields) and both bodies'
+// {im, ii}, with the decoded slot. A lane with no slot (dsc < 0) reads
+// constraint 0 and body 0 (every read stays unconditional) and is not live.
+struct KxSlot {
+  float4 dk;
+  kx_f2v lk;
+  kx_f3v cr;
+  kx_f2v bA, bB;
+  int k, A, B;
+  bool hasB, live;
+};
+
+__device__ __forceinline__ KxSlot kx_slot_read(int dsc, const float4 *dyn, const kx_f2v *lam2, const float4 *t_crec,
+                                               const float4 *t_brec) {
+  KxSlot s;
+  s.live = dsc >= 0;
+  const int dl = s.live ? dsc : 0;
+  s.k = dl & 255;
+  s.A = (dl >> 8) & 255;
+  const int B0 = (dl >> 16) & 255;
+  s.hasB = B0 != 255;
+  s.B = s.hasB ? B0 : s.A;
+  s.dk = dyn[s.k];
+  s.lk = lam2[s.k];
+  s.cr = *(const kx_f3v *)&t_crec[s.k];
+  s.bA = *(const kx_f2v *)&t_brec[s.A];
+  s.bB = *(const kx_f2v *)&t_brec[s.B];
+  return s;
+}
+
 // per-creature LDS layout (word offsets; records of 16 or 8 bytes):
 //   pos[i] = {ppx, ppy, angP, -}   prv[i] = {px, py, ang, -}   vel[i] = {vx, vy, av, -}
 //   dyn[k] = {alpha, broken, lbx, lby} (pin, grip) or {alpha, target, broken, -} (angle):
@@ -257,79 +288,73 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
       KX_WAVE_SYNC();
       for (int it = 0; it < d.I; it++) {
         // constraint rounds (type-homogeneous, wave-uniform branch), four per
-        // group: a lane reads the group's four slot descriptors with one load.
-        // Every LDS read of a slot is issued before any result is needed; a
-        // broken constraint is solved too and its result dropped.
-        // the next group's descriptors are fetched while this group runs
+        // group: a lane reads the group's four slot descriptors with one load,
+        // and the next group's descriptors are fetched while this group runs.
+        // The rounds are software-pipelined: a round first reads its two
+        // body poses (which the previous round may have written), then
+        // issues the next round's other records (dyn, lambda, constraint and
+        // body records: none of them is written by a round), and waits only
+        // for the poses. A broken constraint is solved too and its result
+        // dropped.
         int4 d4n = act ? t_sched4[sub] : make_int4(-1, -1, -1, -1);
+        // one round: its poses, then the next round's records into `nx`
+        // (every load unconditional and ahead of the round-type branch, so
+        // each join sees the same loads in flight), then the solve
+        auto round = [&](const KxSlot &c, KxSlot &nx, int dnext, bool is_ang) {
+          kx_f3v pA = *(const kx_f3v *)&pos[c.A], pB = *(const kx_f3v *)&pos[c.B];
+          // the poses go to LDS first: the wait below is then for them alone
+          __builtin_amdgcn_sched_barrier(0);
+          nx = kx_slot_read(dnext, dyn, lam2, t_crec, t_brec);
+          asm volatile("" : "+v"(pA), "+v"(pB));
+          if (is_ang) {
+            // crec {rest, min, max}, dyn {alpha, target, broken}, lambda,
+            // angles, inverse inertias
+            float tA = pA.z, tB = pB.z, lam = c.lk.x;
+            const bool moved =
+                kx_angle_core(tA, tB, c.bA.y, c.bB.y, c.hasB, c.cr.x, c.dk.y, c.cr.y, c.cr.z, c.dk.x, lam);
+            if (c.live && moved && __float_as_int(c.dk.z) == 0) {
+              if (c.hasB) ((float *)&pos[c.B])[2] = tB;
+              ((float *)&pos[c.A])[2] = tA;
+              ((float *)&lam2[c.k])[0] = lam;
+            }
+          } else {
+            // crec {lax, lay}, dyn {alpha, broken, lbx, lby}, lambda,
+            // positions, {im, ii}; lanes 2j and 2j+1 hold the same pin:
+            // each evaluates sin/cos of one body's angle and the pair swaps
+            // results; the even lane commits
+            const bool odd = (sub & 1) != 0;
+            float s1, c1;
+            kx_sincos(odd ? pB.z : pA.z, &s1, &c1);
+            const float s2 = kx_swap_pair(s1), c2 = kx_swap_pair(c1);
+            float xA = pA.x, yA = pA.y, tA = pA.z, xB = pB.x, yB = pB.y, tB = pB.z;
+            float lx = c.lk.x, ly = c.lk.y;
+            const bool moved = kx_pin_core_sc(xA, yA, tA, c.bA.x, c.bA.y, xB, yB, tB, c.bB.x, c.bB.y, c.hasB, c.cr.x,
+                                              c.cr.y, c.dk.z, c.dk.w, c.dk.x, lx, ly, odd ? s2 : s1, odd ? c2 : c1,
+                                              odd ? s1 : s2, odd ? c1 : c2) &&
+                               !odd;
+            if (c.live && moved && __float_as_int(c.dk.y) == 0) {
+              // B first: for a world anchor B == A and pA must win
+              *(kx_f3v *)&pos[c.B] = kx_f3v{xB, yB, tB};
+              *(kx_f3v *)&pos[c.A] = kx_f3v{xA, yA, tA};
+              lam2[c.k] = kx_f2v{lx, ly};
+            }
+          }
+          KX_WAVE_SYNC();
+        };
+        // slot records alternate between two register sets: sa for the
+        // even rounds of a group, sb for the odd ones
+        KxSlot sa = kx_slot_read(d4n.x, dyn, lam2, t_crec, t_brec), sb;
         for (int r0 = 0; r0 < T.R; r0 += 4) {
           const uint32_t rbits = r0 < 32 ? rb0 : r0 < 64 ? rb1 : (uint32_t)__builtin_amdgcn_readfirstlane(t_rbits[r0 >> 5]);
           const int4 d4 = d4n;
           if (act && r0 + 4 < T.R) d4n = t_sched4[((r0 >> 2) + 1) * G + sub];
-#pragma unroll
-          for (int u = 0; u < 4; u++) {
-            const int r = r0 + u;
-            if (r >= T.R) break;
-            const bool is_ang = (rbits >> (r & 31)) & 1u;
-            const int dsc = u == 0 ? d4.x : u == 1 ? d4.y : u == 2 ? d4.z : d4.w;
-            if (dsc >= 0) {
-              const int k = dsc & 255, A = (dsc >> 8) & 255, B0 = (dsc >> 16) & 255;
-              const bool hasB = B0 != 255;
-              const int B = hasB ? B0 : A;
-              // the constraint's own records (dyn, lambda) are read ahead of
-              // the round-type branch; otherwise each type reads only the
-              // fields it uses
-              float4 dk = dyn[k];
-              kx_f2v lk = lam2[k];
-              bool moved;
-              if (is_ang) {
-                // crec {rest, min, max}, dyn {alpha, target, broken}, lambda,
-                // angles, inverse inertias
-                kx_f3v cr = *(const kx_f3v *)&t_crec[k];
-                float tA = ((const float *)&pos[A])[2], tB = ((const float *)&pos[B])[2];
-                float iA = ((const float *)&t_brec[A])[1], iB = ((const float *)&t_brec[B])[1];
-                KX_HOLD4(dk);
-                asm volatile("" : "+v"(cr), "+v"(lk), "+v"(tA), "+v"(tB), "+v"(iA), "+v"(iB));
-                float lam = lk.x;
-                moved = kx_angle_core(tA, tB, iA, iB, hasB, cr.x, dk.y, cr.y, cr.z, dk.x, lam);
-                if (moved && __float_as_int(dk.z) == 0) {
-                  if (hasB) ((float *)&pos[B])[2] = tB;
-                  ((float *)&pos[A])[2] = tA;
-                  ((float *)&lam2[k])[0] = lam;
-                }
-              } else {
-                // crec {lax, lay}, dyn {alpha, broken, lbx, lby}, lambda,
-                // positions, {im, ii}
-                kx_f2v cr = *(const kx_f2v *)&t_crec[k];
-                kx_f3v pA = *(const kx_f3v *)&pos[A], pB = *(const kx_f3v *)&pos[B];
-                kx_f2v bA = *(const kx_f2v *)&t_brec[A], bB = *(const kx_f2v *)&t_brec[B];
-                KX_HOLD4(dk);
-                asm volatile("" : "+v"(cr), "+v"(lk), "+v"(pA), "+v"(pB), "+v"(bA), "+v"(bB));
-                // lanes 2j and 2j+1 hold the same pin: each evaluates sin/cos
-                // of one body's angle and the pair swaps results; the even
-                // lane commits
-                const bool odd = (sub & 1) != 0;
-                float s1, c1;
-                kx_sincos(odd ? pB.z : pA.z, &s1, &c1);
-                const float s2 = kx_swap_pair(s1), c2 = kx_swap_pair(c1);
-                float xA = pA.x, yA = pA.y, tA = pA.z, xB = pB.x, yB = pB.y, tB = pB.z;
-                float lx = lk.x, ly = lk.y;
-                moved = kx_pin_core_sc(xA, yA, tA, bA.x, bA.y, xB, yB, tB, bB.x, bB.y, hasB, cr.x, cr.y, dk.z, dk.w,
-                                       dk.x, lx, ly, odd ? s2 : s1, odd ? c2 : c1, odd ? s1 : s2,
-                                       odd ? c1 : c2) &&
-                        !odd;
-                pA = kx_f3v{xA, yA, tA};
-                pB = kx_f3v{xB, yB, tB};
-                if (moved && __float_as_int(dk.y) == 0) {
-                  // B first: for a world anchor B == A and pA must win
-                  *(kx_f3v *)&pos[B] = pB;
-                  *(kx_f3v *)&pos[A] = pA;
-                  lam2[k] = kx_f2v{lx, ly};
-                }
-              }
-            }
-            KX_WAVE_SYNC();
-          }
+          round(sa, sb, d4.y, (rbits >> (r0 & 31)) & 1u);
+          if (r0 + 1 >= T.R) break;
+          round(sb, sa, d4.z, (rbits >> ((r0 + 1) & 31)) & 1u);
+          if (r0 + 2 >= T.R) break;
+          round(sa, sb, d4.w, (rbits >> ((r0 + 2) & 31)) & 1u);
+          if (r0 + 3 >= T.R) break;
+          round(sb, sa, d4n.x, (rbits >> ((r0 + 3) & 31)) & 1u);
         }
         // collisions (xpbd.js:441-470: per body, its points in order, each
         // correction moving the body before the next point is tested).
@@ -550,9 +575,11 @@ __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, i
 }
 
 // Lane count per workgroup (per morphology bucket): a wave-uniform switch to
-// the body instantiated for that count.
+// the body instantiated for that count. Held to 168 VGPRs (three waves per
+// SIMD), as the single-count kernels are: these launches are the ones whose
+// waves can share a SIMD.
 template <int WPB>
-__global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist_mix(KxDev d, KxLaw L, int has_law, int step0,
+__global__ __launch_bounds__(KX_WAVE *WPB) __attribute__((amdgpu_waves_per_eu(3))) void kx_k_persist_mix(KxDev d, KxLaw L, int has_law, int step0,
                                                                  const int *step_base, int nsteps, int ev_row0,
                                                                  int with_forces, KxPersistCfg P) {
   const int bid = blockIdx.x;
@@ -580,7 +607,7 @@ __device__ __forceinline__ T *kx_glob(T *p) {
 // workgroup runs its entry's body with that entry's state, tables, law and
 // step count. The entry's HBM pointers are taken as global pointers.
 template <int WPB>
-__global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist_batch(const KxBatchEnt *__restrict__ E,
+__global__ __launch_bounds__(KX_WAVE *WPB) __attribute__((amdgpu_waves_per_eu(3))) void kx_k_persist_batch(const KxBatchEnt *__restrict__ E,
                                                                    const int2 *__restrict__ wgmap) {
   const int2 m = wgmap[blockIdx.x];
   const KxBatchEnt &e = E[m.x];
