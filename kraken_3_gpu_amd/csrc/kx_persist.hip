@@ -43,6 +43,9 @@ typedef float kx_f2v __attribute__((ext_vector_type(2)));
 // constraint record (3 words: both round types' fields) and both bodies'
 // {im, ii}, with the decoded slot. A lane with no slot (dsc < 0) reads
 // constraint 0 and body 0 (every read stays unconditional) and is not live.
+// bodies per lane whose broad-phase reads are batched (sub + k*G, k < KX_NBL)
+#define KX_NBL 4
+
 struct KxSlot {
   float4 dk;
   kx_f2v lk;
@@ -217,6 +220,17 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
   double my_cd = 0.0;
   if (act && sub < T.narms) my_cd = d.cd[cid * KX_MAX_ARMS + sub];
   const double hS = d.hd * d.S;
+  // the lane's collidable bodies among sub + k*G, k < KX_NBL (dynamic, with
+  // collision points): bit k; bodies past KX_NBL*G are tested one by one
+  uint32_t bp_mask = 0u;
+#pragma unroll
+  for (int k = 0; k < KX_NBL; k++) {
+    const int i = sub + k * G;
+    if (act && i < nb) {
+      const float4 br = t_brec[i];
+      if (br.x != 0.0f && (__float_as_int(br.z) >> 16) > 0) bp_mask |= 1u << k;
+    }
+  }
   double comp_last = L.comp;      // motor compliance of the latest control step
   // round types of the first 64 rounds (bit r set = angle round)
   const uint32_t rb0 = (uint32_t)__builtin_amdgcn_readfirstlane(t_rbits[0]);
@@ -370,91 +384,82 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
         // (the owning lane) starts there and re-tests the rest in order. The
         // order of the candidate list does not affect any result.
         {
-          const int nfw = (T.np + 31) >> 5;
-          if (act && sub == 0) evw[1] = 0u;
-          for (int wi = sub; wi < (act ? nfw : 0); wi += G) flg[wi] = 0u;
-          KX_WAVE_SYNC();
           // A body is a candidate when its centre's cell is set in the near
           // bitmap: one LDS read per body. A clear cell lies farther than
           // reach + rmax (+ a margin for the binary32 cell expression) from
           // every rect's box, reach bounding every point's distance from its
           // body centre and rmax every point's radius, so no point of such a
           // body corrects; a centre outside the grid clamps to the edge cell,
-          // which is no farther from any rect than the centre.
+          // which is no farther from any rect than the centre. A body with a
+          // non-finite pose may be dropped too: a non-finite query point
+          // never hits (the fold's distance stays infinite or NaN).
           // The bitmaps are read from LDS only; a map whose bitmaps do not
           // fit there gets no broad phase (every body is a candidate, which
           // is exact). A pointer that may point to LDS or to HBM would compile
           // to flat loads, which wait on both the LDS and the memory counters.
           const bool use_bm = P.bm_words != 0;
           const uint32_t *nearb = bits + (use_bm ? d.cell_words : 0);   // word 0: any word of the LDS allocation
-          // the near-bitmap cell of a body centre: its word (always a valid
-          // index: NaN clamps to 0, +-inf to the edges; word 0 without
-          // bitmaps), its bit, and whether the test may drop the body at all
-          // (bitmaps present and a finite cell coordinate)
-          auto cell = [&](float x, float y, int &wi, int &bi, bool &testable) {
-            const float gx = (x + ox - d.gx0) * d.inv_ns, gy = (y + oy - d.gy0) * d.inv_ns;
-            testable = use_bm && isfinite(gx) && isfinite(gy);
-            const int ix = (int)fminf(fmaxf(floorf(gx), 0.0f), (float)(d.nnx - 1));
-            const int iy = (int)fminf(fmaxf(floorf(gy), 0.0f), (float)(d.nny - 1));
+          // (x + ox) - gx0 as x + (ox - gx0): ox and gx0 are integers, so the
+          // difference is exact and the cell expression rounds once less
+          const float oxg = ox - d.gx0, oyg = oy - d.gy0;
+          // the near-bitmap word and bit of a body centre (clamped into the
+          // grid; NaN converts to cell 0)
+          auto cell = [&](float x, float y, int &wi, int &bi) {
+            const int ix = min(max(__float2int_rd((x + oxg) * d.inv_ns), 0), d.nnx - 1);
+            const int iy = min(max(__float2int_rd((y + oyg) * d.inv_ns), 0), d.nny - 1);
             const int c = iy * d.nnx + ix;
             wi = use_bm ? c >> 5 : 0;
             bi = c & 31;
           };
-          // a lane's bodies sub, sub + G, ...: records, poses and bitmap
-          // words all read before any test (no branch per body), then one
-          // reservation in the creature's candidate list for all of the
-          // lane's points
-          constexpr int NBL = 4;
+          // a lane's bodies sub, sub + G, ...: poses and bitmap words all
+          // read before any test (no branch per body)
           uint32_t mine = 0u;
-          int ntot = 0;
           {
-            kx_f2v brs[NBL], pbs[NBL];   // {im, points}, {x, y}
+            kx_f2v pbs[KX_NBL];
 #pragma unroll
-            for (int k = 0; k < NBL; k++) {
-              const int i = sub + k * G, ic = i < nb ? i : 0;
-              brs[k] = kx_f2v{((const float *)&t_brec[ic])[0], ((const float *)&t_brec[ic])[2]};
-              pbs[k] = *(const kx_f2v *)&pos[ic];
+            for (int k = 0; k < KX_NBL; k++) {
+              const int i = sub + k * G;
+              pbs[k] = *(const kx_f2v *)&pos[i < nb ? i : 0];
             }
-            uint32_t wd[NBL];
-            int bi[NBL];
-            bool testable[NBL];
+            uint32_t wd[KX_NBL];
+            int bi[KX_NBL];
 #pragma unroll
-            for (int k = 0; k < NBL; k++) {
+            for (int k = 0; k < KX_NBL; k++) {
               int wi;
-              cell(pbs[k].x, pbs[k].y, wi, bi[k], testable[k]);
+              cell(pbs[k].x, pbs[k].y, wi, bi[k]);
               wd[k] = nearb[wi];
             }
+            const uint32_t all = use_bm ? 0u : 1u;   // no bitmaps: every body
 #pragma unroll
-            for (int k = 0; k < NBL; k++) {
-              const int i = sub + k * G;
-              const int cnt = __float_as_int(brs[k].y) >> 16;
-              const bool nr = !testable[k] || ((wd[k] >> bi[k]) & 1u) != 0u;
-              const bool c = i < nb && brs[k].x != 0.0f && cnt > 0 && nr;
-              mine |= c ? 1u << k : 0u;
-              ntot += c ? cnt : 0;
-            }
+            for (int k = 0; k < KX_NBL; k++) mine |= (((wd[k] >> bi[k]) | all) & 1u) << k;
+            mine &= bp_mask;
           }
-          for (int k = NBL, i = sub + NBL * G; i < nb; k++, i += G) {
-            const float4 br = t_brec[i], pb = pos[i];
-            const int cnt = __float_as_int(br.z) >> 16;
+          for (int k = KX_NBL, i = sub + KX_NBL * G; i < nb; k++, i += G) {
+            const float4 br = t_brec[i];
+            if (br.x == 0.0f || (__float_as_int(br.z) >> 16) <= 0) continue;
+            const float2 pb = *(const float2 *)&pos[i];
             int wi, bi;
-            bool testable;
-            cell(pb.x, pb.y, wi, bi, testable);
-            if (br.x != 0.0f && cnt > 0 && (!testable || ((nearb[wi] >> bi) & 1u) != 0u)) {
-              mine |= 1u << k;
-              ntot += cnt;
-            }
-          }
-          if (ntot) {
-            int at = (int)atomicAdd(&evw[1], (uint32_t)ntot);
-            for (uint32_t m = mine; m; m &= m - 1u) {
-              const int i = sub + __builtin_ctz(m) * G;
-              const int bp = __float_as_int(t_brec[i].z), cnt = bp >> 16;
-              for (int t = 0; t < cnt; t++) cand[at + t] = (uint32_t)((bp & 0xffff) + t) | ((uint32_t)i << 16);
-              at += cnt;
-            }
+            cell(pb.x, pb.y, wi, bi);
+            if (!use_bm || ((nearb[wi] >> bi) & 1u) != 0u) mine |= 1u << k;
           }
           if (__any(mine != 0u)) {
+            // clear the flags and the candidate count, then one reservation
+            // in the creature's candidate list for all of the lane's points
+            const int nfw = (T.np + 31) >> 5;
+            if (act && sub == 0) evw[1] = 0u;
+            for (int wi = sub; wi < (act ? nfw : 0); wi += G) flg[wi] = 0u;
+            KX_WAVE_SYNC();
+            int ntot = 0;
+            for (uint32_t m = mine; m; m &= m - 1u) ntot += __float_as_int(t_brec[sub + __builtin_ctz(m) * G].z) >> 16;
+            if (ntot) {
+              int at = (int)atomicAdd(&evw[1], (uint32_t)ntot);
+              for (uint32_t m = mine; m; m &= m - 1u) {
+                const int i = sub + __builtin_ctz(m) * G;
+                const int bp = __float_as_int(t_brec[i].z), cnt = bp >> 16;
+                for (int t = 0; t < cnt; t++) cand[at + t] = (uint32_t)((bp & 0xffff) + t) | ((uint32_t)i << 16);
+                at += cnt;
+              }
+            }
             KX_WAVE_SYNC();
             const int ncand = act ? (int)evw[1] : 0;
             for (int j = sub; j < ncand; j += G) {
