@@ -206,10 +206,24 @@ int kx_schedule_info(const kx_handle *h, int32_t *rounds, int32_t max);
  * per workgroup, [1] workgroups, [2] resident workgroups per CU, [3]
  * workgroups per CU the launch needs, [4] LDS words of the staged map (0:
  * map read from HBM), [5] cell-bitmap words, [6] template words, [7]
- * per-creature LDS words, [8] 1 if lane counts differ between workgroups.
- * Returns the number of fields (9). No counterpart in the reference (its path
- * has no persistent kernel). */
+ * per-creature LDS words, [8] 1 if lane counts differ between workgroups,
+ * [9] the collision fold in use (KX_FOLD_PARALLEL or KX_FOLD_OWNER).
+ * Returns the number of fields (10). No counterpart in the reference (its
+ * path has no persistent kernel). */
 int kx_persist_info(const kx_handle *h, int32_t *info, int32_t max);
+/* Persistent path: how a step folds the collision points of the bodies near
+ * a rect (xpbd.js:441-470 either way, with identical results).
+ * KX_FOLD_PARALLEL tests the near bodies' points in parallel over the
+ * creature's lanes and re-runs in order only a body with a correcting point
+ * (a candidate list in LDS); KX_FOLD_OWNER folds each near body in order on
+ * its owning lane. KX_FOLD_AUTO (the default) takes PARALLEL on an open map
+ * (under half of the area within reach of a rect) and OWNER on a dense one
+ * (corridors), where most bodies touch a rect most of the time. No
+ * counterpart in the reference (a launch-shape knob, like waves_per_block). */
+#define KX_FOLD_AUTO 0
+#define KX_FOLD_PARALLEL 1
+#define KX_FOLD_OWNER 2
+int kx_set_collision_fold(kx_handle *h, int32_t fold);
 /* Morphology templates (creatures with identical structure share one):
  * t < 0 returns the template count; else out6 = {bodies, constraints, points,
  * creatures, rounds per iteration, lanes per creature} of template t. */

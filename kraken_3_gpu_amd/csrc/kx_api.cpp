@@ -92,9 +92,11 @@ struct kx_handle {
   std::vector<int> tmpl_rounds;
   KxPersistCfg pc{};
   void *pc_bufs[4] = {nullptr, nullptr, nullptr, nullptr};
-  int pinfo[9] = {};     // see kx_persist_info
+  int pinfo[10] = {};    // see kx_persist_info
   std::vector<int> tmpl_lanes;   // per template lanes per creature, 0 = the handle's G
   int launch_G = 16;             // lane count the uniform kernel is launched with
+  int coll_mode = 0;             // collision fold: 0 = by map density, 1 = parallel, 2 = owning lane
+  double near_frac = 0.0;        // fraction of the near bitmap set (build_grid)
   // scratch host staging
   std::vector<float> hf;
   std::vector<int> hi;
@@ -297,6 +299,11 @@ static int build_grid(kx_handle *h, const float *rects, int nR, float rmax, floa
   h->d.cell_bits = dbits;
   h->d.cell_words = (int)W4;
   h->d.near_words = (int)N4;
+  {
+    size_t set = 0;
+    for (size_t k = 0; k < N4; k++) set += (size_t)__builtin_popcount(bits[W4 + k]);
+    h->near_frac = nnx * nny > 0 ? (double)set / ((double)nnx * nny) : 0.0;
+  }
   h->d.inv_ns = (float)m / cs;
   h->d.nnx = nnx;
   h->d.nny = nny;
@@ -324,18 +331,32 @@ static int build_persist(kx_handle *h) {
   std::vector<KxTmpl> T;
   int words = 0, NB = 1, NC = 1, NP = 1, NG = 0;
   h->tmpl_rounds.clear();
+  std::vector<std::vector<int>> members(h->tmpl_host.size());
+  for (int c = 0; c < h->nCr; c++) members[h->cr_tmpl[c]].push_back(c);
+  int dev = 0, ncu = 1;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  // Collision fold. On an open map (under half of the near bitmap set) a
+  // body is near a rect in few iterations and a correcting point is rare, so
+  // the near bodies' points are tested in parallel over each creature's
+  // lanes and only a body with a correcting point is re-run in order (the
+  // parallel fold: a candidate list in LDS); its worst case per wave stays
+  // short, which is what a launch of one wave per SIMD waits for. On a dense
+  // map (corridors) most bodies are near in most iterations and many
+  // correct, so each near body is folded once, in order, by its owning lane
+  // (the owner fold: no list, no re-run). A handle may fix the fold
+  // (coll_mode 1 = parallel, 2 = owner); both are exact.
+  const bool par = h->coll_mode == 1 || (h->coll_mode == 0 && h->near_frac < 0.5);
   for (size_t t = 0; t < h->tmpl_host.size(); t++) {
     KxTmpl k = kx_pack_template(h->tmpl_host[t], tmpl_lanes_of(h, t), pool);
     // creature regions: 16-byte records, regions offset by 16 bytes modulo
     // 128 so neighbouring creatures' same records fall in different banks
-    k.cr_stride = (kx_persist_cr_words(k.nb, k.nc, k.np, k.ngrip) + 31) / 32 * 32 + 4;
+    k.cr_stride = (kx_persist_cr_words(k.nb, k.nc, k.np, k.ngrip, par) + 31) / 32 * 32 + 4;
     T.push_back(k);
     words = std::max(words, k.words);
     NB = std::max(NB, k.nb); NC = std::max(NC, k.nc); NP = std::max(NP, k.np); NG = std::max(NG, k.ngrip);
     h->tmpl_rounds.push_back(k.R);
   }
-  std::vector<std::vector<int>> members(T.size());
-  for (int c = 0; c < h->nCr; c++) members[h->cr_tmpl[c]].push_back(c);
   std::vector<int> wave_cr, wg(0);
   std::vector<int> wg_tmpl, wg_lanes, wg_off;
   int max_cpw = 1, lanes0 = -1, cr_words = 0;
@@ -372,7 +393,6 @@ static int build_persist(kx_handle *h) {
   }
   const int nwg = (int)wg_tmpl.size();
   const int Gk = mixed ? 0 : (lanes0 > 0 ? lanes0 : h->G);    // kernel: 0 = mixed-lane kernel
-  const int stride = (kx_persist_cr_words(NB, NC, NP, NG) + 31) / 32 * 32 + 4;   // largest region (reported)
   int bmw = h->d.cell_words + h->d.near_words;   // occupancy + near bitmaps
   if ((size_t)bmw * 4 > 32 * 1024) bmw = 0;      // huge grids: no LDS bitmaps
   // Stage the whole map (rects of the cell entries, 16-bit cell offsets) in
@@ -385,18 +405,17 @@ static int build_persist(kx_handle *h) {
   if (base > 160 * 1024)
     return fail(KX_ENOTSUP, "persistent shape (%d creatures per wave) x %d waves needs %zu B of LDS (> 160 KiB)",
                 max_cpw, WPB, base);
-  int dev = 0, ncu = 1;
-  HIPCHK(hipGetDevice(&dev));
-  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   const int need = (std::max(nwg, 1) + std::max(ncu, 1) - 1) / std::max(ncu, 1);
-  const int occ0 = kx_persist_occupancy(Gk, WPB, base);
+  const int occ0 = kx_persist_occupancy(Gk, WPB, base, par);
   const size_t with_map = base + (size_t)mapw * 4;
-  const int occ1 = with_map <= 160 * 1024 ? kx_persist_occupancy(Gk, WPB, with_map) : 0;
+  const int occ1 = with_map <= 160 * 1024 ? kx_persist_occupancy(Gk, WPB, with_map, par) : 0;
   if (!mapw || std::min(occ1, need) < std::min(occ0, need)) mapw = 0;
+  const int stride = (kx_persist_cr_words(NB, NC, NP, NG, par) + 31) / 32 * 32 + 4;   // largest region (reported)
   const size_t lds = base + (size_t)mapw * 4;
   h->pinfo[0] = (int)lds; h->pinfo[1] = nwg; h->pinfo[2] = mapw ? occ1 : occ0; h->pinfo[3] = need;
   h->pinfo[4] = mapw; h->pinfo[5] = bmw; h->pinfo[6] = words; h->pinfo[7] = stride;
   h->pinfo[8] = mixed ? 1 : 0;
+  h->pinfo[9] = par ? KX_FOLD_PARALLEL : KX_FOLD_OWNER;
   std::vector<int> tables;
   tables.insert(tables.end(), wg_tmpl.begin(), wg_tmpl.end());
   tables.insert(tables.end(), wg_lanes.begin(), wg_lanes.end());
@@ -427,6 +446,7 @@ static int build_persist(kx_handle *h) {
   P.map_ids = mids;
   P.cr_words = cr_words;
   P.NB = NB; P.NC = NC; P.NP = NP;
+  P.coll_par = par ? 1 : 0;
   h->launch_G = mixed ? h->G : (lanes0 > 0 ? lanes0 : h->G);
   return 0;
 }
@@ -1076,8 +1096,8 @@ extern "C" int kx_schedule_info(const kx_handle *h, int32_t *rounds, int32_t max
 extern "C" int kx_persist_info(const kx_handle *h, int32_t *info, int32_t max) {
   if (!h || !info) return fail(KX_EINVAL, "null argument");
   if (!h->persist_ok) return fail(KX_ENOTSUP, "persistent path unavailable: %s", h->persist_why.c_str());
-  for (int i = 0; i < 9 && i < max; i++) info[i] = h->pinfo[i];
-  return 9;
+  for (int i = 0; i < 10 && i < max; i++) info[i] = h->pinfo[i];
+  return 10;
 }
 
 extern "C" int kx_template_info(const kx_handle *h, int32_t t, int32_t *out6) {
@@ -1119,6 +1139,27 @@ extern "C" int kx_set_template_lanes(kx_handle *h, int32_t n, const int32_t *lan
     build_persist(h);
     g_err = keep;
     return r;
+  }
+  return 0;
+}
+
+extern "C" int kx_set_collision_fold(kx_handle *h, int32_t fold) {
+  if (fold != KX_FOLD_AUTO && fold != KX_FOLD_PARALLEL && fold != KX_FOLD_OWNER)
+    return fail(KX_EINVAL, "collision fold %d", fold);
+  return kx_set_coll_mode(h, fold);
+}
+
+int kx_set_coll_mode(kx_handle *h, int mode) {
+  if (!h) return fail(KX_EINVAL, "null handle");
+  if (mode < 0 || mode > 2) return fail(KX_EINVAL, "collision mode %d", mode);
+  if (mode == h->coll_mode) return 0;
+  HIPCHK(hipStreamSynchronize(h->st));
+  const int old = h->coll_mode;
+  h->coll_mode = mode;
+  if (h->persist_ok) {
+    int r = build_persist(h);
+    if (r) { h->coll_mode = old; std::string keep = g_err; build_persist(h); g_err = keep; return r; }
+    if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
   }
   return 0;
 }

@@ -110,11 +110,13 @@ struct KxPersistCfg {
                             // then map_cells + 1 16-bit cell offsets
   int cr_words;             // LDS words of the creature regions of the largest workgroup
   int NB, NC, NP;           // per-creature array sizes in LDS (bodies, constraints, points)
+  int coll_par;             // 1: collisions through the per-creature candidate list (phase A/B);
+                            // 0: each near body folded by its owning lane
 };
 int kx_persist_launch(int G, int wpb, const KxDev &d, const KxLaw *law, int step0, const int *step_base,
                       int nsteps, int ev_row0, int with_forces, const KxPersistCfg &P, hipStream_t st);
-int kx_persist_cr_words(int NB, int NC, int NP, int NG);
-int kx_persist_occupancy(int G, int wpb, size_t lds);   // G = 0: the mixed-lane kernel
+int kx_persist_cr_words(int NB, int NC, int NP, int NG, bool par);
+int kx_persist_occupancy(int G, int wpb, size_t lds, bool par);   // G = 0: the mixed-lane kernel
 int kx_bump_counter(int *counter, int delta, hipStream_t st);
 
 // One handle's share of a batched persistent launch: its device state and
@@ -130,7 +132,8 @@ struct KxBatchEnt {
 inline size_t kx_persist_lds_bytes(const KxPersistCfg &P) {
   return ((size_t)P.tmpl_words + P.bm_words + P.map_words + (size_t)P.cr_words) * 4;
 }
-int kx_persist_launch_batch(int wpb, const KxBatchEnt *ents_dev, const int2 *wgmap_dev, int nwg, size_t lds,
+// every entry of one batched launch has the same collision fold `par`
+int kx_persist_launch_batch(int wpb, int par, const KxBatchEnt *ents_dev, const int2 *wgmap_dev, int nwg, size_t lds,
                             hipStream_t st);
 
 // What a batched launch needs of a handle (kx_api.cpp); fails unless the
@@ -144,5 +147,9 @@ struct KxHandleView {
   size_t bytes;             // device memory the handle's state arrays hold
 };
 int kx_handle_view(kx_handle *h, KxHandleView *v);
+// Fixes a handle's collision fold (0 = by map density, 1 = parallel over the
+// creature's lanes, 2 = by the owning lane) and rebuilds its persistent
+// configuration.
+int kx_set_coll_mode(kx_handle *h, int mode);
 // sets the thread's kx_last_error message and returns code
 int kx_error(int code, const char *msg);

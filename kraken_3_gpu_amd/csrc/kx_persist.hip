@@ -80,23 +80,24 @@ __device__ __forceinline__ KxSlot kx_slot_read(int dsc, const float4 *dyn, const
 //   lam[k] = {lambda_x, lambda_y} (pin, grip) or {lambda, -} (angle)
 //   gbf[j] = break force of the j-th grip (the only constraints that break)
 //   ev     = {step events, collision candidate count, -, -}
+// and, for launches that fold collisions in parallel (P.coll_par):
 //   flg    = one bit per collision point: "corrects at the pose after the rounds"
 //   cand   = collision candidates: point | body << 16
 struct KxCrLds {
   int pos, prv, vel, dyn, lam, gbf, ev, flg, cand, words;
-  __host__ __device__ KxCrLds(int NB, int NC, int NP, int NG) {
+  __host__ __device__ KxCrLds(int NB, int NC, int NP, int NG, bool par) {
     int o = 0;
     pos = o; o += 4 * NB; prv = o; o += 4 * NB; vel = o; o += 4 * NB;
     dyn = o; o += 4 * NC; lam = o; o += (2 * NC + 3) / 4 * 4;
     gbf = o; o += (NG + 3) / 4 * 4;
     ev = o; o += 4;
-    flg = o; o += ((NP + 31) / 32 + 3) / 4 * 4;
-    cand = o; o += (NP + 3) / 4 * 4;
+    flg = o; o += par ? ((NP + 31) / 32 + 3) / 4 * 4 : 0;
+    cand = o; o += par ? (NP + 3) / 4 * 4 : 0;
     words = o;
   }
 };
 
-int kx_persist_cr_words(int NB, int NC, int NP, int NG) { return KxCrLds(NB, NC, NP, NG).words; }
+int kx_persist_cr_words(int NB, int NC, int NP, int NG, bool par) { return KxCrLds(NB, NC, NP, NG, par).words; }
 
 // Adds (sx, sy) to a creature's body positions (start and predicted) and to
 // the world anchors of its world-anchored pins and grips; the same binary32
@@ -153,9 +154,42 @@ __device__ __forceinline__ bool kx_query(const KxDev &d, const KxPersistCfg &P, 
                       : kx_sdf_grid(d, wx, wy, ox, oy, hh);
 }
 
+// Two queries against the map staged in LDS, each exactly kx_sdf_cells_lds:
+// both points' cell offsets are read together, then the first two rects of
+// both cells (a cell rarely holds more), so two queries cost two LDS round
+// trips instead of four. Every index read is valid (0 when unused).
+__device__ __forceinline__ void kx_sdf2_cells_lds(const KxDev &d, const uint16_t *coff, const float4 *crect, float x0,
+                                                  float y0, float x1, float y1, float ox, float oy, bool &ok0,
+                                                  KxHit &h0, bool &ok1, KxHit &h1) {
+  auto cellof = [&](float x, float y, int &cell) {
+    const float gx = (x + ox - d.gx0) * d.inv_cs;
+    const float gy = (y + oy - d.gy0) * d.inv_cs;
+    const bool in = gx >= 0.0f && gy >= 0.0f && gx < (float)d.gnx && gy < (float)d.gny;
+    const int ix = min((int)gx, d.gnx - 1), iy = min((int)gy, d.gny - 1);
+    cell = in ? iy * d.gnx + ix : 0;
+    return in;
+  };
+  int c0, c1;
+  const bool in0 = cellof(x0, y0, c0), in1 = cellof(x1, y1, c1);
+  const int b0 = coff[c0], e0 = coff[c0 + 1], b1 = coff[c1], e1 = coff[c1 + 1];
+  const int n0 = in0 ? e0 - b0 : 0, n1 = in1 ? e1 - b1 : 0;
+  const float4 r00 = crect[n0 > 0 ? b0 : 0], r01 = crect[n0 > 1 ? b0 + 1 : 0];
+  const float4 r10 = crect[n1 > 0 ? b1 : 0], r11 = crect[n1 > 1 ? b1 + 1 : 0];
+  kx_hit_init(h0);
+  kx_hit_init(h1);
+  if (n0 > 0) kx_fold_rect(x0, y0, r00, ox, oy, h0);
+  if (n0 > 1) kx_fold_rect(x0, y0, r01, ox, oy, h0);
+  for (int k = 2; k < n0; k++) kx_fold_rect(x0, y0, crect[b0 + k], ox, oy, h0);
+  if (n1 > 0) kx_fold_rect(x1, y1, r10, ox, oy, h1);
+  if (n1 > 1) kx_fold_rect(x1, y1, r11, ox, oy, h1);
+  for (int k = 2; k < n1; k++) kx_fold_rect(x1, y1, crect[b1 + k], ox, oy, h1);
+  ok0 = n0 > 0 && isfinite(h0.d);
+  ok1 = n1 > 0 && isfinite(h1.d);
+}
+
 // The body of one workgroup `bid`: WPB wavefronts of 64/G creatures of one
 // template (workgroup tables P.wg_*).
-template <int G, int WPB>
+template <int G, int WPB, bool PAR>
 __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, int has_law, int step0,
                                                 const int *step_base, int nsteps, int ev_row0, int with_forces,
                                                 const KxPersistCfg &P, int bid) {
@@ -192,7 +226,7 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
 
   const int cid = P.wave_cr[P.wg_cr_off[bid] + w * CPW + q];
   const bool act = cid >= 0;
-  const KxCrLds Lo(T.nb, T.nc, T.np, T.ngrip);
+  const KxCrLds Lo(T.nb, T.nc, T.np, T.ngrip, PAR);
   uint32_t *S = smem + P.tmpl_words + P.bm_words + P.map_words + (size_t)(w * CPW + q) * T.cr_stride;
   float4 *pos = (float4 *)(S + Lo.pos), *prv = (float4 *)(S + Lo.prv), *vel = (float4 *)(S + Lo.vel);
   float4 *dyn = (float4 *)(S + Lo.dyn);
@@ -373,16 +407,9 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
         // collisions (xpbd.js:441-470: per body, its points in order, each
         // correction moving the body before the next point is tested).
         //
-        // Broad phase: a query in a grid cell with no registered rect never
-        // hits, and every point of a body lies within its reach of the body
-        // centre, so only bodies whose reach box touches an occupied cell can
-        // collide. Lane `sub` owns bodies sub, sub + G, ...; the points of its
-        // near bodies are appended to the creature's candidate list.
-        // Phase A tests the candidates in parallel at their body's pose after
-        // the rounds and flags those that correct. Up to a body's first
-        // flagged point the sequential pass sees exactly that pose, so phase B
-        // (the owning lane) starts there and re-tests the rest in order. The
-        // order of the candidate list does not affect any result.
+        // Lane `sub` owns bodies sub, sub + G, ...: a broad phase finds the
+        // bodies that may touch a rect, and the owning lane folds each of
+        // them; bodies are independent of one another here.
         {
           // A body is a candidate when its centre's cell is set in the near
           // bitmap: one LDS read per body. A clear cell lies farther than
@@ -442,70 +469,136 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
             cell(pb.x, pb.y, wi, bi);
             if (!use_bm || ((nearb[wi] >> bi) & 1u) != 0u) mine |= 1u << k;
           }
+          // Two exact ways to fold the near bodies, chosen per launch:
+          //  - parallel (launches of at most one wave per SIMD, whose time is
+          //    the slowest wave's): the points of the near bodies go to a
+          //    per-creature candidate list; phase A tests them in parallel at
+          //    their body's pose after the rounds and flags those that
+          //    correct; up to a body's first flagged point the sequential
+          //    pass sees exactly that pose, so phase B (the owning lane)
+          //    starts there and re-tests the rest in order. The order of the
+          //    list does not affect any result.
+          //  - owner (launches of many waves per SIMD, whose time is the sum):
+          //    the sequential fold of each near body (xpbd.js:447-469) by the
+          //    lane that owns it: its points in order, each queried at the
+          //    body's current pose and correcting it when it penetrates.
+          //    Points go two at a time: both are queried at the pose before
+          //    the pair, and if the first corrects, the second is queried
+          //    again at the corrected pose, so the fold is exactly the
+          //    one-point-at-a-time fold. No candidate list, flags or atomics.
           if (__any(mine != 0u)) {
-            // clear the flags and the candidate count, then one reservation
-            // in the creature's candidate list for all of the lane's points
-            const int nfw = (T.np + 31) >> 5;
-            if (act && sub == 0) evw[1] = 0u;
-            for (int wi = sub; wi < (act ? nfw : 0); wi += G) flg[wi] = 0u;
-            KX_WAVE_SYNC();
-            int ntot = 0;
-            for (uint32_t m = mine; m; m &= m - 1u) ntot += __float_as_int(t_brec[sub + __builtin_ctz(m) * G].z) >> 16;
-            if (ntot) {
-              int at = (int)atomicAdd(&evw[1], (uint32_t)ntot);
+            if constexpr (PAR) {
+              // clear the flags and the candidate count, then one reservation
+              // in the creature's candidate list for all of the lane's points
+              const int nfw = (T.np + 31) >> 5;
+              if (act && sub == 0) evw[1] = 0u;
+              for (int wi = sub; wi < (act ? nfw : 0); wi += G) flg[wi] = 0u;
+              KX_WAVE_SYNC();
+              int ntot = 0;
+              for (uint32_t m = mine; m; m &= m - 1u) ntot += __float_as_int(t_brec[sub + __builtin_ctz(m) * G].z) >> 16;
+              if (ntot) {
+                int at = (int)atomicAdd(&evw[1], (uint32_t)ntot);
+                for (uint32_t m = mine; m; m &= m - 1u) {
+                  const int i = sub + __builtin_ctz(m) * G;
+                  const int bp = __float_as_int(t_brec[i].z), cnt = bp >> 16;
+                  for (int t = 0; t < cnt; t++) cand[at + t] = (uint32_t)((bp & 0xffff) + t) | ((uint32_t)i << 16);
+                  at += cnt;
+                }
+              }
+              KX_WAVE_SYNC();
+              const int ncand = act ? (int)evw[1] : 0;
+              for (int j = sub; j < ncand; j += G) {
+                const uint32_t e = cand[j];
+                const int p = (int)(e & 0xffffu);
+                const float4 pt = t_prec[p], pb = pos[e >> 16];
+                float sa, ca, wx, wy;
+                kx_sincos(pb.z, &sa, &ca);
+                kx_world_point(pb.x, pb.y, ca, sa, pt.x, pt.y, wx, wy);
+                KxHit hh;
+                if (kx_query(d, P, bits, m_off, m_crect, wx, wy, ox, oy, hh) && pt.z - hh.d > KX_SLACK_POS)
+                  atomicOr(&flg[p >> 5], 1u << (p & 31));
+              }
+              KX_WAVE_SYNC();
               for (uint32_t m = mine; m; m &= m - 1u) {
                 const int i = sub + __builtin_ctz(m) * G;
-                const int bp = __float_as_int(t_brec[i].z), cnt = bp >> 16;
-                for (int t = 0; t < cnt; t++) cand[at + t] = (uint32_t)((bp & 0xffff) + t) | ((uint32_t)i << 16);
-                at += cnt;
+                float4 br = t_brec[i];
+                const int bp = __float_as_int(br.z);
+                const int o = bp & 0xffff, cnt = bp >> 16;
+                int pq = cnt;                              // first flagged point of the body
+                for (int wi = o >> 5; (wi << 5) < o + cnt; wi++) {
+                  uint32_t wd = flg[wi];
+                  const int lo = o - (wi << 5), hi = o + cnt - (wi << 5);
+                  if (lo > 0) wd &= ~0u << lo;
+                  if (hi < 32) wd &= (1u << hi) - 1u;
+                  if (wd) { pq = (wi << 5) + __builtin_ctz(wd) - o; break; }
+                }
+                if (pq >= cnt) continue;
+                float4 pb = pos[i];
+                float x = pb.x, y = pb.y, sa, ca;
+                kx_sincos(pb.z, &sa, &ca);
+                for (; pq < cnt; pq++) {
+                  float4 pt = t_prec[o + pq];
+                  float wx, wy;
+                  kx_world_point(x, y, ca, sa, pt.x, pt.y, wx, wy);
+                  KxHit hh;
+                  if (!kx_query(d, P, bits, m_off, m_crect, wx, wy, ox, oy, hh)) continue;
+                  const float pen = pt.z - hh.d;
+                  if (pen > KX_SLACK_POS) {
+                    float nx, ny;
+                    kx_hit_normal(hh, nx, ny);
+                    x += nx * pen; y += ny * pen;
+                  }
+                }
+                pb.x = x; pb.y = y;
+                pos[i] = pb;
               }
-            }
-            KX_WAVE_SYNC();
-            const int ncand = act ? (int)evw[1] : 0;
-            for (int j = sub; j < ncand; j += G) {
-              const uint32_t e = cand[j];
-              const int p = (int)(e & 0xffffu);
-              const float4 pt = t_prec[p], pb = pos[e >> 16];
-              float sa, ca, wx, wy;
-              kx_sincos(pb.z, &sa, &ca);
-              kx_world_point(pb.x, pb.y, ca, sa, pt.x, pt.y, wx, wy);
-              KxHit hh;
-              if (kx_query(d, P, bits, m_off, m_crect, wx, wy, ox, oy, hh) && pt.z - hh.d > KX_SLACK_POS)
-                atomicOr(&flg[p >> 5], 1u << (p & 31));
-            }
-            KX_WAVE_SYNC();
-            for (uint32_t m = mine; m; m &= m - 1u) {
-              const int i = sub + __builtin_ctz(m) * G;
-              float4 br = t_brec[i];
-              const int bp = __float_as_int(br.z);
-              const int o = bp & 0xffff, cnt = bp >> 16;
-              int pq = cnt;                              // first flagged point of the body
-              for (int wi = o >> 5; (wi << 5) < o + cnt; wi++) {
-                uint32_t wd = flg[wi];
-                const int lo = o - (wi << 5), hi = o + cnt - (wi << 5);
-                if (lo > 0) wd &= ~0u << lo;
-                if (hi < 32) wd &= (1u << hi) - 1u;
-                if (wd) { pq = (wi << 5) + __builtin_ctz(wd) - o; break; }
-              }
-              if (pq >= cnt) continue;
-              float4 pb = pos[i];
-              float x = pb.x, y = pb.y, sa, ca;
-              kx_sincos(pb.z, &sa, &ca);
-              for (; pq < cnt; pq++) {
-                float4 pt = t_prec[o + pq];
-                float wx, wy;
-                kx_world_point(x, y, ca, sa, pt.x, pt.y, wx, wy);
-                KxHit hh;
-                if (!kx_query(d, P, bits, m_off, m_crect, wx, wy, ox, oy, hh)) continue;
-                const float pen = pt.z - hh.d;
-                if (pen > KX_SLACK_POS) {
+            } else {
+              const bool lds_map = P.map_words != 0;
+              for (uint32_t m = mine; m; m &= m - 1u) {
+                const int i = sub + __builtin_ctz(m) * G;
+                const int bp = __float_as_int(t_brec[i].z), o = bp & 0xffff, cnt = bp >> 16;
+                float4 pb = pos[i];
+                float x = pb.x, y = pb.y, sa, ca;
+                kx_sincos(pb.z, &sa, &ca);
+                bool moved = false;
+                auto correct = [&](const KxHit &hh, float pen) {
                   float nx, ny;
                   kx_hit_normal(hh, nx, ny);
                   x += nx * pen; y += ny * pen;
+                  moved = true;
+                };
+                for (int t = 0; t < cnt; t += 2) {
+                  const bool two = t + 1 < cnt;
+                  const float4 p0 = t_prec[o + t], p1 = t_prec[o + (two ? t + 1 : t)];
+                  float wx0, wy0, wx1, wy1;
+                  kx_world_point(x, y, ca, sa, p0.x, p0.y, wx0, wy0);
+                  kx_world_point(x, y, ca, sa, p1.x, p1.y, wx1, wy1);
+                  bool q0, q1;
+                  KxHit h0, h1;
+                  if (lds_map) {
+                    kx_sdf2_cells_lds(d, m_off, m_crect, wx0, wy0, wx1, wy1, ox, oy, q0, h0, q1, h1);
+                  } else {
+                    kx_hit_init(h1);
+                    q0 = kx_query(d, P, bits, m_off, m_crect, wx0, wy0, ox, oy, h0);
+                    q1 = two && kx_query(d, P, bits, m_off, m_crect, wx1, wy1, ox, oy, h1);
+                  }
+                  if (q0 && p0.z - h0.d > KX_SLACK_POS) {
+                    correct(h0, p0.z - h0.d);
+                    if (two) {
+                      kx_world_point(x, y, ca, sa, p1.x, p1.y, wx1, wy1);
+                      KxHit h;
+                      if (kx_query(d, P, bits, m_off, m_crect, wx1, wy1, ox, oy, h) && p1.z - h.d > KX_SLACK_POS)
+                        correct(h, p1.z - h.d);
+                    }
+                  } else if (two && q1 && p1.z - h1.d > KX_SLACK_POS) {
+                    correct(h1, p1.z - h1.d);
+                  }
+                }
+                if (moved) {
+                  pb.x = x; pb.y = y;
+                  pos[i] = pb;
                 }
               }
-              pb.x = x; pb.y = y;
-              pos[i] = pb;
             }
           }
         }
@@ -571,28 +664,29 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
   if (act && sub < T.narms) d.cd[cid * KX_MAX_ARMS + sub] = my_cd;
 }
 
-// Every workgroup with the same lane count G.
-template <int G, int WPB>
+// Every workgroup with the same lane count G; PAR: the collision fold
+// (P.coll_par), one kernel per fold so that each compiles alone.
+template <int G, int WPB, bool PAR>
 __global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, int has_law, int step0,
                                                              const int *step_base, int nsteps, int ev_row0,
                                                              int with_forces, KxPersistCfg P) {
-  kx_persist_body<G, WPB>(d, L, has_law, step0, step_base, nsteps, ev_row0, with_forces, P, blockIdx.x);
+  kx_persist_body<G, WPB, PAR>(d, L, has_law, step0, step_base, nsteps, ev_row0, with_forces, P, blockIdx.x);
 }
 
 // Lane count per workgroup (per morphology bucket): a wave-uniform switch to
 // the body instantiated for that count. Held to 168 VGPRs (three waves per
 // SIMD), as the single-count kernels are: these launches are the ones whose
 // waves can share a SIMD.
-template <int WPB>
+template <int WPB, bool PAR>
 __global__ __launch_bounds__(KX_WAVE *WPB) __attribute__((amdgpu_waves_per_eu(3))) void kx_k_persist_mix(KxDev d, KxLaw L, int has_law, int step0,
                                                                  const int *step_base, int nsteps, int ev_row0,
                                                                  int with_forces, KxPersistCfg P) {
   const int bid = blockIdx.x;
   switch (P.wg_lanes[bid]) {
-    case 8: kx_persist_body<8, WPB>(d, L, has_law, step0, step_base, nsteps, ev_row0, with_forces, P, bid); break;
-    case 16: kx_persist_body<16, WPB>(d, L, has_law, step0, step_base, nsteps, ev_row0, with_forces, P, bid); break;
-    case 32: kx_persist_body<32, WPB>(d, L, has_law, step0, step_base, nsteps, ev_row0, with_forces, P, bid); break;
-    default: kx_persist_body<64, WPB>(d, L, has_law, step0, step_base, nsteps, ev_row0, with_forces, P, bid); break;
+    case 8: kx_persist_body<8, WPB, PAR>(d, L, has_law, step0, step_base, nsteps, ev_row0, with_forces, P, bid); break;
+    case 16: kx_persist_body<16, WPB, PAR>(d, L, has_law, step0, step_base, nsteps, ev_row0, with_forces, P, bid); break;
+    case 32: kx_persist_body<32, WPB, PAR>(d, L, has_law, step0, step_base, nsteps, ev_row0, with_forces, P, bid); break;
+    default: kx_persist_body<64, WPB, PAR>(d, L, has_law, step0, step_base, nsteps, ev_row0, with_forces, P, bid); break;
   }
 }
 
@@ -611,7 +705,7 @@ __device__ __forceinline__ T *kx_glob(T *p) {
 // Several handles in one grid (the serving loop's continuous batch): each
 // workgroup runs its entry's body with that entry's state, tables, law and
 // step count. The entry's HBM pointers are taken as global pointers.
-template <int WPB>
+template <int WPB, bool PAR>
 __global__ __launch_bounds__(KX_WAVE *WPB) __attribute__((amdgpu_waves_per_eu(3))) void kx_k_persist_batch(const KxBatchEnt *__restrict__ E,
                                                                    const int2 *__restrict__ wgmap) {
   const int2 m = wgmap[blockIdx.x];
@@ -633,32 +727,34 @@ __global__ __launch_bounds__(KX_WAVE *WPB) __attribute__((amdgpu_waves_per_eu(3)
   P.wg_cr_off = kx_glob(P.wg_cr_off); P.tmpl = kx_glob(P.tmpl); P.pool = kx_glob(P.pool);
   const KxLaw L = e.L;
   switch (P.wg_lanes[bid]) {
-    case 8: kx_persist_body<8, WPB>(d, L, e.has_law, e.step0, nullptr, e.nsteps, 0, 0, P, bid); break;
-    case 16: kx_persist_body<16, WPB>(d, L, e.has_law, e.step0, nullptr, e.nsteps, 0, 0, P, bid); break;
-    case 32: kx_persist_body<32, WPB>(d, L, e.has_law, e.step0, nullptr, e.nsteps, 0, 0, P, bid); break;
-    default: kx_persist_body<64, WPB>(d, L, e.has_law, e.step0, nullptr, e.nsteps, 0, 0, P, bid); break;
+    case 8: kx_persist_body<8, WPB, PAR>(d, L, e.has_law, e.step0, nullptr, e.nsteps, 0, 0, P, bid); break;
+    case 16: kx_persist_body<16, WPB, PAR>(d, L, e.has_law, e.step0, nullptr, e.nsteps, 0, 0, P, bid); break;
+    case 32: kx_persist_body<32, WPB, PAR>(d, L, e.has_law, e.step0, nullptr, e.nsteps, 0, 0, P, bid); break;
+    default: kx_persist_body<64, WPB, PAR>(d, L, e.has_law, e.step0, nullptr, e.nsteps, 0, 0, P, bid); break;
   }
 }
 
-template <int WPB>
+template <int WPB, bool PAR>
 static int launch_batch(const KxBatchEnt *E, const int2 *wgmap, int nwg, size_t lds, hipStream_t st) {
   static bool attr_set = false;
   if (!attr_set) {
-    (void)hipFuncSetAttribute((const void *)kx_k_persist_batch<WPB>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              160 * 1024);
+    (void)hipFuncSetAttribute((const void *)kx_k_persist_batch<WPB, PAR>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     attr_set = true;
   }
-  hipLaunchKernelGGL((kx_k_persist_batch<WPB>), dim3(nwg), dim3(KX_WAVE * WPB), lds, st, E, wgmap);
+  hipLaunchKernelGGL((kx_k_persist_batch<WPB, PAR>), dim3(nwg), dim3(KX_WAVE * WPB), lds, st, E, wgmap);
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 
-int kx_persist_launch_batch(int wpb, const KxBatchEnt *ents_dev, const int2 *wgmap_dev, int nwg, size_t lds,
+int kx_persist_launch_batch(int wpb, int par, const KxBatchEnt *ents_dev, const int2 *wgmap_dev, int nwg, size_t lds,
                             hipStream_t st) {
   if (nwg <= 0) return 0;
   if (lds > 160 * 1024) return -22;
-  switch (wpb) {
-    case 1: return launch_batch<1>(ents_dev, wgmap_dev, nwg, lds, st);
-    case 2: return launch_batch<2>(ents_dev, wgmap_dev, nwg, lds, st);
+  switch (wpb * 2 + (par ? 1 : 0)) {
+    case 2: return launch_batch<1, false>(ents_dev, wgmap_dev, nwg, lds, st);
+    case 3: return launch_batch<1, true>(ents_dev, wgmap_dev, nwg, lds, st);
+    case 4: return launch_batch<2, false>(ents_dev, wgmap_dev, nwg, lds, st);
+    case 5: return launch_batch<2, true>(ents_dev, wgmap_dev, nwg, lds, st);
     default: return -22;
   }
 }
@@ -672,97 +768,111 @@ int kx_bump_counter(int *counter, int delta, hipStream_t st) {
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 
-template <int G, int WPB>
+template <int G, int WPB, bool PAR>
 static int launch_gw(const KxDev &d, const KxLaw *law, int step0, const int *step_base, int nsteps, int ev_row0,
                      int with_forces, const KxPersistCfg &P, hipStream_t st) {
   size_t lds = ((size_t)P.tmpl_words + P.bm_words + P.map_words + (size_t)P.cr_words) * 4;
   if (lds > 160 * 1024) return -22;
   static bool attr_set = false;
   if (!attr_set) {
-    (void)hipFuncSetAttribute((const void *)kx_k_persist<G, WPB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    (void)hipFuncSetAttribute((const void *)kx_k_persist<G, WPB, PAR>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               160 * 1024);
     attr_set = true;
   }
   KxLaw L = law ? *law : KxLaw{};
-  hipLaunchKernelGGL((kx_k_persist<G, WPB>), dim3(P.nwg), dim3(KX_WAVE * WPB), lds, st, d, L, law ? 1 : 0, step0,
-                     step_base, nsteps, ev_row0, with_forces, P);
+  hipLaunchKernelGGL((kx_k_persist<G, WPB, PAR>), dim3(P.nwg), dim3(KX_WAVE * WPB), lds, st, d, L, law ? 1 : 0,
+                     step0, step_base, nsteps, ev_row0, with_forces, P);
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 
-template <int WPB>
+template <int WPB, bool PAR>
 static int launch_mix(const KxDev &d, const KxLaw *law, int step0, const int *step_base, int nsteps, int ev_row0,
                       int with_forces, const KxPersistCfg &P, hipStream_t st) {
   size_t lds = ((size_t)P.tmpl_words + P.bm_words + P.map_words + (size_t)P.cr_words) * 4;
   if (lds > 160 * 1024) return -22;
   static bool attr_set = false;
   if (!attr_set) {
-    (void)hipFuncSetAttribute((const void *)kx_k_persist_mix<WPB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    (void)hipFuncSetAttribute((const void *)kx_k_persist_mix<WPB, PAR>, hipFuncAttributeMaxDynamicSharedMemorySize,
                               160 * 1024);
     attr_set = true;
   }
   KxLaw L = law ? *law : KxLaw{};
-  hipLaunchKernelGGL((kx_k_persist_mix<WPB>), dim3(P.nwg), dim3(KX_WAVE * WPB), lds, st, d, L, law ? 1 : 0, step0,
-                     step_base, nsteps, ev_row0, with_forces, P);
+  hipLaunchKernelGGL((kx_k_persist_mix<WPB, PAR>), dim3(P.nwg), dim3(KX_WAVE * WPB), lds, st, d, L, law ? 1 : 0,
+                     step0, step_base, nsteps, ev_row0, with_forces, P);
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 
-template <int WPB>
+template <int WPB, bool PAR>
 static int occ_mix(size_t lds) {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void *)kx_k_persist_mix<WPB>, KX_WAVE * WPB, lds) !=
-      hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void *)kx_k_persist_mix<WPB, PAR>, KX_WAVE * WPB,
+                                                   lds) != hipSuccess)
     return 0;
   return n;
 }
 
-template <int G, int WPB>
+template <int G, int WPB, bool PAR>
 static int occ_gw(size_t lds) {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void *)kx_k_persist<G, WPB>, KX_WAVE * WPB, lds) !=
-      hipSuccess)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void *)kx_k_persist<G, WPB, PAR>, KX_WAVE * WPB,
+                                                   lds) != hipSuccess)
     return 0;
   return n;
 }
 
-// Resident workgroups per CU for shape (G, wpb) with `lds` bytes of dynamic
-// LDS each (0 when the shape is not instantiated or the query fails).
-int kx_persist_occupancy(int G, int wpb, size_t lds) {
-  if (G == 0) return wpb == 1 ? occ_mix<1>(lds) : wpb == 2 ? occ_mix<2>(lds) : wpb == 4 ? occ_mix<4>(lds) : 0;
+template <bool PAR>
+static int occupancy(int G, int wpb, size_t lds) {
+  if (G == 0)
+    return wpb == 1 ? occ_mix<1, PAR>(lds) : wpb == 2 ? occ_mix<2, PAR>(lds) : wpb == 4 ? occ_mix<4, PAR>(lds) : 0;
 #define KX_OCC(g)                                   \
   case g:                                           \
-    return wpb == 1 ? occ_gw<g, 1>(lds)             \
-         : wpb == 2 ? occ_gw<g, 2>(lds)             \
-         : wpb == 4 ? occ_gw<g, 4>(lds) : 0;
+    return wpb == 1 ? occ_gw<g, 1, PAR>(lds)        \
+         : wpb == 2 ? occ_gw<g, 2, PAR>(lds)        \
+         : wpb == 4 ? occ_gw<g, 4, PAR>(lds) : 0;
   switch (G) { KX_OCC(8) KX_OCC(16) KX_OCC(32) KX_OCC(64) default: return 0; }
 #undef KX_OCC
 }
 
-template <int G>
+// Resident workgroups per CU for shape (G, wpb) and collision fold `par` with
+// `lds` bytes of dynamic LDS each (0 when the shape is not instantiated or
+// the query fails).
+int kx_persist_occupancy(int G, int wpb, size_t lds, bool par) {
+  return par ? occupancy<true>(G, wpb, lds) : occupancy<false>(G, wpb, lds);
+}
+
+template <int G, bool PAR>
 static int launch_g(int wpb, const KxDev &d, const KxLaw *law, int step0, const int *step_base, int nsteps,
                     int ev_row0, int with_forces, const KxPersistCfg &P, hipStream_t st) {
   switch (wpb) {
-    case 1: return launch_gw<G, 1>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
-    case 2: return launch_gw<G, 2>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
-    case 4: return launch_gw<G, 4>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
+    case 1: return launch_gw<G, 1, PAR>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
+    case 2: return launch_gw<G, 2, PAR>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
+    case 4: return launch_gw<G, 4, PAR>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
+    default: return -22;
+  }
+}
+
+template <bool PAR>
+static int launch_par(int G, int wpb, const KxDev &d, const KxLaw *law, int step0, const int *step_base,
+                      int nsteps, int ev_row0, int with_forces, const KxPersistCfg &P, hipStream_t st) {
+  if (P.mixed) {
+    switch (wpb) {
+      case 1: return launch_mix<1, PAR>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
+      case 2: return launch_mix<2, PAR>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
+      case 4: return launch_mix<4, PAR>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
+      default: return -22;
+    }
+  }
+  switch (G) {
+    case 8: return launch_g<8, PAR>(wpb, d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
+    case 16: return launch_g<16, PAR>(wpb, d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
+    case 32: return launch_g<32, PAR>(wpb, d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
+    case 64: return launch_g<64, PAR>(wpb, d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
     default: return -22;
   }
 }
 
 int kx_persist_launch(int G, int wpb, const KxDev &d, const KxLaw *law, int step0, const int *step_base,
                       int nsteps, int ev_row0, int with_forces, const KxPersistCfg &P, hipStream_t st) {
-  if (P.mixed) {
-    switch (wpb) {
-      case 1: return launch_mix<1>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
-      case 2: return launch_mix<2>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
-      case 4: return launch_mix<4>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
-      default: return -22;
-    }
-  }
-  switch (G) {
-    case 8: return launch_g<8>(wpb, d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
-    case 16: return launch_g<16>(wpb, d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
-    case 32: return launch_g<32>(wpb, d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
-    case 64: return launch_g<64>(wpb, d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
-    default: return -22;
-  }
+  return P.coll_par ? launch_par<true>(G, wpb, d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st)
+                    : launch_par<false>(G, wpb, d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
 }

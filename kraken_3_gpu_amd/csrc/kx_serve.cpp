@@ -347,6 +347,7 @@ struct kx_server {
   std::deque<kx_request_done> done;
   int n_admitted_live = 0;          // building + ready + active
   int64_t pending_bytes = 0;        // admission estimates of requests still being built
+  std::atomic<int> fold{0};         // collision fold of every handle (0 = not yet chosen, 1 parallel, 2 owner)
   bool stop = false, sched_exit = false;
   std::atomic<int64_t> launches{0}, creature_steps{0}, peak_active{0}, admit_waits{0};
   std::atomic<int> n_done{0};
@@ -437,6 +438,18 @@ int kx_server::build(Req *r) {
   std::vector<double> cd((size_t)r->ncr * KX_MAX_ARMS, 0.0);
   if ((rc = kx_set_cooldown(r->h, cd.data())) < 0) return rc;
   if ((rc = kx_handle_view(r->h, &r->v)) < 0) return rc;
+  // one batched launch runs one collision fold: the first handle built
+  // chooses it (every request shares the map), later ones follow
+  {
+    const int mine = r->v.P.coll_par ? 1 : 2;
+    int want = 0;
+    if (!fold.compare_exchange_strong(want, mine)) {
+      if (want != mine) {
+        if ((rc = kx_set_coll_mode(r->h, want)) < 0) return rc;
+        if ((rc = kx_handle_view(r->h, &r->v)) < 0) return rc;
+      }
+    }
+  }
   if (r->v.wpb != 1) return server_fail(KX_EINVAL, "request handle is not on the 1-wave shape");
   r->px0.resize(r->ncr);
   for (int c = 0; c < r->ncr; c++) r->px0[c] = f[0][(size_t)c * r->bpc];
@@ -654,7 +667,7 @@ int kx_server::issue(int64_t id, Chunk &c) {
       hipMemcpyAsync(map_dev[slot], M, nwg * sizeof(int2), hipMemcpyHostToDevice, comp) != hipSuccess ||
       hipStreamWaitEvent(comp, c.copied, 0) != hipSuccess || hipEventRecord(c.beg, comp) != hipSuccess)
     return server_fail(KX_EDEVICE, "batch staging failed");
-  if (kx_persist_launch_batch(1, ent_dev[slot], map_dev[slot], (int)nwg, lds, comp) != 0)
+  if (kx_persist_launch_batch(1, fold.load() == 1 ? 1 : 0, ent_dev[slot], map_dev[slot], (int)nwg, lds, comp) != 0)
     return server_fail(KX_EDEVICE, "batched launch failed");
   if (hipEventRecord(c.end, comp) != hipSuccess) return server_fail(KX_EDEVICE, "event record failed");
   int64_t cs = 0;

@@ -292,14 +292,20 @@ def schedule_rounds(g):
 
 
 PERSIST_INFO = ("lds_bytes", "workgroups", "resident_per_cu", "needed_per_cu", "map_words", "bitmap_words",
-                "template_words", "creature_words", "mixed_lanes")
+                "template_words", "creature_words", "mixed_lanes", "collision_fold")
 
 
 def persist_info(g):
     """Launch shape of the persistent path for the handle's (lanes, waves)."""
-    buf = np.zeros(9, dtype=np.int32)
-    K.check(K.lib().kx_persist_info(g.p, K.ptr(buf, C.c_int32), 9))
+    buf = np.zeros(len(PERSIST_INFO), dtype=np.int32)
+    K.check(K.lib().kx_persist_info(g.p, K.ptr(buf, C.c_int32), len(buf)))
     return dict(zip(PERSIST_INFO, buf.tolist()))
+
+
+def set_collision_fold(g, fold):
+    """Persistent path: collision fold (K.KX_FOLD_AUTO, _PARALLEL or _OWNER);
+    every fold gives the same result."""
+    K.check(K.lib().kx_set_collision_fold(g.p, fold))
 
 
 TEMPLATE_INFO = ("bodies", "constraints", "points", "creatures", "rounds", "lanes")

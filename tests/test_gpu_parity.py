@@ -411,6 +411,48 @@ def test_plain_and_persistent_bit_identical(case):
         say(f"    identical, {len(decode_events(evq))} events")
 
 
+@pytest.mark.parametrize("case", ["uniform_walls", "stress_corridors"])
+def test_collision_folds_bit_identical_to_plain(case):
+    """The persistent path's two collision folds (parallel candidate list with
+    an in-order re-run, and the owning lane's in-order fold) reproduce the
+    plain path's sequential fold (xpbd.js:441-470) bit for bit, on worlds
+    where bodies press against rects: the uniform batch run until its
+    creatures reach the walls, and a corridor map where most bodies touch one.
+    Every path runs the same rollout from the same initial world."""
+    law = W.default_law(5)
+    if case == "uniform_walls":
+        make, steps = (lambda: W.world_uniform(1024, seed=2)), 160
+    else:
+        make, steps = (lambda: W.world_stress(2048, seed=4)), 60
+    results = {}
+    for name, path, fold in (("plain", K.KX_PATH_PLAIN, None), ("parallel", K.KX_PATH_PERSISTENT, K.KX_FOLD_PARALLEL),
+                             ("owner", K.KX_PATH_PERSISTENT, K.KX_FOLD_OWNER)):
+        w = make()
+        g = G.create(w, path=path)
+        if fold is not None:
+            G.set_collision_fold(g, fold)
+            assert G.persist_info(g)["collision_fold"] == fold
+        G.upload(g)
+        say(f"[case] collision folds {case}: {name}, {steps} steps")
+        ev = G.rollout(g, law, 0, steps, events=True)
+        G.download(g)
+        G.destroy(g)
+        results[name] = (w, ev)
+    wp, evp = results["plain"]
+    for name in ("parallel", "owner"):
+        wq, evq = results[name]
+        for key in ("px", "py", "ang", "vx", "vy", "av", "broken", "lambda_"):
+            a, b = getattr(wp, key), getattr(wq, key)
+            nd = int(np.count_nonzero(a != b))
+            if nd:
+                say(f"    {name} {key}: {nd} differ, max {np.max(np.abs(a.astype(np.float64) - b)):.3e}")
+            assert nd == 0, (name, key)
+        assert np.array_equal(evp, evq), name
+    w0 = make()
+    say(f"    plain == parallel == owner ({len(decode_events(evp))} events, max |dx| "
+        f"{float(np.max(np.abs(wp.px - w0.px))):.1f} px)")
+
+
 def test_full_size_uniform_properties_and_subset_parity():
     """4096 creatures (the uniform batch): limits, NaN, and one-step parity of
     a 64-creature subset against the oracle."""
