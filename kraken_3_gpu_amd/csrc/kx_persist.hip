@@ -251,6 +251,10 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
     lam2[k] = kx_f2v{0.0f, 0.0f};
   }
   for (int j = sub; j < (act ? T.ngrip : 0); j += G) gbf[j] = d.bf[c0 + t_grip_k[j]];
+  if constexpr (PAR) {
+    // collision flags start at zero; each pass that sets some clears them
+    for (int wi = sub; wi < (act ? (T.np + 31) >> 5 : 0); wi += G) flg[wi] = 0u;
+  }
   double my_cd = 0.0;
   if (act && sub < T.narms) my_cd = d.cd[cid * KX_MAX_ARMS + sub];
   const double hS = d.hd * d.S;
@@ -488,11 +492,10 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
           //    one-point-at-a-time fold. No candidate list, flags or atomics.
           if (__any(mine != 0u)) {
             if constexpr (PAR) {
-              // clear the flags and the candidate count, then one reservation
-              // in the creature's candidate list for all of the lane's points
-              const int nfw = (T.np + 31) >> 5;
+              // clear the candidate count, then one reservation in the
+              // creature's candidate list for all of the lane's points (the
+              // flags are zero here: cleared at launch and after each use)
               if (act && sub == 0) evw[1] = 0u;
-              for (int wi = sub; wi < (act ? nfw : 0); wi += G) flg[wi] = 0u;
               KX_WAVE_SYNC();
               int ntot = 0;
               for (uint32_t m = mine; m; m &= m - 1u) ntot += __float_as_int(t_brec[sub + __builtin_ctz(m) * G].z) >> 16;
@@ -507,6 +510,7 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
               }
               KX_WAVE_SYNC();
               const int ncand = act ? (int)evw[1] : 0;
+              bool flagged = false;
               for (int j = sub; j < ncand; j += G) {
                 const uint32_t e = cand[j];
                 const int p = (int)(e & 0xffffu);
@@ -515,9 +519,14 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
                 kx_sincos(pb.z, &sa, &ca);
                 kx_world_point(pb.x, pb.y, ca, sa, pt.x, pt.y, wx, wy);
                 KxHit hh;
-                if (kx_query(d, P, bits, m_off, m_crect, wx, wy, ox, oy, hh) && pt.z - hh.d > KX_SLACK_POS)
+                if (kx_query(d, P, bits, m_off, m_crect, wx, wy, ox, oy, hh) && pt.z - hh.d > KX_SLACK_POS) {
                   atomicOr(&flg[p >> 5], 1u << (p & 31));
+                  flagged = true;
+                }
               }
+              // phase B only when some point of the wave corrects (rare on
+              // the open maps this fold is chosen for)
+              if (__any(flagged)) {
               KX_WAVE_SYNC();
               for (uint32_t m = mine; m; m &= m - 1u) {
                 const int i = sub + __builtin_ctz(m) * G;
@@ -551,6 +560,11 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
                 }
                 pb.x = x; pb.y = y;
                 pos[i] = pb;
+              }
+              // every lane has read the flags: back to zero for the next pass
+              KX_WAVE_SYNC();
+              const int nfw = (T.np + 31) >> 5;
+              for (int wi = sub; wi < (act ? nfw : 0); wi += G) flg[wi] = 0u;
               }
             } else {
               const bool lds_map = P.map_words != 0;
