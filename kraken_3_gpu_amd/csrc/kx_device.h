@@ -212,9 +212,11 @@ __device__ __forceinline__ bool kx_pin_core(float &xA, float &yA, float &tA, flo
 }
 
 // Angle motor on values: scalar XPBD on the deviation from rest, then the
-// one-sided joint limit (xpbd.js:392-439). Returns true if an angle moved.
+// one-sided joint limit (xpbd.js:392-439). rS is kx_rcp(wA + wB) (wB = 0
+// without a second body), a constant of the constraint that callers may
+// compute once. Returns true if an angle moved.
 __device__ __forceinline__ bool kx_angle_core(float &tA, float &tB, float wA, float wB, bool hasB, float rest,
-                                              float target, float mn, float mx, float alpha, float &lam) {
+                                              float target, float mn, float mx, float alpha, float rS, float &lam) {
   if (!hasB) wB = 0.0f;
   float wSum = wA + wB;
   float rel = hasB ? kx_angle_delta(tB, tA) : tA;
@@ -236,7 +238,7 @@ __device__ __forceinline__ bool kx_angle_core(float &tA, float &tB, float wA, fl
   float dev2 = kx_angle_delta(rel2, rest);
   float over = dev2 > mx ? dev2 - mx : (dev2 < mn ? dev2 - mn : 0.0f);
   const bool ok2 = live && !bad && fabsf(over) >= KX_SLACK_ANG;
-  float dl2 = -over * kx_rcp(wSum);
+  float dl2 = -over * rS;
   float tA2 = hasB ? tA - wA * dl2 : tA + wA * dl2;
   float tB2 = tB + wB * dl2;
   tA = ok2 ? tA2 : tA;
@@ -263,7 +265,8 @@ template <class P, class Q>
 __device__ __forceinline__ void kx_solve_angle(P angP, Q ii, int A, int B, float rest, float target, float mn,
                                                float mx, float alpha, float &lam) {
   float tA = angP[A], tB = B >= 0 ? angP[B] : 0.0f;
-  if (!kx_angle_core(tA, tB, ii[A], B >= 0 ? ii[B] : 0.0f, B >= 0, rest, target, mn, mx, alpha, lam)) return;
+  const float wA = ii[A], wB = B >= 0 ? ii[B] : 0.0f;
+  if (!kx_angle_core(tA, tB, wA, wB, B >= 0, rest, target, mn, mx, alpha, kx_rcp(wA + wB), lam)) return;
   angP[A] = tA;
   if (B >= 0) angP[B] = tB;
 }

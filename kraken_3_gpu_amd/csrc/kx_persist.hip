@@ -75,7 +75,7 @@ __device__ __forceinline__ KxSlot kx_slot_read(int dsc, const float4 *dyn, const
 
 // per-creature LDS layout (word offsets; records of 16 or 8 bytes):
 //   pos[i] = {ppx, ppy, angP, -}   prv[i] = {px, py, ang, -}   vel[i] = {vx, vy, av, -}
-//   dyn[k] = {alpha, broken, lbx, lby} (pin, grip) or {alpha, target, broken, -} (angle):
+//   dyn[k] = {alpha, broken, lbx, lby} (pin, grip) or {alpha, target, broken, 1/(wA+wB)} (angle):
 //            each type keeps its broken flag in the word it does not use
 //   lam[k] = {lambda_x, lambda_y} (pin, grip) or {lambda, -} (angle)
 //   gbf[j] = break force of the j-th grip (the only constraints that break)
@@ -246,8 +246,11 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
   for (int k = sub; k < nc; k += G) {
     const int g = c0 + k;
     const float alpha = d.comp[g] * d.invH2, brk = __int_as_float(d.broken[g]);
-    dyn[k] = (__float_as_int(t_crec[k].w) >> 16) == KX_ANGLE ? make_float4(alpha, d.target[g], brk, 0.0f)
-                                                              : make_float4(alpha, brk, d.lbx[g], d.lby[g]);
+    const int cab = __float_as_int(t_crec[k].w), kA = cab & 255, kB = (cab >> 8) & 255;
+    // an angle's 1 / (wA + wB), as kx_angle_core would compute it
+    const float rS = kx_rcp(t_brec[kA].y + (kB != 255 ? t_brec[kB].y : 0.0f));
+    dyn[k] = (cab >> 16) == KX_ANGLE ? make_float4(alpha, d.target[g], brk, rS)
+                                     : make_float4(alpha, brk, d.lbx[g], d.lby[g]);
     lam2[k] = kx_f2v{0.0f, 0.0f};
   }
   for (int j = sub; j < (act ? T.ngrip : 0); j += G) gbf[j] = d.bf[c0 + t_grip_k[j]];
@@ -363,7 +366,7 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
             // angles, inverse inertias
             float tA = pA.z, tB = pB.z, lam = c.lk.x;
             const bool moved =
-                kx_angle_core(tA, tB, c.bA.y, c.bB.y, c.hasB, c.cr.x, c.dk.y, c.cr.y, c.cr.z, c.dk.x, lam);
+                kx_angle_core(tA, tB, c.bA.y, c.bB.y, c.hasB, c.cr.x, c.dk.y, c.cr.y, c.cr.z, c.dk.x, c.dk.w, lam);
             if (c.live && moved && __float_as_int(c.dk.z) == 0) {
               if (c.hasB) ((float *)&pos[c.B])[2] = tB;
               ((float *)&pos[c.A])[2] = tA;
