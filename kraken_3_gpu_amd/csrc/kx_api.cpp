@@ -329,7 +329,7 @@ static int build_persist(kx_handle *h) {
   const int WPB = h->wpb;
   std::vector<uint32_t> pool;
   std::vector<KxTmpl> T;
-  int words = 0, NB = 1, NC = 1, NP = 1, NG = 0;
+  int words = 0, NB = 1, NC = 1, NP = 1, NG = 0, NA = 0;
   h->tmpl_rounds.clear();
   std::vector<std::vector<int>> members(h->tmpl_host.size());
   for (int c = 0; c < h->nCr; c++) members[h->cr_tmpl[c]].push_back(c);
@@ -348,13 +348,19 @@ static int build_persist(kx_handle *h) {
   // (coll_mode 1 = parallel, 2 = owner); both are exact.
   const bool par = h->coll_mode == 1 || (h->coll_mode == 0 && h->near_frac < 0.5);
   for (size_t t = 0; t < h->tmpl_host.size(); t++) {
-    KxTmpl k = kx_pack_template(h->tmpl_host[t], tmpl_lanes_of(h, t), pool);
+    const int Gt = tmpl_lanes_of(h, t);
+    // each lane holds the start poses and velocities of its bodies in registers
+    if (h->tmpl_host[t].nb > KX_BODY_REGS * Gt)
+      return fail(KX_ENOTSUP, "a creature of %d bodies needs at least %d lanes (at most %d bodies per lane)",
+                  h->tmpl_host[t].nb, (h->tmpl_host[t].nb + KX_BODY_REGS - 1) / KX_BODY_REGS, KX_BODY_REGS);
+    KxTmpl k = kx_pack_template(h->tmpl_host[t], Gt, pool);
     // creature regions: 16-byte records, regions offset by 16 bytes modulo
     // 128 so neighbouring creatures' same records fall in different banks
-    k.cr_stride = (kx_persist_cr_words(k.nb, k.nc, k.np, k.ngrip, par) + 31) / 32 * 32 + 4;
+    k.cr_stride = (kx_persist_cr_words(k.nb, k.nc, k.np, k.ngrip, k.narms, par) + 31) / 32 * 32 + 4;
     T.push_back(k);
     words = std::max(words, k.words);
     NB = std::max(NB, k.nb); NC = std::max(NC, k.nc); NP = std::max(NP, k.np); NG = std::max(NG, k.ngrip);
+    NA = std::max(NA, k.narms);
     h->tmpl_rounds.push_back(k.R);
   }
   std::vector<int> wave_cr, wg(0);
@@ -410,7 +416,7 @@ static int build_persist(kx_handle *h) {
   const size_t with_map = base + (size_t)mapw * 4;
   const int occ1 = with_map <= 160 * 1024 ? kx_persist_occupancy(Gk, WPB, with_map, par) : 0;
   if (!mapw || std::min(occ1, need) < std::min(occ0, need)) mapw = 0;
-  const int stride = (kx_persist_cr_words(NB, NC, NP, NG, par) + 31) / 32 * 32 + 4;   // largest region (reported)
+  const int stride = (kx_persist_cr_words(NB, NC, NP, NG, NA, par) + 31) / 32 * 32 + 4;   // largest region (reported)
   const size_t lds = base + (size_t)mapw * 4;
   h->pinfo[0] = (int)lds; h->pinfo[1] = nwg; h->pinfo[2] = mapw ? occ1 : occ0; h->pinfo[3] = need;
   h->pinfo[4] = mapw; h->pinfo[5] = bmw; h->pinfo[6] = words; h->pinfo[7] = stride;
@@ -702,7 +708,14 @@ static int create_impl(const kx_world_desc *w, const kx_creature_desc *cd, const
     if (p->waves_per_block) h->wpb = p->waves_per_block;
     if (h->G != 8 && h->G != 16 && h->G != 32 && h->G != 64) return fail(KX_EINVAL, "lanes_per_creature %d", h->G);
     if (h->wpb != 1 && h->wpb != 2 && h->wpb != 4) return fail(KX_EINVAL, "waves_per_block %d", h->wpb);
-    TRY(build_persist(h));
+    const int r = build_persist(h);
+    if (r < 0) {
+      // a world the persistent shape cannot hold still runs the plain path;
+      // asking for the persistent path fails with the reason
+      if (h->path == KX_PATH_PERSISTENT) return r;
+      h->persist_ok = false;
+      h->persist_why = g_err;
+    }
   }
   // creatures / arms
   KxCreature *dcr;

@@ -15,6 +15,7 @@
 
 #define KX_MAX_COLORS 16
 #define KX_MAX_ARMS 8
+#define KX_BODY_REGS 8   // persistent path: bodies per lane held in registers (nb <= KX_BODY_REGS * lanes)
 #define KX_SLOTS_PER_ARM 16
 #define KX_SLOT_ADH 256
 #define KX_GRIP_DRAW_KEY 0xFFFFFFFFu   // step key of the once-per-tip grip limit draw
@@ -115,7 +116,7 @@ struct KxPersistCfg {
 };
 int kx_persist_launch(int G, int wpb, const KxDev &d, const KxLaw *law, int step0, const int *step_base,
                       int nsteps, int ev_row0, int with_forces, const KxPersistCfg &P, hipStream_t st);
-int kx_persist_cr_words(int NB, int NC, int NP, int NG, bool par);
+int kx_persist_cr_words(int NB, int NC, int NP, int NG, int NA, bool par);
 int kx_persist_occupancy(int G, int wpb, size_t lds, bool par);   // G = 0: the mixed-lane kernel
 int kx_bump_counter(int *counter, int delta, hipStream_t st);
 

@@ -74,7 +74,11 @@ __device__ __forceinline__ KxSlot kx_slot_read(int dsc, const float4 *dyn, const
 }
 
 // per-creature LDS layout (word offsets; records of 16 or 8 bytes):
-//   pos[i] = {ppx, ppy, angP, -}   prv[i] = {px, py, ang, -}   vel[i] = {vx, vy, av, -}
+// Each body's start-of-substep pose and velocity stay in the registers of the
+// lane that owns the body (lane i % G, at most KX_BODY_REGS bodies per lane);
+// LDS holds what other lanes read:
+//   pos[i] = {ppx, ppy, angP, -}   (predicted pose, read by every solve)
+//   tip[a] = {px, py, ang, vx}, {vy, -, -, -}   (arm a's tip body, for the reflex)
 //   dyn[k] = {alpha, broken, lbx, lby} (pin, grip) or {alpha, target, broken, 1/(wA+wB)} (angle):
 //            each type keeps its broken flag in the word it does not use
 //   lam[k] = {lambda_x, lambda_y} (pin, grip) or {lambda, -} (angle)
@@ -84,10 +88,10 @@ __device__ __forceinline__ KxSlot kx_slot_read(int dsc, const float4 *dyn, const
 //   flg    = one bit per collision point: "corrects at the pose after the rounds"
 //   cand   = collision candidates: point | body << 16
 struct KxCrLds {
-  int pos, prv, vel, dyn, lam, gbf, ev, flg, cand, words;
-  __host__ __device__ KxCrLds(int NB, int NC, int NP, int NG, bool par) {
+  int pos, tip, dyn, lam, gbf, ev, flg, cand, words;
+  __host__ __device__ KxCrLds(int NB, int NC, int NP, int NG, int NA, bool par) {
     int o = 0;
-    pos = o; o += 4 * NB; prv = o; o += 4 * NB; vel = o; o += 4 * NB;
+    pos = o; o += 4 * NB; tip = o; o += 8 * NA;
     dyn = o; o += 4 * NC; lam = o; o += (2 * NC + 3) / 4 * 4;
     gbf = o; o += (NG + 3) / 4 * 4;
     ev = o; o += 4;
@@ -97,18 +101,15 @@ struct KxCrLds {
   }
 };
 
-int kx_persist_cr_words(int NB, int NC, int NP, int NG, bool par) { return KxCrLds(NB, NC, NP, NG, par).words; }
+int kx_persist_cr_words(int NB, int NC, int NP, int NG, int NA, bool par) {
+  return KxCrLds(NB, NC, NP, NG, NA, par).words;
+}
 
-// Adds (sx, sy) to a creature's body positions (start and predicted) and to
-// the world anchors of its world-anchored pins and grips; the same binary32
-// additions as the plain path's shift (an angle constraint has no anchor).
-__device__ __forceinline__ void kx_shift_creature(float4 *prv, float4 *pos, float4 *dyn, const float4 *t_crec, int nb,
-                                                  int nc, int sub, int G, float sx, float sy) {
-  for (int i = sub; i < nb; i += G) {
-    float4 p = prv[i], pp = pos[i];
-    p.x += sx; p.y += sy; pp.x += sx; pp.y += sy;
-    prv[i] = p; pos[i] = pp;
-  }
+// Adds (sx, sy) to the world anchors of a creature's world-anchored pins and
+// grips; with the same additions to the body poses (kx_persist_body) these
+// are the plain path's shift in binary32 (an angle constraint has no anchor).
+__device__ __forceinline__ void kx_shift_anchors(float4 *dyn, const float4 *t_crec, int nc, int sub, int G, float sx,
+                                                 float sy) {
   for (int k = sub; k < nc; k += G) {
     const int v = __float_as_int(t_crec[k].w);
     if (((v >> 8) & 255) == 255 && (v >> 16) != KX_ANGLE) {
@@ -226,9 +227,9 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
 
   const int cid = P.wave_cr[P.wg_cr_off[bid] + w * CPW + q];
   const bool act = cid >= 0;
-  const KxCrLds Lo(T.nb, T.nc, T.np, T.ngrip, PAR);
+  const KxCrLds Lo(T.nb, T.nc, T.np, T.ngrip, T.narms, PAR);
   uint32_t *S = smem + P.tmpl_words + P.bm_words + P.map_words + (size_t)(w * CPW + q) * T.cr_stride;
-  float4 *pos = (float4 *)(S + Lo.pos), *prv = (float4 *)(S + Lo.prv), *vel = (float4 *)(S + Lo.vel);
+  float4 *pos = (float4 *)(S + Lo.pos), *tip = (float4 *)(S + Lo.tip);
   float4 *dyn = (float4 *)(S + Lo.dyn);
   kx_f2v *lam2 = (kx_f2v *)(S + Lo.lam);
   float *gbf = (float *)(S + Lo.gbf);
@@ -238,11 +239,41 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
   const int b0 = act ? d.cr[cid].body_off : 0, c0 = act ? d.cr[cid].cons_off : 0;
 
   // ── load the creature's dynamic state (global coordinates) ──
-  for (int i = sub; i < nb; i += G) {
-    int g = b0 + i;
-    prv[i] = make_float4(d.px[g], d.py[g], d.ang[g], 0.0f);
-    vel[i] = make_float4(d.vx[g], d.vy[g], d.av[g], 0.0f);
+  // pose (rx, ry, ra) and velocity (rvx, rvy, rva) of the lane's bodies
+  // sub + k*G, k < KX_BODY_REGS, in registers; every loop over them is
+  // unrolled (the host guarantees nb <= KX_BODY_REGS * G)
+  float rx[KX_BODY_REGS], ry[KX_BODY_REGS], ra[KX_BODY_REGS], rvx[KX_BODY_REGS], rvy[KX_BODY_REGS],
+      rva[KX_BODY_REGS];
+  // bodies per lane of this workgroup's template (wave-uniform): register
+  // slots past it are skipped by a scalar branch
+  const int nbl = __builtin_amdgcn_readfirstlane((T.nb + G - 1) / G);
+#pragma unroll
+  for (int k = 0; k < KX_BODY_REGS; k++) {
+    const int i = sub + k * G;
+    rx[k] = ry[k] = ra[k] = rvx[k] = rvy[k] = rva[k] = 0.0f;
+    if (i < nb) {
+      const int g = b0 + i;
+      rx[k] = d.px[g]; ry[k] = d.py[g]; ra[k] = d.ang[g];
+      rvx[k] = d.vx[g]; rvy[k] = d.vy[g]; rva[k] = d.av[g];
+    }
   }
+  // each arm's tip body, published by its owning lane for the reflex lane
+  auto publish_tips = [&]() {
+    for (int a = 0; a < (act ? T.narms : 0); a++) {
+      const int t = t_arm_tip[a];
+      if (t % G != sub) continue;
+      const int kt = t / G;
+      float x = 0.0f, y = 0.0f, an = 0.0f, vx = 0.0f, vy = 0.0f;
+#pragma unroll
+      for (int k = 0; k < KX_BODY_REGS; k++) {
+        if (k >= nbl) break;
+        if (k == kt) { x = rx[k]; y = ry[k]; an = ra[k]; vx = rvx[k]; vy = rvy[k]; }
+      }
+      tip[2 * a] = make_float4(x, y, an, vx);
+      tip[2 * a + 1] = make_float4(vy, 0.0f, 0.0f, 0.0f);
+    }
+  };
+  publish_tips();
   for (int k = sub; k < nc; k += G) {
     const int g = c0 + k;
     const float alpha = d.comp[g] * d.invH2, brk = __int_as_float(d.broken[g]);
@@ -302,8 +333,9 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
         const uint32_t ck = (uint32_t)cid + L.cbase;
         double adh = rec ? (double)rec[3 + 3 * sub] : kx_ctl_u(L.seed, ck, (uint32_t)key, KX_SLOT_ADH + sub);
         double gu = (L.flags & KX_LAW_FIXED_GRIP) ? kx_ctl_u(L.seed, ck, KX_GRIP_DRAW_KEY, KX_SLOT_ADH + sub) : 0.0;
-        const int g = t_arm_grip[sub], t = t_arm_tip[sub], j = t_arm_gslot[sub];
-        float4 dk = dyn[g], tp = prv[t], tv = vel[t], cr0 = t_crec[g];   // {lax, lay, -, cab}
+        const int g = t_arm_grip[sub], j = t_arm_gslot[sub];
+        float4 dk = dyn[g], tp = tip[2 * sub], cr0 = t_crec[g];   // tip {px, py, ang, vx}; crec {lax, lay, -, cab}
+        const float4 tv = make_float4(tp.w, tip[2 * sub + 1].x, 0.0f, 0.0f);   // tip {vx, vy}
         const kx_f2v lk = lam2[g];
         int bk = __float_as_int(dk.y);
         float f_bf = gbf[j], gx = dk.z, gy = dk.w, lx = lk.x, ly = lk.y;
@@ -319,25 +351,36 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
     // Between steps the state is held in global binary32 coordinates, as the
     // reference's buffers hold it; a step works relative to the integer
     // origin rint(first body), subtracted before and added back after.
-    float ox = 0.0f, oy = 0.0f;
-    if (act) { float4 p0 = prv[0]; ox = rintf(p0.x); oy = rintf(p0.y); }
-    kx_shift_creature(prv, pos, dyn, t_crec, nb, nc, sub, G, -ox, -oy);
+    // (body 0 is lane 0's first body: its origin reaches the other lanes
+    // through two spare event words)
+    if (act && sub == 0) { evw[2] = __float_as_uint(rintf(rx[0])); evw[3] = __float_as_uint(rintf(ry[0])); }
+    KX_WAVE_SYNC();
+    const float ox = act ? __uint_as_float(evw[2]) : 0.0f, oy = act ? __uint_as_float(evw[3]) : 0.0f;
+#pragma unroll
+    for (int k = 0; k < KX_BODY_REGS; k++) {
+      if (k >= nbl) break;
+      rx[k] -= ox; ry[k] -= oy;
+    }
+    kx_shift_anchors(dyn, t_crec, nc, sub, G, -ox, -oy);
     KX_WAVE_SYNC();
     for (int s = 0; s < d.S; s++) {
       // resetLambda + integrate (xpbd.js:262-287)
       for (int k = sub; k < nc; k += G) lam2[k] = kx_f2v{0.0f, 0.0f};
       const bool forces = with_forces && step == 0 && s == 0;
-      for (int i = sub; i < nb; i += G) {
+#pragma unroll
+      for (int k = 0; k < KX_BODY_REGS; k++) {
+        if (k >= nbl) break;
+        const int i = sub + k * G;
+        if (i >= nb) continue;
         float fx = 0.0f, fy = 0.0f, tq = 0.0f;
         if (forces) {
           int g = b0 + i;
           fx = d.fx[g]; fy = d.fy[g]; tq = d.tq[g];
           d.fx[g] = 0.0f; d.fy[g] = 0.0f; d.tq[g] = 0.0f;
         }
-        float4 p = prv[i], v = vel[i], br = t_brec[i];
+        const float4 br = t_brec[i];
         float q_x, q_y, q_a;
-        kx_integrate(d, v.x, v.y, v.z, p.x, p.y, p.z, br.x, br.y, fx, fy, tq, q_x, q_y, q_a);
-        vel[i] = v;
+        kx_integrate(d, rvx[k], rvy[k], rva[k], rx[k], ry[k], ra[k], br.x, br.y, fx, fy, tq, q_x, q_y, q_a);
         pos[i] = make_float4(q_x, q_y, q_a, 0.0f);
       }
       KX_WAVE_SYNC();
@@ -621,13 +664,14 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
         }
         KX_WAVE_SYNC();
       }
-      for (int i = sub; i < nb; i += G) {
-        float4 p = prv[i], pp = pos[i];
-        float vx, vy, av;
-        kx_finalize(d, p.x, p.y, p.z, vx, vy, av, pp.x, pp.y, pp.z);
-        prv[i] = p;
+#pragma unroll
+      for (int k = 0; k < KX_BODY_REGS; k++) {
+        if (k >= nbl) break;
+        const int i = sub + k * G;
+        if (i >= nb) continue;
+        float4 pp = pos[i];
+        kx_finalize(d, rx[k], ry[k], ra[k], rvx[k], rvy[k], rva[k], pp.x, pp.y, pp.z);
         pos[i] = pp;
-        vel[i] = make_float4(vx, vy, av, 0.0f);
       }
       KX_WAVE_SYNC();
     }
@@ -649,16 +693,32 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
       KX_WAVE_SYNC();
       if (act && sub == 0) d.events[(size_t)(ev_row0 + step) * d.nCr + cid] = evw[0];
     }
-    kx_shift_creature(prv, pos, dyn, t_crec, nb, nc, sub, G, ox, oy);
+#pragma unroll
+    for (int k = 0; k < KX_BODY_REGS; k++) {
+      if (k >= nbl) break;
+      const int i = sub + k * G;
+      rx[k] += ox; ry[k] += oy;
+      if (i < nb) {
+        float4 pp = pos[i];
+        pp.x += ox; pp.y += oy;
+        pos[i] = pp;
+      }
+    }
+    kx_shift_anchors(dyn, t_crec, nc, sub, G, ox, oy);
+    publish_tips();
     KX_WAVE_SYNC();
   }
 
   // ── store ──
-  for (int i = sub; i < nb; i += G) {
-    int g = b0 + i;
-    float4 p = prv[i], pp = pos[i], v = vel[i];
-    d.px[g] = p.x; d.py[g] = p.y; d.ang[g] = p.z;
-    d.vx[g] = v.x; d.vy[g] = v.y; d.av[g] = v.z;
+#pragma unroll
+  for (int k = 0; k < KX_BODY_REGS; k++) {
+    if (k >= nbl) break;
+    const int i = sub + k * G;
+    if (i >= nb) continue;
+    const int g = b0 + i;
+    const float4 pp = pos[i];
+    d.px[g] = rx[k]; d.py[g] = ry[k]; d.ang[g] = ra[k];
+    d.vx[g] = rvx[k]; d.vy[g] = rvy[k]; d.av[g] = rva[k];
     d.ppx[g] = pp.x; d.ppy[g] = pp.y; d.angP[g] = pp.z;
   }
   for (int k = sub; k < nc; k += G) {
@@ -684,18 +744,17 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
 // Every workgroup with the same lane count G; PAR: the collision fold
 // (P.coll_par), one kernel per fold so that each compiles alone.
 template <int G, int WPB, bool PAR>
-__global__ __launch_bounds__(KX_WAVE *WPB) void kx_k_persist(KxDev d, KxLaw L, int has_law, int step0,
+__global__ __launch_bounds__(KX_WAVE *WPB) __attribute__((amdgpu_waves_per_eu(2))) void kx_k_persist(KxDev d, KxLaw L, int has_law, int step0,
                                                              const int *step_base, int nsteps, int ev_row0,
                                                              int with_forces, KxPersistCfg P) {
   kx_persist_body<G, WPB, PAR>(d, L, has_law, step0, step_base, nsteps, ev_row0, with_forces, P, blockIdx.x);
 }
 
 // Lane count per workgroup (per morphology bucket): a wave-uniform switch to
-// the body instantiated for that count. Held to 168 VGPRs (three waves per
-// SIMD), as the single-count kernels are: these launches are the ones whose
-// waves can share a SIMD.
+// the body instantiated for that count. Every persistent kernel is held to
+// 256 VGPRs (two waves per SIMD): no launch here has LDS for more.
 template <int WPB, bool PAR>
-__global__ __launch_bounds__(KX_WAVE *WPB) __attribute__((amdgpu_waves_per_eu(3))) void kx_k_persist_mix(KxDev d, KxLaw L, int has_law, int step0,
+__global__ __launch_bounds__(KX_WAVE *WPB) __attribute__((amdgpu_waves_per_eu(2))) void kx_k_persist_mix(KxDev d, KxLaw L, int has_law, int step0,
                                                                  const int *step_base, int nsteps, int ev_row0,
                                                                  int with_forces, KxPersistCfg P) {
   const int bid = blockIdx.x;
@@ -723,7 +782,7 @@ __device__ __forceinline__ T *kx_glob(T *p) {
 // workgroup runs its entry's body with that entry's state, tables, law and
 // step count. The entry's HBM pointers are taken as global pointers.
 template <int WPB, bool PAR>
-__global__ __launch_bounds__(KX_WAVE *WPB) __attribute__((amdgpu_waves_per_eu(3))) void kx_k_persist_batch(const KxBatchEnt *__restrict__ E,
+__global__ __launch_bounds__(KX_WAVE *WPB) __attribute__((amdgpu_waves_per_eu(2))) void kx_k_persist_batch(const KxBatchEnt *__restrict__ E,
                                                                    const int2 *__restrict__ wgmap) {
   const int2 m = wgmap[blockIdx.x];
   const KxBatchEnt &e = E[m.x];

@@ -190,22 +190,33 @@ def test_graph_replay_matches_eager_launches():
 
 def test_persistent_shapes_are_bit_identical():
     """Any lanes-per-creature / waves-per-workgroup shape runs a topological
-    order of the same constraint dependencies: results must not depend on it."""
+    order of the same constraint dependencies: results must not depend on it.
+    A lane holds at most 8 bodies' start poses in registers, so 8 lanes take
+    creatures of up to 64 bodies: the uneven world (up to 82) runs the wider
+    shapes and refuses 8 lanes with the reason; the uniform world (43 bodies)
+    runs every shape."""
     law = W.default_law(21)
-    ref = None
-    for shape in ((64, 1), (32, 2), (16, 1), (16, 4), (8, 1), (8, 2)):
-        w = W.world_uneven(40, seed=6)
-        g = G.create(w, path=K.KX_PATH_PERSISTENT, lanes_per_creature=shape[0], waves_per_block=shape[1])
-        G.upload(g)
-        say(f"[case] shape {shape}")
-        ev = G.rollout(g, law, 0, 30, events=True)
-        G.download(g)
-        cur = (w.px.copy(), w.py.copy(), w.ang.copy(), w.broken.copy(), ev)
-        if ref is None:
-            ref = cur
-        else:
-            assert all(np.array_equal(x, y) for x, y in zip(ref, cur)), shape
-        G.destroy(g)
+    for make, shapes in ((lambda: W.world_uneven(40, seed=6), ((64, 1), (32, 2), (16, 1), (16, 4))),
+                         (lambda: W.world_uniform(64, seed=6), ((32, 1), (16, 1), (16, 4), (8, 1), (8, 2)))):
+        ref = None
+        for shape in shapes:
+            w = make()
+            g = G.create(w, path=K.KX_PATH_PERSISTENT, lanes_per_creature=shape[0], waves_per_block=shape[1])
+            G.upload(g)
+            say(f"[case] shape {shape}")
+            ev = G.rollout(g, law, 0, 30, events=True)
+            G.download(g)
+            cur = (w.px.copy(), w.py.copy(), w.ang.copy(), w.broken.copy(), ev)
+            if ref is None:
+                ref = cur
+            else:
+                assert all(np.array_equal(x, y) for x, y in zip(ref, cur)), shape
+            G.destroy(g)
+    with pytest.raises(K.KxError, match="needs at least 9 lanes"):
+        G.create(W.world_uneven(40, seed=6), path=K.KX_PATH_PERSISTENT, lanes_per_creature=8)
+    # the same world still runs the plain path
+    g = G.create(W.world_uneven(40, seed=6), path=K.KX_PATH_PLAIN, lanes_per_creature=8)
+    G.destroy(g)
 
 
 def test_mixed_lanes_per_template_are_bit_identical():
@@ -220,9 +231,13 @@ def test_mixed_lanes_per_template_are_bit_identical():
         tinfo = G.template_info(g)
         assert len(tinfo) > 4 and sum(t["creatures"] for t in tinfo) == 256
         if mode == "mixed":
-            G.set_template_lanes(g, [(8, 16, 32, 64)[t % 4] for t in range(len(tinfo))])
+            # 8 lanes only for creatures of up to 64 bodies (8 per lane)
+            want = [16 if (t % 4 == 0 and tinfo[t]["bodies"] > 64) else (8, 16, 32, 64)[t % 4]
+                    for t in range(len(tinfo))]
+            assert 8 in want
+            G.set_template_lanes(g, want)
             assert G.persist_info(g)["mixed_lanes"] == 1
-            assert [t["lanes"] for t in G.template_info(g)] == [(8, 16, 32, 64)[t % 4] for t in range(len(tinfo))]
+            assert [t["lanes"] for t in G.template_info(g)] == want
         G.upload(g)
         say(f"[case] per-template lanes {mode}: {len(tinfo)} templates")
         ev = G.rollout(g, law, 0, 25, events=True)
