@@ -86,6 +86,26 @@ def measured_traffic(n_cr, chunk):
     return t
 
 
+def state_readout_cost(g, w, reps=20):
+    """What a host policy loop pays per step to see the population: the
+    per-creature readout (kx_read_state: readProprio inputs + mantle state)
+    against the full state download the reference's loop mirrors (bodies and
+    constraints), both timed on the host around the call (device work, the
+    copy over PCIe and the synchronisation included)."""
+    def timed(fn):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return (time.perf_counter() - t0) * 1e3 / reps
+    st = G.read_state(g)
+    r_ms = timed(lambda: G.read_state(g))
+    d_ms = timed(lambda: G.download(g))
+    nb, nc = len(w.px), len(w.a)
+    return {"readout_ms": r_ms, "readout_bytes": int(st.nbytes), "floats_per_creature": int(st.shape[1]),
+            "download_ms": d_ms, "download_bytes": int(nb * 24 + nc * 13)}
+
+
 def cpu_baseline(seconds=12.0):
     """The oracle (C restatement of xpbd.js, f64) stepping 64-creature worlds
     under the same control law on the host cores, one world per thread."""
@@ -405,6 +425,7 @@ def main():
     dev_ms = sum(m for m, _ in chunk_ms)
     st = G.download(g)
     shape = G.persist_info(g)
+    readout = state_readout_cost(g, w)
 
     per_cr = dict(pins=42, motors=42, bodies=43, points=92, grips_total=6)
     F = flops_per_creature_step(w.S, w.I, per_cr["pins"], per_cr["motors"], per_cr["bodies"], per_cr["points"],
@@ -473,6 +494,7 @@ def main():
         "rollout_unit": f"{args.chunk} steps x {n_cr} creatures per launch",
         "device_ms_total": dev_ms,
         "final_stats": st,
+        "state_readout": readout,
     }
     if plain:
         out["plain"] = plain

@@ -220,6 +220,26 @@ int kx_persist_info(const kx_handle *h, int32_t *info, int32_t max);
  * (under half of the area within reach of a rect) and OWNER on a dense one
  * (corridors), where most bodies touch a rect most of the time. No
  * counterpart in the reference (a launch-shape knob, like waves_per_block). */
+/* State readout for a host policy, in place of mirroring every body (the
+ * reference's GPU path reads the whole world back; README.md:693-696 names
+ * this compact per-creature stateOut as its next lever). For creature c, at
+ * out + c*stride:
+ *   [0, 6A+6)   readProprio (src/sim/core.js:129-175) of its A arms: shoulder
+ *               angles, curls, tip x/y in the mantle frame, grip loads,
+ *               anchored flags, then acceleration along/across, anchored
+ *               share, mean extension, speed, angular velocity;
+ *   then KX_STATE_EXTRA floats: mantle x, y, angle, vx, vy, av, speed, mean
+ *               extension, mean curl, anchored arms, ax, ay (world frame,
+ *               unclamped).
+ * Computed in binary64 from the device's binary32 state (after the last
+ * step or rollout on the handle's stream), returned as float. Each call moves
+ * the previous mantle velocity the acceleration inputs use, as readProprio
+ * does once per step (zero at kx_create). Needs the creature structure
+ * (kx_creature_desc) with each arm's motors running mantle -> segments ->
+ * tip. Returns 6*maxA + 6 + KX_STATE_EXTRA (54 for six arms); stride must
+ * be at least that. No counterpart in the reference's GPU path. */
+#define KX_STATE_EXTRA 12
+int kx_read_state(kx_handle *h, float *out, int32_t stride);
 #define KX_FOLD_AUTO 0
 #define KX_FOLD_PARALLEL 1
 #define KX_FOLD_OWNER 2

@@ -16,6 +16,7 @@ KX_LAW_TAPE = 2
 KX_TAPE_WORDS = 25
 KX_PATH_PLAIN, KX_PATH_PERSISTENT = 0, 1
 KX_FOLD_AUTO, KX_FOLD_PARALLEL, KX_FOLD_OWNER = 0, 1, 2
+KX_STATE_EXTRA = 12
 
 i32, u32, f64, f32p, i32p, u8p, f64p, u32p = (C.c_int32, C.c_uint32, C.c_double, C.POINTER(C.c_float),
                                               C.POINTER(C.c_int32), C.POINTER(C.c_uint8),
@@ -123,6 +124,7 @@ EXPORTS = {
     "kx_schedule_info": (i32, [C.c_void_p, i32p, i32]),
     "kx_persist_info": (i32, [C.c_void_p, i32p, i32]),
     "kx_set_collision_fold": (i32, [C.c_void_p, i32]),
+    "kx_read_state": (i32, [C.c_void_p, f32p, i32]),
     "kx_template_info": (i32, [C.c_void_p, i32, i32p]),
     "kx_set_template_lanes": (i32, [C.c_void_p, i32, i32p]),
     "kx_creature_templates": (i32, [C.c_void_p, i32p]),

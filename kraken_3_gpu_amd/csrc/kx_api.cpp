@@ -96,6 +96,9 @@ struct kx_handle {
   std::vector<int> tmpl_lanes;   // per template lanes per creature, 0 = the handle's G
   int launch_G = 16;             // lane count the uniform kernel is launched with
   int coll_mode = 0;             // collision fold: 0 = by map density, 1 = parallel, 2 = owning lane
+  std::string state_why;         // why the state readout is unavailable (empty: available)
+  float *state_dev = nullptr;    // readout buffer [nCr * state_stride]
+  int state_stride = 0;
   double near_frac = 0.0;        // fraction of the near bitmap set (build_grid)
   // scratch host staging
   std::vector<float> hf;
@@ -573,6 +576,10 @@ static int create_impl(const kx_world_desc *w, const kx_creature_desc *cd, const
   }
   // creature desc → arms, motor slots
   std::vector<int> cslot(nC, -1), carm(nC, -1), arm_tip, arm_grip;
+  // per arm, for the state readout: the mantle and the segment chain its
+  // motors run along (shoulder motor mantle -> seg0, then seg i -> seg i+1,
+  // kraken.js:103-135)
+  std::vector<int> arm_mantle, arm_seg_off(1, 0), arm_segs;
   std::vector<int> d2i(nC);
   for (int k = 0; k < nC; k++) d2i[h->ord[k]] = h->h2i[k];
   if (cd && cd->n_creatures > 0) {
@@ -599,6 +606,18 @@ static int create_impl(const kx_world_desc *w, const kx_creature_desc *cd, const
             return fail(KX_EINVAL, "arm %d: motor %d is not an angle constraint of creature %d", a, k, c);
           cslot[d2i[k]] = (a - a0) * KX_SLOTS_PER_ARM + (m - m0);
         }
+        bool chain = m1 > m0;
+        for (int m = m0; m < m1 && chain; m++) {
+          const int k = cd->motors[m];
+          chain = w->c_b[k] >= 0 && (m == m0 || w->c_a[k] == w->c_b[cd->motors[m - 1]]);
+        }
+        chain = chain && w->c_b[cd->motors[m1 - 1]] == tip;
+        if (!chain && h->state_why.empty())
+          h->state_why = "arm " + std::to_string(a) + ": its motors do not run mantle -> segments -> tip";
+        arm_mantle.push_back(chain ? w->c_a[cd->motors[m0]] : tip);
+        for (int m = m0; chain && m < m1; m++) arm_segs.push_back(w->c_b[cd->motors[m]]);
+        if (!chain) arm_segs.push_back(tip);
+        arm_seg_off.push_back((int)arm_segs.size());
       }
     }
     h->has_creatures = true;
@@ -722,6 +741,13 @@ static int create_impl(const kx_world_desc *w, const kx_creature_desc *cd, const
   TRY(dalloc(h, &dcr, ncomp)); TRY(dput(dcr, h->cr.data(), ncomp)); d.cr = dcr;
   TRY(dalloc(h, &ip, arm_tip.size())); TRY(dput(ip, arm_tip.data(), arm_tip.size())); d.arm_tip = ip;
   TRY(dalloc(h, &ip, arm_grip.size())); TRY(dput(ip, arm_grip.data(), arm_grip.size())); d.arm_grip = ip;
+  TRY(dalloc(h, &ip, arm_mantle.size())); TRY(dput(ip, arm_mantle.data(), arm_mantle.size())); d.arm_mantle = ip;
+  TRY(dalloc(h, &ip, arm_seg_off.size())); TRY(dput(ip, arm_seg_off.data(), arm_seg_off.size())); d.arm_seg_off = ip;
+  TRY(dalloc(h, &ip, arm_segs.size())); TRY(dput(ip, arm_segs.data(), arm_segs.size())); d.arm_segs = ip;
+  {
+    std::vector<float> zero((size_t)ncomp * 2, 0.0f);
+    TRY(dalloc(h, &d.prev_v, zero.size())); TRY(dput(d.prev_v, zero.data(), zero.size()));
+  }
   TRY(dalloc(h, &d.cd, (size_t)ncomp * KX_MAX_ARMS));
   TRY(dalloc(h, &d.org, (size_t)ncomp * 2));
   TRY(dalloc(h, &ip, nB)); TRY(dput(ip, comp_of.data(), nB)); d.bcr = ip;
@@ -1154,6 +1180,25 @@ extern "C" int kx_set_template_lanes(kx_handle *h, int32_t n, const int32_t *lan
     return r;
   }
   return 0;
+}
+
+extern "C" int kx_read_state(kx_handle *h, float *out, int32_t stride) {
+  if (!h || !out) return fail(KX_EINVAL, "null argument");
+  if (!h->has_creatures) return fail(KX_ENOTSUP, "state readout needs the creature structure (kx_creature_desc)");
+  if (!h->state_why.empty()) return fail(KX_ENOTSUP, "state readout unavailable: %s", h->state_why.c_str());
+  int maxa = 0;
+  for (const KxCreature &c : h->cr) maxa = std::max(maxa, c.narms);
+  const int need = 6 * maxa + 6 + KX_STATE_EXTRA;
+  if (stride < need) return fail(KX_EINVAL, "stride %d: the readout needs %d floats per creature", stride, need);
+  const size_t n = (size_t)h->nCr * stride;
+  if (!h->state_dev || h->state_stride != stride) {
+    TRY(dalloc(h, &h->state_dev, n));
+    h->state_stride = stride;
+  }
+  if (kx_launch_read_state(h->d, h->state_dev, stride, h->st) != 0) return fail(KX_EDEVICE, "state readout launch failed");
+  HIPCHK(hipMemcpyAsync(out, h->state_dev, n * sizeof(float), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  return need;
 }
 
 extern "C" int kx_set_collision_fold(kx_handle *h, int32_t fold) {

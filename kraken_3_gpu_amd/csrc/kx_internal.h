@@ -15,6 +15,7 @@
 
 #define KX_MAX_COLORS 16
 #define KX_MAX_ARMS 8
+#define KX_STATE_EXTRA 12   // include/kxpbd.h kx_read_state
 #define KX_BODY_REGS 8   // persistent path: bodies per lane held in registers (nb <= KX_BODY_REGS * lanes)
 #define KX_SLOTS_PER_ARM 16
 #define KX_SLOT_ADH 256
@@ -67,6 +68,10 @@ struct KxDev {
   float *org;                    // [2*nCr] plain path: integer origin of each creature
   const int *arm_tip;            // global body
   const int *arm_grip;           // internal constraint
+  const int *arm_mantle;         // body the arm's shoulder motor starts at (state readout)
+  const int *arm_seg_off;        // [nArms+1] into arm_segs: the arm's segments, shoulder to tip
+  const int *arm_segs;
+  float *prev_v;                 // [2*nCr] mantle velocity at the previous state readout
   double *cd;                    // [nCr * KX_MAX_ARMS]
   uint32_t *events;              // [nsteps * nCr] or null
   const float *tape;             // [rows][nCr][KX_TAPE_WORDS] controls under KX_LAW_TAPE, or null
@@ -92,6 +97,7 @@ struct KxLaw {
 // Launchers (kx_plain.hip / kx_persist.hip). All asynchronous on `st`.
 // step key = (step_base ? *step_base : 0) + step_key
 // tape_row: the control-tape row of this step (KX_LAW_TAPE)
+int kx_launch_read_state(const KxDev &d, float *out, int stride, hipStream_t st);
 int kx_plain_step(const KxDev &d, const KxLaw *law, int step_key, const int *step_base, int with_forces,
                   int ev_row, int tape_row, hipStream_t st, int *launches);
 // Persistent-path launch description (see kx_persist.hip).

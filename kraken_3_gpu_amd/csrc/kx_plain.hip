@@ -201,3 +201,101 @@ int kx_plain_step(const KxDev &d, const KxLaw *law, int step_key, const int *ste
   if (launches) *launches += n;
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
+
+// ── state readout ───────────────────────────────────────────────────
+// readProprio (src/sim/core.js:129-175; tipPosition, armCurl, armExtension
+// src/body/kraken.js:150-183) of every creature, in binary64 from the
+// binary32 state, then its mantle state; one thread per creature. Math.min /
+// Math.max give NaN for a NaN argument, unlike fmin / fmax, so the clamps are
+// written as comparisons, as the reference's own `cl` is.
+__device__ __forceinline__ double kx_js_min(double a, double b) { return (a != a || b != b) ? a + b : (a < b ? a : b); }
+__device__ __forceinline__ double kx_js_max(double a, double b) { return (a != a || b != b) ? a + b : (a > b ? a : b); }
+__device__ __forceinline__ double kx_cl(double v, double m) { return v > m ? m : (v < -m ? -m : v); }
+
+__global__ __launch_bounds__(PLAIN_BLOCK) void kx_k_read_state(KxDev d, float *out, int stride) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= d.nCr) return;
+  const KxCreature cr = d.cr[c];
+  const int A = cr.narms;
+  if (A <= 0) return;
+  // kraken.js:24-49: segLen, shoulderLimit, jointLimit, ARM_REACH
+  const double kPi = 3.141592653589793, seg_len = 13.5;
+  const double shoulder_lim = kPi * 0.62, joint_lim = kPi * 0.42, reach = 24.0 + 7.0 * seg_len;
+  const int *so = d.arm_seg_off + cr.arm_off;
+  const int m = d.arm_mantle[cr.arm_off];
+  const double x = d.px[m], y = d.py[m], vx = d.vx[m], vy = d.vy[m], angle = d.ang[m], av = d.av[m];
+  const double speed = hypot(vx, vy), ca = cos(-angle), sa = sin(-angle);
+  auto tip_pos = [&](int a, double &tx, double &ty) {
+    const int b = d.arm_segs[so[a + 1] - 1];
+    const double L = seg_len * 0.5, t = d.ang[b];
+    tx = (double)d.px[b] + L * cos(t);
+    ty = (double)d.py[b] + L * sin(t);
+  };
+  auto curl = [&](int a) {
+    double s = 0.0;
+    int n = 0;
+    for (int j = so[a] + 1; j < so[a + 1]; j++) {
+      const double dd = (double)d.ang[d.arm_segs[j]] - (double)d.ang[d.arm_segs[j - 1]];
+      s += atan2(sin(dd), cos(dd));
+      n++;
+    }
+    return n ? s / n : 0.0;
+  };
+  auto extension = [&](int a) {
+    double tx, ty;
+    tip_pos(a, tx, ty);
+    const int s0 = d.arm_segs[so[a]];
+    return kx_js_min(1.0, hypot(tx - (double)d.px[s0], ty - (double)d.py[s0]) / kx_js_max(1.0, reach));
+  };
+  auto shoulder = [&](int a) {
+    const double dd = (double)d.ang[d.arm_segs[so[a]]] - angle;
+    return atan2(sin(dd), cos(dd));
+  };
+  auto broken = [&](int a) { return d.broken[d.arm_grip[cr.arm_off + a]] != 0; };
+  int anchored = 0;
+  double ext_sum = 0.0, curl_sum = 0.0;
+  for (int a = 0; a < A; a++) {
+    ext_sum += extension(a);
+    curl_sum += curl(a);
+    if (!broken(a)) anchored++;
+  }
+  float *o = out + (size_t)c * stride;
+  int i = 0;
+  for (int a = 0; a < A; a++) o[i++] = (float)kx_cl(shoulder(a) / shoulder_lim, 1.2);
+  for (int a = 0; a < A; a++) o[i++] = (float)kx_cl(curl(a) / joint_lim, 1.2);
+  for (int a = 0; a < A; a++) {
+    double tx, ty;
+    tip_pos(a, tx, ty);
+    const double dx = tx - x, dy = ty - y;
+    o[i++] = (float)kx_cl((dx * ca - dy * sa) / reach, 1.2);
+    o[i++] = (float)kx_cl((dx * sa + dy * ca) / reach, 1.2);
+  }
+  for (int a = 0; a < A; a++) {
+    const int g = d.arm_grip[cr.arm_off + a];
+    // the grip's force as _postGripBreak last measured it: |lambda| / h of
+    // the step's last substep (xpbd.js:496-504)
+    const double f = hypot((double)d.lamx[g], (double)d.lamy[g]) * (1.0 / d.hd);
+    const double fz = (f == f && f != 0.0) ? f : 0.0;
+    o[i++] = broken(a) ? 0.0f : (float)kx_js_min(1.0, fz / kx_js_max(1.0, (double)d.bf[g]));
+  }
+  for (int a = 0; a < A; a++) o[i++] = broken(a) ? 0.0f : 1.0f;
+  const double pvx = d.prev_v[2 * c], pvy = d.prev_v[2 * c + 1];
+  const double ax = (vx - pvx) / (d.hd * d.S), ay = (vy - pvy) / (d.hd * d.S);
+  o[i++] = (float)kx_js_max(-1.0, kx_js_min(1.0, (ax * ca - ay * sa) / 600));
+  o[i++] = (float)kx_js_max(-1.0, kx_js_min(1.0, (ax * sa + ay * ca) / 600));
+  const double mean_ext = ext_sum / A, mean_curl = curl_sum / A;
+  o[i++] = (float)((double)anchored / A);
+  o[i++] = (float)mean_ext;
+  o[i++] = (float)kx_js_min(1.0, speed / 300);
+  o[i++] = (float)kx_js_max(-1.0, kx_js_min(1.0, av / 18));
+  // mantle state and the readout's side values
+  const double extra[KX_STATE_EXTRA] = {x, y, angle, vx, vy, av, speed, mean_ext, mean_curl, (double)anchored, ax, ay};
+  for (int k = 0; k < KX_STATE_EXTRA; k++) o[i++] = (float)extra[k];
+  d.prev_v[2 * c] = (float)vx;
+  d.prev_v[2 * c + 1] = (float)vy;
+}
+
+int kx_launch_read_state(const KxDev &d, float *out, int stride, hipStream_t st) {
+  hipLaunchKernelGGL(kx_k_read_state, dim3(nblk(d.nCr)), dim3(PLAIN_BLOCK), 0, st, d, out, stride);
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}

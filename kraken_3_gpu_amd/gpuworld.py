@@ -302,6 +302,23 @@ def persist_info(g):
     return dict(zip(PERSIST_INFO, buf.tolist()))
 
 
+STATE_EXTRA = ("x", "y", "angle", "vx", "vy", "av", "speed", "mean_extension", "mean_curl", "anchored", "ax", "ay")
+
+
+def read_state(g, stride=0):
+    """Per-creature state readout (kx_read_state): readProprio's 6*A + 6
+    inputs (src/sim/core.js:129-175) followed by the mantle state
+    STATE_EXTRA, float32 [nCr, stride]. Advances the previous mantle velocity
+    the acceleration inputs use, as readProprio does once per step."""
+    off = np.asarray(g.cpu.arm_off)
+    arms = int(np.diff(off).max()) if len(off) > 1 else 0
+    need = 6 * arms + 6 + K.KX_STATE_EXTRA
+    stride = max(stride, need)
+    out = np.zeros((g.nCr, stride), np.float32)
+    K.check(K.lib().kx_read_state(g.p, K.ptr(out, C.c_float), stride))
+    return out
+
+
 def set_collision_fold(g, fold):
     """Persistent path: collision fold (K.KX_FOLD_AUTO, _PARALLEL or _OWNER);
     every fold gives the same result."""

@@ -306,8 +306,16 @@ function rollout(g, law, step0, nsteps, events) {
   return lib().rollout(g.handle, law || {}, step0 | 0, nsteps | 0, events);
 }
 
+/** Per-creature state readout in place of a full download: readProprio's
+ * 6A+6 inputs (src/sim/core.js:129-175) then the mantle state (12 floats),
+ * `stride` floats per creature into `out` (Float32Array of creatures x
+ * stride). Returns the floats per creature the readout writes. */
+function readState(g, out, stride) {
+  return lib().readState(g.handle, { out }, stride | 0);
+}
+
 module.exports = {
   GPU_STRIDES, C_PIN, C_ANGLE, C_GRIP,
   isAvailable, create, upload, uploadControls, packControls, step, download, mirrorInto,
-  groupCollPoints, destroy, rollout,
+  groupCollPoints, destroy, rollout, readState,
 };

@@ -4,6 +4,6 @@ import gw from './gpuworld.js';
 export const {
   GPU_STRIDES, C_PIN, C_ANGLE, C_GRIP,
   isAvailable, create, upload, uploadControls, packControls, step, download, mirrorInto,
-  groupCollPoints, destroy, rollout,
+  groupCollPoints, destroy, rollout, readState,
 } = gw;
 export default gw;

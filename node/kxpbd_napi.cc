@@ -241,7 +241,7 @@ napi_value Counts(napi_env env, napi_callback_info info) {
   return arr;
 }
 
-// constraintOrder(handle, order:Int32Array, color:Int32Array)
+// constraintOrder(handle, {order:Int32Array, color:Int32Array})
 napi_value ConstraintOrder(napi_env env, napi_callback_info info) {
   size_t argc = 2;
   napi_value a[2];
@@ -453,6 +453,35 @@ napi_value Download(napi_env env, napi_callback_info info) {
   return queue(env, w, {a[0], a[1], a[2]});
 }
 
+// readState(handle, {out:Float32Array}, stride) -> floats per creature written
+// (kx_read_state: readProprio's inputs then the mantle state, per creature)
+napi_value ReadState(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value a[3];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
+  if (argc < 3) {
+    napi_throw_type_error(env, nullptr, "kxpbd.readState(handle, {out}, stride)");
+    return nullptr;
+  }
+  kx_handle *h = handle_of(env, a[0]);
+  if (!h) return nullptr;
+  int32_t stride = 0, nb, nc, np, ncr;
+  NAPI_OK(napi_get_value_int32(env, a[2], &stride));
+  int rc = kx_counts(h, &nb, &nc, &np, &ncr);
+  if (rc < 0) return throw_kx(env, rc);
+  if (stride <= 0) {
+    napi_throw_range_error(env, nullptr, "kxpbd.readState: stride must be positive");
+    return nullptr;
+  }
+  float *out;
+  if (!typed(env, a[1], "out", napi_float32_array, (size_t)ncr * (size_t)stride, &out, true)) return nullptr;
+  rc = kx_read_state(h, out, stride);
+  if (rc < 0) return throw_kx(env, rc);
+  napi_value r;
+  NAPI_OK(napi_create_int32(env, rc, &r));
+  return r;
+}
+
 // rollout(handle, law, step0, nsteps, events?:Uint32Array) -> Promise<void>
 napi_value Rollout(napi_env env, napi_callback_info info) {
   size_t argc = 5;
@@ -570,6 +599,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"step", nullptr, Step, nullptr, nullptr, nullptr, kProp, nullptr},
       {"download", nullptr, Download, nullptr, nullptr, nullptr, kProp, nullptr},
       {"rollout", nullptr, Rollout, nullptr, nullptr, nullptr, kProp, nullptr},
+      {"readState", nullptr, ReadState, nullptr, nullptr, nullptr, kProp, nullptr},
       {"groupPoints", nullptr, GroupPoints, nullptr, nullptr, nullptr, kProp, nullptr},
       {"lastError", nullptr, LastError, nullptr, nullptr, nullptr, kProp, nullptr},
       {"abiVersion", nullptr, AbiVersion, nullptr, nullptr, nullptr, kProp, nullptr},

@@ -291,3 +291,89 @@ class World:
 
     def bench_threads(self, threads, steps, seed=1):
         return lib().or_bench_threads(self.p, threads, steps, seed & 0xFFFFFFFF)
+
+
+# ── proprioception readout (restated from the reference; checker only) ──
+
+SEG_LEN = 13.5
+
+
+def read_proprio(px, py, ang, vx, vy, av, mantle, arms, grips, prev, h, substeps):
+    """readProprio (src/sim/core.js:129-175) of one creature, in f64.
+
+    arms: per arm its segment bodies (segs[0] .. tip); grips: per arm
+    (broken, force, breakForce or None for unbreakable); prev: the previous
+    call's mantle velocity (prevVx, prevVy). Returns (out, side) with out the
+    6*A + 6 inputs and side = (meanExtension, meanCurl, anchoredCount, speed,
+    (vx, vy) to carry as the next prev). Helpers: tipPosition, armCurl and
+    armExtension (src/body/kraken.js:150-183), readState (core.js:100-108).
+    """
+    import math
+    shoulder_limit, joint_limit = math.pi * 0.62, math.pi * 0.42
+    x, y = float(px[mantle]), float(py[mantle])
+    cvx, cvy = float(vx[mantle]), float(vy[mantle])
+    angle, cav = float(ang[mantle]), float(av[mantle])
+    speed = math.hypot(cvx, cvy)
+    ca, sa = math.cos(-angle), math.sin(-angle)
+
+    def tip(segs):
+        b = segs[-1]
+        L = SEG_LEN * 0.5
+        return float(px[b]) + L * math.cos(float(ang[b])), float(py[b]) + L * math.sin(float(ang[b]))
+
+    def curl(segs):
+        s, n = 0.0, 0
+        for i in range(1, len(segs)):
+            d = float(ang[segs[i]]) - float(ang[segs[i - 1]])
+            s += math.atan2(math.sin(d), math.cos(d))
+            n += 1
+        return s / n if n else 0.0
+
+    def extension(segs):
+        tx, ty = tip(segs)
+        s0 = segs[0]
+        return min(1.0, math.hypot(tx - float(px[s0]), ty - float(py[s0])) / max(1.0, ARM_REACH))
+
+    def cl(v, m):
+        return m if v > m else (-m if v < -m else v)
+
+    def jsmin(a, b):
+        return math.nan if (a != a or b != b) else (a if a < b else b)
+
+    def jsmax(a, b):
+        return math.nan if (a != a or b != b) else (a if a > b else b)
+
+    A = len(arms)
+    anchored, ext_sum, curl_sum = 0, 0.0, 0.0
+    shoulder = []
+    for a in range(A):
+        d = float(ang[arms[a][0]]) - angle
+        shoulder.append(math.atan2(math.sin(d), math.cos(d)))
+        ext_sum += extension(arms[a])
+        curl_sum += curl(arms[a])
+        if not grips[a][0]:
+            anchored += 1
+    out = []
+    out += [cl(shoulder[a] / shoulder_limit, 1.2) for a in range(A)]
+    out += [cl(curl(arms[a]) / joint_limit, 1.2) for a in range(A)]
+    for a in range(A):
+        tx, ty = tip(arms[a])
+        dx, dy = tx - x, ty - y
+        out.append(cl((dx * ca - dy * sa) / ARM_REACH, 1.2))
+        out.append(cl((dx * sa + dy * ca) / ARM_REACH, 1.2))
+    for a in range(A):
+        broken, force, bf = grips[a]
+        f = force if (force == force and force) else 0.0
+        bfv = math.inf if bf is None else bf
+        out.append(0.0 if broken else jsmin(1.0, f / jsmax(1.0, bfv)))
+    out += [0.0 if grips[a][0] else 1.0 for a in range(A)]
+    ax = (cvx - prev[0]) / (h * substeps)
+    ay = (cvy - prev[1]) / (h * substeps)
+    out.append(jsmax(-1.0, jsmin(1.0, (ax * ca - ay * sa) / 600)))
+    out.append(jsmax(-1.0, jsmin(1.0, (ax * sa + ay * ca) / 600)))
+    out.append(anchored / A)
+    mean_ext, mean_curl = ext_sum / A, curl_sum / A
+    out.append(mean_ext)
+    out.append(jsmin(1.0, speed / 300))
+    out.append(jsmax(-1.0, jsmin(1.0, cav / 18)))
+    return out, (mean_ext, mean_curl, anchored, speed, (cvx, cvy))

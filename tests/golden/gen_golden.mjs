@@ -11,7 +11,7 @@ import { XpbdWorld, ensureColored, SLACK_POS } from '/root/reference/src/physics
 import { makeMap, makeSdf, findSpawn, mulberry32 } from '/root/reference/src/world/map.js';
 import { spawnKraken, attachGrip, tipPosition, adhesionToBreakForce, toneToCompliance,
   BODY, ARM_REACH } from '/root/reference/src/body/kraken.js';
-import { createSim, applyControl, stepSim } from '/root/reference/src/sim/core.js';
+import { createSim, applyControl, stepSim, readProprio } from '/root/reference/src/sim/core.js';
 import { f32ify } from '/root/reference/tools/f32solver.mjs';
 import { packControls, mirrorInto, groupCollPoints, GPU_STRIDES } from '/root/reference/src/gpu/gpuworld.js';
 
@@ -325,5 +325,32 @@ function runTraj(mode, S, I, h, nSteps, record, seed) {
     bodyIn: Array.from(new Uint32Array(bd.buffer)), consIn: Array.from(new Uint32Array(cdm.buffer)), mirrored,
     group: { bodies: pts, order: Array.from(g1.order), range: Array.from(g1.range),
       shuffledBodies: shuffled.map(p => p.body), shuffledOrder: Array.from(g2.order), shuffledRange: Array.from(g2.range) } });
+}
+// ── 10. proprioception (core.js:129-175 readProprio) after every step of the
+// trajectory world under the seeded law, the acceleration input following the
+// previous call's mantle velocity as in the sim loop ──
+{
+  const S = 4, I = 8, h = 1 / 240, seed = 2024, nSteps = 32, rec = [1, 2, 4, 8, 16, 32];
+  const { w, ks } = buildTrajWorld(S, I, h);
+  const crs = ks.map(k => reflexCreature(w, k));
+  const cs = ks.map(k => ({ world: w, mantle: k.mantle, arms: k.arms, prevVx: 0, prevVy: 0 }));
+  const frames = [];
+  for (let s = 0; s < nSteps; s++) {
+    for (let ci = 0; ci < ks.length; ci++) applyLaw(crs[ci], lawControl(seed, ci, s, ks[ci].arms));
+    w.step();
+    const prev = cs.map(c => [c.prevVx, c.prevVy]);
+    const outs = cs.map(c => Array.from(readProprio(c, new Float64Array(42))));
+    if (rec.indexOf(s + 1) >= 0) {
+      frames.push({ step: s + 1, px: arr(w.px, w.n), py: arr(w.py, w.n), ang: arr(w.ang, w.n),
+        vx: arr(w.vx, w.n), vy: arr(w.vy, w.n), av: arr(w.av, w.n),
+        grips: ks.map(k => k.arms.map(a => [a.grip.broken ? 1 : 0, a.grip.force,
+          Number.isFinite(a.grip.breakForce) ? a.grip.breakForce : null])),
+        prev, proprio: outs,
+        side: cs.map(c => [c.meanExtension, c.meanCurl, c.anchoredCount, c.speed]) });
+    }
+  }
+  dump('proprio.json', { S, I, h, seed, armReach: ARM_REACH, segLen: BODY.segLen,
+    shoulderLimit: BODY.shoulderLimit, jointLimit: BODY.jointLimit,
+    creatures: ks.map(k => ({ mantle: k.mantle, arms: k.arms.map(a => a.segs) })), frames });
 }
 console.log('done');

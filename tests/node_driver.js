@@ -112,7 +112,11 @@ async function gpu(job) {
     const ev = new Uint32Array(job.rolloutSteps * ncr);
     await gw.rollout(g2, job.law, 0, job.rolloutSteps, ev);
     await gw.download(g2);
-    out.rollout = Object.assign(state(cpu2), { events: Array.from(ev) });
+    // per-creature readout after the rollout (6A+6 proprio inputs + 12 mantle floats)
+    const stride = job.stateStride;
+    const st = new Float32Array(ncr * stride);
+    const wrote = gw.readState(g2, st, stride);
+    out.rollout = Object.assign(state(cpu2), { events: Array.from(ev), state: Array.from(st), stateStride: wrote });
     gw.destroy(g2);
   }
   // error behaviour: a bad body index throws like the reference's guard

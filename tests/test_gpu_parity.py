@@ -566,3 +566,55 @@ def _apply_law_keyed(o, seed, step, keys):
     import ctypes as C
     for local, key in enumerate(keys):
         lib.or_apply_law_one(o.p, C.c_uint32(seed), C.c_uint32(step), local, key)
+
+
+@pytest.mark.parametrize("world", ["uniform", "uneven"])
+def test_state_readout_matches_oracle(world):
+    """kx_read_state after each of six steps: readProprio (the oracle's
+    restatement of src/sim/core.js:129-175, pinned to the reference's own
+    outputs) of every creature on the downloaded binary32 state, the grip
+    load from |lambda| / h as _postGripBreak measures it, the acceleration
+    from the previous readout's mantle velocity; then the mantle state. Bar:
+    binary32 rounding of binary64 values (1e-6 relative + 1e-6)."""
+    w = W.world_uniform(64, seed=3) if world == "uniform" else W.world_uneven(64, seed=5)
+    g = G.create(w, path=K.KX_PATH_PERSISTENT)
+    G.upload(g)
+    law = W.default_law(9)
+    h, S = w.solver["h"], w.solver["substeps"]
+    arms_of = []
+    for c in range(g.nCr):
+        arms = []
+        for a in range(w.arm_off[c], w.arm_off[c + 1]):
+            ms = w.motors[w.motor_off[a]:w.motor_off[a + 1]]
+            arms.append((int(w.a[ms[0]]), [int(w.b[m]) for m in ms], int(w.arm_grip[a])))
+        arms_of.append(arms)
+    prev = np.zeros((g.nCr, 2))
+    worst = 0.0
+    loads = 0
+    for s in range(6):
+        G.rollout(g, law, s, 1)
+        st = G.read_state(g)
+        G.download(g)
+        bf = G.download_controls(g)["breakForce"]   # the reflex sets it on the device; download() leaves it
+        say(f"[case] state readout {world} step {s + 1}")
+        for c in range(g.nCr):
+            arms = arms_of[c]
+            A = len(arms)
+            grips = []
+            for (_, _, gk) in arms:
+                f = float(np.hypot(np.float64(w.lambdaX[gk]), np.float64(w.lambdaY[gk]))) * (1.0 / h)
+                grips.append((int(w.broken[gk]), f, float(bf[gk])))
+            out, side = po.read_proprio(w.px, w.py, w.ang, w.vx, w.vy, w.av, arms[0][0], [a[1] for a in arms], grips,
+                                        prev[c], h, S)
+            m = arms[0][0]
+            ax, ay = (w.vx[m] - prev[c][0]) / (h * S), (w.vy[m] - prev[c][1]) / (h * S)
+            extra = [w.px[m], w.py[m], w.ang[m], w.vx[m], w.vy[m], w.av[m], side[3], side[0], side[1], side[2], ax, ay]
+            want = np.array(out + extra, np.float64)
+            got = st[c, :6 * A + 6 + K.KX_STATE_EXTRA].astype(np.float64)
+            err = np.abs(got - want) / (1e-6 * np.abs(want) + 1e-6)
+            worst = max(worst, float(err.max()))
+            assert err.max() <= 1.0, (c, s, int(np.argmax(err)), got[np.argmax(err)], want[np.argmax(err)])
+            loads += sum(1 for v in out[4 * A:5 * A] if 0.0 < v)
+            prev[c] = (w.vx[m], w.vy[m])
+    say(f"    {g.nCr} creatures x 6 steps within the bar (worst {worst:.2f} of it), {loads} loaded grips")
+    G.destroy(g)

@@ -52,10 +52,11 @@ def test_addon_loads_and_exports_the_surface():
     assert r.returncode == 0, r.stderr
     d = json.loads(r.stdout)
     assert d["abi"] == 3 and d["avail"] is True
-    for name in ("create", "destroy", "upload", "packControls", "step", "download", "rollout", "groupPoints"):
+    for name in ("create", "destroy", "upload", "packControls", "step", "download", "rollout", "groupPoints",
+                 "readState"):
         assert name in d["k"]
     for name in ("create", "upload", "uploadControls", "packControls", "step", "download", "mirrorInto",
-                 "groupCollPoints", "destroy", "GPU_STRIDES", "isAvailable"):
+                 "groupCollPoints", "destroy", "GPU_STRIDES", "isAvailable", "readState"):
         assert name in d["g"]
 
 
@@ -114,8 +115,9 @@ def test_node_loop_and_rollout_match_c_abi(tmp_path):
                 slipMax=law.slip_max, cooldown=law.cooldown, attachMin=law.attach_min,
                 releaseBelow=law.release_below)
     tg = _targets(steps, w0.n_creatures)
+    stride = 6 * int(np.diff(np.asarray(w0.arm_off)).max()) + 6 + K.KX_STATE_EXTRA
     job = dict(world=world_to_js(w0), creatures=creatures_to_js(w0), steps=steps, targets=tg, law=lawd,
-               rolloutSteps=rsteps)
+               rolloutSteps=rsteps, stateStride=stride)
     r = run_node("gpu", job, tmp_path, timeout=240)
 
     # the same through the C ABI
@@ -144,10 +146,13 @@ def test_node_loop_and_rollout_match_c_abi(tmp_path):
     G.upload(g2)
     ev = G.rollout(g2, law, 0, rsteps, events=True)
     G.download(g2)
+    rs = G.read_state(g2, stride)
     G.destroy(g2)
     ro = r["rollout"]
     for key in ("px", "py", "ang"):
         assert np.array_equal(np.array(ro[key]), getattr(w2, key)), key
     assert ro["events"] == ev.reshape(-1).tolist()
+    assert ro["stateStride"] == rs.shape[1]
+    assert np.array_equal(np.array(ro["state"], dtype=np.float32).reshape(rs.shape), rs)
     print(f"    node rollout identical over {rsteps} steps, {int((ev != 0).sum())} event words", flush=True)
     assert r["badThrows"] is True and "kxpbd error" in r["badMessage"]

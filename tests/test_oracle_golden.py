@@ -363,3 +363,27 @@ def test_spawn_pose_has_zero_violation():
     w = po.World()
     w.spawn(720, 405, 0.7)
     assert _worst_pin_violation(w) < 1e-9
+
+
+def test_read_proprio_matches_reference(golden):
+    """The oracle's readProprio restatement (src/sim/core.js:129-175) gives the
+    reference's own outputs on the trajectory world after each recorded step
+    (grips loaded, joints on their limits, the acceleration input following
+    the previous call's velocity), side values included."""
+    g = golden("proprio.json")
+    worst = 0.0
+    for fr in g["frames"]:
+        for ci, cr in enumerate(g["creatures"]):
+            out, side = po.read_proprio(fr["px"], fr["py"], fr["ang"], fr["vx"], fr["vy"], fr["av"], cr["mantle"],
+                                        cr["arms"], fr["grips"][ci], fr["prev"][ci], g["h"], g["S"])
+            ref = fr["proprio"][ci]
+            assert len(out) == len(ref) == 6 * len(cr["arms"]) + 6
+            worst = max(worst, max(abs(a - b) for a, b in zip(out, ref)))
+            me, mc, anc, spd = fr["side"][ci]
+            assert abs(side[0] - me) < 1e-12 and abs(side[1] - mc) < 1e-12 and side[2] == anc
+            assert abs(side[3] - spd) < 1e-9
+    assert worst < 1e-12
+    # the fixture exercises the clamps and the grip loads
+    flat = [v for fr in g["frames"] for o in fr["proprio"] for v in o]
+    assert max(flat) == 1.2 and min(flat) == -1.2
+    assert any(0.0 < v < 1.0 for fr in g["frames"] for o in fr["proprio"] for v in o[24:30])
