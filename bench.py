@@ -98,6 +98,7 @@ def state_readout_cost(g, w, reps=20):
         for _ in range(reps):
             fn()
         return (time.perf_counter() - t0) * 1e3 / reps
+    from kraken_3_gpu_amd import gpuworld as G
     st = G.read_state(g)
     r_ms = timed(lambda: G.read_state(g))
     d_ms = timed(lambda: G.download(g))
