@@ -355,6 +355,55 @@ int  kx_mem_info(int64_t *free_bytes, int64_t *total_bytes);
  * steps step0 .. step0+nsteps-1, tape[nsteps][2*pairs][KX_TAPE_WORDS] */
 int  kx_policy_tape(const kx_request *req, int32_t step0, int32_t nsteps, double step_seconds, float *tape);
 
+/* ── policy network on the device (src/brain/net.js) ─────────────────────
+ * A population of n creatures, each with its own genome of
+ * kx_policy_weights(n_in, hidden, n_out) floats in randomGenome's layout
+ * (net.js:30-50; gputrain.js:55-58 holds them as one flat array). The forward
+ * pass is forwardInto (net.js:71-98) for every creature at once: sums in
+ * float64 in the reference's order, activations and outputs stored as
+ * float32, the layer-2 bias read where net.js:84-86 reads it. hidden and
+ * n_out at most 64; the genome, the input row and the activations must fit
+ * 64 KiB. */
+typedef struct kx_policy kx_policy;
+int32_t kx_policy_weights(int32_t n_in, int32_t hidden, int32_t n_out);   /* weightCount, net.js:30-32 */
+int  kx_policy_create(int32_t n, int32_t n_in, int32_t hidden, int32_t n_out, kx_policy **out);
+int  kx_policy_destroy(kx_policy *p);
+/* genomes[n][weights] host <-> device (the population's genomes stay resident) */
+int  kx_policy_set_genomes(kx_policy *p, const float *genomes);
+int  kx_policy_get_genomes(const kx_policy *p, float *genomes);
+/* gputrain.js:77-86 for every creature: x[n][n_in] in (the caller places the
+ * previous outputs in the last n_out inputs, gputrain.js:82), out[n][n_out]
+ * out; scratch[n][2*hidden] (h1 | h2, may be NULL); controls[n][KX_TAPE_WORDS]
+ * (may be NULL): decode (net.js:114-124) of `arms` arms (<= 8) into control-
+ * tape records ready for kx_upload_tape; emitEcho stays in out[n_out-1]. */
+int  kx_policy_forward(kx_policy *p, const float *x, float *out, float *scratch, float *controls, int32_t arms);
+/* device time of the last forward kernel (HIP events), milliseconds */
+int  kx_policy_last_ms(const kx_policy *p, float *ms);
+/* reps launches of the forward kernel over the resident inputs and genomes
+ * (no host copies); ms = mean kernel time. For measurement. */
+int  kx_policy_forward_repeat(kx_policy *p, int32_t reps, float *ms);
+
+/* ── evolution strategy on the device (src/brain/es.js) ────────────────── */
+typedef struct {
+  double sigma, lr, min_sigma, sigma_decay, beta1, beta2, adam_eps, clip;   /* es.js:23-38 option names */
+  int32_t pop_size;                                                        /* made even as es.js:37 does */
+} kx_es_opts;
+typedef struct kx_es kx_es;
+/* theta (n_weights floats) may be NULL for zeros (createES, es.js:22-58) */
+int  kx_es_create(int32_t n_weights, const kx_es_opts *opts, const float *theta, kx_es **out);
+int  kx_es_destroy(kx_es *e);
+/* beginGeneration (train.js:154-158): genomeAt (es.js:67-88) for i = 0 ..
+ * pop_size-1 into the policy's resident genomes (its n must equal pop_size),
+ * drawing from the trainer's mulberry32 stream (train.js:65-72) whose state is
+ * *rnd_state; on return *rnd_state is the state after those draws. */
+int  kx_es_generation(kx_es *e, kx_policy *p, uint32_t *rnd_state);
+/* update (es.js:92-149) from fitness[pop_size] of the last generation;
+ * stats4 = {best, mean, sd, sigma} as update returns them (may be NULL) */
+int  kx_es_update(kx_es *e, const double *fitness, double *stats4);
+/* theta, m, v, best_theta (n_weights floats each, any may be NULL) and
+ * scalars3 = {sigma, t, bestEver} (may be NULL) */
+int  kx_es_get(const kx_es *e, float *theta, float *m, float *v, float *best_theta, double *scalars3);
+
 #ifdef __cplusplus
 }
 #endif

@@ -90,6 +90,11 @@ class ServerInfo(C.Structure):
                 ("admit_waits", C.c_int64)]
 
 
+class EsOpts(C.Structure):
+    _fields_ = [("sigma", f64), ("lr", f64), ("min_sigma", f64), ("sigma_decay", f64), ("beta1", f64),
+                ("beta2", f64), ("adam_eps", f64), ("clip", f64), ("pop_size", i32)]
+
+
 EXPORTS = {
     "kx_create": (i32, [C.POINTER(WorldDesc), C.POINTER(CreatureDesc), C.POINTER(Params), C.POINTER(C.c_void_p)]),
     "kx_destroy": (None, [C.c_void_p]),
@@ -129,6 +134,19 @@ EXPORTS = {
     "kx_set_template_lanes": (i32, [C.c_void_p, i32, i32p]),
     "kx_creature_templates": (i32, [C.c_void_p, i32p]),
     "kx_set_plain_sdf": (i32, [C.c_void_p, i32]),
+    "kx_policy_weights": (i32, [i32, i32, i32]),
+    "kx_policy_create": (i32, [i32, i32, i32, i32, C.POINTER(C.c_void_p)]),
+    "kx_policy_destroy": (i32, [C.c_void_p]),
+    "kx_policy_set_genomes": (i32, [C.c_void_p, f32p]),
+    "kx_policy_get_genomes": (i32, [C.c_void_p, f32p]),
+    "kx_policy_forward": (i32, [C.c_void_p, f32p, f32p, f32p, f32p, i32]),
+    "kx_policy_last_ms": (i32, [C.c_void_p, f32p]),
+    "kx_policy_forward_repeat": (i32, [C.c_void_p, i32, f32p]),
+    "kx_es_create": (i32, [i32, C.POINTER(EsOpts), f32p, C.POINTER(C.c_void_p)]),
+    "kx_es_destroy": (i32, [C.c_void_p]),
+    "kx_es_generation": (i32, [C.c_void_p, C.c_void_p, u32p]),
+    "kx_es_update": (i32, [C.c_void_p, f64p, f64p]),
+    "kx_es_get": (i32, [C.c_void_p, f32p, f32p, f32p, f32p, f64p]),
     "kx_builder_new": (C.c_void_p, []),
     "kx_builder_free": (None, [C.c_void_p]),
     "kx_builder_map": (i32, [C.c_void_p, f64, f64, i32, i32, f64, f64, u32]),

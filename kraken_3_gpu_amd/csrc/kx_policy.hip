@@ -1,0 +1,485 @@
+// Policy network and evolution strategy on the device: the batched forward
+// pass of src/brain/net.js:71-124 (forwardInto + decode) over a population
+// with one genome per creature, and the ES generation of src/brain/es.js:67-149
+// (genomeAt for every member, update) with the trainer's mulberry32 stream
+// (src/sim/train.js:65-72) drawn by index instead of in sequence.
+//
+// Numerics follow the reference's JS arithmetic: float64 operations on
+// float32-stored values, each result rounded where the reference stores into
+// a Float32Array. Sums run in the reference's order; a product of two float32
+// values is exact in float64, so fma(x, w, s) equals JS's s + x*w. The ES
+// kernels compile with contraction off so every float64 operation rounds as
+// its JS counterpart does.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "../../include/kxpbd.h"
+#include "kx_internal.h"
+
+namespace {
+
+constexpr int KX_POL_MAX_ROWS = 64;      // hidden and n_out: one lane per row
+constexpr int KX_POL_MAX_IN = 1024;
+constexpr size_t KX_POL_MAX_LDS = 64 * 1024;
+
+int pfail(int code, const char *fmt, ...) {
+  char buf[384];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  return kx_error(code, buf);
+}
+
+#define PCHK(expr)                                                                   \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess) return pfail(KX_EDEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+struct PolDims {
+  int n_in, hidden, n_out, nw;
+  int b1, b2, b4, b5;   // offsets of net.js:73-77 (b3 = b2 + hidden^2 is never read)
+};
+
+PolDims pol_dims(int n_in, int hidden, int n_out) {
+  PolDims d;
+  d.n_in = n_in;
+  d.hidden = hidden;
+  d.n_out = n_out;
+  d.b1 = n_in * hidden;
+  d.b2 = d.b1 + hidden;
+  int b3 = d.b2 + hidden * hidden;
+  d.b4 = b3 + hidden;
+  d.b5 = d.b4 + hidden * n_out;
+  d.nw = d.b5 + n_out;
+  return d;
+}
+
+__device__ __forceinline__ double kx_act(double s) {
+  // net.js:81: s > 20 -> 1, s < -20 -> -1, else tanh(s) (NaN passes through)
+  return s > 20.0 ? 1.0 : (s < -20.0 ? -1.0 : tanh(s));
+}
+
+// One wave per creature. The genome (nW floats), the input row, both hidden
+// activations and the outputs are staged in LDS; lane j computes row j of each
+// layer.
+__global__ __launch_bounds__(64) void kx_k_policy_fwd(PolDims d, int n, const float *__restrict__ genomes,
+                                                      const float *__restrict__ x, float *__restrict__ out,
+                                                      float *__restrict__ scratch, float *__restrict__ controls,
+                                                      int arms, int vec4) {
+  extern __shared__ float lds[];
+  const int c = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (c >= n) return;
+  float *W = lds;
+  float *X = W + d.nw;
+  float *H1 = X + d.n_in;
+  float *H2 = H1 + d.hidden;
+  float *O = H2 + d.hidden;   // n_out floats: the outputs decode reads
+  const float *g = genomes + (size_t)c * d.nw;
+  if (vec4) {
+    const float4 *g4 = reinterpret_cast<const float4 *>(g);
+    float4 *W4 = reinterpret_cast<float4 *>(W);
+    const int n4 = d.nw >> 2;
+#pragma unroll 4
+    for (int k = lane; k < n4; k += 64) W4[k] = g4[k];
+  } else {
+#pragma unroll 4
+    for (int k = lane; k < d.nw; k += 64) W[k] = g[k];
+  }
+  const float *xr = x + (size_t)c * d.n_in;
+  for (int k = lane; k < d.n_in; k += 64) X[k] = xr[k];
+  __syncthreads();
+
+  if (lane < d.hidden) {
+    double s = (double)W[d.b1 + lane];
+    const float *row = W + lane * d.n_in;
+    for (int i = 0; i < d.n_in; i++) s = fma((double)X[i], (double)row[i], s);
+    H1[lane] = (float)kx_act(s);
+  }
+  __syncthreads();
+  if (lane < d.hidden) {
+    // layer-2 bias read at b2 + j as net.js:84-86 does (the first weight row)
+    double s = (double)W[d.b2 + lane];
+    const float *row = W + d.b2 + lane * d.hidden;
+    for (int i = 0; i < d.hidden; i++) s = fma((double)H1[i], (double)row[i], s);
+    H2[lane] = (float)kx_act(s);
+  }
+  __syncthreads();
+  float o = 0.f;
+  if (lane < d.n_out) {
+    double s = (double)W[d.b5 + lane];
+    const float *row = W + d.b4 + lane * d.hidden;
+    for (int i = 0; i < d.hidden; i++) s = fma((double)H2[i], (double)row[i], s);
+    o = (float)kx_act(s);
+    out[(size_t)c * d.n_out + lane] = o;
+  }
+  if (scratch) {
+    if (lane < d.hidden) {
+      scratch[(size_t)c * 2 * d.hidden + lane] = H1[lane];
+      scratch[(size_t)c * 2 * d.hidden + d.hidden + lane] = H2[lane];
+    }
+  }
+  if (controls) {
+    // decode (net.js:114-124) into a control-tape record: [0] tone, then
+    // shoulder, curl, adhesion per arm; words of absent arms are zero
+    if (lane < d.n_out) O[lane] = o;
+    __syncthreads();
+    float *rec = controls + (size_t)c * KX_TAPE_WORDS;
+    if (lane < KX_TAPE_WORDS) {
+      float v = 0.f;
+      if (lane == 0) {
+        v = (float)(0.5 + 0.5 * (double)O[0]);
+      } else {
+        const int a = (lane - 1) / 3, f = (lane - 1) % 3;
+        if (a < arms) {
+          const float u = O[1 + a * 3 + f];   // index < 1 + 3*arms <= n_out (host check)
+          v = (f == 2) ? (u > 0.f ? u : 0.f) : u;
+        }
+      }
+      rec[lane] = v;
+    }
+  }
+}
+
+// The value of call idx of a mulberry32 stream with state `st` before its
+// first call: the state advances by 0x6D2B79F5 per call (src/world/map.js:11-20).
+__device__ __forceinline__ double kx_mb32_at(uint32_t st, uint64_t idx) {
+  uint32_t a = st + (uint32_t)(idx + 1) * 0x6D2B79F5u;
+  uint32_t t = a;
+  t = (t ^ (t >> 15)) * (t | 1u);
+  t ^= t + (t ^ (t >> 7)) * (t | 61u);
+  return (double)(t ^ (t >> 14)) * (1.0 / 4294967296.0);
+}
+
+// genomeAt for every member of a generation (es.js:67-88 as called by
+// train.js:154-158): thread (p, k) draws eps_p[k] (four calls, k-major within
+// the pair, pairs in order) and writes both mirrored genomes.
+__global__ __launch_bounds__(256) void kx_k_es_genomes(int nw, int pairs, uint32_t st, double sigma,
+                                                       const float *__restrict__ theta, float *__restrict__ eps_bank,
+                                                       float *__restrict__ genomes) {
+#pragma clang fp contract(off)
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)pairs * nw) return;
+  const int p = (int)(t / nw), k = (int)(t - (int64_t)p * nw);
+  const uint64_t base = (uint64_t)t * 4;
+  const double r0 = kx_mb32_at(st, base), r1 = kx_mb32_at(st, base + 1);
+  const double r2 = kx_mb32_at(st, base + 2), r3 = kx_mb32_at(st, base + 3);
+  const float e = (float)(((((r0 + r1) + r2) + r3) - 2.0) * 1.7320508);   // gauss01, es.js:84-86
+  eps_bank[t] = e;
+  const double th = (double)theta[k];
+  genomes[(size_t)(2 * p) * nw + k] = (float)(th + (double)e * sigma);
+  genomes[(size_t)(2 * p + 1) * nw + k] = (float)(th + (double)e * -sigma);
+}
+
+struct KxEsStep {
+  double inv, clip, b1, b2, omb1, omb2, c1, c2, lr, eps;
+};
+
+// update (es.js:107-135) per weight: the gradient accumulated in float32 over
+// the pairs in order, scaled and clipped, then one Adam step.
+__global__ __launch_bounds__(256) void kx_k_es_update(int nw, int half, KxEsStep q, const double *__restrict__ diffs,
+                                                      const float *__restrict__ eps_bank, float *__restrict__ theta,
+                                                      float *__restrict__ m, float *__restrict__ v) {
+#pragma clang fp contract(off)
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nw) return;
+  float g = 0.f;
+  for (int p = 0; p < half; p++) g = (float)((double)g + diffs[p] * (double)eps_bank[(size_t)p * nw + k]);
+  double a = (double)g * q.inv;
+  if (a > q.clip) a = q.clip;
+  else if (a < -q.clip) a = -q.clip;
+  const double gk = (double)(float)a;
+  const float mk = (float)(q.b1 * (double)m[k] + q.omb1 * gk);
+  const float vk = (float)(q.b2 * (double)v[k] + q.omb2 * gk * gk);
+  m[k] = mk;
+  v[k] = vk;
+  const double step = q.lr * ((double)mk / q.c1) / (sqrt((double)vk / q.c2) + q.eps);
+  if (isfinite(step)) theta[k] = (float)((double)theta[k] + step);
+}
+
+}  // namespace
+
+struct kx_policy {
+  int n = 0;
+  PolDims d{};
+  float *genomes = nullptr, *x = nullptr, *out = nullptr, *scratch = nullptr, *controls = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  float last_ms = 0.f;
+};
+
+struct kx_es {
+  int nw = 0;
+  kx_es_opts o{};
+  double sigma = 0, best_ever = -INFINITY;
+  int t = 0, have_eps = 0, have_best = 0;
+  float *theta = nullptr, *m = nullptr, *v = nullptr, *eps = nullptr, *best = nullptr;
+  double *diffs = nullptr;
+  hipStream_t st = nullptr;
+};
+
+extern "C" int32_t kx_policy_weights(int32_t n_in, int32_t hidden, int32_t n_out) {
+  if (n_in <= 0 || hidden <= 0 || n_out <= 0) return 0;
+  return pol_dims(n_in, hidden, n_out).nw;
+}
+
+extern "C" int kx_policy_create(int32_t n, int32_t n_in, int32_t hidden, int32_t n_out, kx_policy **out) {
+  if (!out) return pfail(KX_EINVAL, "null argument");
+  *out = nullptr;
+  if (n <= 0 || n_in <= 0 || hidden <= 0 || n_out <= 0)
+    return pfail(KX_EINVAL, "policy of %d creatures, %d -> %d -> %d", n, n_in, hidden, n_out);
+  if (hidden > KX_POL_MAX_ROWS || n_out > KX_POL_MAX_ROWS || n_in > KX_POL_MAX_IN)
+    return pfail(KX_EINVAL, "policy %d -> %d -> %d: hidden and outputs at most %d, inputs at most %d", n_in, hidden,
+                 n_out, KX_POL_MAX_ROWS, KX_POL_MAX_IN);
+  PolDims d = pol_dims(n_in, hidden, n_out);
+  size_t lds = (size_t)(d.nw + n_in + 2 * hidden + n_out) * sizeof(float);
+  if (lds > KX_POL_MAX_LDS) return pfail(KX_EINVAL, "genome of %d weights exceeds the %zu-byte LDS stage", d.nw, KX_POL_MAX_LDS);
+  kx_policy *p = new kx_policy();
+  p->n = n;
+  p->d = d;
+  auto cleanup = [&](int rc) {
+    kx_policy_destroy(p);
+    return rc;
+  };
+  if (hipMalloc((void **)&p->genomes, (size_t)n * d.nw * sizeof(float)) != hipSuccess ||
+      hipMalloc((void **)&p->x, (size_t)n * n_in * sizeof(float)) != hipSuccess ||
+      hipMalloc((void **)&p->out, (size_t)n * n_out * sizeof(float)) != hipSuccess ||
+      hipMalloc((void **)&p->scratch, (size_t)n * 2 * hidden * sizeof(float)) != hipSuccess ||
+      hipMalloc((void **)&p->controls, (size_t)n * KX_TAPE_WORDS * sizeof(float)) != hipSuccess)
+    return cleanup(pfail(KX_ENOMEM, "policy buffers for %d creatures", n));
+  if (hipStreamCreate(&p->st) != hipSuccess || hipEventCreate(&p->ev0) != hipSuccess ||
+      hipEventCreate(&p->ev1) != hipSuccess)
+    return cleanup(pfail(KX_EDEVICE, "policy stream"));
+  if (hipMemsetAsync(p->genomes, 0, (size_t)n * d.nw * sizeof(float), p->st) != hipSuccess ||
+      hipMemsetAsync(p->out, 0, (size_t)n * n_out * sizeof(float), p->st) != hipSuccess ||
+      hipStreamSynchronize(p->st) != hipSuccess)
+    return cleanup(pfail(KX_EDEVICE, "policy clear"));
+  *out = p;
+  return 0;
+}
+
+extern "C" int kx_policy_destroy(kx_policy *p) {
+  if (!p) return 0;
+  if (p->st) (void)hipStreamSynchronize(p->st);
+  (void)hipFree(p->genomes);
+  (void)hipFree(p->x);
+  (void)hipFree(p->out);
+  (void)hipFree(p->scratch);
+  (void)hipFree(p->controls);
+  if (p->ev0) (void)hipEventDestroy(p->ev0);
+  if (p->ev1) (void)hipEventDestroy(p->ev1);
+  if (p->st) (void)hipStreamDestroy(p->st);
+  delete p;
+  return 0;
+}
+
+extern "C" int kx_policy_set_genomes(kx_policy *p, const float *genomes) {
+  if (!p || !genomes) return pfail(KX_EINVAL, "null argument");
+  PCHK(hipMemcpyAsync(p->genomes, genomes, (size_t)p->n * p->d.nw * sizeof(float), hipMemcpyHostToDevice, p->st));
+  PCHK(hipStreamSynchronize(p->st));
+  return 0;
+}
+
+extern "C" int kx_policy_get_genomes(const kx_policy *p, float *genomes) {
+  if (!p || !genomes) return pfail(KX_EINVAL, "null argument");
+  PCHK(hipMemcpyAsync(genomes, p->genomes, (size_t)p->n * p->d.nw * sizeof(float), hipMemcpyDeviceToHost, p->st));
+  PCHK(hipStreamSynchronize(p->st));
+  return 0;
+}
+
+static int policy_launch(kx_policy *p, int arms, bool want_controls) {
+  const PolDims &d = p->d;
+  const size_t lds = (size_t)(d.nw + d.n_in + 2 * d.hidden + d.n_out) * sizeof(float);
+  const int vec4 = (d.nw & 3) == 0;   // hipMalloc bases are 256-byte aligned
+  PCHK(hipEventRecord(p->ev0, p->st));
+  hipLaunchKernelGGL(kx_k_policy_fwd, dim3(p->n), dim3(64), lds, p->st, d, p->n, p->genomes, p->x, p->out,
+                     p->scratch, want_controls ? p->controls : nullptr, arms, vec4);
+  PCHK(hipGetLastError());
+  PCHK(hipEventRecord(p->ev1, p->st));
+  return 0;
+}
+
+extern "C" int kx_policy_forward(kx_policy *p, const float *x, float *out, float *scratch, float *controls,
+                                 int32_t arms) {
+  if (!p || !x || !out) return pfail(KX_EINVAL, "null argument");
+  if (controls && (arms < 0 || arms > KX_MAX_ARMS || p->d.n_out < 1 + 3 * arms))
+    return pfail(KX_EINVAL, "decode of %d arms from %d outputs (at most %d arms)", arms, p->d.n_out, KX_MAX_ARMS);
+  const PolDims &d = p->d;
+  PCHK(hipMemcpyAsync(p->x, x, (size_t)p->n * d.n_in * sizeof(float), hipMemcpyHostToDevice, p->st));
+  int rc = policy_launch(p, arms, controls != nullptr);
+  if (rc) return rc;
+  PCHK(hipMemcpyAsync(out, p->out, (size_t)p->n * d.n_out * sizeof(float), hipMemcpyDeviceToHost, p->st));
+  if (scratch)
+    PCHK(hipMemcpyAsync(scratch, p->scratch, (size_t)p->n * 2 * d.hidden * sizeof(float), hipMemcpyDeviceToHost,
+                        p->st));
+  if (controls)
+    PCHK(hipMemcpyAsync(controls, p->controls, (size_t)p->n * KX_TAPE_WORDS * sizeof(float), hipMemcpyDeviceToHost,
+                        p->st));
+  PCHK(hipStreamSynchronize(p->st));
+  PCHK(hipEventElapsedTime(&p->last_ms, p->ev0, p->ev1));
+  return 0;
+}
+
+extern "C" int kx_policy_forward_repeat(kx_policy *p, int32_t reps, float *ms) {
+  if (!p || reps <= 0) return pfail(KX_EINVAL, "policy repeat of %d", reps);
+  float tot = 0.f;
+  for (int r = 0; r < reps; r++) {
+    int rc = policy_launch(p, 0, false);
+    if (rc) return rc;
+    PCHK(hipStreamSynchronize(p->st));
+    float t = 0.f;
+    PCHK(hipEventElapsedTime(&t, p->ev0, p->ev1));
+    tot += t;
+  }
+  p->last_ms = tot / reps;
+  if (ms) *ms = p->last_ms;
+  return 0;
+}
+
+extern "C" int kx_policy_last_ms(const kx_policy *p, float *ms) {
+  if (!p || !ms) return pfail(KX_EINVAL, "null argument");
+  *ms = p->last_ms;
+  return 0;
+}
+
+extern "C" int kx_es_create(int32_t n_weights, const kx_es_opts *opts, const float *theta, kx_es **out) {
+  if (!out || !opts) return pfail(KX_EINVAL, "null argument");
+  *out = nullptr;
+  if (n_weights <= 0) return pfail(KX_EINVAL, "ES over %d weights", n_weights);
+  kx_es_opts o = *opts;
+  if (o.pop_size <= 0) return pfail(KX_EINVAL, "ES population %d", o.pop_size);
+  if (o.pop_size % 2) o.pop_size++;   // es.js:37 keeps the population even
+  kx_es *e = new kx_es();
+  e->nw = n_weights;
+  e->o = o;
+  e->sigma = o.sigma;
+  auto cleanup = [&](int rc) {
+    kx_es_destroy(e);
+    return rc;
+  };
+  const size_t wb = (size_t)n_weights * sizeof(float);
+  if (hipMalloc((void **)&e->theta, wb) != hipSuccess || hipMalloc((void **)&e->m, wb) != hipSuccess ||
+      hipMalloc((void **)&e->v, wb) != hipSuccess || hipMalloc((void **)&e->best, wb) != hipSuccess ||
+      hipMalloc((void **)&e->eps, (size_t)(o.pop_size / 2) * wb) != hipSuccess ||
+      hipMalloc((void **)&e->diffs, (size_t)(o.pop_size / 2) * sizeof(double)) != hipSuccess)
+    return cleanup(pfail(KX_ENOMEM, "ES state of %d weights", n_weights));
+  if (hipStreamCreate(&e->st) != hipSuccess) return cleanup(pfail(KX_EDEVICE, "ES stream"));
+  bool ok = hipMemsetAsync(e->m, 0, wb, e->st) == hipSuccess && hipMemsetAsync(e->v, 0, wb, e->st) == hipSuccess;
+  if (ok && theta) ok = hipMemcpyAsync(e->theta, theta, wb, hipMemcpyHostToDevice, e->st) == hipSuccess;
+  else if (ok) ok = hipMemsetAsync(e->theta, 0, wb, e->st) == hipSuccess;
+  if (!ok || hipStreamSynchronize(e->st) != hipSuccess) return cleanup(pfail(KX_EDEVICE, "ES init"));
+  *out = e;
+  return 0;
+}
+
+extern "C" int kx_es_destroy(kx_es *e) {
+  if (!e) return 0;
+  if (e->st) (void)hipStreamSynchronize(e->st);
+  (void)hipFree(e->theta);
+  (void)hipFree(e->m);
+  (void)hipFree(e->v);
+  (void)hipFree(e->eps);
+  (void)hipFree(e->best);
+  (void)hipFree(e->diffs);
+  if (e->st) (void)hipStreamDestroy(e->st);
+  delete e;
+  return 0;
+}
+
+extern "C" int kx_es_generation(kx_es *e, kx_policy *p, uint32_t *rnd_state) {
+  if (!e || !p || !rnd_state) return pfail(KX_EINVAL, "null argument");
+  if (p->d.nw != e->nw || p->n != e->o.pop_size)
+    return pfail(KX_EINVAL, "policy of %d genomes x %d weights for an ES of %d x %d", p->n, p->d.nw, e->o.pop_size,
+                 e->nw);
+  const int pairs = e->o.pop_size / 2;
+  const int64_t tot = (int64_t)pairs * e->nw;
+  // the policy's stream waits for nothing else of the ES; order the two by
+  // running the generation on the policy's stream after the ES's work
+  PCHK(hipStreamSynchronize(e->st));
+  hipLaunchKernelGGL(kx_k_es_genomes, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, p->st, e->nw, pairs,
+                     *rnd_state, e->sigma, e->theta, e->eps, p->genomes);
+  PCHK(hipGetLastError());
+  PCHK(hipStreamSynchronize(p->st));
+  *rnd_state = (uint32_t)(*rnd_state + (uint32_t)((uint64_t)tot * 4) * 0x6D2B79F5u);
+  e->have_eps = 1;
+  return 0;
+}
+
+extern "C" int kx_es_update(kx_es *e, const double *fitness, double *stats4) {
+  if (!e || !fitness) return pfail(KX_EINVAL, "null argument");
+  if (!e->have_eps) return pfail(KX_EINVAL, "ES update before any generation (kx_es_generation)");
+  const kx_es_opts &o = e->o;
+  const int pop = o.pop_size, half = pop >> 1;
+  // advantages (es.js:96-106), in float64 on the host as the reference does
+  double diffs[4096];
+  if (half > 4096) return pfail(KX_EINVAL, "ES population %d exceeds 8192", pop);
+  double mean = 0;
+  for (int p = 0; p < half; p++) {
+    diffs[p] = (fitness[p * 2] - fitness[p * 2 + 1]) / 2;
+    mean += fitness[p * 2] + fitness[p * 2 + 1];
+  }
+  mean /= pop;
+  double sd = 0;
+  for (int p = 0; p < half; p++) sd += diffs[p] * diffs[p];
+  sd = std::sqrt(sd / (half > 1 ? half : 1));
+  if (sd == 0 || std::isnan(sd)) sd = 1;   // `|| 1`
+  KxEsStep q;
+  q.inv = 1 / (half * e->sigma * sd);
+  q.clip = o.clip;
+  e->t++;
+  q.b1 = o.beta1;
+  q.b2 = o.beta2;
+  q.omb1 = 1 - o.beta1;
+  q.omb2 = 1 - o.beta2;
+  q.c1 = 1 - std::pow(o.beta1, (double)e->t);
+  q.c2 = 1 - std::pow(o.beta2, (double)e->t);
+  q.lr = o.lr;
+  q.eps = o.adam_eps;
+  PCHK(hipMemcpyAsync(e->diffs, diffs, (size_t)half * sizeof(double), hipMemcpyHostToDevice, e->st));
+  hipLaunchKernelGGL(kx_k_es_update, dim3((e->nw + 255) / 256), dim3(256), 0, e->st, e->nw, half, q, e->diffs, e->eps,
+                     e->theta, e->m, e->v);
+  PCHK(hipGetLastError());
+  e->sigma = std::max(o.min_sigma, e->sigma * o.sigma_decay);
+  double best = -INFINITY;
+  for (int i = 0; i < pop; i++)
+    if (fitness[i] > best) best = fitness[i];
+  if (best > e->best_ever) {
+    e->best_ever = best;
+    PCHK(hipMemcpyAsync(e->best, e->theta, (size_t)e->nw * sizeof(float), hipMemcpyDeviceToDevice, e->st));
+    e->have_best = 1;
+  }
+  PCHK(hipStreamSynchronize(e->st));
+  if (stats4) {
+    stats4[0] = best;
+    stats4[1] = mean;
+    stats4[2] = sd * 2;
+    stats4[3] = e->sigma;
+  }
+  return 0;
+}
+
+extern "C" int kx_es_get(const kx_es *e, float *theta, float *m, float *v, float *best_theta, double *scalars3) {
+  if (!e) return pfail(KX_EINVAL, "null argument");
+  const size_t wb = (size_t)e->nw * sizeof(float);
+  if (theta) PCHK(hipMemcpyAsync(theta, e->theta, wb, hipMemcpyDeviceToHost, e->st));
+  if (m) PCHK(hipMemcpyAsync(m, e->m, wb, hipMemcpyDeviceToHost, e->st));
+  if (v) PCHK(hipMemcpyAsync(v, e->v, wb, hipMemcpyDeviceToHost, e->st));
+  if (best_theta) {
+    if (e->have_best) PCHK(hipMemcpyAsync(best_theta, e->best, wb, hipMemcpyDeviceToHost, e->st));
+    else std::memset(best_theta, 0, wb);
+  }
+  PCHK(hipStreamSynchronize(e->st));
+  if (scalars3) {
+    scalars3[0] = e->sigma;
+    scalars3[1] = e->t;
+    scalars3[2] = e->best_ever;
+  }
+  return 0;
+}

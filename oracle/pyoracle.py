@@ -377,3 +377,148 @@ def read_proprio(px, py, ang, vx, vy, av, mantle, arms, grips, prev, h, substeps
     out.append(jsmin(1.0, speed / 300))
     out.append(jsmax(-1.0, jsmin(1.0, cav / 18)))
     return out, (mean_ext, mean_curl, anchored, speed, (cvx, cvy))
+
+
+# ── policy network and ES (src/brain/net.js, src/brain/es.js), numpy f64 ──
+
+HIDDEN = 24                      # src/brain/net.js:20
+OUT_DIM = 1 + 6 * 3 + 1          # src/brain/net.js:23
+
+
+def weight_count(n_in, hidden=HIDDEN, n_out=OUT_DIM):
+    """src/brain/net.js:30-32."""
+    return n_in * hidden + hidden + hidden * hidden + hidden + hidden * n_out + n_out
+
+
+def _act(s):
+    # net.js:81/87/93: s > 20 -> 1, s < -20 -> -1, else tanh(s); NaN stays NaN
+    return np.where(s > 20, 1.0, np.where(s < -20, -1.0, np.tanh(s)))
+
+
+def forward_into(w, x, n_in, hidden=HIDDEN, n_out=OUT_DIM):
+    """forwardInto (src/brain/net.js:71-98) for every creature of a population:
+    w float32 [pop, nW] (per-creature genomes), x float32 [pop, n_in] ->
+    (out float32 [pop, n_out], scratch float32 [pop, 2*hidden]). Each sum
+    starts at its bias and adds x[i]*w[row+i] for i = 0, 1, ... in float64
+    (products of two float32 values are exact in float64), the activations are
+    stored as float32 as the reference's Float32Array scratch and outputs are.
+    The layer-2 bias is read at offset b2 + j, the first row of the layer-2
+    weights, as net.js:84-86 reads it (the stored layer-2 biases at b3 are
+    never used)."""
+    w = np.asarray(w, np.float32).astype(np.float64)
+    x = np.asarray(x, np.float32).astype(np.float64)
+    pop = w.shape[0]
+    b1 = n_in * hidden
+    b2 = b1 + hidden
+    b3 = b2 + hidden * hidden
+    b4 = b3 + hidden
+    b5 = b4 + hidden * n_out
+    w1 = w[:, :b1].reshape(pop, hidden, n_in)
+    s = w[:, b1:b1 + hidden].copy()
+    for i in range(n_in):
+        s = s + x[:, None, i] * w1[:, :, i]
+    h1 = _act(s).astype(np.float32).astype(np.float64)
+    w2 = w[:, b2:b2 + hidden * hidden].reshape(pop, hidden, hidden)
+    s = w[:, b2:b2 + hidden].copy()
+    for i in range(hidden):
+        s = s + h1[:, None, i] * w2[:, :, i]
+    h2 = _act(s).astype(np.float32).astype(np.float64)
+    w3 = w[:, b4:b5].reshape(pop, n_out, hidden)
+    s = w[:, b5:b5 + n_out].copy()
+    for i in range(hidden):
+        s = s + h2[:, None, i] * w3[:, :, i]
+    out = _act(s).astype(np.float32)
+    return out, np.concatenate([h1, h2], axis=1).astype(np.float32)
+
+
+def decode(out, arms):
+    """decode (src/brain/net.js:114-124) as a row per creature:
+    [tone, (shoulder, curl, adhesion) per arm, emitEcho] in float64."""
+    out = np.asarray(out, np.float32).astype(np.float64)
+    rec = np.zeros((out.shape[0], 2 + 3 * arms))
+    rec[:, 0] = 0.5 + 0.5 * out[:, 0]
+    for a in range(arms):
+        o = 1 + a * 3
+        rec[:, 1 + 3 * a] = out[:, o]
+        rec[:, 2 + 3 * a] = out[:, o + 1]
+        rec[:, 3 + 3 * a] = np.where(out[:, o + 2] > 0, out[:, o + 2], 0.0)
+    rec[:, -1] = out[:, OUT_DIM - 1] > 0.3
+    return rec
+
+
+def mulberry32_at(state, idx):
+    """The value of call idx (0-based) of a mulberry32 stream whose state is
+    `state` before the first call (src/world/map.js:11-20; the trainer's stream
+    of src/sim/train.js:65-72 is the same generator): the state advances by
+    0x6D2B79F5 per call, so any call's value follows from its index."""
+    a = (np.uint64(state) + (np.asarray(idx, np.uint64) + np.uint64(1)) * np.uint64(0x6D2B79F5)) \
+        & np.uint64(0xFFFFFFFF)
+    t = a.astype(np.uint32)
+    with np.errstate(over="ignore"):
+        t = (t ^ (t >> np.uint32(15))) * (t | np.uint32(1))
+        t = t ^ (t + (t ^ (t >> np.uint32(7))) * (t | np.uint32(61)))
+        t = t ^ (t >> np.uint32(14))
+    return t.astype(np.float64) / 4294967296.0
+
+
+def genome_generation(theta, sigma, pop, state):
+    """beginGeneration's genomeAt loop (src/sim/train.js:154-158,
+    src/brain/es.js:67-88) for i = 0 .. pop-1 from a trainer stream whose
+    state is `state`: pair p = i >> 1 draws eps_p (Irwin-Hall, four calls per
+    weight, k-major) on its first member; genome i = theta + eps_p * sigma *
+    (+1 for even i, -1 for odd), rounded to float32. Returns (genomes float32
+    [pop, nW], eps_bank float32 [pop//2, nW], state after the draws)."""
+    theta = np.asarray(theta, np.float32)
+    nw = theta.shape[0]
+    pairs = pop >> 1
+    base = np.arange(pairs * nw, dtype=np.uint64) * np.uint64(4)
+    r = [mulberry32_at(state, base + np.uint64(d)) for d in range(4)]
+    eps = ((((r[0] + r[1]) + r[2]) + r[3]) - 2) * 1.7320508
+    eps = eps.astype(np.float32).reshape(pairs, nw)
+    gen = np.zeros((pop, nw), np.float32)
+    th = theta.astype(np.float64)
+    for i in range(pop):
+        s = sigma * (-1.0 if i & 1 else 1.0)
+        gen[i] = (th + eps[i >> 1].astype(np.float64) * s).astype(np.float32)
+    state2 = (int(state) + pairs * nw * 4 * 0x6D2B79F5) & 0xFFFFFFFF
+    return gen, eps, state2
+
+
+def es_update(theta, m, v, t, eps_bank, fitness, opts):
+    """update (src/brain/es.js:92-149): mirrored-pair advantages normalised
+    over the generation, the clipped gradient accumulated in float32 pair by
+    pair, one Adam step. theta, m, v float32 [nW] (copies are returned), t the
+    step count before the update, opts the es.js option names. Returns
+    (theta, m, v, t, sigma, stats{best, mean, sd, sigma})."""
+    fitness = np.asarray(fitness, np.float64)
+    pop = fitness.shape[0]
+    half = pop >> 1
+    diffs = (fitness[0::2][:half] - fitness[1::2][:half]) / 2
+    mean = 0.0
+    for p in range(half):
+        mean += fitness[p * 2] + fitness[p * 2 + 1]
+    mean /= opts["popSize"]
+    sd = 0.0
+    for p in range(half):
+        sd += diffs[p] * diffs[p]
+    sd = float(np.sqrt(sd / max(1, half))) or 1.0
+    inv = 1 / (half * opts["sigma"] * sd)
+    g = np.zeros(theta.shape[0], np.float32)
+    eb = np.asarray(eps_bank, np.float32).astype(np.float64)
+    for p in range(half):
+        g = (g.astype(np.float64) + diffs[p] * eb[p]).astype(np.float32)
+    a = g.astype(np.float64) * inv
+    a = np.where(a > opts["clip"], opts["clip"], np.where(a < -opts["clip"], -opts["clip"], a))
+    g = a.astype(np.float32).astype(np.float64)
+    t = t + 1
+    b1, b2 = opts["adamBeta1"], opts["adamBeta2"]
+    c1, c2 = 1 - b1 ** t, 1 - b2 ** t
+    m = (b1 * np.asarray(m, np.float32).astype(np.float64) + (1 - b1) * g).astype(np.float32)
+    v = (b2 * np.asarray(v, np.float32).astype(np.float64) + (1 - b2) * g * g).astype(np.float32)
+    with np.errstate(all="ignore"):
+        step = opts["lr"] * (m.astype(np.float64) / c1) / (np.sqrt(v.astype(np.float64) / c2) + opts["adamEps"])
+    step = np.where(np.isfinite(step), step, 0.0)
+    theta = (np.asarray(theta, np.float32).astype(np.float64) + step).astype(np.float32)
+    sigma = max(opts["minSigma"], opts["sigma"] * opts["sigmaDecay"])
+    best = float(np.max(fitness))
+    return theta, m, v, t, sigma, dict(best=best, mean=mean, sd=sd * 2, sigma=sigma)

@@ -12,6 +12,9 @@ import { makeMap, makeSdf, findSpawn, mulberry32 } from '/root/reference/src/wor
 import { spawnKraken, attachGrip, tipPosition, adhesionToBreakForce, toneToCompliance,
   BODY, ARM_REACH } from '/root/reference/src/body/kraken.js';
 import { createSim, applyControl, stepSim, readProprio } from '/root/reference/src/sim/core.js';
+import { PROPRIO_DIM, SENSE_DIM } from '/root/reference/src/sim/core.js';
+import { HIDDEN, OUT_DIM, inputDim, weightCount, randomGenome, forwardInto, decode } from '/root/reference/src/brain/net.js';
+import { createES, genomeAt, update } from '/root/reference/src/brain/es.js';
 import { f32ify } from '/root/reference/tools/f32solver.mjs';
 import { packControls, mirrorInto, groupCollPoints, GPU_STRIDES } from '/root/reference/src/gpu/gpuworld.js';
 
@@ -352,5 +355,66 @@ function runTraj(mode, S, I, h, nSteps, record, seed) {
   dump('proprio.json', { S, I, h, seed, armReach: ARM_REACH, segLen: BODY.segLen,
     shoulderLimit: BODY.shoulderLimit, jointLimit: BODY.jointLimit,
     creatures: ks.map(k => ({ mantle: k.mantle, arms: k.arms.map(a => a.segs) })), frames });
+}
+// ── 11. policy network (net.js:71-124 forwardInto, decode) over a small
+// population with per-creature genomes and the recurrent output feedback of
+// gputrain.js:77-86, then the ES generation (es.js:67-149 genomeAt, update)
+// driven by the trainer's stream (train.js:65-72 is mulberry32 seeded with
+// seed*2654435761) ──
+{
+  const b64 = (a) => Buffer.from(a.buffer, a.byteOffset, a.byteLength).toString('base64');
+  const nIn = inputDim(PROPRIO_DIM, SENSE_DIM), nW = weightCount(nIn);
+  const pop = 6, steps = 3, arms = 6;
+  const rnd = mulberry32(12345);
+  const genomes = new Float32Array(pop * nW);
+  for (let i = 0; i < pop; i++) {
+    const g = randomGenome(nIn, HIDDEN, OUT_DIM, rnd);
+    const sc = i >= 4 ? 25 : 1;                      // two genomes large enough to hit the +-20 clamp
+    for (let k = 0; k < nW; k++) genomes[i * nW + k] = g[k] * sc;
+  }
+  const outputs = new Float32Array(pop * OUT_DIM);
+  const scratch = new Float32Array(pop * 2 * HIDDEN);
+  const frames = [];
+  for (let s = 0; s < steps; s++) {
+    const inputs = new Float32Array(pop * nIn);
+    for (let i = 0; i < pop; i++) {
+      const x = inputs.subarray(i * nIn, (i + 1) * nIn);
+      for (let k = 0; k < PROPRIO_DIM + SENSE_DIM; k++) x[k] = (rnd() * 2 - 1) * (k % 7 === 0 ? 4 : 1.5);
+      const oPrev = outputs.subarray(i * OUT_DIM, (i + 1) * OUT_DIM);
+      for (let k = 0; k < OUT_DIM; k++) x[PROPRIO_DIM + SENSE_DIM + k] = oPrev[k];
+      forwardInto(genomes.subarray(i * nW, (i + 1) * nW), x, oPrev,
+        scratch.subarray(i * 2 * HIDDEN, (i + 1) * 2 * HIDDEN), nIn, HIDDEN, OUT_DIM);
+    }
+    const controls = [];
+    for (let i = 0; i < pop; i++) {
+      const c = { tone: 0, emitEcho: false, arms: [] };
+      for (let a = 0; a < arms; a++) c.arms.push({ shoulder: 0, curl: 0, adhesion: 0 });
+      decode(outputs.subarray(i * OUT_DIM, (i + 1) * OUT_DIM), c, arms);
+      controls.push([c.tone].concat(...c.arms.map(a => [a.shoulder, a.curl, a.adhesion]), [c.emitEcho ? 1 : 0]));
+    }
+    frames.push({ inputs: b64(inputs), outputs: b64(outputs), scratch: b64(scratch), controls });
+  }
+  dump('net.json', { nIn, hidden: HIDDEN, nOut: OUT_DIM, nW, pop, proprioDim: PROPRIO_DIM, senseDim: SENSE_DIM,
+    genomes: b64(genomes), frames });
+
+  const popSize = 4, seed = 7;
+  const es = createES(nW, { popSize });
+  es.theta.set(randomGenome(nIn, HIDDEN, OUT_DIM, mulberry32(99)));
+  const theta0 = Float32Array.from(es.theta);
+  const s0 = (seed * 2654435761) >>> 0;
+  const trnd = mulberry32(s0);
+  const gens = [];
+  const frnd = mulberry32(31337);
+  for (let gen = 0; gen < 2; gen++) {
+    const sigma = es.opts.sigma;
+    const gs = new Float32Array(popSize * nW);
+    for (let i = 0; i < popSize; i++) genomeAt(es, i, gs.subarray(i * nW, (i + 1) * nW), trnd);
+    const fitness = new Float64Array(popSize);
+    for (let i = 0; i < popSize; i++) fitness[i] = (frnd() - 0.3) * 40;
+    const stats = update(es, fitness);
+    gens.push({ sigma, genomes: b64(gs), fitness: Array.from(fitness), theta: b64(es.theta), m: b64(es.m),
+      v: b64(es.v), t: es.t, stats });
+  }
+  dump('es.json', { nW, popSize, seed, s0, opts: es.opts, theta0: b64(theta0), gens });
 }
 console.log('done');

@@ -192,3 +192,22 @@ def test_create_sanity_guards_raise(case):
     rc = K.lib().kx_create(C.byref(d), C.byref(c), C.byref(prm), C.byref(h))
     assert rc == K.KX_EINVAL and not h.value
     assert K.lib().kx_last_error()
+
+
+def test_policy_shapes_and_argument_checks_on_the_host():
+    """weightCount (src/brain/net.js:30-32) and the argument checks that run
+    before any device call."""
+    import ctypes as C
+    L = K.lib()
+    assert L.kx_policy_weights(105, 24, 20) == 3644 == po.weight_count(105)
+    assert L.kx_policy_weights(0, 24, 20) == 0
+    h = C.c_void_p()
+    assert L.kx_policy_create(0, 105, 24, 20, C.byref(h)) == K.KX_EINVAL
+    assert L.kx_policy_create(4, 105, 65, 20, C.byref(h)) == K.KX_EINVAL
+    assert "at most 64" in L.kx_last_error().decode()
+    assert L.kx_policy_create(4, 1000, 64, 64, C.byref(h)) == K.KX_EINVAL
+    assert "LDS" in L.kx_last_error().decode()
+    eo = K.EsOpts(sigma=0.06, lr=0.03, min_sigma=0.004, sigma_decay=0.999, beta1=0.9, beta2=0.999,
+                  adam_eps=1e-8, clip=2.5, pop_size=0)
+    assert L.kx_es_create(3644, C.byref(eo), None, C.byref(h)) == K.KX_EINVAL
+    assert L.kx_es_create(0, C.byref(eo), None, C.byref(h)) == K.KX_EINVAL

@@ -387,3 +387,85 @@ def test_read_proprio_matches_reference(golden):
     flat = [v for fr in g["frames"] for o in fr["proprio"] for v in o]
     assert max(flat) == 1.2 and min(flat) == -1.2
     assert any(0.0 < v < 1.0 for fr in g["frames"] for o in fr["proprio"] for v in o[24:30])
+
+
+# ── policy network and ES (src/brain/net.js, src/brain/es.js) ──
+
+def _f32(s):
+    import base64
+    return np.frombuffer(base64.b64decode(s), np.float32)
+
+
+def test_forward_into_and_decode_match_reference(golden):
+    """Three recurrent steps of forwardInto over six genomes (two scaled so the
+    +-20 clamp saturates) and decode: outputs, h1|h2 and controls exactly as
+    the reference wrote them."""
+    d = golden("net.json")
+    n_in, nw, pop = d["nIn"], d["nW"], d["pop"]
+    assert po.weight_count(n_in) == nw and n_in == 105
+    gen = _f32(d["genomes"]).reshape(pop, nw)
+    sat = 0
+    for fr in d["frames"]:
+        x = _f32(fr["inputs"]).reshape(pop, n_in)
+        out, sc = po.forward_into(gen, x, n_in)
+        assert np.array_equal(out, _f32(fr["outputs"]).reshape(pop, -1))
+        assert np.array_equal(sc, _f32(fr["scratch"]).reshape(pop, -1))
+        assert np.array_equal(po.decode(out, 6), np.array(fr["controls"]))
+        sat += int((np.abs(out) == 1).sum())
+    assert sat > 0, "the fixture must exercise the +-20 clamp"
+
+
+def test_forward_into_layer2_bias_quirk_is_pinned(golden):
+    """The reference reads the layer-2 bias at b2 + j (net.js:84-86), the first
+    weight row, not the stored biases at b3. An oracle that 'fixed' this
+    would fail the fixture: planting that change must change the outputs."""
+    d = golden("net.json")
+    n_in, nw, pop = d["nIn"], d["nW"], d["pop"]
+    gen = _f32(d["genomes"]).reshape(pop, nw).copy()
+    fr = d["frames"][0]
+    x = _f32(fr["inputs"]).reshape(pop, n_in)
+    h = po.HIDDEN
+    b2 = n_in * h + h
+    b3 = b2 + h * h
+    planted = gen.copy()
+    planted[:, b3:b3 + h] = 0.5            # stored layer-2 biases: never read by the reference
+    out_p, _ = po.forward_into(planted, x, n_in)
+    assert np.array_equal(out_p, _f32(fr["outputs"]).reshape(pop, -1))
+    planted = gen.copy()
+    planted[:, b2:b2 + h] += 0.5           # the words the reference does read as layer-2 bias
+    out_q, _ = po.forward_into(planted, x, n_in)
+    assert not np.array_equal(out_q, _f32(fr["outputs"]).reshape(pop, -1))
+
+
+def test_mulberry32_jump_ahead_matches_the_sequential_stream():
+    """Call idx of a stream, computed from its index, equals the idx-th value of
+    the sequential generator (the oracle's C mulberry32, itself pinned by the
+    reference's streams)."""
+    for seed in (1, 7 * 2654435761 & 0xFFFFFFFF, 0xFFFFFFFF):
+        seq = po.mulberry32(seed, 5000)
+        assert np.array_equal(po.mulberry32_at(seed, np.arange(5000)), seq)
+
+
+def test_es_generation_and_update_match_reference(golden):
+    """Two generations of genomeAt over the trainer's stream, each followed by
+    update (Adam, clipping, sigma decay): genomes, theta, m, v and the returned
+    statistics exactly as es.js produced them."""
+    e = golden("es.json")
+    theta = _f32(e["theta0"]).copy()
+    m = np.zeros_like(theta)
+    v = np.zeros_like(theta)
+    t = 0
+    opts = dict(e["opts"])
+    opts["sigma"] = e["gens"][0]["sigma"]          # e["opts"] was written after both updates
+    state = e["s0"]
+    assert state == (e["seed"] * 2654435761) & 0xFFFFFFFF
+    for g in e["gens"]:
+        assert g["sigma"] == opts["sigma"]
+        gens, eps, state = po.genome_generation(theta, g["sigma"], e["popSize"], state)
+        assert np.array_equal(gens, _f32(g["genomes"]).reshape(e["popSize"], -1))
+        theta, m, v, t, sigma, st = po.es_update(theta, m, v, t, eps, g["fitness"], opts)
+        opts["sigma"] = sigma
+        assert np.array_equal(theta, _f32(g["theta"])) and np.array_equal(m, _f32(g["m"]))
+        assert np.array_equal(v, _f32(g["v"])) and t == g["t"]
+        for k in ("best", "mean", "sd", "sigma"):
+            assert st[k] == g["stats"][k], k
