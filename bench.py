@@ -107,6 +107,49 @@ def state_readout_cost(g, w, reps=20):
             "download_ms": d_ms, "download_bytes": int(nb * 24 + nc * 13)}
 
 
+def policy_cost(n, reps=50):
+    """The policy network and ES over a population of n creatures with the
+    reference's network (105 -> 24 -> 24 -> 20, one genome per creature): the
+    forward kernel's device time over resident genomes (HIP events, mean of
+    `reps` launches) against its algorithmic HBM bytes (genomes + inputs +
+    outputs), the host-inclusive forward (inputs in, outputs and decoded
+    control records out), and one ES generation (genomeAt for every member)
+    and update."""
+    from kraken_3_gpu_amd import brain as B
+    n_in = B.input_dim()
+    rng = np.random.default_rng(3)
+    nw = B.weight_count(n_in)
+    res = {}
+    with B.Population(n, n_in) as P:
+        P.set_genomes((rng.standard_normal((n, nw)) / np.sqrt(n_in)).astype(np.float32))
+        x = rng.uniform(-1, 1, (n, n_in)).astype(np.float32)
+        P.forward_into(x, arms=6)
+        k_ms = P.forward_repeat(reps)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            P.forward_into(x, arms=6)
+        host_ms = (time.perf_counter() - t0) * 1e3 / reps
+        alg = n * (nw + n_in + B.OUT_DIM) * 4
+        res["forward"] = {"kernel": "kx_k_policy_fwd", "creatures": n, "weights_per_creature": nw,
+                          "kernel_ms": k_ms, "algorithmic_bytes": alg, "achieved_gbs": alg / (k_ms * 1e6),
+                          "frac_hbm": alg / (k_ms * 1e6) / HBM_PEAK_GBS, "host_inclusive_ms": host_ms}
+        with B.ES(nw, popSize=n) as es:
+            state = B.trainer_stream_state(1)
+            es.begin_generation(P, state)
+            t0 = time.perf_counter()
+            for _ in range(5):
+                state = es.begin_generation(P, state)
+            gen_ms = (time.perf_counter() - t0) * 1e3 / 5
+            fit = rng.normal(0, 10, es.pop_size)
+            t0 = time.perf_counter()
+            for _ in range(5):
+                es.update(fit)
+            upd_ms = (time.perf_counter() - t0) * 1e3 / 5
+        res["es"] = {"pop": n, "generation_ms": gen_ms, "update_ms": upd_ms,
+                     "generation_bytes": n * nw * 4 + (n // 2) * nw * 4}
+    return res
+
+
 def cpu_baseline(seconds=12.0):
     """The oracle (C restatement of xpbd.js, f64) stepping 64-creature worlds
     under the same control law on the host cores, one world per thread."""
@@ -348,7 +391,7 @@ def bench_serve(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["uniform", "uneven", "serve", "stress"], default="uniform",
+    ap.add_argument("--workload", choices=["uniform", "uneven", "serve", "stress", "policy"], default="uniform",
                     help="uniform batch (the headline line), the uneven batch, the serving loop or the stress run")
     ap.add_argument("--requests", type=int, default=120, help="serve: requests in the Poisson stream")
     ap.add_argument("--load", type=float, default=0.7, help="serve: offered load as a fraction of capacity")
@@ -383,6 +426,9 @@ def main():
         return
     if args.workload == "serve":
         bench_serve(args)
+        return
+    if args.workload == "policy":
+        print(json.dumps({"workload": "policy network + ES", **policy_cost(args.creatures or 4096)}), flush=True)
         return
     args.steps = args.steps or 1000
     args.creatures = args.creatures or 4096
@@ -427,6 +473,7 @@ def main():
     st = G.download(g)
     shape = G.persist_info(g)
     readout = state_readout_cost(g, w)
+    policy = policy_cost(n_cr)
 
     per_cr = dict(pins=42, motors=42, bodies=43, points=92, grips_total=6)
     F = flops_per_creature_step(w.S, w.I, per_cr["pins"], per_cr["motors"], per_cr["bodies"], per_cr["points"],
@@ -496,6 +543,7 @@ def main():
         "device_ms_total": dev_ms,
         "final_stats": st,
         "state_readout": readout,
+        "policy": policy,
     }
     if plain:
         out["plain"] = plain

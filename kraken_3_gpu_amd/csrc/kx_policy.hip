@@ -66,70 +66,53 @@ __device__ __forceinline__ double kx_act(double s) {
   return s > 20.0 ? 1.0 : (s < -20.0 ? -1.0 : tanh(s));
 }
 
-// One wave per creature. The genome (nW floats), the input row, both hidden
-// activations and the outputs are staged in LDS; lane j computes row j of each
-// layer.
-__global__ __launch_bounds__(64) void kx_k_policy_fwd(PolDims d, int n, const float *__restrict__ genomes,
-                                                      const float *__restrict__ x, float *__restrict__ out,
-                                                      float *__restrict__ scratch, float *__restrict__ controls,
-                                                      int arms, int vec4) {
-  extern __shared__ float lds[];
-  const int c = blockIdx.x;
-  const int lane = threadIdx.x;
-  if (c >= n) return;
-  float *W = lds;
-  float *X = W + d.nw;
-  float *H1 = X + d.n_in;
-  float *H2 = H1 + d.hidden;
-  float *O = H2 + d.hidden;   // n_out floats: the outputs decode reads
-  const float *g = genomes + (size_t)c * d.nw;
-  if (vec4) {
-    const float4 *g4 = reinterpret_cast<const float4 *>(g);
-    float4 *W4 = reinterpret_cast<float4 *>(W);
-    const int n4 = d.nw >> 2;
-#pragma unroll 4
-    for (int k = lane; k < n4; k += 64) W4[k] = g4[k];
-  } else {
-#pragma unroll 4
-    for (int k = lane; k < d.nw; k += 64) W[k] = g[k];
-  }
-  const float *xr = x + (size_t)c * d.n_in;
-  for (int k = lane; k < d.n_in; k += 64) X[k] = xr[k];
-  __syncthreads();
+typedef float kx_v4f __attribute__((ext_vector_type(4)));   // native 16-byte vector (register-resident)
+constexpr int KX_POL_V4 = 16;   // genome float4 per lane held in registers (up to 4096 weights)
+constexpr int KX_POL_XR = 4;    // input floats per lane held in registers (up to 256 inputs)
 
-  if (lane < d.hidden) {
-    double s = (double)W[d.b1 + lane];
-    const float *row = W + lane * d.n_in;
-    for (int i = 0; i < d.n_in; i++) s = fma((double)X[i], (double)row[i], s);
-    H1[lane] = (float)kx_act(s);
-  }
-  __syncthreads();
-  if (lane < d.hidden) {
-    // layer-2 bias read at b2 + j as net.js:84-86 does (the first weight row)
-    double s = (double)W[d.b2 + lane];
-    const float *row = W + d.b2 + lane * d.hidden;
-    for (int i = 0; i < d.hidden; i++) s = fma((double)H1[i], (double)row[i], s);
-    H2[lane] = (float)kx_act(s);
-  }
-  __syncthreads();
-  float o = 0.f;
-  if (lane < d.n_out) {
-    double s = (double)W[d.b5 + lane];
-    const float *row = W + d.b4 + lane * d.hidden;
-    for (int i = 0; i < d.hidden; i++) s = fma((double)H2[i], (double)row[i], s);
-    o = (float)kx_act(s);
-    out[(size_t)c * d.n_out + lane] = o;
-  }
-  if (scratch) {
-    if (lane < d.hidden) {
-      scratch[(size_t)c * 2 * d.hidden + lane] = H1[lane];
-      scratch[(size_t)c * 2 * d.hidden + d.hidden + lane] = H2[lane];
+// s + sum_i a[i]*b[i] for i = 0 .. n-1 in that order, in float64 (a float32
+// product is exact in float64, so each fma is the reference's s += a*b). The
+// LDS reads go in batches of 16 ahead of their fmas, so one LDS latency covers
+// sixteen steps of the chain.
+__device__ __forceinline__ double kx_dot_seq(double s, const float *a, const float *b, int n) {
+  int i = 0;
+  for (; i + 16 <= n; i += 16) {
+    float av[16], bv[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      av[u] = a[i + u];
+      bv[u] = b[i + u];
     }
+#pragma unroll
+    for (int u = 0; u < 16; u++) s = fma((double)av[u], (double)bv[u], s);
+  }
+  for (; i < n; i++) s = fma((double)a[i], (double)b[i], s);
+  return s;
+}
+
+// The three layers and decode for the creature whose genome (W), inputs (X) are
+// staged in LDS; lane j computes row j of each layer.
+__device__ __forceinline__ void kx_policy_eval(const PolDims &d, int c, int lane, const float *W, const float *X,
+                                               float *H1, float *H2, float *O, float *__restrict__ out,
+                                               float *__restrict__ scratch, float *__restrict__ controls, int arms) {
+  if (lane < d.hidden) H1[lane] = (float)kx_act(kx_dot_seq((double)W[d.b1 + lane], X, W + lane * d.n_in, d.n_in));
+  __syncthreads();
+  // layer-2 bias read at b2 + j as net.js:84-86 does (the first weight row)
+  if (lane < d.hidden)
+    H2[lane] = (float)kx_act(kx_dot_seq((double)W[d.b2 + lane], H1, W + d.b2 + lane * d.hidden, d.hidden));
+  __syncthreads();
+  if (lane < d.n_out) {
+    const float o = (float)kx_act(kx_dot_seq((double)W[d.b5 + lane], H2, W + d.b4 + lane * d.hidden, d.hidden));
+    out[(size_t)c * d.n_out + lane] = o;
+    O[lane] = o;
+  }
+  if (scratch && lane < d.hidden) {
+    scratch[(size_t)c * 2 * d.hidden + lane] = H1[lane];
+    scratch[(size_t)c * 2 * d.hidden + d.hidden + lane] = H2[lane];
   }
   if (controls) {
     // decode (net.js:114-124) into a control-tape record: [0] tone, then
     // shoulder, curl, adhesion per arm; words of absent arms are zero
-    if (lane < d.n_out) O[lane] = o;
     __syncthreads();
     float *rec = controls + (size_t)c * KX_TAPE_WORDS;
     if (lane < KX_TAPE_WORDS) {
@@ -144,6 +127,80 @@ __global__ __launch_bounds__(64) void kx_k_policy_fwd(PolDims d, int n, const fl
         }
       }
       rec[lane] = v;
+    }
+  }
+}
+
+// Lane `lane`'s share of creature cc's genome (float4 k = lane + 64 r) and
+// inputs, loaded into registers.
+__device__ __forceinline__ void kx_pol_fetch(kx_v4f (&gr)[KX_POL_V4], float (&xr)[KX_POL_XR],
+                                             const float *__restrict__ genomes, const float *__restrict__ x,
+                                             const PolDims &d, int cc, int n4, int lane) {
+  const kx_v4f *g4 = reinterpret_cast<const kx_v4f *>(genomes + (size_t)cc * d.nw);
+#pragma unroll
+  for (int r = 0; r < KX_POL_V4; r++) {
+    const int k = lane + 64 * r;
+    if (k < n4) gr[r] = g4[k];
+  }
+  const float *xs = x + (size_t)cc * d.n_in;
+#pragma unroll
+  for (int r = 0; r < KX_POL_XR; r++) {
+    const int k = lane + 64 * r;
+    if (k < d.n_in) xr[r] = xs[k];
+  }
+}
+
+// One wave per creature at a time, looping over creatures c = blockIdx.x,
+// + gridDim.x, ... The genome (nW floats), the input row, both hidden
+// activations and the outputs are staged in LDS. With `pipe` (nW a multiple
+// of 4, at most 64 * KX_POL_V4 float4, n_in <= 64 * KX_POL_XR), each lane
+// holds its share of the next creature's genome and inputs in registers: the
+// loads are issued before the current creature is evaluated, so HBM reads
+// overlap the arithmetic; otherwise the genome is copied through LDS directly.
+__global__ __launch_bounds__(64) void kx_k_policy_fwd(PolDims d, int n, const float *__restrict__ genomes,
+                                                      const float *__restrict__ x, float *__restrict__ out,
+                                                      float *__restrict__ scratch, float *__restrict__ controls,
+                                                      int arms, int pipe) {
+  extern __shared__ float lds[];
+  const int lane = threadIdx.x;
+  float *W = lds;
+  float *X = W + d.nw;
+  float *H1 = X + d.n_in;
+  float *H2 = H1 + d.hidden;
+  float *O = H2 + d.hidden;   // n_out floats: the outputs decode reads
+  int c = blockIdx.x;
+  if (c >= n) return;
+  if (pipe) {
+    const int n4 = d.nw >> 2;
+    kx_v4f gr[KX_POL_V4];
+    float xr[KX_POL_XR];
+    kx_pol_fetch(gr, xr, genomes, x, d, c, n4, lane);
+    kx_v4f *W4 = reinterpret_cast<kx_v4f *>(W);
+    for (; c < n; c += gridDim.x) {
+#pragma unroll
+      for (int r = 0; r < KX_POL_V4; r++) {
+        const int k = lane + 64 * r;
+        if (k < n4) W4[k] = gr[r];
+      }
+#pragma unroll
+      for (int r = 0; r < KX_POL_XR; r++) {
+        const int k = lane + 64 * r;
+        if (k < d.n_in) X[k] = xr[r];
+      }
+      __syncthreads();
+      if (c + (int)gridDim.x < n) kx_pol_fetch(gr, xr, genomes, x, d, c + gridDim.x, n4, lane);
+      kx_policy_eval(d, c, lane, W, X, H1, H2, O, out, scratch, controls, arms);
+      __syncthreads();   // LDS is rewritten for the next creature
+    }
+  } else {
+    for (; c < n; c += gridDim.x) {
+      const float *g = genomes + (size_t)c * d.nw;
+      for (int k = lane; k < d.nw; k += 64) W[k] = g[k];
+      const float *xs = x + (size_t)c * d.n_in;
+      for (int k = lane; k < d.n_in; k += 64) X[k] = xs[k];
+      __syncthreads();
+      kx_policy_eval(d, c, lane, W, X, H1, H2, O, out, scratch, controls, arms);
+      __syncthreads();
     }
   }
 }
@@ -213,6 +270,7 @@ struct kx_policy {
   hipStream_t st = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_ms = 0.f;
+  int grid = 0;   // workgroups of the forward launch (set at its first launch)
 };
 
 struct kx_es {
@@ -259,6 +317,7 @@ extern "C" int kx_policy_create(int32_t n, int32_t n_in, int32_t hidden, int32_t
     return cleanup(pfail(KX_EDEVICE, "policy stream"));
   if (hipMemsetAsync(p->genomes, 0, (size_t)n * d.nw * sizeof(float), p->st) != hipSuccess ||
       hipMemsetAsync(p->out, 0, (size_t)n * n_out * sizeof(float), p->st) != hipSuccess ||
+      hipMemsetAsync(p->x, 0, (size_t)n * n_in * sizeof(float), p->st) != hipSuccess ||
       hipStreamSynchronize(p->st) != hipSuccess)
     return cleanup(pfail(KX_EDEVICE, "policy clear"));
   *out = p;
@@ -294,13 +353,24 @@ extern "C" int kx_policy_get_genomes(const kx_policy *p, float *genomes) {
   return 0;
 }
 
-static int policy_launch(kx_policy *p, int arms, bool want_controls) {
+static int policy_launch(kx_policy *p, int arms, bool want_controls, bool want_scratch) {
   const PolDims &d = p->d;
   const size_t lds = (size_t)(d.nw + d.n_in + 2 * d.hidden + d.n_out) * sizeof(float);
-  const int vec4 = (d.nw & 3) == 0;   // hipMalloc bases are 256-byte aligned
+  // register pipeline when the genome is whole float4s (hipMalloc bases are
+  // 256-byte aligned, so every genome then starts 16-byte aligned) and fits
+  const int pipe = (d.nw & 3) == 0 && (d.nw >> 2) <= 64 * KX_POL_V4 && d.n_in <= 64 * KX_POL_XR;
+  if (p->grid == 0) {
+    int dev = 0, cus = 0, per_cu = 0;
+    PCHK(hipGetDevice(&dev));
+    PCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    PCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kx_k_policy_fwd, 64, lds));
+    const long slots = (long)std::max(1, cus) * std::max(1, per_cu);
+    const long rounds = (p->n + slots - 1) / slots;   // creatures per wave
+    p->grid = (int)((p->n + rounds - 1) / rounds);
+  }
   PCHK(hipEventRecord(p->ev0, p->st));
-  hipLaunchKernelGGL(kx_k_policy_fwd, dim3(p->n), dim3(64), lds, p->st, d, p->n, p->genomes, p->x, p->out,
-                     p->scratch, want_controls ? p->controls : nullptr, arms, vec4);
+  hipLaunchKernelGGL(kx_k_policy_fwd, dim3(p->grid), dim3(64), lds, p->st, d, p->n, p->genomes, p->x, p->out,
+                     want_scratch ? p->scratch : nullptr, want_controls ? p->controls : nullptr, arms, pipe);
   PCHK(hipGetLastError());
   PCHK(hipEventRecord(p->ev1, p->st));
   return 0;
@@ -313,7 +383,7 @@ extern "C" int kx_policy_forward(kx_policy *p, const float *x, float *out, float
     return pfail(KX_EINVAL, "decode of %d arms from %d outputs (at most %d arms)", arms, p->d.n_out, KX_MAX_ARMS);
   const PolDims &d = p->d;
   PCHK(hipMemcpyAsync(p->x, x, (size_t)p->n * d.n_in * sizeof(float), hipMemcpyHostToDevice, p->st));
-  int rc = policy_launch(p, arms, controls != nullptr);
+  int rc = policy_launch(p, arms, controls != nullptr, scratch != nullptr);
   if (rc) return rc;
   PCHK(hipMemcpyAsync(out, p->out, (size_t)p->n * d.n_out * sizeof(float), hipMemcpyDeviceToHost, p->st));
   if (scratch)
@@ -331,7 +401,7 @@ extern "C" int kx_policy_forward_repeat(kx_policy *p, int32_t reps, float *ms) {
   if (!p || reps <= 0) return pfail(KX_EINVAL, "policy repeat of %d", reps);
   float tot = 0.f;
   for (int r = 0; r < reps; r++) {
-    int rc = policy_launch(p, 0, false);
+    int rc = policy_launch(p, 0, false, false);
     if (rc) return rc;
     PCHK(hipStreamSynchronize(p->st));
     float t = 0.f;
