@@ -70,24 +70,25 @@ def test_forward_matches_reference_vectors(golden):
             assert np.abs(ctl[:, :19] - ref[:, :19]).max() <= 1.2e-7
 
 
-@pytest.mark.parametrize("n,scale", [(4096, 1.0), (1000, 12.0)])
-def test_forward_matches_oracle_population(n, scale):
+@pytest.mark.parametrize("n,scale,hidden", [(4096, 1.0, 24), (1001, 12.0, 24), (777, 3.0, 40)])
+def test_forward_matches_oracle_population(n, scale, hidden):
     """A whole population against the oracle (random genomes of randomGenome's
     scale, some large enough to saturate), with the recurrent tail fed back for
-    three steps; decode's records against the oracle's decode."""
+    three steps; decode's records against the oracle's decode, at the
+    reference's shape, an odd population, and a wider hidden layer."""
     n_in = B.input_dim()
     rng = np.random.default_rng(n)
-    nw = B.weight_count(n_in)
+    nw = B.weight_count(n_in, hidden)
     gen = (rng.standard_normal((n, nw)) * scale / np.sqrt(n_in)).astype(np.float32)
-    print(f"[case] policy forward vs oracle: {n} creatures, scale {scale}", flush=True)
-    with B.Population(n, n_in) as P:
+    print(f"[case] policy forward vs oracle: {n} creatures, scale {scale}, hidden {hidden}", flush=True)
+    with B.Population(n, n_in, hidden=hidden) as P:
         P.set_genomes(gen)
         prev = np.zeros((n, B.OUT_DIM), np.float32)
         for step in range(3):
             x = (rng.uniform(-1.5, 1.5, (n, n_in))).astype(np.float32)
             x[:, n_in - B.OUT_DIM:] = prev
             out, ctl, sc = P.forward_into(x, arms=6, want_scratch=True)
-            ref, ref_sc = po.forward_into(gen, x, n_in)
+            ref, ref_sc = po.forward_into(gen, x, n_in, hidden=hidden)
             _within_one_ulp(out, ref, f"step {step} outputs")
             _within_one_ulp(sc, ref_sc, f"step {step} h1|h2")
             assert np.array_equal(ctl, _tape_of(po.decode(out, 6), 6))
