@@ -239,16 +239,31 @@ struct KxEsStep {
   double inv, clip, b1, b2, omb1, omb2, c1, c2, lr, eps;
 };
 
+constexpr int KX_ES_BATCH = 64;   // eps loads in flight per thread ahead of the additions
+
 // update (es.js:107-135) per weight: the gradient accumulated in float32 over
-// the pairs in order, scaled and clipped, then one Adam step.
-__global__ __launch_bounds__(256) void kx_k_es_update(int nw, int half, KxEsStep q, const double *__restrict__ diffs,
-                                                      const float *__restrict__ eps_bank, float *__restrict__ theta,
-                                                      float *__restrict__ m, float *__restrict__ v) {
+// the pairs in order, scaled and clipped, then one Adam step. The additions
+// are a chain per weight (each rounds to float32), so the loads run ahead of
+// them in batches; the pair advantages sit in LDS.
+__global__ __launch_bounds__(64) void kx_k_es_update(int nw, int half, KxEsStep q, const double *__restrict__ diffs,
+                                                     const float *__restrict__ eps_bank, float *__restrict__ theta,
+                                                     float *__restrict__ m, float *__restrict__ v) {
 #pragma clang fp contract(off)
+  extern __shared__ double dsh[];   // half advantages
+  for (int p = threadIdx.x; p < half; p += blockDim.x) dsh[p] = diffs[p];
+  __syncthreads();
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nw) return;
   float g = 0.f;
-  for (int p = 0; p < half; p++) g = (float)((double)g + diffs[p] * (double)eps_bank[(size_t)p * nw + k]);
+  int p = 0;
+  for (; p + KX_ES_BATCH <= half; p += KX_ES_BATCH) {
+    float e[KX_ES_BATCH];
+#pragma unroll
+    for (int u = 0; u < KX_ES_BATCH; u++) e[u] = eps_bank[(size_t)(p + u) * nw + k];
+#pragma unroll
+    for (int u = 0; u < KX_ES_BATCH; u++) g = (float)((double)g + dsh[p + u] * (double)e[u]);
+  }
+  for (; p < half; p++) g = (float)((double)g + dsh[p] * (double)eps_bank[(size_t)p * nw + k]);
   double a = (double)g * q.inv;
   if (a > q.clip) a = q.clip;
   else if (a < -q.clip) a = -q.clip;
@@ -513,8 +528,8 @@ extern "C" int kx_es_update(kx_es *e, const double *fitness, double *stats4) {
   q.lr = o.lr;
   q.eps = o.adam_eps;
   PCHK(hipMemcpyAsync(e->diffs, diffs, (size_t)half * sizeof(double), hipMemcpyHostToDevice, e->st));
-  hipLaunchKernelGGL(kx_k_es_update, dim3((e->nw + 255) / 256), dim3(256), 0, e->st, e->nw, half, q, e->diffs, e->eps,
-                     e->theta, e->m, e->v);
+  hipLaunchKernelGGL(kx_k_es_update, dim3((e->nw + 63) / 64), dim3(64), (size_t)half * sizeof(double), e->st, e->nw,
+                     half, q, e->diffs, e->eps, e->theta, e->m, e->v);
   PCHK(hipGetLastError());
   e->sigma = std::max(o.min_sigma, e->sigma * o.sigma_decay);
   double best = -INFINITY;
