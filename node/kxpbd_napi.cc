@@ -9,6 +9,7 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <unordered_set>
 #include <vector>
 
 #include "../include/kxpbd.h"
@@ -482,6 +483,285 @@ napi_value ReadState(napi_env env, napi_callback_info info) {
   return r;
 }
 
+// ── policy network and ES (kx_policy_*, kx_es_*) ──────────────────────
+// Their externals hold a BrainSlot, registered in g_brain so a foreign
+// external is refused before it is read; the leading null word makes
+// handle_of refuse a BrainSlot passed where a world handle is expected.
+enum { KIND_POLICY = 1, KIND_ES = 2 };
+struct BrainSlot {
+  void *world_guard = nullptr;
+  int kind = 0;
+  void *obj = nullptr;
+};
+std::unordered_set<BrainSlot *> g_brain;
+
+void finalize_brain(napi_env, void *data, void *) {
+  BrainSlot *b = static_cast<BrainSlot *>(data);
+  if (b->obj) {
+    if (b->kind == KIND_POLICY) kx_policy_destroy(static_cast<kx_policy *>(b->obj));
+    else kx_es_destroy(static_cast<kx_es *>(b->obj));
+  }
+  g_brain.erase(b);
+  delete b;
+}
+
+void *brain_of(napi_env env, napi_value v, int kind) {
+  void *p = nullptr;
+  BrainSlot *b = nullptr;
+  if (napi_get_value_external(env, v, &p) == napi_ok && p && g_brain.count(static_cast<BrainSlot *>(p)))
+    b = static_cast<BrainSlot *>(p);
+  if (!b || b->kind != kind || !b->obj) {
+    napi_throw_error(env, nullptr, kind == KIND_POLICY ? "kxpbd: not a live policy" : "kxpbd: not a live ES");
+    return nullptr;
+  }
+  return b->obj;
+}
+
+napi_value wrap_brain(napi_env env, int kind, void *obj) {
+  BrainSlot *b = new BrainSlot();
+  b->kind = kind;
+  b->obj = obj;
+  napi_value ext;
+  if (napi_create_external(env, b, finalize_brain, nullptr, &ext) != napi_ok) {
+    if (kind == KIND_POLICY) kx_policy_destroy(static_cast<kx_policy *>(obj));
+    else kx_es_destroy(static_cast<kx_es *>(obj));
+    delete b;
+    napi_throw_error(env, nullptr, "kxpbd: napi_create_external failed");
+    return nullptr;
+  }
+  g_brain.insert(b);
+  return ext;
+}
+
+bool get_i32(napi_env env, napi_value v, int32_t *out) {
+  if (napi_get_value_int32(env, v, out) != napi_ok) {
+    napi_throw_type_error(env, nullptr, "kxpbd: expected an integer");
+    return false;
+  }
+  return true;
+}
+
+bool get_f64_prop(napi_env env, napi_value obj, const char *name, double *out) {
+  napi_value v;
+  if (!get_prop(env, obj, name, &v)) return true;   // keep the default
+  if (napi_get_value_double(env, v, out) != napi_ok) {
+    napi_throw_type_error(env, nullptr, (std::string("kxpbd: not a number: ") + name).c_str());
+    return false;
+  }
+  return true;
+}
+
+// policyCreate(n, nIn, hidden, nOut) -> policy (src/brain/net.js layout)
+napi_value PolicyCreate(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value a[4];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
+  if (argc < 4) {
+    napi_throw_type_error(env, nullptr, "kxpbd.policyCreate(n, nIn, hidden, nOut)");
+    return nullptr;
+  }
+  int32_t v[4];
+  for (int i = 0; i < 4; i++)
+    if (!get_i32(env, a[i], &v[i])) return nullptr;
+  kx_policy *p = nullptr;
+  int rc = kx_policy_create(v[0], v[1], v[2], v[3], &p);
+  if (rc < 0) return throw_kx(env, rc);
+  return wrap_brain(env, KIND_POLICY, p);
+}
+
+// policyWeights(nIn, hidden, nOut) -> weightCount (net.js:30-32)
+napi_value PolicyWeights(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value a[3];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
+  int32_t v[3] = {0, 24, 20};
+  for (size_t i = 0; i < argc && i < 3; i++)
+    if (!get_i32(env, a[i], &v[i])) return nullptr;
+  napi_value r;
+  NAPI_OK(napi_create_int32(env, kx_policy_weights(v[0], v[1], v[2]), &r));
+  return r;
+}
+
+// policyGenomes(policy, {genomes:Float32Array}, upload:boolean)
+napi_value PolicyGenomes(napi_env env, napi_callback_info info) {
+  size_t argc = 5;
+  napi_value a[5];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
+  if (argc < 5) {
+    napi_throw_type_error(env, nullptr, "kxpbd.policyGenomes(policy, {genomes}, upload, n, nWeights)");
+    return nullptr;
+  }
+  kx_policy *p = static_cast<kx_policy *>(brain_of(env, a[0], KIND_POLICY));
+  if (!p) return nullptr;
+  bool up = false;
+  NAPI_OK(napi_get_value_bool(env, a[2], &up));
+  int32_t n = 0, nw = 0;
+  if (!get_i32(env, a[3], &n) || !get_i32(env, a[4], &nw)) return nullptr;
+  float *g;
+  if (!typed(env, a[1], "genomes", napi_float32_array, (size_t)n * (size_t)nw, &g, true)) return nullptr;
+  int rc = up ? kx_policy_set_genomes(p, g) : kx_policy_get_genomes(p, g);
+  if (rc < 0) return throw_kx(env, rc);
+  return nullptr;
+}
+
+// policyForward(policy, {x, out, scratch?, controls?}, arms, n, nIn, nOut,
+// hidden) -> kernel milliseconds; lengths are checked against n and the shape
+napi_value PolicyForward(napi_env env, napi_callback_info info) {
+  size_t argc = 7;
+  napi_value a[7];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
+  if (argc < 7) {
+    napi_throw_type_error(env, nullptr, "kxpbd.policyForward(policy, {x, out, scratch, controls}, arms, n, nIn, nOut, hidden)");
+    return nullptr;
+  }
+  kx_policy *p = static_cast<kx_policy *>(brain_of(env, a[0], KIND_POLICY));
+  if (!p) return nullptr;
+  int32_t arms, n, n_in, n_out, hidden;
+  if (!get_i32(env, a[2], &arms) || !get_i32(env, a[3], &n) || !get_i32(env, a[4], &n_in) ||
+      !get_i32(env, a[5], &n_out) || !get_i32(env, a[6], &hidden))
+    return nullptr;
+  float *x, *out, *scratch, *controls;
+  if (!typed(env, a[1], "x", napi_float32_array, (size_t)n * n_in, &x, true) ||
+      !typed(env, a[1], "out", napi_float32_array, (size_t)n * n_out, &out, true) ||
+      !typed(env, a[1], "scratch", napi_float32_array, (size_t)n * 2 * hidden, &scratch, false) ||
+      !typed(env, a[1], "controls", napi_float32_array, (size_t)n * KX_TAPE_WORDS, &controls, false))
+    return nullptr;
+  int rc = kx_policy_forward(p, x, out, scratch, controls, arms);
+  if (rc < 0) return throw_kx(env, rc);
+  float ms = 0.f;
+  kx_policy_last_ms(p, &ms);
+  napi_value r;
+  NAPI_OK(napi_create_double(env, ms, &r));
+  return r;
+}
+
+// esCreate(nWeights, opts, {theta?}) -> es (src/brain/es.js createES option names)
+napi_value EsCreate(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value a[3];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
+  if (argc < 2) {
+    napi_throw_type_error(env, nullptr, "kxpbd.esCreate(nWeights, opts, {theta})");
+    return nullptr;
+  }
+  int32_t nw;
+  if (!get_i32(env, a[0], &nw)) return nullptr;
+  kx_es_opts o{0.06, 0.030, 0.004, 0.999, 0.9, 0.999, 1e-8, 2.5, 64};   // es.js:23-38
+  double pop = o.pop_size;
+  if (!get_f64_prop(env, a[1], "sigma", &o.sigma) || !get_f64_prop(env, a[1], "lr", &o.lr) ||
+      !get_f64_prop(env, a[1], "minSigma", &o.min_sigma) || !get_f64_prop(env, a[1], "sigmaDecay", &o.sigma_decay) ||
+      !get_f64_prop(env, a[1], "adamBeta1", &o.beta1) || !get_f64_prop(env, a[1], "adamBeta2", &o.beta2) ||
+      !get_f64_prop(env, a[1], "adamEps", &o.adam_eps) || !get_f64_prop(env, a[1], "clip", &o.clip) ||
+      !get_f64_prop(env, a[1], "popSize", &pop))
+    return nullptr;
+  o.pop_size = (int32_t)pop;
+  float *theta = nullptr;
+  if (argc >= 3 && !typed(env, a[2], "theta", napi_float32_array, (size_t)nw, &theta, false)) return nullptr;
+  kx_es *e = nullptr;
+  int rc = kx_es_create(nw, &o, theta, &e);
+  if (rc < 0) return throw_kx(env, rc);
+  return wrap_brain(env, KIND_ES, e);
+}
+
+// esGeneration(es, policy, rndState) -> rndState after the draws
+napi_value EsGeneration(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value a[3];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
+  if (argc < 3) {
+    napi_throw_type_error(env, nullptr, "kxpbd.esGeneration(es, policy, rndState)");
+    return nullptr;
+  }
+  kx_es *e = static_cast<kx_es *>(brain_of(env, a[0], KIND_ES));
+  if (!e) return nullptr;
+  kx_policy *p = static_cast<kx_policy *>(brain_of(env, a[1], KIND_POLICY));
+  if (!p) return nullptr;
+  uint32_t st = 0;
+  NAPI_OK(napi_get_value_uint32(env, a[2], &st));
+  int rc = kx_es_generation(e, p, &st);
+  if (rc < 0) return throw_kx(env, rc);
+  napi_value r;
+  NAPI_OK(napi_create_uint32(env, st, &r));
+  return r;
+}
+
+napi_value stats_object(napi_env env, const char *const *names, const double *v, int n) {
+  napi_value o;
+  NAPI_OK(napi_create_object(env, &o));
+  for (int i = 0; i < n; i++) {
+    napi_value x;
+    NAPI_OK(napi_create_double(env, v[i], &x));
+    NAPI_OK(napi_set_named_property(env, o, names[i], x));
+  }
+  return o;
+}
+
+// esUpdate(es, {fitness:Float64Array}, popSize) -> {best, mean, sd, sigma}
+napi_value EsUpdate(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value a[3];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
+  if (argc < 3) {
+    napi_throw_type_error(env, nullptr, "kxpbd.esUpdate(es, {fitness}, popSize)");
+    return nullptr;
+  }
+  kx_es *e = static_cast<kx_es *>(brain_of(env, a[0], KIND_ES));
+  if (!e) return nullptr;
+  int32_t pop;
+  if (!get_i32(env, a[2], &pop)) return nullptr;
+  double *fit;
+  if (!typed(env, a[1], "fitness", napi_float64_array, (size_t)pop, &fit, true)) return nullptr;
+  double st[4];
+  int rc = kx_es_update(e, fit, st);
+  if (rc < 0) return throw_kx(env, rc);
+  static const char *const names[4] = {"best", "mean", "sd", "sigma"};
+  return stats_object(env, names, st, 4);
+}
+
+// esGet(es, {theta?, m?, v?, bestTheta?}, nWeights) -> {sigma, t, bestEver}
+napi_value EsGet(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value a[3];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
+  if (argc < 3) {
+    napi_throw_type_error(env, nullptr, "kxpbd.esGet(es, {theta, m, v, bestTheta}, nWeights)");
+    return nullptr;
+  }
+  kx_es *e = static_cast<kx_es *>(brain_of(env, a[0], KIND_ES));
+  if (!e) return nullptr;
+  int32_t nw;
+  if (!get_i32(env, a[2], &nw)) return nullptr;
+  float *th, *m, *v, *best;
+  if (!typed(env, a[1], "theta", napi_float32_array, (size_t)nw, &th, false) ||
+      !typed(env, a[1], "m", napi_float32_array, (size_t)nw, &m, false) ||
+      !typed(env, a[1], "v", napi_float32_array, (size_t)nw, &v, false) ||
+      !typed(env, a[1], "bestTheta", napi_float32_array, (size_t)nw, &best, false))
+    return nullptr;
+  double sc[3];
+  int rc = kx_es_get(e, th, m, v, best, sc);
+  if (rc < 0) return throw_kx(env, rc);
+  static const char *const names[3] = {"sigma", "t", "bestEver"};
+  return stats_object(env, names, sc, 3);
+}
+
+// brainDestroy(policy or es): frees now; the finalizer then has nothing left
+napi_value BrainDestroy(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value a[1];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
+  void *p = nullptr;
+  if (argc >= 1 && napi_get_value_external(env, a[0], &p) == napi_ok && p &&
+      g_brain.count(static_cast<BrainSlot *>(p))) {
+    BrainSlot *b = static_cast<BrainSlot *>(p);
+    if (b->obj) {
+      if (b->kind == KIND_POLICY) kx_policy_destroy(static_cast<kx_policy *>(b->obj));
+      else kx_es_destroy(static_cast<kx_es *>(b->obj));
+    }
+    b->obj = nullptr;
+  }
+  return nullptr;
+}
+
 // rollout(handle, law, step0, nsteps, events?:Uint32Array) -> Promise<void>
 napi_value Rollout(napi_env env, napi_callback_info info) {
   size_t argc = 5;
@@ -600,6 +880,15 @@ napi_value Init(napi_env env, napi_value exports) {
       {"download", nullptr, Download, nullptr, nullptr, nullptr, kProp, nullptr},
       {"rollout", nullptr, Rollout, nullptr, nullptr, nullptr, kProp, nullptr},
       {"readState", nullptr, ReadState, nullptr, nullptr, nullptr, kProp, nullptr},
+      {"policyCreate", nullptr, PolicyCreate, nullptr, nullptr, nullptr, kProp, nullptr},
+      {"policyWeights", nullptr, PolicyWeights, nullptr, nullptr, nullptr, kProp, nullptr},
+      {"policyGenomes", nullptr, PolicyGenomes, nullptr, nullptr, nullptr, kProp, nullptr},
+      {"policyForward", nullptr, PolicyForward, nullptr, nullptr, nullptr, kProp, nullptr},
+      {"esCreate", nullptr, EsCreate, nullptr, nullptr, nullptr, kProp, nullptr},
+      {"esGeneration", nullptr, EsGeneration, nullptr, nullptr, nullptr, kProp, nullptr},
+      {"esUpdate", nullptr, EsUpdate, nullptr, nullptr, nullptr, kProp, nullptr},
+      {"esGet", nullptr, EsGet, nullptr, nullptr, nullptr, kProp, nullptr},
+      {"brainDestroy", nullptr, BrainDestroy, nullptr, nullptr, nullptr, kProp, nullptr},
       {"groupPoints", nullptr, GroupPoints, nullptr, nullptr, nullptr, kProp, nullptr},
       {"lastError", nullptr, LastError, nullptr, nullptr, nullptr, kProp, nullptr},
       {"abiVersion", nullptr, AbiVersion, nullptr, nullptr, nullptr, kProp, nullptr},

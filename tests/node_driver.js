@@ -132,10 +132,47 @@ async function gpu(job) {
   return out;
 }
 
+// policy network and ES through node/brain.js: forward steps with the
+// recurrent tail fed back, then ES generations with updates
+function brainJob(job) {
+  const B = require(path.join(__dirname, '..', 'node', 'brain.js'));
+  const out = { nW: B.weightCount(job.nIn, job.hidden, B.OUT_DIM) };
+  const pop = B.createPopulation(job.n, job.nIn, job.hidden, B.OUT_DIM);
+  B.setGenomes(pop, Float32Array.from(job.genomes));
+  const o = new Float32Array(job.n * B.OUT_DIM);
+  const ctl = new Float32Array(job.n * 25);
+  out.steps = [];
+  for (const xs of job.inputs) {
+    const x = Float32Array.from(xs);
+    for (let i = 0; i < job.n; i++)
+      for (let k = 0; k < B.OUT_DIM; k++) x[i * job.nIn + job.nIn - B.OUT_DIM + k] = o[i * B.OUT_DIM + k];
+    B.forwardInto(pop, x, o, { controls: ctl, arms: 6 });
+    out.steps.push({ out: Array.from(o), controls: Array.from(ctl) });
+  }
+  B.destroy(pop);
+  const es = B.createES(out.nW, job.esOpts, Float32Array.from(job.theta));
+  const epop = B.createPopulation(es.popSize, job.nIn, job.hidden, B.OUT_DIM);
+  let st = B.trainerStreamState(job.seed);
+  out.gens = [];
+  for (const fit of job.fitness) {
+    st = B.beginGeneration(es, epop, st);
+    const genomes = Array.from(B.getGenomes(epop));
+    const stats = B.update(es, fit);
+    const s = B.esState(es);
+    out.gens.push({ state: st, genomes, stats, theta: Array.from(s.theta), m: Array.from(s.m), v: Array.from(s.v),
+      t: s.t, sigma: s.sigma });
+  }
+  // misuse throws like the rest of the surface
+  try { B.beginGeneration(es, pop, 1); out.deadThrows = false; } catch (e) { out.deadThrows = String(e.message); }
+  B.destroy(es);
+  B.destroy(epop);
+  return out;
+}
+
 (async () => {
   const [mode, file] = process.argv.slice(2);
   const job = JSON.parse(fs.readFileSync(file, 'utf8'), revive);
-  const res = mode === 'pure' ? await pure(job) : await gpu(job);
+  const res = mode === 'pure' ? await pure(job) : mode === 'brain' ? brainJob(job) : await gpu(job);
   process.stdout.write(JSON.stringify(res) + '\n');
 })().catch((e) => {
   process.stderr.write(String(e && e.stack ? e.stack : e) + '\n');

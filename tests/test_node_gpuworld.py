@@ -53,7 +53,8 @@ def test_addon_loads_and_exports_the_surface():
     d = json.loads(r.stdout)
     assert d["abi"] == 3 and d["avail"] is True
     for name in ("create", "destroy", "upload", "packControls", "step", "download", "rollout", "groupPoints",
-                 "readState"):
+                 "readState", "policyCreate", "policyWeights", "policyGenomes", "policyForward", "esCreate",
+                 "esGeneration", "esUpdate", "esGet", "brainDestroy"):
         assert name in d["k"]
     for name in ("create", "upload", "uploadControls", "packControls", "step", "download", "mirrorInto",
                  "groupCollPoints", "destroy", "GPU_STRIDES", "isAvailable", "readState"):
@@ -156,3 +157,64 @@ def test_node_loop_and_rollout_match_c_abi(tmp_path):
     assert np.array_equal(np.array(ro["state"], dtype=np.float32).reshape(rs.shape), rs)
     print(f"    node rollout identical over {rsteps} steps, {int((ev != 0).sum())} event words", flush=True)
     assert r["badThrows"] is True and "kxpbd error" in r["badMessage"]
+
+
+@needs_node
+def test_brain_module_exports_the_net_and_es_surface():
+    code = ("const b=require(process.argv[1]);"
+            "console.log(JSON.stringify({k:Object.keys(b),w:b.weightCount(105),s:b.trainerStreamState(7)}))")
+    r = subprocess.run([NODE, "-e", code, os.path.join(ROOT, "node", "brain.js")],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    d = json.loads(r.stdout)
+    for name in ("inputDim", "weightCount", "createPopulation", "setGenomes", "getGenomes", "forwardInto",
+                 "createES", "beginGeneration", "update", "esState", "trainerStreamState", "destroy"):
+        assert name in d["k"], name
+    assert d["w"] == 3644 and d["s"] == (7 * 2654435761) & 0xFFFFFFFF
+
+
+@pytest.mark.gpu
+@needs_node
+def test_node_brain_matches_c_abi(tmp_path):
+    """forwardInto over a population and two ES generations through node/brain.js
+    give exactly what the same calls give through the C ABI from Python."""
+    from kraken_3_gpu_amd import brain as B
+    assert os.path.exists(ADDON), "node/kxpbd.node must be built"
+    n, n_in, hidden = 6, B.input_dim(), B.HIDDEN
+    nw = B.weight_count(n_in)
+    rng = np.random.default_rng(42)
+    gen = (rng.standard_normal((n, nw)) * 3 / np.sqrt(n_in)).astype(np.float32)
+    xs = [rng.uniform(-1.5, 1.5, (n, n_in)).astype(np.float32) for _ in range(3)]
+    theta = (rng.standard_normal(nw) * 0.1).astype(np.float32)
+    fits = [rng.normal(0, 10, 4).tolist() for _ in range(2)]
+    es_opts = dict(popSize=4, sigma=0.05, lr=0.02)
+    job = dict(n=n, nIn=n_in, hidden=hidden, genomes=gen.reshape(-1).tolist(),
+               inputs=[x.reshape(-1).tolist() for x in xs], theta=theta.tolist(), esOpts=es_opts, seed=9,
+               fitness=fits)
+    r = run_node("brain", job, tmp_path, timeout=120)
+    assert r["nW"] == nw
+    with B.Population(n, n_in) as P:
+        P.set_genomes(gen)
+        prev = np.zeros((n, B.OUT_DIM), np.float32)
+        for x, st in zip(xs, r["steps"]):
+            x = x.copy()
+            x[:, n_in - B.OUT_DIM:] = prev
+            out, ctl = P.forward_into(x, arms=6)
+            assert np.array_equal(np.array(st["out"], np.float32).reshape(out.shape), out)
+            assert np.array_equal(np.array(st["controls"], np.float32).reshape(ctl.shape), ctl)
+            prev = out
+    with B.ES(nw, theta=theta, **es_opts) as es, B.Population(4, n_in) as P:
+        state = B.trainer_stream_state(9)
+        for f, g in zip(fits, r["gens"]):
+            state = es.begin_generation(P, state)
+            assert g["state"] == state
+            assert np.array_equal(np.array(g["genomes"], np.float32).reshape(4, nw), P.genomes())
+            stats = es.update(np.array(f))
+            s = es.state()
+            for k in ("theta", "m", "v"):
+                assert np.array_equal(np.array(g[k], np.float32), s[k]), k
+            assert g["t"] == s["t"] and g["sigma"] == s["sigma"]
+            for k in ("best", "mean", "sd", "sigma"):
+                assert g["stats"][k] == stats[k], k
+    assert "not a live policy" in r["deadThrows"]
+    print("    node brain identical: 3 forward steps, 2 ES generations", flush=True)
