@@ -377,6 +377,16 @@ int  kx_policy_get_genomes(const kx_policy *p, float *genomes);
  * (may be NULL): decode (net.js:114-124) of `arms` arms (<= 8) into control-
  * tape records ready for kx_upload_tape; emitEcho stays in out[n_out-1]. */
 int  kx_policy_forward(kx_policy *p, const float *x, float *out, float *scratch, float *controls, int32_t arms);
+/* One step of gputrain.js:77-100 on the device, with the policy's creatures
+ * being the world's (same count, one arm count for all): readProprio of every
+ * creature (as kx_read_state), the inputs [proprio | senses | previous
+ * outputs] (senses[n][sense_dim] from the host, NULL for zeros;
+ * 6*arms + 6 + sense_dim + n_out must equal n_in), the forward pass, decode
+ * into control-tape row 0, then a one-step rollout with KX_LAW_TAPE set in a
+ * copy of `law` at `step` (applyControl and the step on the device). The
+ * outputs stay resident as the next step's recurrent inputs. */
+int  kx_policy_drive(kx_policy *p, kx_handle *h, const float *senses, int32_t sense_dim, const kx_law *law,
+                     int32_t step);
 /* device time of the last forward kernel (HIP events), milliseconds */
 int  kx_policy_last_ms(const kx_policy *p, float *ms);
 /* reps launches of the forward kernel over the resident inputs and genomes

@@ -141,6 +141,7 @@ EXPORTS = {
     "kx_policy_get_genomes": (i32, [C.c_void_p, f32p]),
     "kx_policy_forward": (i32, [C.c_void_p, f32p, f32p, f32p, f32p, i32]),
     "kx_policy_last_ms": (i32, [C.c_void_p, f32p]),
+    "kx_policy_drive": (i32, [C.c_void_p, C.c_void_p, f32p, i32, C.POINTER(Law), i32]),
     "kx_policy_forward_repeat": (i32, [C.c_void_p, i32, f32p]),
     "kx_es_create": (i32, [i32, C.POINTER(EsOpts), f32p, C.POINTER(C.c_void_p)]),
     "kx_es_destroy": (i32, [C.c_void_p]),

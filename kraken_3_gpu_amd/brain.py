@@ -89,6 +89,21 @@ class Population:
             res.append(sc)
         return res[0] if len(res) == 1 else tuple(res)
 
+    def drive(self, g, law, step, senses=None):
+        """One step of src/sim/gputrain.js:77-100 on the device for the world
+        handle g (gpuworld.create): readProprio, inputs [proprio | senses |
+        previous outputs], forwardInto, decode into the control tape, one
+        rollout step with applyControl (kx_policy_drive). senses float32
+        [n, sense_dim] or None (zeros, sense_dim = n_in - 42 - n_out at six
+        arms)."""
+        if senses is None:
+            sd = self.n_in - self.n_out - 42
+            sp = None
+        else:
+            senses = np.ascontiguousarray(senses, np.float32).reshape(self.n, -1)
+            sd, sp = senses.shape[1], K.ptr(senses, C.c_float)
+        K.check(K.lib().kx_policy_drive(self.p, g.p, sp, int(sd), C.byref(law), int(step)))
+
     def last_ms(self):
         v = C.c_float()
         K.check(K.lib().kx_policy_last_ms(self.p, C.byref(v)))

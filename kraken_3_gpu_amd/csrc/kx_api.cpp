@@ -1201,6 +1201,42 @@ extern "C" int kx_read_state(kx_handle *h, float *out, int32_t stride) {
   return need;
 }
 
+int kx_policy_io(kx_handle *h, int32_t stride, float **state_dev, float **tape_dev, hipStream_t *st, int32_t *n_cr,
+                 int32_t *arms, int launch) {
+  if (!h || !state_dev || !tape_dev || !st || !n_cr || !arms) return fail(KX_EINVAL, "null argument");
+  if (!h->has_creatures) return fail(KX_ENOTSUP, "the policy step needs the creature structure (kx_creature_desc)");
+  if (!h->state_why.empty()) return fail(KX_ENOTSUP, "state readout unavailable: %s", h->state_why.c_str());
+  const int a0 = h->cr.empty() ? 0 : h->cr[0].narms;
+  for (const KxCreature &c : h->cr)
+    if (c.narms != a0) return fail(KX_ENOTSUP, "the policy step needs one arm count for every creature (%d and %d)", a0, c.narms);
+  const int need = 6 * a0 + 6 + KX_STATE_EXTRA;
+  if (stride < need) return fail(KX_EINVAL, "stride %d: the readout needs %d floats per creature", stride, need);
+  *n_cr = h->nCr;
+  *arms = a0;
+  if (!launch) return need;
+  if (!h->state_dev || h->state_stride != stride) {
+    TRY(dalloc(h, &h->state_dev, (size_t)h->nCr * stride));
+    h->state_stride = stride;
+  }
+  const size_t row = (size_t)h->nCr * KX_TAPE_WORDS;
+  if (row > h->tape_cap) {
+    HIPCHK(hipStreamSynchronize(h->st));
+    if (h->tape_dev) HIPCHK(hipFree(h->tape_dev));
+    h->tape_dev = nullptr;
+    h->tape_cap = 0;
+    HIPCHK(hipMalloc(&h->tape_dev, row * 4));
+    h->tape_cap = row;
+    h->d.tape = h->tape_dev;
+    if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; h->g_nsteps = -1; }
+  }
+  if (h->tape_rows < 1) h->tape_rows = 1;
+  if (kx_launch_read_state(h->d, h->state_dev, stride, h->st) != 0) return fail(KX_EDEVICE, "state readout launch failed");
+  *state_dev = h->state_dev;
+  *tape_dev = h->tape_dev;
+  *st = h->st;
+  return need;
+}
+
 extern "C" int kx_set_collision_fold(kx_handle *h, int32_t fold) {
   if (fold != KX_FOLD_AUTO && fold != KX_FOLD_PARALLEL && fold != KX_FOLD_OWNER)
     return fail(KX_EINVAL, "collision fold %d", fold);

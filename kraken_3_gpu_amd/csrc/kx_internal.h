@@ -160,3 +160,11 @@ int kx_handle_view(kx_handle *h, KxHandleView *v);
 int kx_set_coll_mode(kx_handle *h, int mode);
 // sets the thread's kx_last_error message and returns code
 int kx_error(int code, const char *msg);
+// Device side of one policy step (kx_api.cpp): reports the creature count and
+// the arm count every creature shares; with `launch`, also launches the state
+// readout into the handle's device readout buffer (stride floats per
+// creature), makes sure the control tape holds at least one row, and returns
+// both buffers and the handle's stream. Returns the floats per creature the
+// readout writes, or a negative code (nothing launched).
+int kx_policy_io(kx_handle *h, int32_t stride, float **state_dev, float **tape_dev, hipStream_t *st, int32_t *n_cr,
+                 int32_t *arms, int launch);

@@ -276,6 +276,23 @@ __global__ __launch_bounds__(64) void kx_k_es_update(int nw, int half, KxEsStep 
   if (isfinite(step)) theta[k] = (float)((double)theta[k] + step);
 }
 
+// The policy inputs of gputrain.js:77-82 for every creature: readProprio's
+// floats from the device readout, then the senses (zeros when none are
+// given), then the creature's previous outputs.
+__global__ __launch_bounds__(256) void kx_k_policy_inputs(int n, int n_in, int n_out, int proprio, int sense_dim,
+                                                          const float *__restrict__ state, int stride,
+                                                          const float *__restrict__ senses,
+                                                          const float *__restrict__ prev, float *__restrict__ x) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * n_in) return;
+  const int c = (int)(t / n_in), k = (int)(t - (int64_t)c * n_in);
+  float v;
+  if (k < proprio) v = state[(size_t)c * stride + k];
+  else if (k < proprio + sense_dim) v = senses ? senses[(size_t)c * sense_dim + (k - proprio)] : 0.f;
+  else v = prev[(size_t)c * n_out + (k - proprio - sense_dim)];
+  x[t] = v;
+}
+
 }  // namespace
 
 struct kx_policy {
@@ -286,6 +303,8 @@ struct kx_policy {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_ms = 0.f;
   int grid = 0;   // workgroups of the forward launch (set at its first launch)
+  float *senses = nullptr;
+  int senses_cap = 0;   // floats
 };
 
 struct kx_es {
@@ -347,6 +366,7 @@ extern "C" int kx_policy_destroy(kx_policy *p) {
   (void)hipFree(p->out);
   (void)hipFree(p->scratch);
   (void)hipFree(p->controls);
+  (void)hipFree(p->senses);
   if (p->ev0) (void)hipEventDestroy(p->ev0);
   if (p->ev1) (void)hipEventDestroy(p->ev1);
   if (p->st) (void)hipStreamDestroy(p->st);
@@ -368,7 +388,8 @@ extern "C" int kx_policy_get_genomes(const kx_policy *p, float *genomes) {
   return 0;
 }
 
-static int policy_launch(kx_policy *p, int arms, bool want_controls, bool want_scratch) {
+static int policy_launch(kx_policy *p, int arms, bool want_controls, bool want_scratch,
+                         hipStream_t st = nullptr, float *controls_dst = nullptr) {
   const PolDims &d = p->d;
   const size_t lds = (size_t)(d.nw + d.n_in + 2 * d.hidden + d.n_out) * sizeof(float);
   // register pipeline when the genome is whole float4s (hipMalloc bases are
@@ -383,11 +404,13 @@ static int policy_launch(kx_policy *p, int arms, bool want_controls, bool want_s
     const long rounds = (p->n + slots - 1) / slots;   // creatures per wave
     p->grid = (int)((p->n + rounds - 1) / rounds);
   }
-  PCHK(hipEventRecord(p->ev0, p->st));
-  hipLaunchKernelGGL(kx_k_policy_fwd, dim3(p->grid), dim3(64), lds, p->st, d, p->n, p->genomes, p->x, p->out,
-                     want_scratch ? p->scratch : nullptr, want_controls ? p->controls : nullptr, arms, pipe);
+  if (!st) st = p->st;
+  float *ctl = want_controls ? (controls_dst ? controls_dst : p->controls) : nullptr;
+  PCHK(hipEventRecord(p->ev0, st));
+  hipLaunchKernelGGL(kx_k_policy_fwd, dim3(p->grid), dim3(64), lds, st, d, p->n, p->genomes, p->x, p->out,
+                     want_scratch ? p->scratch : nullptr, ctl, arms, pipe);
   PCHK(hipGetLastError());
-  PCHK(hipEventRecord(p->ev1, p->st));
+  PCHK(hipEventRecord(p->ev1, st));
   return 0;
 }
 
@@ -410,6 +433,55 @@ extern "C" int kx_policy_forward(kx_policy *p, const float *x, float *out, float
   PCHK(hipStreamSynchronize(p->st));
   PCHK(hipEventElapsedTime(&p->last_ms, p->ev0, p->ev1));
   return 0;
+}
+
+extern "C" int kx_policy_drive(kx_policy *p, kx_handle *h, const float *senses, int32_t sense_dim,
+                               const kx_law *law, int32_t step) {
+  if (!p || !h || !law) return pfail(KX_EINVAL, "null argument");
+  if (sense_dim < 0) return pfail(KX_EINVAL, "sense_dim %d", sense_dim);
+  const PolDims &d = p->d;
+  float *state = nullptr, *tape = nullptr;
+  hipStream_t st = nullptr;
+  int32_t n_cr = 0, arms = 0;
+  const int stride = KX_STATE_EXTRA + 6 * KX_MAX_ARMS + 6;
+  // every check before anything is launched (the readout advances the
+  // previous mantle velocity, so a refused call must not run it)
+  int need = kx_policy_io(h, stride, &state, &tape, &st, &n_cr, &arms, 0);
+  if (need < 0) return need;
+  const int proprio = 6 * arms + 6;
+  if (n_cr != p->n) return pfail(KX_EINVAL, "policy of %d creatures for a world of %d", p->n, n_cr);
+  if (proprio + sense_dim + d.n_out != d.n_in)
+    return pfail(KX_EINVAL, "inputs: %d proprio + %d senses + %d outputs != %d", proprio, sense_dim, d.n_out, d.n_in);
+  if (d.n_out < 1 + 3 * arms) return pfail(KX_EINVAL, "decode of %d arms from %d outputs", arms, d.n_out);
+  need = kx_policy_io(h, stride, &state, &tape, &st, &n_cr, &arms, 1);
+  if (need < 0) return need;
+  const float *sd = nullptr;
+  if (senses && sense_dim > 0) {
+    const int nf = p->n * sense_dim;
+    if (nf > p->senses_cap) {
+      PCHK(hipStreamSynchronize(st));
+      (void)hipFree(p->senses);
+      p->senses = nullptr;
+      p->senses_cap = 0;
+      if (hipMalloc((void **)&p->senses, (size_t)nf * sizeof(float)) != hipSuccess)
+        return pfail(KX_ENOMEM, "senses of %d creatures", p->n);
+      p->senses_cap = nf;
+    }
+    PCHK(hipMemcpyAsync(p->senses, senses, (size_t)nf * sizeof(float), hipMemcpyHostToDevice, st));
+    sd = p->senses;
+  }
+  const int64_t tot = (int64_t)p->n * d.n_in;
+  hipLaunchKernelGGL(kx_k_policy_inputs, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, p->n, d.n_in, d.n_out,
+                     proprio, sense_dim, state, stride, sd, p->out, p->x);
+  PCHK(hipGetLastError());
+  int rc = policy_launch(p, arms, true, false, st, tape);
+  if (rc) return rc;
+  // later work on the policy's own stream (forward, ES generation) touches
+  // the same inputs, outputs and genomes: order it after this forward pass
+  PCHK(hipStreamWaitEvent(p->st, p->ev1, 0));
+  kx_law L = *law;
+  L.flags |= KX_LAW_TAPE;
+  return kx_rollout(h, &L, step, 1, nullptr);
 }
 
 extern "C" int kx_policy_forward_repeat(kx_policy *p, int32_t reps, float *ms) {
