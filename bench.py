@@ -150,6 +150,59 @@ def policy_cost(n, reps=50):
     return res
 
 
+def policy_loop_cost(n, steps=40):
+    """gputrain.js's per-step loop (readProprio -> forwardInto -> decode ->
+    applyControl -> step) at n creatures with the reference's network and
+    random senses, two ways: kx_policy_drive (everything but the senses on
+    the device) and the host-driven loop over the same library (read_state
+    down, inputs assembled on the host, forward_into, upload_tape, a one-step
+    tape rollout). Host-timed per step, the stream drained at the end; the
+    physics-only one-step rollout under the device law is timed beside them."""
+    from kraken_3_gpu_amd import _lib as K
+    from kraken_3_gpu_amd import brain as B
+    from kraken_3_gpu_amd import gpuworld as G
+    from kraken_3_gpu_amd import world as W
+    n_in = B.input_dim()
+    rng = np.random.default_rng(4)
+    gen = (rng.standard_normal((n, B.weight_count(n_in))) / np.sqrt(n_in)).astype(np.float32)
+    senses = rng.uniform(-1, 1, (n, B.SENSE_DIM)).astype(np.float32)
+    law = W.default_law(5)
+    tlaw = W.default_law(5)
+    tlaw.flags |= K.KX_LAW_TAPE
+    res = {"creatures": n, "steps": steps}
+    w = W.world_uniform(n, seed=2)
+    g = G.create(w)
+    G.upload(g)
+    with B.Population(n, n_in) as P:
+        P.set_genomes(gen)
+        for s in range(3):
+            P.drive(g, law, s, senses)
+        G.sync(g)
+        t0 = time.perf_counter()
+        for s in range(steps):
+            P.drive(g, law, s, senses)
+        G.sync(g)
+        res["device_loop_ms_per_step"] = (time.perf_counter() - t0) * 1e3 / steps
+        prev = np.zeros((n, B.OUT_DIM), np.float32)
+        t0 = time.perf_counter()
+        for s in range(steps):
+            st = G.read_state(g)
+            x = np.concatenate([st[:, :42], senses, prev], axis=1)
+            prev, ctl = P.forward_into(x, arms=6)
+            G.upload_tape(g, ctl[None])
+            G.rollout(g, tlaw, s, 1)
+        G.sync(g)
+        res["host_loop_ms_per_step"] = (time.perf_counter() - t0) * 1e3 / steps
+    t0 = time.perf_counter()
+    for s in range(steps):
+        G.rollout(g, law, s, 1)
+    G.sync(g)
+    res["physics_only_ms_per_step"] = (time.perf_counter() - t0) * 1e3 / steps
+    G.destroy(g)
+    res["device_loop_creature_steps_per_s"] = n / (res["device_loop_ms_per_step"] * 1e-3)
+    return res
+
+
 def cpu_baseline(seconds=12.0):
     """The oracle (C restatement of xpbd.js, f64) stepping 64-creature worlds
     under the same control law on the host cores, one world per thread."""
@@ -428,7 +481,9 @@ def main():
         bench_serve(args)
         return
     if args.workload == "policy":
-        print(json.dumps({"workload": "policy network + ES", **policy_cost(args.creatures or 4096)}), flush=True)
+        n = args.creatures or 4096
+        print(json.dumps({"workload": "policy network + ES", **policy_cost(n), "loop": policy_loop_cost(n)}),
+              flush=True)
         return
     args.steps = args.steps or 1000
     args.creatures = args.creatures or 4096
