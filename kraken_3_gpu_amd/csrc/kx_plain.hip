@@ -205,80 +205,81 @@ int kx_plain_step(const KxDev &d, const KxLaw *law, int step_key, const int *ste
 // ── state readout ───────────────────────────────────────────────────
 // readProprio (src/sim/core.js:129-175; tipPosition, armCurl, armExtension
 // src/body/kraken.js:150-183) of every creature, in binary64 from the
-// binary32 state, then its mantle state; one thread per creature. Math.min /
+// binary32 state, then its mantle state; one thread per arm (eight arm slots
+// per creature, eight creatures per block), the arm sums taken in arm order by
+// the creature's first thread. Math.min /
 // Math.max give NaN for a NaN argument, unlike fmin / fmax, so the clamps are
 // written as comparisons, as the reference's own `cl` is.
 __device__ __forceinline__ double kx_js_min(double a, double b) { return (a != a || b != b) ? a + b : (a < b ? a : b); }
 __device__ __forceinline__ double kx_js_max(double a, double b) { return (a != a || b != b) ? a + b : (a > b ? a : b); }
 __device__ __forceinline__ double kx_cl(double v, double m) { return v > m ? m : (v < -m ? -m : v); }
 
-__global__ __launch_bounds__(PLAIN_BLOCK) void kx_k_read_state(KxDev d, float *out, int stride) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= d.nCr) return;
-  const KxCreature cr = d.cr[c];
-  const int A = cr.narms;
-  if (A <= 0) return;
+constexpr int KX_RS_CR = 64 / KX_MAX_ARMS;   // creatures per 64-thread block, one thread per arm slot
+
+__global__ __launch_bounds__(64) void kx_k_read_state(KxDev d, float *out, int stride) {
+  __shared__ double s_ext[KX_RS_CR][KX_MAX_ARMS], s_curl[KX_RS_CR][KX_MAX_ARMS];
+  __shared__ int s_brk[KX_RS_CR][KX_MAX_ARMS];
+  const int lc = threadIdx.x / KX_MAX_ARMS, a = threadIdx.x % KX_MAX_ARMS;
+  const int c = blockIdx.x * KX_RS_CR + lc;
+  const bool have = c < d.nCr;
+  const KxCreature cr = have ? d.cr[c] : KxCreature{};
+  const int A = have ? cr.narms : 0;
   // kraken.js:24-49: segLen, shoulderLimit, jointLimit, ARM_REACH
   const double kPi = 3.141592653589793, seg_len = 13.5;
   const double shoulder_lim = kPi * 0.62, joint_lim = kPi * 0.42, reach = 24.0 + 7.0 * seg_len;
-  const int *so = d.arm_seg_off + cr.arm_off;
-  const int m = d.arm_mantle[cr.arm_off];
-  const double x = d.px[m], y = d.py[m], vx = d.vx[m], vy = d.vy[m], angle = d.ang[m], av = d.av[m];
-  const double speed = hypot(vx, vy), ca = cos(-angle), sa = sin(-angle);
-  auto tip_pos = [&](int a, double &tx, double &ty) {
-    const int b = d.arm_segs[so[a + 1] - 1];
-    const double L = seg_len * 0.5, t = d.ang[b];
-    tx = (double)d.px[b] + L * cos(t);
-    ty = (double)d.py[b] + L * sin(t);
-  };
-  auto curl = [&](int a) {
-    double s = 0.0;
-    int n = 0;
+  float *o = out + (size_t)(have ? c : 0) * stride;
+  if (A > 0 && a < A) {
+    // this arm's inputs (core.js:140-160), each computed once
+    const int *so = d.arm_seg_off + cr.arm_off;
+    const int m = d.arm_mantle[cr.arm_off];
+    const double x = d.px[m], y = d.py[m], angle = d.ang[m];
+    const double ca = cos(-angle), sa = sin(-angle);
+    const double ds = (double)d.ang[d.arm_segs[so[a]]] - angle;
+    const double shoulder = atan2(sin(ds), cos(ds));
+    double cs = 0.0;
+    int nj = 0;
     for (int j = so[a] + 1; j < so[a + 1]; j++) {
       const double dd = (double)d.ang[d.arm_segs[j]] - (double)d.ang[d.arm_segs[j - 1]];
-      s += atan2(sin(dd), cos(dd));
-      n++;
+      cs += atan2(sin(dd), cos(dd));
+      nj++;
     }
-    return n ? s / n : 0.0;
-  };
-  auto extension = [&](int a) {
-    double tx, ty;
-    tip_pos(a, tx, ty);
+    const double curl = nj ? cs / nj : 0.0;
+    const int tb = d.arm_segs[so[a + 1] - 1];
+    const double L = seg_len * 0.5, t = d.ang[tb];
+    const double tx = (double)d.px[tb] + L * cos(t), ty = (double)d.py[tb] + L * sin(t);
     const int s0 = d.arm_segs[so[a]];
-    return kx_js_min(1.0, hypot(tx - (double)d.px[s0], ty - (double)d.py[s0]) / kx_js_max(1.0, reach));
-  };
-  auto shoulder = [&](int a) {
-    const double dd = (double)d.ang[d.arm_segs[so[a]]] - angle;
-    return atan2(sin(dd), cos(dd));
-  };
-  auto broken = [&](int a) { return d.broken[d.arm_grip[cr.arm_off + a]] != 0; };
-  int anchored = 0;
-  double ext_sum = 0.0, curl_sum = 0.0;
-  for (int a = 0; a < A; a++) {
-    ext_sum += extension(a);
-    curl_sum += curl(a);
-    if (!broken(a)) anchored++;
-  }
-  float *o = out + (size_t)c * stride;
-  int i = 0;
-  for (int a = 0; a < A; a++) o[i++] = (float)kx_cl(shoulder(a) / shoulder_lim, 1.2);
-  for (int a = 0; a < A; a++) o[i++] = (float)kx_cl(curl(a) / joint_lim, 1.2);
-  for (int a = 0; a < A; a++) {
-    double tx, ty;
-    tip_pos(a, tx, ty);
-    const double dx = tx - x, dy = ty - y;
-    o[i++] = (float)kx_cl((dx * ca - dy * sa) / reach, 1.2);
-    o[i++] = (float)kx_cl((dx * sa + dy * ca) / reach, 1.2);
-  }
-  for (int a = 0; a < A; a++) {
+    const double ext = kx_js_min(1.0, hypot(tx - (double)d.px[s0], ty - (double)d.py[s0]) / kx_js_max(1.0, reach));
     const int g = d.arm_grip[cr.arm_off + a];
+    const bool brk = d.broken[g] != 0;
+    o[a] = (float)kx_cl(shoulder / shoulder_lim, 1.2);
+    o[A + a] = (float)kx_cl(curl / joint_lim, 1.2);
+    const double dx = tx - x, dy = ty - y;
+    o[2 * A + 2 * a] = (float)kx_cl((dx * ca - dy * sa) / reach, 1.2);
+    o[2 * A + 2 * a + 1] = (float)kx_cl((dx * sa + dy * ca) / reach, 1.2);
     // the grip's force as _postGripBreak last measured it: |lambda| / h of
     // the step's last substep (xpbd.js:496-504)
     const double f = hypot((double)d.lamx[g], (double)d.lamy[g]) * (1.0 / d.hd);
     const double fz = (f == f && f != 0.0) ? f : 0.0;
-    o[i++] = broken(a) ? 0.0f : (float)kx_js_min(1.0, fz / kx_js_max(1.0, (double)d.bf[g]));
+    o[4 * A + a] = brk ? 0.0f : (float)kx_js_min(1.0, fz / kx_js_max(1.0, (double)d.bf[g]));
+    o[5 * A + a] = brk ? 0.0f : 1.0f;
+    s_ext[lc][a] = ext;
+    s_curl[lc][a] = curl;
+    s_brk[lc][a] = brk;
   }
-  for (int a = 0; a < A; a++) o[i++] = broken(a) ? 0.0f : 1.0f;
+  __syncthreads();
+  if (A <= 0 || a != 0) return;
+  // the creature's sums over its arms in arm order, then its mantle values
+  const int m = d.arm_mantle[cr.arm_off];
+  const double x = d.px[m], y = d.py[m], vx = d.vx[m], vy = d.vy[m], angle = d.ang[m], av = d.av[m];
+  const double speed = hypot(vx, vy), ca = cos(-angle), sa = sin(-angle);
+  int anchored = 0;
+  double ext_sum = 0.0, curl_sum = 0.0;
+  for (int k = 0; k < A; k++) {
+    ext_sum += s_ext[lc][k];
+    curl_sum += s_curl[lc][k];
+    if (!s_brk[lc][k]) anchored++;
+  }
+  int i = 6 * A;
   const double pvx = d.prev_v[2 * c], pvy = d.prev_v[2 * c + 1];
   const double ax = (vx - pvx) / (d.hd * d.S), ay = (vy - pvy) / (d.hd * d.S);
   o[i++] = (float)kx_js_max(-1.0, kx_js_min(1.0, (ax * ca - ay * sa) / 600));
@@ -296,6 +297,6 @@ __global__ __launch_bounds__(PLAIN_BLOCK) void kx_k_read_state(KxDev d, float *o
 }
 
 int kx_launch_read_state(const KxDev &d, float *out, int stride, hipStream_t st) {
-  hipLaunchKernelGGL(kx_k_read_state, dim3(nblk(d.nCr)), dim3(PLAIN_BLOCK), 0, st, d, out, stride);
+  hipLaunchKernelGGL(kx_k_read_state, dim3((d.nCr + KX_RS_CR - 1) / KX_RS_CR), dim3(64), 0, st, d, out, stride);
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
