@@ -529,6 +529,7 @@ def main():
     shape = G.persist_info(g)
     readout = state_readout_cost(g, w)
     policy = policy_cost(n_cr)
+    policy["loop"] = policy_loop_cost(n_cr)
 
     per_cr = dict(pins=42, motors=42, bodies=43, points=92, grips_total=6)
     F = flops_per_creature_step(w.S, w.I, per_cr["pins"], per_cr["motors"], per_cr["bodies"], per_cr["points"],
