@@ -396,7 +396,7 @@ int  kx_policy_forward_repeat(kx_policy *p, int32_t reps, float *ms);
 /* ── evolution strategy on the device (src/brain/es.js) ────────────────── */
 typedef struct {
   double sigma, lr, min_sigma, sigma_decay, beta1, beta2, adam_eps, clip;   /* es.js:23-38 option names */
-  int32_t pop_size;                                                        /* made even as es.js:37 does */
+  int32_t pop_size;                                     /* 1 .. 8192, made even as es.js:37 does */
 } kx_es_opts;
 typedef struct kx_es kx_es;
 /* theta (n_weights floats) may be NULL for zeros (createES, es.js:22-58) */

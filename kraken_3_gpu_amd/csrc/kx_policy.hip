@@ -511,7 +511,8 @@ extern "C" int kx_es_create(int32_t n_weights, const kx_es_opts *opts, const flo
   *out = nullptr;
   if (n_weights <= 0) return pfail(KX_EINVAL, "ES over %d weights", n_weights);
   kx_es_opts o = *opts;
-  if (o.pop_size <= 0) return pfail(KX_EINVAL, "ES population %d", o.pop_size);
+  if (o.pop_size <= 0 || o.pop_size > 8192)
+    return pfail(KX_EINVAL, "ES population %d (1 .. 8192: the update keeps up to 4096 pair advantages)", o.pop_size);
   if (o.pop_size % 2) o.pop_size++;   // es.js:37 keeps the population even
   kx_es *e = new kx_es();
   e->nw = n_weights;

@@ -211,3 +211,6 @@ def test_policy_shapes_and_argument_checks_on_the_host():
                   adam_eps=1e-8, clip=2.5, pop_size=0)
     assert L.kx_es_create(3644, C.byref(eo), None, C.byref(h)) == K.KX_EINVAL
     assert L.kx_es_create(0, C.byref(eo), None, C.byref(h)) == K.KX_EINVAL
+    eo.pop_size = 8194
+    assert L.kx_es_create(3644, C.byref(eo), None, C.byref(h)) == K.KX_EINVAL
+    assert "8192" in L.kx_last_error().decode()
