@@ -304,7 +304,9 @@ struct kx_policy {
   float last_ms = 0.f;
   int grid = 0;   // workgroups of the forward launch (set at its first launch)
   float *senses = nullptr;
-  int senses_cap = 0;   // floats
+  float *senses_host = nullptr;   // pinned staging of the senses (hipHostMalloc)
+  int senses_cap = 0;             // floats of both
+  hipEvent_t senses_done = nullptr;   // the last upload from the staging buffer
 };
 
 struct kx_es {
@@ -367,6 +369,8 @@ extern "C" int kx_policy_destroy(kx_policy *p) {
   (void)hipFree(p->scratch);
   (void)hipFree(p->controls);
   (void)hipFree(p->senses);
+  if (p->senses_host) (void)hipHostFree(p->senses_host);
+  if (p->senses_done) (void)hipEventDestroy(p->senses_done);
   if (p->ev0) (void)hipEventDestroy(p->ev0);
   if (p->ev1) (void)hipEventDestroy(p->ev1);
   if (p->st) (void)hipStreamDestroy(p->st);
@@ -461,13 +465,23 @@ extern "C" int kx_policy_drive(kx_policy *p, kx_handle *h, const float *senses, 
     if (nf > p->senses_cap) {
       PCHK(hipStreamSynchronize(st));
       (void)hipFree(p->senses);
+      if (p->senses_host) (void)hipHostFree(p->senses_host);
       p->senses = nullptr;
+      p->senses_host = nullptr;
       p->senses_cap = 0;
-      if (hipMalloc((void **)&p->senses, (size_t)nf * sizeof(float)) != hipSuccess)
+      if (hipMalloc((void **)&p->senses, (size_t)nf * sizeof(float)) != hipSuccess ||
+          hipHostMalloc((void **)&p->senses_host, (size_t)nf * sizeof(float), hipHostMallocDefault) != hipSuccess)
         return pfail(KX_ENOMEM, "senses of %d creatures", p->n);
+      if (!p->senses_done) PCHK(hipEventCreateWithFlags(&p->senses_done, hipEventDisableTiming));
       p->senses_cap = nf;
+    } else if (p->senses_done) {
+      PCHK(hipEventSynchronize(p->senses_done));   // the previous upload has left the staging buffer
     }
-    PCHK(hipMemcpyAsync(p->senses, senses, (size_t)nf * sizeof(float), hipMemcpyHostToDevice, st));
+    // the caller's (pageable) senses go through the pinned staging buffer, so
+    // the upload itself is an asynchronous DMA on the world's stream
+    std::memcpy(p->senses_host, senses, (size_t)nf * sizeof(float));
+    PCHK(hipMemcpyAsync(p->senses, p->senses_host, (size_t)nf * sizeof(float), hipMemcpyHostToDevice, st));
+    PCHK(hipEventRecord(p->senses_done, st));
     sd = p->senses;
   }
   const int64_t tot = (int64_t)p->n * d.n_in;
