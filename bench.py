@@ -468,7 +468,7 @@ def main():
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (creatures shard across "
-                                   "GPUs with no per-step exchange, see DESIGN.md)"}))
+                                   "GPUs with no per-step exchange; one process per GPU, each with its own share)"}))
         sys.exit(2)
 
     if args.workload == "stress":
