@@ -218,3 +218,22 @@ def test_node_brain_matches_c_abi(tmp_path):
                 assert g["stats"][k] == stats[k], k
     assert "not a live policy" in r["deadThrows"]
     print("    node brain identical: 3 forward steps, 2 ES generations", flush=True)
+
+
+@needs_node
+def test_brain_module_errors_without_a_gpu_call():
+    """Shape checks and foreign handles throw with the library's message before
+    any device work, as gpuworld.js throws."""
+    code = ("const b=require(process.argv[1]);const out={};"
+            "try{b.createPopulation(4,105,65);out.wide='no throw'}catch(e){out.wide=e.message}"
+            "try{b.createES(3644,{popSize:9000});out.pop='no throw'}catch(e){out.pop=e.message}"
+            "try{b.forwardInto({handle:5,n:1,nIn:105,nOut:20,hidden:24},new Float32Array(105),new Float32Array(20));"
+            "out.dead='no throw'}catch(e){out.dead=e.message}"
+            "console.log(JSON.stringify(out))")
+    r = subprocess.run([NODE, "-e", code, os.path.join(ROOT, "node", "brain.js")],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    d = json.loads(r.stdout)
+    assert "at most 64" in d["wide"] and "kxpbd error" in d["wide"]
+    assert "8192" in d["pop"]
+    assert "not a live policy" in d["dead"]
