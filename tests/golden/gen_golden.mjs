@@ -12,6 +12,7 @@ import { makeMap, makeSdf, findSpawn, mulberry32 } from '/root/reference/src/wor
 import { spawnKraken, attachGrip, tipPosition, adhesionToBreakForce, toneToCompliance,
   BODY, ARM_REACH } from '/root/reference/src/body/kraken.js';
 import { createSim, applyControl, stepSim, readProprio } from '/root/reference/src/sim/core.js';
+import { crnSpawns, respawnCreature } from '/root/reference/src/sim/train.js';
 import { PROPRIO_DIM, SENSE_DIM } from '/root/reference/src/sim/core.js';
 import { HIDDEN, OUT_DIM, inputDim, weightCount, randomGenome, forwardInto, decode } from '/root/reference/src/brain/net.js';
 import { createES, genomeAt, update } from '/root/reference/src/brain/es.js';
@@ -416,5 +417,124 @@ function runTraj(mode, S, I, h, nSteps, record, seed) {
       v: b64(es.v), t: es.t, stats });
   }
   dump('es.json', { nW, popSize, seed, s0, opts: es.opts, theta0: b64(theta0), gens });
+}
+// ── 12. generalised morphology. spawnKraken reads BODY.arms and
+// BODY.segsPerArm at call time (kraken.js:64-148; the mantle keeps its 8
+// collision points), so setting them gives the reference's own creature for
+// each (arms, segs): bodies, points, constraints, the greedy colouring, a
+// trajectory under the seeded law with its grip events (f64, and the f32
+// emulator for two shapes) and readProprio (core.js:129-175) after each
+// recorded step. BODY is restored afterwards. ──
+{
+  const saved = [BODY.arms, BODY.segsPerArm];
+  const morphs = [[4, 5], [4, 9], [5, 8], [7, 6], [8, 9], [8, 5], [5, 5], [6, 7]];
+  const cases = [];
+  const S = 4, I = 8, h = 1 / 240, seed = 4242, nSteps = 24, rec = [1, 8, 24];
+  for (const [A, SG] of morphs) {
+    BODY.arms = A; BODY.segsPerArm = SG;
+    for (const mode of (A === 8 && SG === 9) || (A === 4 && SG === 5) ? ['f64', 'f32'] : ['f64']) {
+      const map = makeMap(1200, 800, { seed: 1 });
+      const w = new XpbdWorld({ substeps: S, iterations: I, h });
+      w.sdf = map.sdf;
+      const r4 = map.rects[4];
+      const poses = [[70, 300, 0.3], [r4.x + r4.w + 40, r4.y + Math.min(r4.h, 200) * 0.5, 2.1], [600, 420, 4.4]];
+      const ks = poses.map(p => spawnKraken(w, p[0], p[1], p[2]));
+      const creation = worldDump(w);
+      const ids = new Map(); w.constraints.forEach((c, i) => ids.set(c, i));
+      ensureColored(w);
+      const colouring = { sorted: w.constraints.map(c => ids.get(c)), colorOf: w.constraints.map(c => c.color),
+        colorOff: arr(w.colorOff, 16), colorCnt: arr(w.colorCnt, 16) };
+      let emu = null;
+      if (mode === 'f32') { emu = f32ify(w, map); emu.roundState(); }
+      const crs = ks.map(k => reflexCreature(w, k));
+      const cs = ks.map(k => ({ world: w, mantle: k.mantle, arms: k.arms, prevVx: 0, prevVy: 0 }));
+      const frames = [], events = [];
+      for (let s = 0; s < nSteps; s++) {
+        const before = ks.map(k => k.arms.map(a => a.grip.broken));
+        for (let ci = 0; ci < ks.length; ci++) applyLaw(crs[ci], lawControl(seed, ci, s, ks[ci].arms));
+        const mid = ks.map(k => k.arms.map(a => a.grip.broken));
+        w.step();
+        for (let ci = 0; ci < ks.length; ci++) for (let a = 0; a < A; a++) {
+          if (before[ci][a] && !mid[ci][a]) events.push([s, ci, a, 'attach']);
+          if (!before[ci][a] && mid[ci][a]) events.push([s, ci, a, 'release']);
+          if (!mid[ci][a] && ks[ci].arms[a].grip.broken) events.push([s, ci, a, 'break']);
+        }
+        const prev = cs.map(c => [c.prevVx, c.prevVy]);
+        const outs = mode === 'f64' ? cs.map(c => Array.from(readProprio(c, new Float64Array(6 * A + 6)))) : null;
+        if (rec.indexOf(s + 1) >= 0) {
+          const fr = { step: s + 1, state: stateDump(w, ks, crs) };
+          if (outs) { fr.prev = prev; fr.proprio = outs; fr.side = cs.map(c => [c.meanExtension, c.meanCurl, c.anchoredCount, c.speed]); }
+          frames.push(fr);
+        }
+      }
+      cases.push({ arms: A, segs: SG, mode, S, I, h, seed, poses, mapSeed: 1, creation: mode === 'f64' ? creation : null,
+        colouring: mode === 'f64' ? colouring : null, frames, events,
+        creatures: ks.map(k => ({ mantle: k.mantle, arms: k.arms.map(a => a.segs) })) });
+    }
+  }
+  BODY.arms = saved[0]; BODY.segsPerArm = saved[1];
+  dump('morph.json', { armReach: ARM_REACH, cases });
+}
+
+// ── 13. crnSpawns (train.js:103-122) drawn from the trainer's stream
+// (train.js:65-72: mulberry32 seeded with seed*2654435761), pair-shared and
+// not, and respawnCreature (train.js:188-229) on a running world: the full
+// body and constraint state before and after the respawn, then 8 further steps
+// under the seeded law ──
+{
+  const crn = [];
+  for (const [W, H, o, seed, pop, pairShared] of [[1200, 800, { seed: 1 }, 1, 64, true], [1440, 810, { seed: 7 }, 3, 17, false],
+    [4096, 4096, { corridorCols: 32, corridorRows: 32, seed: 1 }, 2, 48, true], [1200, 800, { seed: 2 }, 9, 40, true]]) {
+    const map = makeMap(W, H, o);
+    const s0 = (seed * 2654435761) >>> 0;
+    const rnd = mulberry32(s0);
+    let calls = 0;
+    const counted = () => { calls++; return rnd(); };
+    const sp = crnSpawns({ map }, pop, counted, pairShared);
+    crn.push({ width: W, height: H, opts: o, seed, s0, pop, pairShared, calls, spawns: sp.map(p => [p.x, p.y, p.angle]) });
+  }
+  {
+    const rnd = mulberry32(5);
+    const sp = crnSpawns({ map: null }, 6, rnd, true);
+    crn.push({ width: 240, height: 240, opts: null, seed: null, s0: 5, pop: 6, pairShared: true, calls: null, spawns: sp.map(p => [p.x, p.y, p.angle]) });
+  }
+  const full = (w) => ({
+    px: arr(w.px, w.n), py: arr(w.py, w.n), ppx: arr(w.ppx, w.n), ppy: arr(w.ppy, w.n), vx: arr(w.vx, w.n), vy: arr(w.vy, w.n),
+    ang: arr(w.ang, w.n), angP: arr(w.angP, w.n), av: arr(w.av, w.n),
+    cons: w.constraints.map(c => [c.target, c.broken ? 1 : 0, c.force || 0, c.lambdaX, c.lambdaY, c.lambda, c.compliance,
+      c.lbx, c.lby, Number.isFinite(c.breakForce) ? c.breakForce : null]),
+  });
+  const resp = [];
+  const saved = [BODY.arms, BODY.segsPerArm];
+  for (const [A, SG, seed] of [[6, 7, 21], [5, 8, 22]]) {
+    BODY.arms = A; BODY.segsPerArm = SG;
+    const sim = createSim({ seed, creatures: 3, substeps: 4, iterations: 8, h: 1 / 240, width: 1200, height: 800 });
+    const w = sim.world;
+    const spawns = sim.creatures.map(c => [c.spawn.x, c.spawn.y, c.spawn.angle]);
+    const law = 777;
+    const run = (s0, n) => {
+      for (let s = s0; s < s0 + n; s++) {
+        for (let ci = 0; ci < sim.creatures.length; ci++) applyLaw(sim.creatures[ci], lawControl(law, ci, s, sim.creatures[ci].arms));
+        w.step();
+      }
+    };
+    run(0, 40);
+    const before = full(w);
+    const cdBefore = sim.creatures.map(c => c.perArm.map(p => p.cd));
+    const map = sim.map;
+    const targets = crnSpawns(sim, 4, mulberry32(99), true);
+    respawnCreature(sim, sim.creatures[0], targets[0].x, targets[0].y, targets[0].angle);
+    respawnCreature(sim, sim.creatures[2], targets[2].x, targets[2].y, targets[2].angle);
+    const after = full(w);
+    const cdAfter = sim.creatures.map(c => c.perArm.map(p => p.cd));
+    const prevAfter = sim.creatures.map(c => [c.prevVx, c.prevVy]);
+    run(40, 8);
+    resp.push({ arms: A, segs: SG, seed, law, spawns, mapRects: map.rects.map(r => [r.x, r.y, r.w, r.h]),
+      respawned: [[0, targets[0].x, targets[0].y, targets[0].angle], [2, targets[2].x, targets[2].y, targets[2].angle]],
+      before, after, cdBefore, cdAfter, prevAfter, later: full(w), laterCd: sim.creatures.map(c => c.perArm.map(p => p.cd)),
+      colorOf: w.constraints.map(c => c.color) });
+  }
+  BODY.arms = saved[0]; BODY.segsPerArm = saved[1];
+  dump('respawn.json', { spawnClear: 26, armReach: ARM_REACH, crn, respawn: resp });
 }
 console.log('done');
