@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define KXPBD_ABI_VERSION 3
+#define KXPBD_ABI_VERSION 4
 
 /* error codes */
 #define KX_OK 0
@@ -175,6 +175,10 @@ int kx_set_cooldown(kx_handle *h, const double *cd);
 /* ── stepping (gpuworld.js step 374-382): S substeps + grip break per step;
  *    asynchronous on the handle's stream. ─────────────────────────────── */
 int kx_step(kx_handle *h, int32_t nsteps);
+/* n substeps without the grip break (gpuworld.js:344-372 substep: reset
+ * lambda, integrate, I x (colours, collisions), finalize), on either path's
+ * state; asynchronous on the handle's stream. */
+int kx_substep(kx_handle *h, int32_t n);
 /* nsteps of {control law + reflex, S substeps, grip break}; steps are keyed
  * step0 .. step0+nsteps-1 in the control PRNG. Needs a creature desc.
  * events (optional, device-resident until kx_sync): per step and creature a
@@ -240,6 +244,20 @@ int kx_persist_info(const kx_handle *h, int32_t *info, int32_t max);
  * be at least that. No counterpart in the reference's GPU path. */
 #define KX_STATE_EXTRA 12
 int kx_read_state(kx_handle *h, float *out, int32_t stride);
+/* respawnCreature (src/sim/train.js:188-229) for n creatures in place on the
+ * device, no rebuild or full upload: creature creatures[i] gets its mantle at
+ * pose[3i], pose[3i+1] with heading pose[3i+2], every arm re-chained from
+ * 24 px out along heading + 2*pi*a/A in 13.5 px segments turning 0.28 rad
+ * each (float64, rounded to the state's binary32), all velocities 0, every
+ * motor target 0, every grip released (broken, lambdaX/Y 0, so its force
+ * reads 0), and the readout's previous mantle velocity cleared (readProprio's
+ * prevVx/prevVy). Compliances, break limits and reflex cooldowns are kept, as
+ * the reference keeps them. Ordered on the handle's stream after earlier
+ * work; the host arrays may be reused on return. Ids must be distinct and in
+ * range; needs the creature structure with motors running mantle -> segments
+ * -> tip (KX_ENOTSUP otherwise). Replaces the host respawn + full upload of
+ * a generation start (gputrain.js:130-133 beginGenerationGpu). */
+int kx_respawn(kx_handle *h, int32_t n, const int32_t *creatures, const double *pose);
 #define KX_FOLD_AUTO 0
 #define KX_FOLD_PARALLEL 1
 #define KX_FOLD_OWNER 2
@@ -267,6 +285,14 @@ void kx_builder_free(kx_builder *b);
 int  kx_builder_map(kx_builder *b, double width, double height, int32_t cols, int32_t rows,
                     double density, double wall, uint32_t seed);
 int  kx_builder_find_spawn(kx_builder *b, double clearance, uint32_t *rng_state, double *xy);
+/* crnSpawns (src/sim/train.js:103-122): pop training spawns drawn from the
+ * trainer's stream rng_state (advanced in place; train.js:65-72 seeds it
+ * with seed*2654435761): per slot a point in the map's 60 px inner margin
+ * whose wall distance lies in [26, ARM_REACH] (500 tries, then an unchecked
+ * point) and a heading; with pair_shared, slot 2p+1 repeats slot 2p without
+ * drawing. No map built: a 240 x 240 area, untested. out[3*pop] = x, y,
+ * angle. Returns the stream calls made, or a negative code. */
+int  kx_builder_crn_spawns(kx_builder *b, int32_t pop, int32_t pair_shared, uint32_t *rng_state, double *out);
 int  kx_builder_spawn(kx_builder *b, double x, double y, double angle, int32_t arms,
                       int32_t segs, int32_t ring, double curl);
 /* views valid until the next builder call */

@@ -112,6 +112,7 @@ EXPORTS = {
     "kx_get_cooldown": (i32, [C.c_void_p, f64p]),
     "kx_set_cooldown": (i32, [C.c_void_p, f64p]),
     "kx_step": (i32, [C.c_void_p, i32]),
+    "kx_substep": (i32, [C.c_void_p, i32]),
     "kx_rollout": (i32, [C.c_void_p, C.POINTER(Law), i32, i32, u32p]),
     "kx_sync": (i32, [C.c_void_p]),
     "kx_upload_tape": (i32, [C.c_void_p, C.POINTER(C.c_float), i32]),
@@ -130,6 +131,7 @@ EXPORTS = {
     "kx_persist_info": (i32, [C.c_void_p, i32p, i32]),
     "kx_set_collision_fold": (i32, [C.c_void_p, i32]),
     "kx_read_state": (i32, [C.c_void_p, f32p, i32]),
+    "kx_respawn": (i32, [C.c_void_p, i32, i32p, f64p]),
     "kx_template_info": (i32, [C.c_void_p, i32, i32p]),
     "kx_set_template_lanes": (i32, [C.c_void_p, i32, i32p]),
     "kx_creature_templates": (i32, [C.c_void_p, i32p]),
@@ -152,6 +154,7 @@ EXPORTS = {
     "kx_builder_free": (None, [C.c_void_p]),
     "kx_builder_map": (i32, [C.c_void_p, f64, f64, i32, i32, f64, f64, u32]),
     "kx_builder_find_spawn": (i32, [C.c_void_p, f64, u32p, f64p]),
+    "kx_builder_crn_spawns": (i32, [C.c_void_p, i32, i32, u32p, f64p]),
     "kx_builder_spawn": (i32, [C.c_void_p, f64, f64, f64, i32, i32, i32, f64]),
     "kx_builder_desc": (i32, [C.c_void_p, C.POINTER(WorldDesc), C.POINTER(CreatureDesc)]),
     "kx_builder_state": (i32, [C.c_void_p, f64p, f64p, f64p]),

@@ -20,20 +20,25 @@ FLAGS = [
 ]
 
 
+def _compile(s, verbose):
+    src = os.path.join(CSRC, s)
+    obj = os.path.join(CSRC, "_build", s + ".o")
+    os.makedirs(os.path.dirname(obj), exist_ok=True)
+    lang = ["-x", "hip"] if s.endswith(".hip") else []
+    if s in HOST_ONLY:
+        lang = ["--offload-host-only"]
+    cmd = [HIPCC] + FLAGS + lang + ["-c", src, "-o", obj]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return obj
+
+
 def build(verbose=False):
-    objs = []
-    for s in SOURCES:
-        src = os.path.join(CSRC, s)
-        obj = os.path.join(CSRC, "_build", s + ".o")
-        os.makedirs(os.path.dirname(obj), exist_ok=True)
-        lang = ["-x", "hip"] if s.endswith(".hip") else []
-        if s in HOST_ONLY:
-            lang = ["--offload-host-only"]
-        cmd = [HIPCC] + FLAGS + lang + ["-c", src, "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
-        objs.append(obj)
+    from concurrent.futures import ThreadPoolExecutor
+    jobs = max(1, min(len(SOURCES), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(jobs) as ex:
+        objs = list(ex.map(lambda s: _compile(s, verbose), SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", tmp] + objs
     subprocess.run(cmd, check=True)

@@ -49,6 +49,12 @@ extern "C" int kx_abi_version(void) { return KXPBD_ABI_VERSION; }
     if (e_ != hipSuccess) return fail(KX_EDEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
   } while (0)
 
+#define TRY(x)            \
+  do {                    \
+    int r_ = (x);         \
+    if (r_) return r_;    \
+  } while (0)
+
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
@@ -97,8 +103,11 @@ struct kx_handle {
   int launch_G = 16;             // lane count the uniform kernel is launched with
   int coll_mode = 0;             // collision fold: 0 = by map density, 1 = parallel, 2 = owning lane
   std::string state_why;         // why the state readout is unavailable (empty: available)
-  float *state_dev = nullptr;    // readout buffer [nCr * state_stride]
+  float *state_dev = nullptr;    // readout buffer [nCr * state_stride], state_cap floats allocated
   int state_stride = 0;
+  size_t state_cap = 0;
+  void *resp_dev = nullptr;      // kx_respawn staging: ids then poses, resp_cap bytes
+  size_t resp_cap = 0;
   double near_frac = 0.0;        // fraction of the near bitmap set (build_grid)
   // scratch host staging
   std::vector<float> hf;
@@ -115,16 +124,32 @@ static int dalloc(kx_handle *h, T **p, size_t n) {
   *p = (T *)q;
   return 0;
 }
+// A handle buffer that grows in place of being duplicated: when `need`
+// bytes exceed `*cap`, the old allocation is released (after the handle's
+// queued work) and dropped from the handle's accounting, and a larger one
+// takes its place. Contents are not kept.
+static int regrow(kx_handle *h, void **p, size_t *cap, size_t need) {
+  if (*p && need <= *cap) return 0;
+  if (*p) {
+    HIPCHK(hipStreamSynchronize(h->st));
+    for (size_t i = 0; i < h->bufs.size(); i++)
+      if (h->bufs[i].p == *p) { h->bufs.erase(h->bufs.begin() + (long)i); break; }
+    HIPCHK(hipFree(*p));
+    *p = nullptr;
+    *cap = 0;
+  }
+  char *q = nullptr;
+  TRY(dalloc(h, &q, need));
+  *p = q;
+  *cap = need;
+  return 0;
+}
+
 template <class T>
 static int dput(T *dst, const T *src, size_t n) {
   if (n) HIPCHK(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyHostToDevice));
   return 0;
 }
-#define TRY(x)            \
-  do {                    \
-    int r_ = (x);         \
-    if (r_) return r_;    \
-  } while (0)
 
 // ── colouring (xpbd.js:98-126) ──────────────────────────────────────
 static int colour_constraints(int nB, int nC, const int32_t *ca, const int32_t *cb, const int32_t *given,
@@ -997,6 +1022,22 @@ extern "C" int kx_step(kx_handle *h, int32_t nsteps) {
   return 0;
 }
 
+extern "C" int kx_substep(kx_handle *h, int32_t n) {
+  if (!h) return fail(KX_EINVAL, "null handle");
+  if (n < 0) return fail(KX_EINVAL, "n < 0");
+  if (n == 0) return 0;
+  const int with_forces = h->forces_pending ? 1 : 0;
+  h->forces_pending = false;
+  int launches = 0;
+  HIPCHK(hipEventRecord(h->e0, h->st));
+  if (kx_plain_substeps(h->d, n, with_forces, h->st, &launches) != 0)
+    return fail(KX_EDEVICE, "substep launch failed: %s", hipGetErrorString(hipGetLastError()));
+  HIPCHK(hipEventRecord(h->e1, h->st));
+  h->last_launches = launches;
+  h->timed = true;
+  return 0;
+}
+
 extern "C" int kx_rollout(kx_handle *h, const kx_law *law, int32_t step0, int32_t nsteps, uint32_t *events_host) {
   if (!h || !law) return fail(KX_EINVAL, "null argument");
   if (!h->has_creatures) return fail(KX_ENOTSUP, "rollout needs a creature description at create");
@@ -1182,6 +1223,18 @@ extern "C" int kx_set_template_lanes(kx_handle *h, int32_t n, const int32_t *lan
   return 0;
 }
 
+// the readout buffer at `stride` floats per creature: one allocation, grown
+// (never duplicated) when a call asks for a wider stride, reused for narrower
+static int state_buffer(kx_handle *h, int stride) {
+  void *p = h->state_dev;
+  size_t cap = h->state_cap * sizeof(float);
+  TRY(regrow(h, &p, &cap, (size_t)h->nCr * stride * sizeof(float)));
+  h->state_dev = (float *)p;
+  h->state_cap = cap / sizeof(float);
+  h->state_stride = stride;
+  return 0;
+}
+
 extern "C" int kx_read_state(kx_handle *h, float *out, int32_t stride) {
   if (!h || !out) return fail(KX_EINVAL, "null argument");
   if (!h->has_creatures) return fail(KX_ENOTSUP, "state readout needs the creature structure (kx_creature_desc)");
@@ -1191,14 +1244,35 @@ extern "C" int kx_read_state(kx_handle *h, float *out, int32_t stride) {
   const int need = 6 * maxa + 6 + KX_STATE_EXTRA;
   if (stride < need) return fail(KX_EINVAL, "stride %d: the readout needs %d floats per creature", stride, need);
   const size_t n = (size_t)h->nCr * stride;
-  if (!h->state_dev || h->state_stride != stride) {
-    TRY(dalloc(h, &h->state_dev, n));
-    h->state_stride = stride;
-  }
+  TRY(state_buffer(h, stride));
   if (kx_launch_read_state(h->d, h->state_dev, stride, h->st) != 0) return fail(KX_EDEVICE, "state readout launch failed");
   HIPCHK(hipMemcpyAsync(out, h->state_dev, n * sizeof(float), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
   return need;
+}
+
+extern "C" int kx_respawn(kx_handle *h, int32_t n, const int32_t *creatures, const double *pose) {
+  if (!h || (n > 0 && (!creatures || !pose)) || n < 0) return fail(KX_EINVAL, "null argument");
+  if (!h->has_creatures) return fail(KX_ENOTSUP, "respawn needs the creature structure (kx_creature_desc)");
+  if (!h->state_why.empty()) return fail(KX_ENOTSUP, "respawn unavailable: %s", h->state_why.c_str());
+  if (n == 0) return 0;
+  std::vector<char> seen(h->nCr, 0);
+  for (int i = 0; i < n; i++) {
+    const int c = creatures[i];
+    if (c < 0 || c >= h->nCr) return fail(KX_EINVAL, "respawn: creature %d out of range [0, %d)", c, h->nCr);
+    if (seen[c]) return fail(KX_EINVAL, "respawn: creature %d listed twice", c);
+    seen[c] = 1;
+  }
+  const size_t ids_bytes = ((size_t)n * 4 + 15) / 16 * 16, need = ids_bytes + (size_t)n * 3 * sizeof(double);
+  // the staging buffer may still feed an earlier respawn on the stream
+  if (h->resp_dev) HIPCHK(hipStreamSynchronize(h->st));
+  TRY(regrow(h, &h->resp_dev, &h->resp_cap, need));
+  char *q = (char *)h->resp_dev;
+  HIPCHK(hipMemcpy(q, creatures, (size_t)n * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(q + ids_bytes, pose, (size_t)n * 3 * sizeof(double), hipMemcpyHostToDevice));
+  if (kx_launch_respawn(h->d, (const int *)q, (const double *)(q + ids_bytes), n, h->st) != 0)
+    return fail(KX_EDEVICE, "respawn launch failed");
+  return 0;
 }
 
 int kx_policy_io(kx_handle *h, int32_t stride, float **state_dev, float **tape_dev, hipStream_t *st, int32_t *n_cr,
@@ -1214,10 +1288,7 @@ int kx_policy_io(kx_handle *h, int32_t stride, float **state_dev, float **tape_d
   *n_cr = h->nCr;
   *arms = a0;
   if (!launch) return need;
-  if (!h->state_dev || h->state_stride != stride) {
-    TRY(dalloc(h, &h->state_dev, (size_t)h->nCr * stride));
-    h->state_stride = stride;
-  }
+  TRY(state_buffer(h, stride));
   const size_t row = (size_t)h->nCr * KX_TAPE_WORDS;
   if (row > h->tape_cap) {
     HIPCHK(hipStreamSynchronize(h->st));

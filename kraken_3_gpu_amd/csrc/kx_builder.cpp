@@ -134,6 +134,39 @@ extern "C" int kx_builder_find_spawn(kx_builder *B, double clearance, uint32_t *
   return 0;
 }
 
+// crnSpawns (src/sim/train.js:103-122): training spawns on a ring around the
+// walls. Each drawn slot tries up to 500 points in the map's 60 px inner
+// margin for one whose fold distance d satisfies 26 <= d <= ARM_REACH (the
+// mantle clear, a wall within the arms' reach), falls back to one unchecked
+// point, then draws the heading; pair-shared slots 2p+1 copy slot 2p. With
+// no map built the area is 240 x 240 and no point is tested.
+extern "C" int kx_builder_crn_spawns(kx_builder *B, int32_t pop, int32_t pair_shared, uint32_t *s, double *out) {
+  if (!B || !s || (pop > 0 && !out) || pop < 0) return KX_EINVAL;
+  constexpr double kClear = 26, kReach = kShoulderR + 7 * kSegLen;
+  const bool has_map = B->map_w > 0;
+  const double W = has_map ? B->map_w : 240, H = has_map ? B->map_h : 240;
+  int calls = 0;
+  for (int i = 0; i < pop; i++) {
+    double *o = out + 3 * (size_t)i;
+    if (pair_shared && (i & 1)) { std::memcpy(o, o - 3, 3 * sizeof(double)); continue; }
+    bool found = false;
+    double x = 0, y = 0, d = 0;
+    for (int t = 0; has_map && t < 500 && !found; t++, calls += 2) {
+      x = 60 + rng_next(*s) * jmax(1, W - 120);
+      y = 60 + rng_next(*s) * jmax(1, H - 120);
+      found = B->sdf(x, y, d) && d >= kClear && d <= kReach;
+    }
+    if (!found) {
+      x = 60 + rng_next(*s) * jmax(1, W - 120);
+      y = 60 + rng_next(*s) * jmax(1, H - 120);
+      calls += 2;
+    }
+    o[0] = x; o[1] = y; o[2] = rng_next(*s) * M_PI * 2;
+    calls++;
+  }
+  return calls;
+}
+
 extern "C" int kx_builder_spawn(kx_builder *B, double x, double y, double angle, int32_t arms, int32_t segs,
                                 int32_t ring, double curl) {
   if (!B || arms < 1 || arms > 8 || segs < 1 || segs > 15 || ring < 0) return KX_EINVAL;

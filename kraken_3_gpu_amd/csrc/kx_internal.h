@@ -98,8 +98,11 @@ struct KxLaw {
 // step key = (step_base ? *step_base : 0) + step_key
 // tape_row: the control-tape row of this step (KX_LAW_TAPE)
 int kx_launch_read_state(const KxDev &d, float *out, int stride, hipStream_t st);
+// in-place respawn of creatures ids[0..n) at pose[3*i..3*i+3) = x, y, angle
+int kx_launch_respawn(const KxDev &d, const int *ids, const double *pose, int n, hipStream_t st);
 int kx_plain_step(const KxDev &d, const KxLaw *law, int step_key, const int *step_base, int with_forces,
                   int ev_row, int tape_row, hipStream_t st, int *launches);
+int kx_plain_substeps(const KxDev &d, int n, int with_forces, hipStream_t st, int *launches);
 // Persistent-path launch description (see kx_persist.hip).
 struct KxPersistCfg {
   const int *wave_cr;       // creature ids of every wavefront slot, -1 = empty slot

@@ -202,6 +202,29 @@ int kx_plain_step(const KxDev &d, const KxLaw *law, int step_key, const int *ste
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 
+// n substeps without the grip break (gpuworld.js:344-372 substep), in the
+// creature-local frame like a full step; forces consumed by the first.
+int kx_plain_substeps(const KxDev &d, int n, int with_forces, hipStream_t st, int *launches) {
+  int k = 0;
+  const int nbc = d.nB > d.nC ? d.nB : d.nC;
+  hipLaunchKernelGGL(kx_k_origin, dim3(nblk(d.nCr)), dim3(PLAIN_BLOCK), 0, st, d); k++;
+  hipLaunchKernelGGL(kx_k_shift, dim3(nblk(nbc)), dim3(PLAIN_BLOCK), 0, st, d, -1.0f); k++;
+  for (int s = 0; s < n; s++) {
+    hipLaunchKernelGGL(kx_k_reset, dim3(nblk(d.nC)), dim3(PLAIN_BLOCK), 0, st, d); k++;
+    hipLaunchKernelGGL(kx_k_integrate, dim3(nblk(d.nB)), dim3(PLAIN_BLOCK), 0, st, d, with_forces && s == 0); k++;
+    for (int it = 0; it < d.I; it++) {
+      for (int c = 0; c < d.ncol; c++) {
+        hipLaunchKernelGGL(kx_k_colour, dim3(nblk(d.col_cnt[c])), dim3(PLAIN_BLOCK), 0, st, d, d.col_off[c], d.col_cnt[c]); k++;
+      }
+      hipLaunchKernelGGL(kx_k_collide, dim3(nblk(d.nB)), dim3(PLAIN_BLOCK), 0, st, d); k++;
+    }
+    hipLaunchKernelGGL(kx_k_finalize, dim3(nblk(d.nB)), dim3(PLAIN_BLOCK), 0, st, d); k++;
+  }
+  hipLaunchKernelGGL(kx_k_shift, dim3(nblk(nbc)), dim3(PLAIN_BLOCK), 0, st, d, 1.0f); k++;
+  if (launches) *launches += k;
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
 // ── state readout ───────────────────────────────────────────────────
 // readProprio (src/sim/core.js:129-175; tipPosition, armCurl, armExtension
 // src/body/kraken.js:150-183) of every creature, in binary64 from the
@@ -298,5 +321,60 @@ __global__ __launch_bounds__(64) void kx_k_read_state(KxDev d, float *out, int s
 
 int kx_launch_read_state(const KxDev &d, float *out, int stride, hipStream_t st) {
   hipLaunchKernelGGL(kx_k_read_state, dim3((d.nCr + KX_RS_CR - 1) / KX_RS_CR), dim3(64), 0, st, d, out, stride);
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+// respawnCreature (src/sim/train.js:188-229) in place: one thread per
+// (listed creature, arm), eight creatures per 64-thread block. Arm a's
+// thread re-chains the arm from 24 px out along angle + 2*pi*a/A in 13.5 px
+// segments turning 0.28 rad each (float64, then rounded to the state's
+// binary32 as the reference's upload rounds its float64 pose), zeroes its
+// velocities, sets every motor target of the arm to 0 and releases its grip
+// (broken, lambdaX/Y 0). Arm 0's thread also places the mantle and clears
+// the readout's previous mantle velocity (readProprio's prevVx/prevVy).
+// Compliances, other multipliers and reflex cooldowns stay as they were.
+// Creature ids are in range and distinct (checked on the host).
+__global__ __launch_bounds__(64) void kx_k_respawn(KxDev d, const int *ids, const double *pose, int n) {
+  const int li = blockIdx.x * (64 / KX_MAX_ARMS) + threadIdx.x / KX_MAX_ARMS, a = threadIdx.x % KX_MAX_ARMS;
+  if (li >= n) return;
+  const int c = ids[li];
+  const KxCreature cr = d.cr[c];
+  const int A = cr.narms;
+  if (a >= A) return;
+  const double x = pose[3 * li], y = pose[3 * li + 1], angle = pose[3 * li + 2];
+  const double kPi = 3.141592653589793, seg_half = 13.5 * 0.5;
+  if (a == 0) {
+    const int m = d.arm_mantle[cr.arm_off];
+    d.px[m] = (float)x; d.py[m] = (float)y; d.ppx[m] = (float)x; d.ppy[m] = (float)y;
+    d.vx[m] = 0.0f; d.vy[m] = 0.0f; d.av[m] = 0.0f;
+    d.ang[m] = (float)angle; d.angP[m] = (float)angle;
+    d.prev_v[2 * c] = 0.0f;
+    d.prev_v[2 * c + 1] = 0.0f;
+  }
+  const double th0 = angle + ((double)a / A) * kPi * 2;
+  double jx = x + cos(th0) * 24, jy = y + sin(th0) * 24, th = th0;
+  const int *so = d.arm_seg_off + cr.arm_off;
+  for (int j = so[a]; j < so[a + 1]; j++) {
+    const int b = d.arm_segs[j];
+    const double ux = cos(th), uy = sin(th);
+    const double cx = jx + ux * seg_half, cy = jy + uy * seg_half;
+    d.px[b] = (float)cx; d.py[b] = (float)cy; d.ppx[b] = (float)cx; d.ppy[b] = (float)cy;
+    d.vx[b] = 0.0f; d.vy[b] = 0.0f; d.av[b] = 0.0f;
+    d.ang[b] = (float)th; d.angP[b] = (float)th;
+    jx = cx + ux * seg_half;
+    jy = cy + uy * seg_half;
+    th += 0.28;
+  }
+  for (int k = cr.cons_off; k < cr.cons_off + cr.nc; k++)
+    if (d.ctype[k] == KX_ANGLE && d.cslot[k] >= 0 && d.cslot[k] / KX_SLOTS_PER_ARM == a) d.target[k] = 0.0f;
+  const int g = d.arm_grip[cr.arm_off + a];
+  d.broken[g] = 1;
+  d.lamx[g] = 0.0f;
+  d.lamy[g] = 0.0f;
+}
+
+int kx_launch_respawn(const KxDev &d, const int *ids, const double *pose, int n, hipStream_t st) {
+  const int per = 64 / KX_MAX_ARMS;
+  hipLaunchKernelGGL(kx_k_respawn, dim3((n + per - 1) / per), dim3(64), 0, st, d, ids, pose, n);
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }

@@ -175,6 +175,12 @@ def step(g, n=1):
     g.dispatches += n * (1 + g.cpu.S * (3 + g.cpu.I * (int((g.cpu.colorCnt > 0).sum()) + 1)))
 
 
+def substep(g, n=1):
+    """One (or n) substeps without the grip break (gpuworld.js:344-372)."""
+    K.check(K.lib().kx_substep(g.p, n))
+    g.dispatches += n * (3 + g.cpu.I * (int((g.cpu.colorCnt > 0).sum()) + 1))
+
+
 def download(g):
     """Device -> CPU world, returns {maxSpeed, maxAngSpeed, nan}
     (gpuworld.js:385-396 + mirrorInto 417-456)."""
@@ -354,6 +360,17 @@ def set_template_lanes(g, lanes):
 
 def set_plain_sdf(g, use_grid):
     K.check(K.lib().kx_set_plain_sdf(g.p, 1 if use_grid else 0))
+
+
+def respawn(g, creatures, poses):
+    """respawnCreature (src/sim/train.js:188-229) in place on the device for
+    the listed creatures (poses [n, 3]: x, y, heading); the caller's world
+    is not touched until the next download."""
+    ids = np.ascontiguousarray(creatures, dtype=np.int32)
+    pz = np.ascontiguousarray(poses, dtype=np.float64).reshape(-1, 3)
+    if len(pz) != len(ids):
+        raise ValueError(f"{len(ids)} creatures but {len(pz)} poses")
+    K.check(K.lib().kx_respawn(g.p, len(ids), K.ptr(ids, C.c_int32), K.ptr(pz, C.c_double)))
 
 
 def download_cooldown(g):

@@ -162,6 +162,17 @@ class Builder:
         rng_state[0] = st.value
         return xy
 
+    def crn_spawns(self, pop, rng_state, pair_shared=True):
+        """crnSpawns (src/sim/train.js:103-122) on this builder's map from the
+        trainer's stream state rng_state[0] (advanced in place): [pop, 3]
+        x, y, heading."""
+        st = C.c_uint32(rng_state[0])
+        out = np.zeros(3 * max(pop, 1))
+        K.check(K.lib().kx_builder_crn_spawns(self.p, pop, 1 if pair_shared else 0, C.byref(st),
+                                              K.ptr(out, C.c_double)))
+        rng_state[0] = st.value
+        return out[: 3 * pop].reshape(pop, 3)
+
     def spawn(self, x, y, angle, arms=6, segs=7, ring=8, curl=0.28):
         self.spawns.append((x, y, angle, arms, segs, ring, curl))
         return K.check(K.lib().kx_builder_spawn(self.p, x, y, angle, arms, segs, ring, curl))
@@ -241,6 +252,24 @@ def world_from_spawns(spawns, map_args, solver=None):
     for s in spawns:
         b.spawn(*s)
     return b.world(solver)
+
+
+def trainer_stream(seed):
+    """The trainer's stream state (src/sim/train.js:65-72): seed*2654435761."""
+    return [(seed * 2654435761) & 0xFFFFFFFF]
+
+
+def world_crn(pop, seed=1, map_seed=1, width=1200, height=800, arms=6, segs=7, ring=8, solver=None):
+    """An ES population on crnSpawns (src/sim/train.js:103-122): mirror pairs
+    share spawn and heading; returns the world and the spawn table."""
+    b = Builder()
+    b.map(width, height, map_seed)
+    sp = b.crn_spawns(pop, trainer_stream(seed), True)
+    for x, y, a in sp:
+        b.spawn(x, y, a, arms, segs, ring)
+    w = b.world(solver)
+    w.crn = sp
+    return w, sp
 
 
 def world_stress(n=65536, seed=1, solver=None):

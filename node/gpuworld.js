@@ -7,7 +7,9 @@
 //
 //   create(cpu, map, wgslSource?, device?, opts?) -> Promise<g>
 //   upload(g), uploadControls(g), packControls(cpu, consData)
-//   step(g) -> Promise, download(g) -> Promise<{maxSpeed, maxAngSpeed, nan}>
+//   step(g) -> Promise, substep(g) -> Promise,
+//   download(g) -> Promise<{maxSpeed, maxAngSpeed, nan}>
+//   respawn(g, creatures, poses), rollout(g, law, step0, n, events?), readState
 //   mirrorInto(cpu, bodyData, consData) -> stats, groupCollPoints(collPts, n)
 //   destroy(g), GPU_STRIDES, isAvailable()
 //
@@ -208,6 +210,26 @@ function mirrorFields(cpu, i, c, lam, lamX, lamY, broken) {
   }
 }
 
+/** One substep (gpuworld.js:344-372): lambda reset, integrate, the
+ * iterations of colour passes and collisions, finalize; no grip break. */
+function substep(g) {
+  const cpu = g.cpu;
+  let colors = 0;
+  if (cpu.colorCnt) for (let c = 0; c < MAX_COLORS; c++) if (cpu.colorCnt[c] > 0) colors++;
+  g.dispatches += 3 + cpu.iterations * (colors + 1);
+  return lib().substep(g.handle, 1);
+}
+
+/** respawnCreature (src/sim/train.js:188-229) on the device, in place of a
+ * host respawn and a full upload: creatures[i] goes to poses[3i..3i+2]
+ * (x, y, heading); arms re-chained, velocities 0, motor targets 0, grips
+ * released. The caller's world sees it at the next download. */
+function respawn(g, creatures, poses) {
+  const ids = creatures instanceof Int32Array ? creatures : Int32Array.from(creatures);
+  const p = poses instanceof Float64Array ? poses : Float64Array.from(poses);
+  lib().respawn(g.handle, ids, p);
+}
+
 /** Device -> the caller's world (gpuworld.js:385-396 + mirrorInto): bodies
  * px, py, vx, vy, ang, av; lambda and broken of every constraint; lambdaX,
  * lambdaY and force = |lambda|/h of grips. Returns the stats. */
@@ -316,6 +338,6 @@ function readState(g, out, stride) {
 
 module.exports = {
   GPU_STRIDES, C_PIN, C_ANGLE, C_GRIP,
-  isAvailable, create, upload, uploadControls, packControls, step, download, mirrorInto,
-  groupCollPoints, destroy, rollout, readState,
+  isAvailable, create, upload, uploadControls, packControls, step, substep, download, mirrorInto,
+  groupCollPoints, destroy, rollout, readState, respawn,
 };

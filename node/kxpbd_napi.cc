@@ -9,6 +9,7 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <deque>
 #include <unordered_set>
 #include <vector>
 
@@ -89,19 +90,56 @@ bool get_f64(napi_env env, napi_value obj, const char *name, double *out, double
   return napi_get_value_double(env, v, out) == napi_ok;
 }
 
-kx_handle *handle_of(napi_env env, napi_value v) {
+// A world handle's external. Asynchronous calls on one handle run one at a
+// time in call order (the WebGPU queue's FIFO of the reference); a destroy
+// while any is queued or running takes effect when the last one completes,
+// and synchronous calls are refused meanwhile instead of touching a handle a
+// worker thread is using.
+struct Work;
+struct WorldSlot {
+  kx_handle *h = nullptr;
+  std::deque<Work *> pending;   // queued, not started
+  bool running = false;         // one Work on a worker thread
+  bool destroy_pending = false;
+};
+
+WorldSlot *slot_of(napi_env env, napi_value v) {
   void *p = nullptr;
   if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
     napi_throw_error(env, nullptr, "kxpbd: not a live handle");
     return nullptr;
   }
-  return *static_cast<kx_handle **>(p);
+  WorldSlot *s = static_cast<WorldSlot *>(p);
+  if (!s->h || s->destroy_pending) {
+    napi_throw_error(env, nullptr, "kxpbd: not a live handle");
+    return nullptr;
+  }
+  return s;
+}
+
+// the handle for an asynchronous call (queued behind the handle's work)
+kx_handle *handle_of(napi_env env, napi_value v) {
+  WorldSlot *s = slot_of(env, v);
+  return s ? s->h : nullptr;
+}
+
+// the handle for a synchronous call: refused while asynchronous work on it
+// is queued or running
+kx_handle *idle_handle_of(napi_env env, napi_value v) {
+  WorldSlot *s = slot_of(env, v);
+  if (!s) return nullptr;
+  if (s->running || !s->pending.empty()) {
+    napi_throw_error(env, nullptr, "kxpbd: handle busy: await its pending step/download/rollout first");
+    return nullptr;
+  }
+  return s->h;
 }
 
 void finalize_handle(napi_env, void *data, void *) {
-  kx_handle **slot = static_cast<kx_handle **>(data);
-  if (*slot) kx_destroy(*slot);
-  delete slot;
+  // the external stays reachable from every queued Work, so no work is left
+  WorldSlot *s = static_cast<WorldSlot *>(data);
+  if (s->h) kx_destroy(s->h);
+  delete s;
 }
 
 // create(desc, params[, creatures]) -> handle; `creatures` (optional) carries
@@ -198,7 +236,8 @@ napi_value Create(napi_env env, napi_callback_info info) {
   kx_handle *h = nullptr;
   int rc = kx_create(&d, has_cr ? &cr : nullptr, &p, &h);
   if (rc < 0) return throw_kx(env, rc);
-  kx_handle **slot = new kx_handle *(h);
+  WorldSlot *slot = new WorldSlot();
+  slot->h = h;
   napi_value ext;
   if (napi_create_external(env, slot, finalize_handle, nullptr, &ext) != napi_ok) {
     kx_destroy(h);
@@ -209,16 +248,22 @@ napi_value Create(napi_env env, napi_callback_info info) {
   return ext;
 }
 
-// destroy(handle): idempotent
+// destroy(handle): idempotent; with asynchronous work queued or running the
+// release happens when the last of it completes (the handle is dead to new
+// calls at once)
 napi_value Destroy(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value a[1];
   NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
   void *p = nullptr;
   if (argc >= 1 && napi_get_value_external(env, a[0], &p) == napi_ok && p) {
-    kx_handle **slot = static_cast<kx_handle **>(p);
-    if (*slot) kx_destroy(*slot);
-    *slot = nullptr;
+    WorldSlot *slot = static_cast<WorldSlot *>(p);
+    if (slot->running || !slot->pending.empty()) {
+      slot->destroy_pending = true;
+    } else if (slot->h) {
+      kx_destroy(slot->h);
+      slot->h = nullptr;
+    }
   }
   return nullptr;
 }
@@ -227,7 +272,7 @@ napi_value Counts(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value a[1];
   NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
-  kx_handle *h = handle_of(env, a[0]);
+  kx_handle *h = idle_handle_of(env, a[0]);
   if (!h) return nullptr;
   int32_t c[4];
   int rc = kx_counts(h, &c[0], &c[1], &c[2], &c[3]);
@@ -247,7 +292,7 @@ napi_value ConstraintOrder(napi_env env, napi_callback_info info) {
   size_t argc = 2;
   napi_value a[2];
   NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
-  kx_handle *h = handle_of(env, a[0]);
+  kx_handle *h = idle_handle_of(env, a[0]);
   if (!h) return nullptr;
   int32_t nb, nc, np, ncr;
   int rc = kx_counts(h, &nb, &nc, &np, &ncr);
@@ -266,7 +311,7 @@ napi_value ColorInfo(napi_env env, napi_callback_info info) {
   size_t argc = 2;
   napi_value a[2];
   NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
-  kx_handle *h = handle_of(env, a[0]);
+  kx_handle *h = idle_handle_of(env, a[0]);
   if (!h) return nullptr;
   uint32_t *off, *cnt;
   if (!typed(env, a[1], "off", napi_uint32_array, 16, &off, true) ||
@@ -311,7 +356,7 @@ napi_value Upload(napi_env env, napi_callback_info info) {
   size_t argc = 3;
   napi_value a[3];
   NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
-  kx_handle *h = handle_of(env, a[0]);
+  kx_handle *h = idle_handle_of(env, a[0]);
   if (!h) return nullptr;
   int32_t nb, nc, np, ncr;
   int rc = kx_counts(h, &nb, &nc, &np, &ncr);
@@ -329,7 +374,7 @@ napi_value PackControls(napi_env env, napi_callback_info info) {
   size_t argc = 2;
   napi_value a[2];
   NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
-  kx_handle *h = handle_of(env, a[0]);
+  kx_handle *h = idle_handle_of(env, a[0]);
   if (!h) return nullptr;
   int32_t nb, nc, np, ncr;
   int rc = kx_counts(h, &nb, &nc, &np, &ncr);
@@ -345,9 +390,10 @@ napi_value PackControls(napi_env env, napi_callback_info info) {
 struct Work {
   napi_async_work work = nullptr;
   napi_deferred deferred = nullptr;
-  std::vector<napi_ref> keep;         // typed arrays alive until completion
+  std::vector<napi_ref> keep;         // the handle and typed arrays, alive until completion
+  WorldSlot *slot = nullptr;
   kx_handle *h = nullptr;
-  int kind = 0;                       // 0 step, 1 download, 2 rollout
+  int kind = 0;                       // 0 step, 1 download, 2 rollout, 3 substep
   int32_t nsteps = 0, step0 = 0;
   kx_body_state b{};
   kx_cons_state c{};
@@ -360,8 +406,8 @@ struct Work {
 
 void exec_work(napi_env, void *data) {
   Work *w = static_cast<Work *>(data);
-  if (w->kind == 0) {
-    w->rc = kx_step(w->h, w->nsteps);
+  if (w->kind == 0 || w->kind == 3) {
+    w->rc = w->kind == 0 ? kx_step(w->h, w->nsteps) : kx_substep(w->h, w->nsteps);
     if (w->rc >= 0) w->rc = kx_sync(w->h);
   } else if (w->kind == 1) {
     w->rc = kx_download(w->h, &w->b, &w->c, &w->st);
@@ -372,6 +418,23 @@ void exec_work(napi_env, void *data) {
   if (w->rc < 0) {
     const char *e = kx_last_error();   // thread-local: read on this thread
     w->err = "kxpbd error " + std::to_string(w->rc) + ": " + (e ? e : "");
+  }
+}
+
+// starts the handle's next queued call when none is running; releases a
+// destroyed handle once its work has drained
+void start_next(napi_env env, WorldSlot *slot) {
+  if (slot->running) return;
+  if (!slot->pending.empty()) {
+    Work *w = slot->pending.front();
+    slot->pending.pop_front();
+    slot->running = true;
+    napi_queue_async_work(env, w->work);
+    return;
+  }
+  if (slot->destroy_pending && slot->h) {
+    kx_destroy(slot->h);
+    slot->h = nullptr;
   }
 }
 
@@ -398,18 +461,29 @@ void done_work(napi_env env, napi_status, void *data) {
     }
     napi_resolve_deferred(env, w->deferred, result);
   }
+  WorldSlot *slot = w->slot;
   for (napi_ref r : w->keep) napi_delete_reference(env, r);
   napi_delete_async_work(env, w->work);
   delete w;
+  slot->running = false;
+  start_next(env, slot);
 }
 
+// `keep[0]` is the handle's external
 napi_value queue(napi_env env, Work *w, const std::vector<napi_value> &keep) {
   napi_value promise, name;
+  void *p = nullptr;
+  if (napi_get_value_external(env, keep[0], &p) != napi_ok || !p) {
+    delete w;
+    napi_throw_error(env, nullptr, "kxpbd: not a live handle");
+    return nullptr;
+  }
   if (napi_create_promise(env, &w->deferred, &promise) != napi_ok) {
     delete w;
     napi_throw_error(env, nullptr, "kxpbd: napi_create_promise failed");
     return nullptr;
   }
+  w->slot = static_cast<WorldSlot *>(p);
   for (napi_value v : keep) {
     napi_ref r;
     napi_create_reference(env, v, 1, &r);
@@ -417,7 +491,8 @@ napi_value queue(napi_env env, Work *w, const std::vector<napi_value> &keep) {
   }
   napi_create_string_utf8(env, "kxpbd", NAPI_AUTO_LENGTH, &name);
   napi_create_async_work(env, nullptr, name, exec_work, done_work, w, &w->work);
-  napi_queue_async_work(env, w->work);
+  w->slot->pending.push_back(w);
+  start_next(env, w->slot);
   return promise;
 }
 
@@ -433,6 +508,50 @@ napi_value Step(napi_env env, napi_callback_info info) {
   Work *w = new Work;
   w->h = h; w->kind = 0; w->nsteps = n;
   return queue(env, w, {a[0]});
+}
+
+// substep(handle, n) -> Promise<void> (gpuworld.js:344-372: no grip break)
+napi_value Substep(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value a[2];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
+  kx_handle *h = handle_of(env, a[0]);
+  if (!h) return nullptr;
+  int32_t n = 1;
+  if (argc >= 2) napi_get_value_int32(env, a[1], &n);
+  Work *w = new Work;
+  w->h = h; w->kind = 3; w->nsteps = n;
+  return queue(env, w, {a[0]});
+}
+
+// respawn(handle, creatures:Int32Array, poses:Float64Array(3n)) — in place on
+// the device (kx_respawn; train.js:188-229 respawnCreature)
+napi_value Respawn(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value a[3];
+  NAPI_OK(napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
+  if (argc < 3) {
+    napi_throw_type_error(env, nullptr, "kxpbd.respawn(handle, creatures, poses)");
+    return nullptr;
+  }
+  kx_handle *h = idle_handle_of(env, a[0]);
+  if (!h) return nullptr;
+  napi_typedarray_type t1, t2;
+  size_t n1, n2, off;
+  void *d1, *d2;
+  napi_value ab;
+  bool is1 = false, is2 = false;
+  napi_is_typedarray(env, a[1], &is1);
+  napi_is_typedarray(env, a[2], &is2);
+  if (!is1 || !is2 || napi_get_typedarray_info(env, a[1], &t1, &n1, &d1, &ab, &off) != napi_ok ||
+      napi_get_typedarray_info(env, a[2], &t2, &n2, &d2, &ab, &off) != napi_ok || t1 != napi_int32_array ||
+      t2 != napi_float64_array || n2 < 3 * n1) {
+    napi_throw_type_error(env, nullptr, "kxpbd.respawn: an Int32Array of creatures and a Float64Array of 3 per creature");
+    return nullptr;
+  }
+  int rc = kx_respawn(h, (int32_t)n1, static_cast<int32_t *>(d1), static_cast<double *>(d2));
+  if (rc < 0) return throw_kx(env, rc);
+  return nullptr;
 }
 
 // download(handle, bodies, cons) -> Promise<{maxSpeed, maxAngSpeed, nan}>
@@ -464,7 +583,7 @@ napi_value ReadState(napi_env env, napi_callback_info info) {
     napi_throw_type_error(env, nullptr, "kxpbd.readState(handle, {out}, stride)");
     return nullptr;
   }
-  kx_handle *h = handle_of(env, a[0]);
+  kx_handle *h = idle_handle_of(env, a[0]);
   if (!h) return nullptr;
   int32_t stride = 0, nb, nc, np, ncr;
   NAPI_OK(napi_get_value_int32(env, a[2], &stride));
@@ -492,6 +611,9 @@ struct BrainSlot {
   void *world_guard = nullptr;
   int kind = 0;
   void *obj = nullptr;
+  // shapes fixed at create: policy n, nIn, hidden, nOut, nWeights; ES
+  // nWeights and the (even) population in dims[4], dims[5]
+  int32_t dims[6] = {0, 0, 0, 0, 0, 0};
 };
 std::unordered_set<BrainSlot *> g_brain;
 
@@ -505,7 +627,7 @@ void finalize_brain(napi_env, void *data, void *) {
   delete b;
 }
 
-void *brain_of(napi_env env, napi_value v, int kind) {
+BrainSlot *brain_slot(napi_env env, napi_value v, int kind) {
   void *p = nullptr;
   BrainSlot *b = nullptr;
   if (napi_get_value_external(env, v, &p) == napi_ok && p && g_brain.count(static_cast<BrainSlot *>(p)))
@@ -514,13 +636,29 @@ void *brain_of(napi_env env, napi_value v, int kind) {
     napi_throw_error(env, nullptr, kind == KIND_POLICY ? "kxpbd: not a live policy" : "kxpbd: not a live ES");
     return nullptr;
   }
-  return b->obj;
+  return b;
 }
 
-napi_value wrap_brain(napi_env env, int kind, void *obj) {
+void *brain_of(napi_env env, napi_value v, int kind) {
+  BrainSlot *b = brain_slot(env, v, kind);
+  return b ? b->obj : nullptr;
+}
+
+// a shape argument from JS must equal the object's own (a mismatch would
+// size the typed-array checks wrongly)
+bool same_dim(napi_env env, const char *what, int32_t given, int32_t own) {
+  if (given == own) return true;
+  std::string m = std::string("kxpbd: ") + what + " " + std::to_string(given) + " does not match the object's " +
+                  std::to_string(own);
+  napi_throw_range_error(env, nullptr, m.c_str());
+  return false;
+}
+
+napi_value wrap_brain(napi_env env, int kind, void *obj, const int32_t *dims) {
   BrainSlot *b = new BrainSlot();
   b->kind = kind;
   b->obj = obj;
+  for (int i = 0; i < 6; i++) b->dims[i] = dims[i];
   napi_value ext;
   if (napi_create_external(env, b, finalize_brain, nullptr, &ext) != napi_ok) {
     if (kind == KIND_POLICY) kx_policy_destroy(static_cast<kx_policy *>(obj));
@@ -566,7 +704,8 @@ napi_value PolicyCreate(napi_env env, napi_callback_info info) {
   kx_policy *p = nullptr;
   int rc = kx_policy_create(v[0], v[1], v[2], v[3], &p);
   if (rc < 0) return throw_kx(env, rc);
-  return wrap_brain(env, KIND_POLICY, p);
+  const int32_t dims[6] = {v[0], v[1], v[2], v[3], kx_policy_weights(v[1], v[2], v[3]), 0};
+  return wrap_brain(env, KIND_POLICY, p, dims);
 }
 
 // policyWeights(nIn, hidden, nOut) -> weightCount (net.js:30-32)
@@ -591,12 +730,14 @@ napi_value PolicyGenomes(napi_env env, napi_callback_info info) {
     napi_throw_type_error(env, nullptr, "kxpbd.policyGenomes(policy, {genomes}, upload, n, nWeights)");
     return nullptr;
   }
-  kx_policy *p = static_cast<kx_policy *>(brain_of(env, a[0], KIND_POLICY));
-  if (!p) return nullptr;
+  BrainSlot *b = brain_slot(env, a[0], KIND_POLICY);
+  if (!b) return nullptr;
+  kx_policy *p = static_cast<kx_policy *>(b->obj);
   bool up = false;
   NAPI_OK(napi_get_value_bool(env, a[2], &up));
   int32_t n = 0, nw = 0;
   if (!get_i32(env, a[3], &n) || !get_i32(env, a[4], &nw)) return nullptr;
+  if (!same_dim(env, "population", n, b->dims[0]) || !same_dim(env, "weights", nw, b->dims[4])) return nullptr;
   float *g;
   if (!typed(env, a[1], "genomes", napi_float32_array, (size_t)n * (size_t)nw, &g, true)) return nullptr;
   int rc = up ? kx_policy_set_genomes(p, g) : kx_policy_get_genomes(p, g);
@@ -614,11 +755,15 @@ napi_value PolicyForward(napi_env env, napi_callback_info info) {
     napi_throw_type_error(env, nullptr, "kxpbd.policyForward(policy, {x, out, scratch, controls}, arms, n, nIn, nOut, hidden)");
     return nullptr;
   }
-  kx_policy *p = static_cast<kx_policy *>(brain_of(env, a[0], KIND_POLICY));
-  if (!p) return nullptr;
+  BrainSlot *b = brain_slot(env, a[0], KIND_POLICY);
+  if (!b) return nullptr;
+  kx_policy *p = static_cast<kx_policy *>(b->obj);
   int32_t arms, n, n_in, n_out, hidden;
   if (!get_i32(env, a[2], &arms) || !get_i32(env, a[3], &n) || !get_i32(env, a[4], &n_in) ||
       !get_i32(env, a[5], &n_out) || !get_i32(env, a[6], &hidden))
+    return nullptr;
+  if (!same_dim(env, "population", n, b->dims[0]) || !same_dim(env, "inputs", n_in, b->dims[1]) ||
+      !same_dim(env, "hidden", hidden, b->dims[2]) || !same_dim(env, "outputs", n_out, b->dims[3]))
     return nullptr;
   float *x, *out, *scratch, *controls;
   if (!typed(env, a[1], "x", napi_float32_array, (size_t)n * n_in, &x, true) ||
@@ -660,7 +805,9 @@ napi_value EsCreate(napi_env env, napi_callback_info info) {
   kx_es *e = nullptr;
   int rc = kx_es_create(nw, &o, theta, &e);
   if (rc < 0) return throw_kx(env, rc);
-  return wrap_brain(env, KIND_ES, e);
+  // the library keeps the population even (es.js:37)
+  const int32_t dims[6] = {0, 0, 0, 0, nw, o.pop_size + (o.pop_size & 1)};
+  return wrap_brain(env, KIND_ES, e, dims);
 }
 
 // esGeneration(es, policy, rndState) -> rndState after the draws
@@ -705,10 +852,12 @@ napi_value EsUpdate(napi_env env, napi_callback_info info) {
     napi_throw_type_error(env, nullptr, "kxpbd.esUpdate(es, {fitness}, popSize)");
     return nullptr;
   }
-  kx_es *e = static_cast<kx_es *>(brain_of(env, a[0], KIND_ES));
-  if (!e) return nullptr;
+  BrainSlot *b = brain_slot(env, a[0], KIND_ES);
+  if (!b) return nullptr;
+  kx_es *e = static_cast<kx_es *>(b->obj);
   int32_t pop;
   if (!get_i32(env, a[2], &pop)) return nullptr;
+  if (!same_dim(env, "popSize", pop, b->dims[5])) return nullptr;
   double *fit;
   if (!typed(env, a[1], "fitness", napi_float64_array, (size_t)pop, &fit, true)) return nullptr;
   double st[4];
@@ -727,10 +876,12 @@ napi_value EsGet(napi_env env, napi_callback_info info) {
     napi_throw_type_error(env, nullptr, "kxpbd.esGet(es, {theta, m, v, bestTheta}, nWeights)");
     return nullptr;
   }
-  kx_es *e = static_cast<kx_es *>(brain_of(env, a[0], KIND_ES));
-  if (!e) return nullptr;
+  BrainSlot *b = brain_slot(env, a[0], KIND_ES);
+  if (!b) return nullptr;
+  kx_es *e = static_cast<kx_es *>(b->obj);
   int32_t nw;
   if (!get_i32(env, a[2], &nw)) return nullptr;
+  if (!same_dim(env, "nWeights", nw, b->dims[4])) return nullptr;
   float *th, *m, *v, *best;
   if (!typed(env, a[1], "theta", napi_float32_array, (size_t)nw, &th, false) ||
       !typed(env, a[1], "m", napi_float32_array, (size_t)nw, &m, false) ||
@@ -877,6 +1028,8 @@ napi_value Init(napi_env env, napi_value exports) {
       {"upload", nullptr, Upload, nullptr, nullptr, nullptr, kProp, nullptr},
       {"packControls", nullptr, PackControls, nullptr, nullptr, nullptr, kProp, nullptr},
       {"step", nullptr, Step, nullptr, nullptr, nullptr, kProp, nullptr},
+      {"substep", nullptr, Substep, nullptr, nullptr, nullptr, kProp, nullptr},
+      {"respawn", nullptr, Respawn, nullptr, nullptr, nullptr, kProp, nullptr},
       {"download", nullptr, Download, nullptr, nullptr, nullptr, kProp, nullptr},
       {"rollout", nullptr, Rollout, nullptr, nullptr, nullptr, kProp, nullptr},
       {"readState", nullptr, ReadState, nullptr, nullptr, nullptr, kProp, nullptr},

@@ -53,6 +53,12 @@ void     or_world_set_limits(OrWorld *w, double linDrag, double angDrag, double 
 int      or_spawn_kraken(OrWorld *w, double x, double y, double angle, int arms, int segs, int ring, double curl);
 int      or_find_spawn(OrWorld *w, double clearance, uint32_t *rng_state, double *xy);
 double   or_rng_next(uint32_t *rng_state);     /* mulberry32 with explicit state */
+/* crnSpawns (train.js:103-122) on the world's map (has_map 0: a 240 x 240
+ * area, no test); out[3*pop] = x, y, angle; returns the stream calls made */
+int      or_crn_spawns(OrWorld *w, int has_map, int pop, int pair_shared, uint32_t *rng_state, double *out);
+/* respawnCreature (train.js:188-229): pose chain, velocities, motor targets,
+ * grips released; cooldowns untouched */
+void     or_respawn(OrWorld *w, int ci, double x, double y, double angle);
 void     or_ensure_colored(OrWorld *w);
 void     or_permute_cons(OrWorld *w, const int *order);
 int      or_counts(const OrWorld *w, int *out4);            /* n, nc, np, ncreatures */

@@ -324,6 +324,74 @@ int or_find_spawn(OrWorld *w, double clearance, uint32_t *s, double *xy) {
   return 0;
 }
 
+/* crnSpawns (train.js:103-122): per slot, up to 500 draws of a point in the
+ * map's 60 px inner margin whose fold distance lies in [SPAWN_CLEAR,
+ * ARM_REACH] (a wall within reach, the mantle clear), else an unchecked draw;
+ * then the heading. With pair_shared the odd slot repeats its even
+ * neighbour's spawn and draws nothing. Without a map (has_map 0) the area is
+ * 240 x 240 and no point is tested. out: x, y, angle per slot. Returns the
+ * number of stream calls made. */
+#define OR_SPAWN_CLEAR 26.0
+#define OR_ARM_REACH (B_SHOULDER_R + 7 * B_SEG_LEN)
+int or_crn_spawns(OrWorld *w, int has_map, int pop, int pair_shared, uint32_t *s, double *out) {
+  double W = has_map ? w->map_w : 240, H = has_map ? w->map_h : 240, hit[3];
+  int calls = 0;
+  for (int i = 0; i < pop; i++) {
+    if (pair_shared && (i & 1)) { memcpy(out + 3 * i, out + 3 * (i - 1), 3 * sizeof(double)); continue; }
+    int found = 0;
+    double x = 0, y = 0;
+    if (has_map) {
+      for (int t = 0; t < 500 && !found; t++) {
+        x = 60 + or_rng_next(s) * or_js_max(1, W - 120);
+        y = 60 + or_rng_next(s) * or_js_max(1, H - 120);
+        calls += 2;
+        if (or_sdf(w->rects, w->nr, x, y, hit) && hit[0] >= OR_SPAWN_CLEAR && hit[0] <= OR_ARM_REACH) found = 1;
+      }
+    }
+    if (!found) {
+      x = 60 + or_rng_next(s) * or_js_max(1, W - 120);
+      y = 60 + or_rng_next(s) * or_js_max(1, H - 120);
+      calls += 2;
+    }
+    out[3 * i] = x; out[3 * i + 1] = y; out[3 * i + 2] = or_rng_next(s) * M_PI * 2;
+    calls++;
+  }
+  return calls;
+}
+
+/* respawnCreature (train.js:188-229), physics part: the mantle at (x, y,
+ * angle), every arm re-chained from 24 px out along angle + 2*pi*a/arms in
+ * 13.5 px segments turning 0.28 rad each; poses and predictions set,
+ * velocities zeroed; every motor target 0; every grip broken with force and
+ * lambdaX/Y cleared. Reflex cooldowns, compliances and the other
+ * multipliers stay as they were (the reference leaves them). */
+void or_respawn(OrWorld *w, int ci, double x, double y, double angle) {
+  OrCreature *K = &w->cr[ci];
+  int m = K->mantle;
+  PX[m] = x; PY[m] = y; PPX[m] = x; PPY[m] = y; VX[m] = 0; VY[m] = 0;
+  ANG[m] = angle; ANGP[m] = angle; AV[m] = 0;
+  for (int a = 0; a < K->arms; a++) {
+    OrArm *arm = &K->arm[a];
+    double relAng = ((double)a / K->arms) * M_PI * 2;
+    double th0 = angle + relAng;
+    double jx = x + cos(th0) * 24, jy = y + sin(th0) * 24;
+    double th = th0;
+    for (int i = 0; i < K->segs; i++) {
+      int b = K->first_body + 1 + a * K->segs + i;
+      double ux = cos(th), uy = sin(th);
+      double cx = jx + ux * 13.5 * 0.5, cy = jy + uy * 13.5 * 0.5;
+      PX[b] = cx; PY[b] = cy; PPX[b] = cx; PPY[b] = cy;
+      VX[b] = 0; VY[b] = 0;
+      ANG[b] = th; ANGP[b] = th; AV[b] = 0;
+      jx = cx + ux * 13.5 * 0.5; jy = cy + uy * 13.5 * 0.5;
+      th += 0.28;
+    }
+    for (int j = 0; j < arm->nmotors; j++) CONS_ID(w, arm->motor[j])->target = 0;
+    OrCons *g = CONS_ID(w, arm->grip);
+    g->broken = 1; g->force = 0; g->lambdaX = 0; g->lambdaY = 0;
+  }
+}
+
 /* ── colouring (xpbd.js:98-126) ──────────────────────────────────── */
 void or_ensure_colored(OrWorld *w) {
   if (w->colored) return;

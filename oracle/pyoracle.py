@@ -47,6 +47,8 @@ def lib():
             "or_spawn_kraken": (i, [P, d, d, d, i, i, i, d]),
             "or_find_spawn": (i, [P, d, C.POINTER(u32), dp]),
             "or_rng_next": (d, [C.POINTER(u32)]),
+            "or_crn_spawns": (i, [P, i, i, i, C.POINTER(u32), dp]),
+            "or_respawn": (None, [P, i, d, d, d]),
             "or_ensure_colored": (None, [P]),
             "or_permute_cons": (None, [P, ip]),
             "or_counts": (i, [P, ip]),
@@ -147,6 +149,20 @@ class World:
         lib().or_find_spawn(self.p, clearance, C.byref(st), _dp(xy))
         rng_state[0] = st.value
         return xy
+
+    def crn_spawns(self, pop, rng_state, pair_shared=True, has_map=True):
+        """crnSpawns (src/sim/train.js:103-122) from the stream state
+        rng_state[0] (advanced in place): [pop, 3] x, y, angle, and the
+        number of stream calls made."""
+        st = C.c_uint32(rng_state[0])
+        out = np.zeros(3 * max(pop, 1))
+        calls = lib().or_crn_spawns(self.p, 1 if has_map else 0, pop, 1 if pair_shared else 0, C.byref(st), _dp(out))
+        rng_state[0] = st.value
+        return out[: 3 * pop].reshape(pop, 3), calls
+
+    def respawn(self, ci, x, y, angle):
+        """respawnCreature (src/sim/train.js:188-229) of creature ci."""
+        lib().or_respawn(self.p, ci, x, y, angle)
 
     def color(self):
         lib().or_ensure_colored(self.p)

@@ -119,6 +119,32 @@ async function gpu(job) {
     out.rollout = Object.assign(state(cpu2), { events: Array.from(ev), state: Array.from(st), stateStride: wrote });
     gw.destroy(g2);
   }
+  // respawn in place then one substep (train.js:188-229; gpuworld.js:344-372)
+  if (job.respawn) {
+    const cpu3 = makeWorld(job.world);
+    const cr = {};
+    for (const [k, v] of Object.entries(job.creatures)) cr[k] = Array.isArray(v) ? Int32Array.from(v) : v;
+    const g3 = await gw.create(cpu3, { rects: job.world.rects }, null, null, { creatures: cr });
+    gw.upload(g3);
+    await gw.rollout(g3, job.law, 0, job.respawn.warm);
+    gw.respawn(g3, job.respawn.creatures, job.respawn.poses);
+    await gw.substep(g3);
+    await gw.download(g3);
+    out.respawn = state(cpu3);
+    // handle lifetime: calls queue in order; a synchronous call on a busy
+    // handle is refused; destroy while work is queued releases after it
+    const k = require(path.join(__dirname, '..', 'node', 'kxpbd.node'));
+    const h = g3.handle;
+    const p1 = k.step(h, 2);
+    const p2 = k.substep(h, 1);
+    try { k.counts(h); out.busy = 'no throw'; } catch (e) { out.busy = String(e.message); }
+    k.destroy(h);
+    try { k.step(h, 1); out.dead = 'no throw'; } catch (e) { out.dead = String(e.message); }
+    await p1;
+    await p2;
+    out.drained = true;
+    g3.handle = null;
+  }
   // error behaviour: a bad body index throws like the reference's guard
   const bad = makeWorld(job.world);
   bad.constraints[0].a = bad.n + 5;
@@ -164,6 +190,13 @@ function brainJob(job) {
   }
   // misuse throws like the rest of the surface
   try { B.beginGeneration(es, pop, 1); out.deadThrows = false; } catch (e) { out.deadThrows = String(e.message); }
+  // a shape that is not the object's own is refused before any copy
+  const small = Object.assign({}, epop, { n: 1 });
+  try {
+    B.forwardInto(small, new Float32Array(job.nIn), new Float32Array(B.OUT_DIM));
+    out.shapeThrows = false;
+  } catch (e) { out.shapeThrows = String(e.message); }
+  try { B.update(Object.assign({}, es, { popSize: 2 }), [1, 2]); out.popThrows = false; } catch (e) { out.popThrows = String(e.message); }
   B.destroy(es);
   B.destroy(epop);
   return out;

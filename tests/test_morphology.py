@@ -344,3 +344,108 @@ def test_read_proprio_straight_arms_closed_form(A, S):
     want += [0.0] * A + [0.0] * A + [0.0, 0.0, 0.0, ext, 0.0, 0.0]
     np.testing.assert_allclose(out, want, atol=1e-12)
     assert abs(side[0] - ext) < 1e-12 and abs(side[1]) < 1e-12 and side[2] == 0
+
+
+# ── spawn draws and in-place respawn (src/sim/train.js:103-122, 188-229) ──
+
+def test_crn_spawns_match_reference(golden):
+    """crnSpawns from the trainer's stream: every slot's x, y, heading and
+    the number of stream calls, pair-shared and not, with and without a map."""
+    for case in golden("respawn.json")["crn"]:
+        o = po.World()
+        if case["opts"] is not None:
+            op = case["opts"]
+            rects = po.make_map(case["width"], case["height"], op["seed"], op.get("corridorCols", 4),
+                                op.get("corridorRows", 3))
+            o.set_map(case["width"], case["height"], rects)
+        st = [case["s0"]]
+        sp, calls = o.crn_spawns(case["pop"], st, case["pairShared"], has_map=case["opts"] is not None)
+        assert sp.tolist() == case["spawns"], case["seed"]
+        if case["calls"] is not None:
+            assert calls == case["calls"]
+        if case["pairShared"]:
+            assert np.array_equal(sp[0::2][: len(sp[1::2])], sp[1::2])
+
+
+def _creature_of(o, bodies):
+    """Creature index of every body index in `bodies`."""
+    first = [o.creature(i)["first_body"] for i in range(o.counts()[3])]
+    return np.searchsorted(first, bodies, side="right") - 1
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_respawn_matches_reference(golden, which):
+    """respawnCreature on a world 40 steps into the seeded law (grips held,
+    joints driven, bodies moving): the respawned creatures' full state, the
+    untouched creature, multipliers and cooldowns, then 8 more steps."""
+    r = golden("respawn.json")["respawn"][which]
+    A, S = r["arms"], r["segs"]
+    o = po.World(4, 8, 1 / 240)
+    o.set_map(1200, 800, np.array(r["mapRects"], dtype=np.float64))
+    for x, y, a in r["spawns"]:
+        o.spawn(x, y, a, A, S, 8, 0.28)
+    o.color()
+    assert o.cons()["color"].tolist() == r["colorOf"]
+
+    def check(ref, tol, cd=None):
+        b, p = o.bodies(), o.pred()
+        for key, got in (("px", b["px"]), ("py", b["py"]), ("ang", b["ang"]), ("vx", b["vx"]), ("vy", b["vy"]),
+                         ("av", b["av"]), ("ppx", p["ppx"]), ("ppy", p["ppy"]), ("angP", p["angP"])):
+            t = tol if key in ("px", "py", "ang", "ppx", "ppy", "angP") else 100 * tol
+            assert np.abs(got - np.array(ref[key])).max() <= t, key
+        c = o.cons()
+        g = np.array([[v if v is not None else np.inf for v in row] for row in ref["cons"]])
+        assert c["broken"].tolist() == g[:, 1].astype(int).tolist()
+        for j, key in ((0, "target"), (6, "compliance"), (9, "breakForce")):
+            assert np.allclose(c[key], g[:, j], rtol=1e-12, atol=100 * tol), key
+        grip = c["type"] == po.GRIP
+        for j, key in ((3, "lambdaX"), (4, "lambdaY")):
+            assert np.abs(c[key][grip] - g[grip, j]).max() <= 1e4 * tol, key
+        if cd is not None:
+            assert np.array_equal(o.cd()[:, :A], np.array(cd)), "cooldowns"
+
+    for s in range(40):
+        o.apply_law(r["law"], s)
+        o.step()
+    check(r["before"], 1e-8, r["cdBefore"])
+    # continue from the reference's own state, so the untouched creature and
+    # the respawned poses compare to the last bit of libm vs V8
+    o.set_bodies(*(np.array(r["before"][k]) for k in ("px", "py", "ang", "vx", "vy", "av")))
+    o.set_pred(*(np.array(r["before"][k]) for k in ("ppx", "ppy", "angP")))
+    for ci, x, y, a in r["respawned"]:
+        o.respawn(ci, x, y, a)
+    b1, p1 = o.bodies(), o.pred()
+    for key in ("px", "py", "ang", "vx", "vy", "av"):
+        assert np.abs(b1[key] - np.array(r["after"][key])).max() <= 1e-12, key
+    for key in ("ppx", "ppy", "angP"):
+        assert np.abs(p1[key] - np.array(r["after"][key])).max() <= 1e-12, key
+    c = o.cons()
+    g = np.array([[v if v is not None else np.inf for v in row] for row in r["after"]["cons"]])
+    assert c["broken"].tolist() == g[:, 1].astype(int).tolist()
+    assert c["target"].tolist() == g[:, 0].tolist()
+    grip = c["type"] == po.GRIP
+    released = grip & np.isin(_creature_of(o, c["a"]), [ci for ci, *_ in r["respawned"]])
+    assert released.sum() == A * len(r["respawned"])
+    assert not np.any(c["force"][released]) and not np.any(c["lambdaX"][released]) and not np.any(c["lambdaY"][released])
+    assert np.all(c["broken"][released] == 1) and not np.any(g[released, 2])
+    assert np.array_equal(o.cd()[:, :A], np.array(r["cdAfter"]))
+    assert r["cdAfter"] == r["cdBefore"] and r["prevAfter"] == [[0, 0]] * 3
+    for s in range(40, 48):
+        o.apply_law(r["law"], s)
+        o.step()
+    check(r["later"], 1e-7, r["laterCd"])
+
+
+def test_product_crn_spawns_match_reference(golden):
+    """The product builder's crnSpawns against the reference's draws."""
+    for case in golden("respawn.json")["crn"]:
+        b = W.Builder()
+        if case["opts"] is not None:
+            op = case["opts"]
+            b.map(case["width"], case["height"], op["seed"], op.get("corridorCols", 4), op.get("corridorRows", 3))
+        st = [case["s0"]]
+        sp = b.crn_spawns(case["pop"], st, case["pairShared"])
+        assert sp.tolist() == case["spawns"], case["seed"]
+    w, sp = W.world_crn(8, seed=1)
+    assert w.n_creatures == 8 and np.array_equal(sp[0::2], sp[1::2])
+    assert np.array_equal(w.px[[0, 43]], sp[:2, 0])

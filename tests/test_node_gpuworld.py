@@ -51,13 +51,13 @@ def test_addon_loads_and_exports_the_surface():
                        capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stderr
     d = json.loads(r.stdout)
-    assert d["abi"] == 3 and d["avail"] is True
+    assert d["abi"] == 4 and d["avail"] is True
     for name in ("create", "destroy", "upload", "packControls", "step", "download", "rollout", "groupPoints",
                  "readState", "policyCreate", "policyWeights", "policyGenomes", "policyForward", "esCreate",
                  "esGeneration", "esUpdate", "esGet", "brainDestroy"):
         assert name in d["k"]
-    for name in ("create", "upload", "uploadControls", "packControls", "step", "download", "mirrorInto",
-                 "groupCollPoints", "destroy", "GPU_STRIDES", "isAvailable", "readState"):
+    for name in ("create", "upload", "uploadControls", "packControls", "step", "substep", "download", "mirrorInto",
+                 "groupCollPoints", "destroy", "GPU_STRIDES", "isAvailable", "readState", "respawn"):
         assert name in d["g"]
 
 
@@ -117,8 +117,9 @@ def test_node_loop_and_rollout_match_c_abi(tmp_path):
                 releaseBelow=law.release_below)
     tg = _targets(steps, w0.n_creatures)
     stride = 6 * int(np.diff(np.asarray(w0.arm_off)).max()) + 6 + K.KX_STATE_EXTRA
+    resp = dict(warm=12, creatures=[0, 5, 2], poses=[300.25, 410.5, 1.0, 820.0, 200.0, 5.5, 640.0, 480.0, 3.0])
     job = dict(world=world_to_js(w0), creatures=creatures_to_js(w0), steps=steps, targets=tg, law=lawd,
-               rolloutSteps=rsteps, stateStride=stride)
+               rolloutSteps=rsteps, stateStride=stride, respawn=resp)
     r = run_node("gpu", job, tmp_path, timeout=240)
 
     # the same through the C ABI
@@ -157,6 +158,21 @@ def test_node_loop_and_rollout_match_c_abi(tmp_path):
     assert np.array_equal(np.array(ro["state"], dtype=np.float32).reshape(rs.shape), rs)
     print(f"    node rollout identical over {rsteps} steps, {int((ev != 0).sum())} event words", flush=True)
     assert r["badThrows"] is True and "kxpbd error" in r["badMessage"]
+
+    # respawn + substep through Node == through the C ABI
+    w3 = mk()
+    g3 = G.create(w3)
+    G.upload(g3)
+    G.rollout(g3, law, 0, resp["warm"])
+    G.respawn(g3, resp["creatures"], np.array(resp["poses"]).reshape(-1, 3))
+    G.substep(g3)
+    G.download(g3)
+    G.destroy(g3)
+    for key in ("px", "py", "vx", "vy", "ang", "av"):
+        assert np.array_equal(np.array(r["respawn"][key]), getattr(w3, key)), key
+    assert r["respawn"]["broken"] == w3.broken.astype(int).tolist()
+    # handle lifetime
+    assert "busy" in r["busy"] and "not a live handle" in r["dead"] and r["drained"] is True
 
 
 @needs_node
@@ -217,6 +233,7 @@ def test_node_brain_matches_c_abi(tmp_path):
             for k in ("best", "mean", "sd", "sigma"):
                 assert g["stats"][k] == stats[k], k
     assert "not a live policy" in r["deadThrows"]
+    assert "does not match" in r["shapeThrows"] and "does not match" in r["popThrows"]
     print("    node brain identical: 3 forward steps, 2 ES generations", flush=True)
 
 
