@@ -93,6 +93,12 @@ __device__ __forceinline__ void kx_fold_rect(float x, float y, float4 r, float o
 
 __device__ __forceinline__ void kx_hit_init(KxHit &h) { h.d = INFINITY; h.ux = 0.0f; h.uy = 0.0f; h.ext = false; }
 
+// A creature's integer origin for a step: its first body's coordinate
+// rounded; 0 when that coordinate is not finite, so a non-finite body stays
+// alone (the reference's solves skip it) instead of shifting its whole
+// creature by NaN.
+__device__ __forceinline__ float kx_origin(float v) { return isfinite(v) ? rintf(v) : 0.0f; }
+
 // World-frame offset of a body-local point: (x, y) + R(angle) (lx, ly).
 __device__ __forceinline__ void kx_world_point(float x, float y, float ca, float sa, float lx, float ly, float &wx,
                                                float &wy) {

@@ -350,10 +350,11 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
     }
     // Between steps the state is held in global binary32 coordinates, as the
     // reference's buffers hold it; a step works relative to the integer
-    // origin rint(first body), subtracted before and added back after.
+    // origin rint(first body) (0 if that is not finite), subtracted before
+    // and added back after.
     // (body 0 is lane 0's first body: its origin reaches the other lanes
     // through two spare event words)
-    if (act && sub == 0) { evw[2] = __float_as_uint(rintf(rx[0])); evw[3] = __float_as_uint(rintf(ry[0])); }
+    if (act && sub == 0) { evw[2] = __float_as_uint(kx_origin(rx[0])); evw[3] = __float_as_uint(kx_origin(ry[0])); }
     KX_WAVE_SYNC();
     const float ox = act ? __uint_as_float(evw[2]) : 0.0f, oy = act ? __uint_as_float(evw[3]) : 0.0f;
 #pragma unroll

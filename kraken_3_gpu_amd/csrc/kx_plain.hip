@@ -56,13 +56,14 @@ __global__ __launch_bounds__(PLAIN_BLOCK) void kx_k_motors(KxDev d, KxLaw L, int
   d.comp[k] = (float)L.comp;
 }
 
-// Integer origin per creature: its first body's position, rounded.
+// Integer origin per creature: its first body's position, rounded (0 when
+// not finite).
 __global__ __launch_bounds__(PLAIN_BLOCK) void kx_k_origin(KxDev d) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= d.nCr) return;
   int b = d.cr[c].body_off;
-  d.org[2 * c] = rintf(d.px[b]);
-  d.org[2 * c + 1] = rintf(d.py[b]);
+  d.org[2 * c] = kx_origin(d.px[b]);
+  d.org[2 * c + 1] = kx_origin(d.py[b]);
 }
 
 // Shift bodies and world anchors by -origin (sign=-1) or +origin (sign=+1).

@@ -559,6 +559,53 @@ def test_stress_map_full_size_properties_and_subset_parity():
     G.destroy(g)
 
 
+def test_uneven_8192_autotuned_graph_subset_parity():
+    """The uneven workload as benchmarked: 8192 creatures of 125 shapes, the
+    autotuner's per-bucket lane counts, waves per workgroup and steps per
+    launch, a hipGraph-captured launch replayed; then one step of a
+    64-creature subset (every shape bucket) against the oracle under the same
+    law, and limits / NaN over the whole batch."""
+    from kraken_3_gpu_amd import autotune as T
+    seed = 13
+    w = W.world_uneven(8192, seed=1)
+    law = W.default_law(seed)
+    say("[case] uneven 8192 autotune")
+    g, rep = T.tune_buckets(w, law, steps=4, reps=1, passes=1, tiles=(25, 50))
+    tile = rep["steps_per_launch"]
+    lanes = [b["lanes"] for b in rep["buckets"]]
+    say(f"    tuned lanes {lanes}, waves {rep['waves_per_block']}, tile {tile}, mixed "
+        f"{rep['launch_shape']['mixed_lanes']}")
+    say("[case] uneven 8192 graph replay")
+    G.rollout_graph(g, law, 0, tile, 3)
+    steps = 3 * tile
+    st = G.download(g)
+    assert st["nan"] == 0
+    assert st["maxSpeed"] <= 360 * 1.001 and st["maxAngSpeed"] <= 18 * 1.001
+    _pull_state(g)
+    sub = list(range(0, 8192, 128))
+    o, bidx, cidx = oracle_subset(w, sub)
+    load_state_subset(o, w, sub, bidx, cidx)
+    o.set_step_counter(steps)
+    o.clear_events()
+    _apply_law_keyed(o, law.seed, steps, sub)
+    o.step()
+    say("[case] uneven 8192 one more step")
+    ev = G.rollout(g, law, steps, 1, events=True)
+    G.download(g)
+    ev_gpu = {(0, sub.index(c), a, k) for (_, c, a, k) in decode_events(ev) if c in sub}
+    ev_or = {(0, c, a, k) for (_, c, a, k) in o.events()}
+    flips = {e[1] for e in ev_gpu ^ ev_or}
+    b = o.bodies()
+    err = np.maximum(np.abs(w.px[bidx] - b["px"]), np.abs(w.py[bidx] - b["py"]))
+    crb = np.concatenate([np.full(1 + w.spawns[c][3] * w.spawns[c][4], i) for i, c in enumerate(sub)])
+    ok = ~np.isin(crb, list(flips))
+    shapes = {tuple(w.spawns[c][3:6]) for c in sub}
+    say(f"    uneven subset ({len(shapes)} shapes) max|dpos|={err[ok].max():.3e} flips={sorted(flips)}")
+    assert len(flips) <= 1
+    assert err[ok].max() <= TOL_PX
+    G.destroy(g)
+
+
 def _apply_law_keyed(o, seed, step, keys):
     """or_apply_law keys the PRNG by the oracle's own creature index; a
     subset needs the batch index, so remap through a seed-preserving loop."""

@@ -14,6 +14,7 @@ from kraken_3_gpu_amd import _lib as K
 from kraken_3_gpu_amd import gpuworld as G
 from kraken_3_gpu_amd import serve as S
 from kraken_3_gpu_amd import world as W
+from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
 KEYS = ("px", "py", "ang", "vx", "vy", "av")
@@ -98,6 +99,97 @@ def test_each_result_same_alone_and_in_mix():
         for k in KEYS:
             assert np.array_equal(a["final_state"][k], m["final_state"][k]), (d["tag"], k)
         assert np.array_equal(a["fitness"], m["fitness"])
+
+
+def _oracle_request(d):
+    """The request's population in the oracle: 2*pairs creatures of its
+    shape at its pose on the server's map (binary32 rect values), the
+    starting poses rounded to binary32 as the server's upload rounds them."""
+    s = W.SOLVER
+    o = po.World(s["substeps"], s["iterations"], s["h"])
+    m = S.MAP
+    rects = po.make_map(m["width"], m["height"], m["seed"], m["cols"], m["rows"], m["density"], m["wall"])
+    o.set_map(m["width"], m["height"], np.float32(rects).astype(np.float64))
+    for _ in range(2 * d["pairs"]):
+        o.spawn(d["x"], d["y"], d["angle"], d["arms"], d["segs"], d["ring"])
+    o.color()
+    b = o.bodies()
+    z = np.zeros(o.n)
+    f = lambda a: np.float32(a).astype(np.float64)
+    o.set_bodies(f(b["px"]), f(b["py"]), f(b["ang"]), z, z, z)
+    return o
+
+
+def _oracle_steps(o, d, step0, n):
+    """n steps of applyControl (src/sim/core.js:183-220) on each creature's
+    control-tape record from the request's policy, then the physics step."""
+    tape = S.policy_tape(d, step0, n).astype(np.float64)
+    A = d["arms"]
+    for s in range(n):
+        for ci in range(2 * d["pairs"]):
+            r = tape[s, ci]
+            o.apply_control(ci, r[0], r[1::3][:A], r[2::3][:A], r[3::3][:A])
+        o.step()
+
+
+def test_served_requests_match_the_oracle_one_step():
+    """Three requests of different shape served together in one batched
+    launch, one step each: every served creature within 1e-3 px of the
+    oracle's applyControl of the same policy record and step."""
+    reqs = [_req(11, 4, 5, 6, 4, 1, x=420, y=330, angle=0.7), _req(12, 8, 9, 10, 2, 1, x=610, y=420, angle=2.0),
+            _req(13, 5, 8, 7, 3, 1, x=300, y=500, angle=4.1)]
+    say("[case] served one-step requests vs oracle")
+    with S.Server(S.default_opts(chunk_steps=50)) as srv:
+        for d in reqs:
+            srv.submit(d, want_state=True)
+        res = _drain(srv, len(reqs))
+    for d in reqs:
+        r = res[d["tag"]]
+        assert r["status"] == 0
+        o = _oracle_request(d)
+        _oracle_steps(o, d, 0, 1)
+        b = o.bodies()
+        err = max(float(np.abs(r["final_state"][k] - b[k]).max()) for k in ("px", "py"))
+        say(f"    request {d['tag']} ({d['arms']}x{d['segs']}, ring {d['ring']}, {2 * d['pairs']} creatures): "
+            f"max|dpos| {err:.3e} px")
+        assert err <= 1e-3
+
+
+def test_served_request_event_window_matches_the_oracle():
+    """A served 24-step request equals a handle's rollout of the same tape
+    bit for bit; that rollout's grip events (attach, release, break) over
+    the 24 steps equal the oracle's exactly, and the final poses stay within
+    0.05 px of the oracle's."""
+    from helpers import decode_events
+    d = _req(21, 6, 7, 8, 6, 24, x=150.0, y=300.0, angle=3.0)
+    say("[case] served 24-step request vs handle vs oracle")
+    with S.Server(S.default_opts(chunk_steps=50)) as srv:
+        srv.submit(d, want_state=True)
+        r = _drain(srv, 1)[d["tag"]]
+    b = W.Builder()
+    m = S.MAP
+    b.map(m["width"], m["height"], m["seed"], m["cols"], m["rows"], m["density"], m["wall"])
+    for _ in range(2 * d["pairs"]):
+        b.spawn(d["x"], d["y"], d["angle"], d["arms"], d["segs"], d["ring"])
+    w = b.world()
+    g = G.create(w, path=K.KX_PATH_PERSISTENT, lanes_per_creature=16, waves_per_block=1)
+    G.upload(g)
+    G.upload_tape(g, S.policy_tape(d, 0, d["steps"]))
+    law = W.default_law(0)
+    law.flags = K.KX_LAW_TAPE
+    ev = decode_events(G.rollout(g, law, 0, d["steps"], events=True))
+    G.download(g)
+    G.destroy(g)
+    for k in KEYS:
+        assert np.array_equal(r["final_state"][k], np.float32(getattr(w, k))), k
+    o = _oracle_request(d)
+    _oracle_steps(o, d, 0, d["steps"])
+    ev_or = sorted(o.events())
+    bo = o.bodies()
+    err = max(float(np.abs(r["final_state"][k] - bo[k]).max()) for k in ("px", "py"))
+    say(f"    24 steps: {len(ev_or)} oracle events, {len(ev)} served, final max|dpos| {err:.3e} px")
+    assert len(ev_or) > 10 and ev == ev_or
+    assert err <= 0.05
 
 
 def _free_bytes():
