@@ -73,17 +73,19 @@ def algorithmic_launch_bytes(w, nsteps):
 
 
 def measured_traffic(n_cr, chunk):
-    """HBM bytes per launch from the committed rocprofv3 counter summary
-    (FETCH_SIZE + WRITE_SIZE, copy-calibrated), when it covers this workload."""
-    path = os.path.join(ROOT, "profiles", "kx_k_persist_uniform4096_traffic.json")
-    try:
-        with open(path) as f:
-            t = json.load(f)
-    except (OSError, ValueError):
-        return None
-    if t.get("creatures") != n_cr or t.get("steps_per_launch") != chunk:
-        return None
-    return t
+    """HBM bytes per launch from the committed rocprofv3 counter summaries
+    (FETCH_SIZE + WRITE_SIZE, copy-calibrated), when one covers this
+    workload and launch length."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "kx_k_persist_uniform4096*_traffic.json"))):
+        try:
+            with open(path) as f:
+                t = json.load(f)
+        except (OSError, ValueError):
+            continue
+        if t.get("creatures") == n_cr and t.get("steps_per_launch") == chunk:
+            return dict(t, source=os.path.relpath(path, ROOT))
+    return None
 
 
 def state_readout_cost(g, w, reps=20):
@@ -204,9 +206,67 @@ def policy_loop_cost(n, steps=40):
     return res
 
 
-def cpu_baseline(seconds=12.0):
+def device_parity(g, w, law, step, every=64):
+    """The device's error against the f64 oracle on the benchmarked world: a
+    creature subset (every `every`-th) pulled from the device after the timed
+    run, one more step of the same law on both sides from the identical
+    binary32 state. Returns max |dpos| (px) over creatures whose grip events
+    agree, and the creatures whose events differ (flips). The oracle is the
+    checker here, as in the parity tests."""
+    import ctypes as C
+    from oracle import pyoracle as po
+    from kraken_3_gpu_amd import gpuworld as G
+    G.download(g)
+    ctl = G.download_controls(g)
+    G.download_cooldown(g)
+    f32 = lambda a: np.asarray(a, np.float32).astype(np.float64)
+    sub = list(range(0, w.n_creatures, every))
+    o = po.World(w.S, w.I, w.h)
+    o.set_map(w.width, w.height, w.rects.astype(np.float64))
+    lo = np.concatenate([[0], np.cumsum([1 + sp[3] * sp[4] for sp in w.spawns])])
+    bidx = []
+    for c in sub:
+        x, y, a, arms, segs, ring, curl = w.spawns[c]
+        o.spawn(x, y, a, arms, segs, ring, curl)
+        bidx.extend(range(lo[c], lo[c + 1]))
+    o.color()
+    bidx = np.asarray(bidx)
+    crb = np.searchsorted(lo, np.arange(w.n), side="right") - 1
+    cidx = np.nonzero(np.isin(crb[w.a], sub))[0]
+    o.set_bodies(*(f32(getattr(w, k)[bidx]) for k in ("px", "py", "ang", "vx", "vy", "av")))
+    o.set_cons_dynamic(broken=w.broken[cidx].astype(np.int32), lbx=f32(ctl["lbx"][cidx]), lby=f32(ctl["lby"][cidx]),
+                       compliance=f32(ctl["compliance"][cidx]), target=f32(ctl["target"][cidx]),
+                       breakForce=f32(ctl["breakForce"][cidx]))
+    o.set_cd(np.ascontiguousarray(w.cd[sub]))
+    o.set_step_counter(step)
+    o.clear_events()
+    for local, key in enumerate(sub):
+        po.lib().or_apply_law_one(o.p, C.c_uint32(law.seed), C.c_uint32(step), local, key)
+    o.step()
+    ev = G.rollout(g, law, step, 1, events=True)
+    G.download(g)
+    ev_gpu = set()
+    for c in sub:
+        v = int(ev[0, c])
+        for bit in range(24):
+            if v >> bit & 1:
+                ev_gpu.add((sub.index(c), bit // 3, bit % 3))
+    ev_or = {(c, a, k) for (_, c, a, k) in o.events()}
+    flips = sorted({e[0] for e in ev_gpu ^ ev_or})
+    b = o.bodies()
+    err = np.maximum(np.abs(w.px[bidx] - b["px"]), np.abs(w.py[bidx] - b["py"]))
+    local = np.searchsorted(lo[sub], bidx, side="right") - 1
+    ok = ~np.isin(local, flips)
+    return {"max_dpos_px": float(err[ok].max()), "max_dpos_px_all": float(err.max()), "event_flips": len(flips),
+            "events": len(ev_or), "creatures_checked": len(sub), "step": step,
+            "check": "one step of the device vs the f64 oracle from identical binary32 state"}
+
+
+def cpu_baseline(seconds=12.0, device=None):
     """The oracle (C restatement of xpbd.js, f64) stepping 64-creature worlds
-    under the same control law on the host cores, one world per thread."""
+    under the same control law on the host cores, one world per thread; with
+    `device` = (handle, world, law, step), also the device's one-step error
+    against it on a subset of the benchmarked world (device_parity)."""
     from oracle import pyoracle as po
     from kraken_3_gpu_amd import world as W
     threads = max(1, min(16, os.cpu_count() or 1))
@@ -219,8 +279,11 @@ def cpu_baseline(seconds=12.0):
     rate1 = o.bench_threads(1, 2, 5)           # calibration: creature-steps/s on one thread
     steps = max(2, int(seconds * rate1 / 64 / 1.0))
     rate = o.bench_threads(threads, steps, 5)
-    return {"value": rate, "unit": "creature-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{threads} threads x 64 creatures x {steps} steps (uniform batch recipe, f64 oracle)"}
+    out = {"value": rate, "unit": "creature-steps/s", "cores": threads, "kind": "port",
+           "sample": f"{threads} threads x 64 creatures x {steps} steps (uniform batch recipe, f64 oracle)"}
+    if device is not None:
+        out["device_parity"] = device_parity(*device)
+    return out
 
 
 def copy_probe(mib):
@@ -288,6 +351,43 @@ def persistent_run(g, law, steps, warmup, chunk, step0=0):
     return dict(wall=wall, chunk_ms=chunk_ms, launch_ms=float(np.mean(full)), step=step,
                 p50=float(np.percentile(lat, 50)), p99=float(np.percentile(lat, 99)),
                 dev_ms=sum(m for m, _ in chunk_ms))
+
+
+def rollout_latency(g, law, step, chunk, launches):
+    """Rollout latency: `launches` launches of `chunk` steps each, one at a
+    time, timed on the host from submission to completion and on the device
+    with HIP events. Returns percentiles of both and the next step key."""
+    from kraken_3_gpu_amd import gpuworld as G
+    host, dev = [], []
+    G.sync(g)
+    for _ in range(launches):
+        t0 = time.perf_counter()
+        G.rollout(g, law, step, chunk)
+        G.sync(g)
+        host.append((time.perf_counter() - t0) * 1e3)
+        dev.append(G.last_timing(g)[0])
+        step += chunk
+    h, d = np.asarray(host), np.asarray(dev)
+    return dict(samples=launches, steps_per_launch=chunk, host_p50_ms=float(np.percentile(h, 50)),
+                host_p99_ms=float(np.percentile(h, 99)), device_p50_ms=float(np.percentile(d, 50)),
+                device_p99_ms=float(np.percentile(d, 99)), device_mean_ms=float(d.mean()),
+                device_min_ms=float(d.min()), device_max_ms=float(d.max()), step=step)
+
+
+def serving_fixed_rate(n_req, rate, seed=7):
+    """ES evaluation requests (16-512 antithetic pairs, 300-1500 steps, 4-8
+    arms x 5-9 segments, ring 6-10) arriving as a Poisson stream at a fixed
+    `rate` per second into the continuous-batching server: request latency
+    (arrival -> result) percentiles and the aggregate creature-steps/s."""
+    from kraken_3_gpu_amd import serve as S
+    t, reqs = S.es_stream(n_req, rate=rate, seed=seed)
+    with S.Server(S.default_opts(chunk_steps=50)) as srv:
+        _, summ = S.run_stream(srv, t, reqs)
+    mean_cs = float(np.mean([2 * r["pairs"] * r["steps"] for r in reqs]))
+    return dict(requests=n_req, arrival_rate_per_s=rate, offered_creature_steps_per_s=rate * mean_cs,
+                p50_request_ms=summ["p50_ms"], p99_request_ms=summ["p99_ms"], max_request_ms=summ["max_ms"],
+                creature_steps_per_s=summ["creature_steps_per_s"], p99_admit_wait_ms=summ["p99_admit_wait_ms"],
+                device_busy_frac=summ["device_busy_frac"], failed=summ["failed"], peak_active=summ["peak_active"])
 
 
 def warm_accounting(g, w):
@@ -455,6 +555,11 @@ def main():
     ap.add_argument("--chunk", type=int, default=0,
                     help="simulation steps per persistent launch (default 50; uneven: tuned)")
     ap.add_argument("--plain-steps", type=int, default=100)
+    ap.add_argument("--latency-launches", type=int, default=100,
+                    help="launches timed one by one for the rollout latency and the roofline's launch duration")
+    ap.add_argument("--serve-requests", type=int, default=100,
+                    help="requests of the fixed-rate serving leg (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="arrivals per second of the serving leg")
     ap.add_argument("--lanes", type=int, default=0, help="lanes per creature (0 = default)")
     ap.add_argument("--wpb", type=int, default=0, help="wavefronts per workgroup (0 = default)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -497,11 +602,13 @@ def main():
     law = W.default_law(seed)
     w = W.world_uniform(args.creatures, seed=1)
     n_cr = w.n_creatures
+    # every launch runs `chunk` steps (the last one of the timed region may run fewer)
+    chunk = min(args.chunk, args.steps)
     g = G.create(w, path=K.KX_PATH_PERSISTENT, lanes_per_creature=args.lanes, waves_per_block=args.wpb)
     G.upload(g)
     step = 0
     while step < args.warmup:
-        n = min(args.chunk, args.warmup - step)
+        n = min(chunk, args.warmup - step)
         G.rollout(g, law, step, n)
         step += n
     G.sync(g)
@@ -516,7 +623,7 @@ def main():
     t0 = time.perf_counter()
     done = 0
     while done < args.steps:
-        n = min(args.chunk, args.steps - done)
+        n = min(chunk, args.steps - done)
         G.rollout(g, law, step, n)
         ms, _ = G.last_timing(g)
         chunk_ms.append((ms, n))
@@ -528,6 +635,9 @@ def main():
     dev_ms = sum(m for m, _ in chunk_ms)
     st = G.download(g)
     shape = G.persist_info(g)
+    # launch duration and rollout latency over many launches of the same length
+    lat = rollout_latency(g, law, step, chunk, args.latency_launches)
+    step = lat.pop("step")
     readout = state_readout_cost(g, w)
     policy = policy_cost(n_cr)
     policy["loop"] = policy_loop_cost(n_cr)
@@ -535,16 +645,12 @@ def main():
     per_cr = dict(pins=42, motors=42, bodies=43, points=92, grips_total=6)
     F = flops_per_creature_step(w.S, w.I, per_cr["pins"], per_cr["motors"], per_cr["bodies"], per_cr["points"],
                                 g_att, r_near, per_cr["grips_total"])
-    full = [m for m, n in chunk_ms if n == args.chunk] or [chunk_ms[0][0]]
-    launch_ms = float(np.mean(full))
-    achieved_tflops = F * n_cr * args.chunk / (launch_ms * 1e-3) / 1e12
-    launch_bytes = algorithmic_launch_bytes(w, args.chunk)
+    launch_ms = lat["device_mean_ms"]
+    achieved_tflops = F * n_cr * chunk / (launch_ms * 1e-3) / 1e12
+    timed_tflops = F * n_cr * args.steps / (dev_ms * 1e-3) / 1e12
+    launch_bytes = algorithmic_launch_bytes(w, chunk)
     hbm_gbs = launch_bytes / (launch_ms * 1e-3) / 1e9
-    traffic = measured_traffic(n_cr, args.chunk)
-    lat = sorted(m for m, n in chunk_ms if n == args.chunk) or [launch_ms]
-    p50 = float(np.percentile(lat, 50))
-    p99 = float(np.percentile(lat, 99))
-
+    traffic = measured_traffic(n_cr, chunk)
     plain = None
     if not args.no_plain:
         wp = W.world_uniform(args.creatures, seed=1)
@@ -582,21 +688,26 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"uniform batch: {n_cr} creatures (6 arms x 7 segments), makeMap 1200x800 seed 1, "
                                f"S=4 x I=8, h=1/240, seeded control law + adhesion reflex",
-                   "creatures": n_cr, "sim_steps": args.steps, "steps_per_launch": args.chunk,
+                   "creatures": n_cr, "sim_steps": args.steps, "steps_per_launch": chunk,
                    "path": "persistent (one wavefront per creature, LDS-resident)"},
         "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tflops / FP32_PEAK_TFLOPS,
                      "traffic": traffic["traffic_bytes"] if traffic else None,
-                     "traffic_source": "profiles/kx_k_persist_uniform4096_traffic.json" if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None,
                      "algorithmic_bytes_per_launch": launch_bytes,
-                     "flops_per_creature_step": F, "grips_attached_mean": g_att, "rects_within_16px_mean": r_near,
+                     "flops_per_creature_step": F, "steps_per_launch": chunk, "creatures_per_launch": n_cr,
+                     "grips_attached_mean": g_att, "rects_within_16px_mean": r_near,
                      "kernel": "kx_k_persist", "launch_ms": launch_ms,
+                     "launch_ms_source": f"mean of {lat['samples']} launches of {chunk} steps (HIP events)",
+                     "achieved_timed_region": timed_tflops,
                      "hbm_algorithmic_gbs": hbm_gbs, "hbm_peak_gbs": HBM_PEAK_GBS,
                      "peak_unpacked": FP32_UNPACKED_TFLOPS, "frac_unpacked": achieved_tflops / FP32_UNPACKED_TFLOPS},
         "launch_shape": dict(shape, lanes_per_creature=args.lanes or None, waves_per_block=args.wpb or None),
-        "p50_rollout_ms": p50,
-        "p99_rollout_ms": p99,
-        "rollout_unit": f"{args.chunk} steps x {n_cr} creatures per launch",
+        "p50_rollout_ms": lat["host_p50_ms"],
+        "p99_rollout_ms": lat["host_p99_ms"],
+        "rollout_unit": f"{chunk} steps x {n_cr} creatures per launch, submission to completion, "
+                        f"{lat['samples']} launches",
+        "rollout_latency": lat,
         "device_ms_total": dev_ms,
         "final_stats": st,
         "state_readout": readout,
@@ -605,8 +716,12 @@ def main():
     if plain:
         out["plain"] = plain
         out["speedup_vs_plain"] = value / plain["value"]
+        out["speedup_vs_plain_grid_sdf"] = value / plain["value_grid_sdf"]
+    if args.serve_requests:
+        out["serving"] = serving_fixed_rate(args.serve_requests, args.serve_rate)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline(device=(g, w, law, step))
+        out["parity"] = out["cpu_baseline"]["device_parity"]
     print(json.dumps(out), flush=True)
 
 
