@@ -85,6 +85,7 @@ struct kx_handle {
   size_t tape_cap = 0;
   int tape_rows = 0;
   hipStream_t st = nullptr;
+  int dev = 0;                   // the device current at kx_create
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int last_launches = 0;
   bool timed = false;
@@ -793,6 +794,7 @@ static int create_impl(const kx_world_desc *w, const kx_creature_desc *cd, const
     if (h->colorCnt[c]) { d.col_off[d.ncol] = h->colorOff[c]; d.col_cnt[d.ncol] = h->colorCnt[c]; d.ncol++; }
   d.use_grid = 0;
   d.events = nullptr;
+  HIPCHK(hipGetDevice(&h->dev));
   HIPCHK(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking));
   HIPCHK(hipEventCreate(&h->e0));
   HIPCHK(hipEventCreate(&h->e1));
@@ -990,9 +992,9 @@ static KxLaw to_law(const kx_law *l) {
   return L;
 }
 
-static int run_steps(kx_handle *h, const KxLaw *law, int step0, const int *step_base, int nsteps, bool events) {
+static int run_steps(kx_handle *h, const KxLaw *law, int step0, const int *step_base, int nsteps, uint32_t *events) {
   KxDev d = h->d;
-  d.events = events ? h->ev_dev : nullptr;
+  d.events = events;
   int launches = 0;
   int with_forces = h->forces_pending ? 1 : 0;
   h->forces_pending = false;
@@ -1016,7 +1018,7 @@ extern "C" int kx_step(kx_handle *h, int32_t nsteps) {
   if (nsteps < 0) return fail(KX_EINVAL, "nsteps < 0");
   if (nsteps == 0) return 0;
   HIPCHK(hipEventRecord(h->e0, h->st));
-  TRY(run_steps(h, nullptr, 0, nullptr, nsteps, false));
+  TRY(run_steps(h, nullptr, 0, nullptr, nsteps, nullptr));
   HIPCHK(hipEventRecord(h->e1, h->st));
   h->timed = true;
   return 0;
@@ -1038,14 +1040,45 @@ extern "C" int kx_substep(kx_handle *h, int32_t n) {
   return 0;
 }
 
-extern "C" int kx_rollout(kx_handle *h, const kx_law *law, int32_t step0, int32_t nsteps, uint32_t *events_host) {
+static int rollout_checks(kx_handle *h, const kx_law *law, int32_t step0, int32_t nsteps) {
   if (!h || !law) return fail(KX_EINVAL, "null argument");
   if (!h->has_creatures) return fail(KX_ENOTSUP, "rollout needs a creature description at create");
   if (nsteps < 0 || step0 < 0) return fail(KX_EINVAL, "bad step range");
+  if ((law->flags & KX_LAW_TAPE) && nsteps > h->tape_rows)
+    return fail(KX_EINVAL, "tape rollout of %d steps with %d tape rows", nsteps, h->tape_rows);
+  return 0;
+}
+
+// The rollout proper: the steps on the handle's stream with the event words
+// written to `ev_dev` (zeroed by the caller, nsteps * creatures words) when
+// given, then copied to `events_host`. Allocates nothing.
+int kx_rollout_into(kx_handle *h, const kx_law *law, int32_t step0, int32_t nsteps, uint32_t *ev_dev,
+                    uint32_t *events_host) {
+  TRY(rollout_checks(h, law, step0, nsteps));
   if (nsteps == 0) return 0;
   KxLaw L = to_law(law);
-  if ((L.flags & KX_LAW_TAPE) && nsteps > h->tape_rows)
-    return fail(KX_EINVAL, "tape rollout of %d steps with %d tape rows", nsteps, h->tape_rows);
+  HIPCHK(hipEventRecord(h->e0, h->st));
+  TRY(run_steps(h, &L, step0, nullptr, nsteps, events_host ? ev_dev : nullptr));
+  HIPCHK(hipEventRecord(h->e1, h->st));
+  h->timed = true;
+  if (events_host) {
+    HIPCHK(hipMemcpyAsync(events_host, ev_dev, (size_t)nsteps * h->nCr * 4, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+  }
+  return 0;
+}
+
+int kx_handle_basics(const kx_handle *h, int *device, int *n_creatures, hipStream_t *st) {
+  if (!h) return fail(KX_EINVAL, "null handle");
+  *device = h->dev;
+  *n_creatures = h->nCr;
+  *st = h->st;
+  return 0;
+}
+
+extern "C" int kx_rollout(kx_handle *h, const kx_law *law, int32_t step0, int32_t nsteps, uint32_t *events_host) {
+  TRY(rollout_checks(h, law, step0, nsteps));
+  if (nsteps == 0) return 0;
   size_t need = (size_t)nsteps * h->nCr;
   if (events_host) {
     if (need > h->ev_cap) {
@@ -1056,15 +1089,7 @@ extern "C" int kx_rollout(kx_handle *h, const kx_law *law, int32_t step0, int32_
     }
     HIPCHK(hipMemsetAsync(h->ev_dev, 0, need * 4, h->st));
   }
-  HIPCHK(hipEventRecord(h->e0, h->st));
-  TRY(run_steps(h, &L, step0, nullptr, nsteps, events_host != nullptr));
-  HIPCHK(hipEventRecord(h->e1, h->st));
-  h->timed = true;
-  if (events_host) {
-    HIPCHK(hipStreamSynchronize(h->st));
-    HIPCHK(hipMemcpy(events_host, h->ev_dev, need * 4, hipMemcpyDeviceToHost));
-  }
-  return 0;
+  return kx_rollout_into(h, law, step0, nsteps, h->ev_dev, events_host);
 }
 
 extern "C" int kx_upload_tape(kx_handle *h, const float *tape, int32_t n_rows) {

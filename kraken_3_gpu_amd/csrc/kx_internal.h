@@ -163,6 +163,12 @@ int kx_handle_view(kx_handle *h, KxHandleView *v);
 int kx_set_coll_mode(kx_handle *h, int mode);
 // sets the thread's kx_last_error message and returns code
 int kx_error(int code, const char *msg);
+// kx_rollout with the event words in `ev_dev` (nsteps x creatures words,
+// zeroed by the caller) when events_host is given; allocates nothing
+int kx_rollout_into(kx_handle *h, const kx_law *law, int32_t step0, int32_t nsteps, uint32_t *ev_dev,
+                    uint32_t *events_host);
+// the device a handle was created on, its creature count and stream
+int kx_handle_basics(const kx_handle *h, int *device, int *n_creatures, hipStream_t *st);
 // Device side of one policy step (kx_api.cpp): reports the creature count and
 // the arm count every creature shares; with `launch`, also launches the state
 // readout into the handle's device readout buffer (stride floats per

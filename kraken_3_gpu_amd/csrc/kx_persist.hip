@@ -16,6 +16,8 @@
 // exact for coordinates of magnitude >= 1, so the solver works with finer
 // position ulps. Between steps the state is global binary32, exactly as the
 // plain path (kx_plain.hip) holds it, and the two paths agree bit for bit.
+#include <atomic>
+
 #include "../../include/kxpbd.h"
 #include "kx_device.h"
 #include "kx_sched.h"
@@ -811,14 +813,22 @@ __global__ __launch_bounds__(KX_WAVE *WPB) __attribute__((amdgpu_waves_per_eu(2)
   }
 }
 
+// Raises a kernel's dynamic LDS limit to 160 KiB once per device; safe to
+// call from several threads at once (the attribute call is idempotent and
+// the per-device flags are atomic).
+static void allow_full_lds(const void *fn, std::atomic<uint64_t> &done) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+  const uint64_t bit = 1ull << dev;
+  if (done.load(std::memory_order_acquire) & bit) return;
+  (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  done.fetch_or(bit, std::memory_order_acq_rel);
+}
+
 template <int WPB, bool PAR>
 static int launch_batch(const KxBatchEnt *E, const int2 *wgmap, int nwg, size_t lds, hipStream_t st) {
-  static bool attr_set = false;
-  if (!attr_set) {
-    (void)hipFuncSetAttribute((const void *)kx_k_persist_batch<WPB, PAR>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    attr_set = true;
-  }
+  static std::atomic<uint64_t> lds_set{0};
+  allow_full_lds((const void *)kx_k_persist_batch<WPB, PAR>, lds_set);
   hipLaunchKernelGGL((kx_k_persist_batch<WPB, PAR>), dim3(nwg), dim3(KX_WAVE * WPB), lds, st, E, wgmap);
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
@@ -850,12 +860,8 @@ static int launch_gw(const KxDev &d, const KxLaw *law, int step0, const int *ste
                      int with_forces, const KxPersistCfg &P, hipStream_t st) {
   size_t lds = ((size_t)P.tmpl_words + P.bm_words + P.map_words + (size_t)P.cr_words) * 4;
   if (lds > 160 * 1024) return -22;
-  static bool attr_set = false;
-  if (!attr_set) {
-    (void)hipFuncSetAttribute((const void *)kx_k_persist<G, WPB, PAR>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              160 * 1024);
-    attr_set = true;
-  }
+  static std::atomic<uint64_t> lds_set{0};
+  allow_full_lds((const void *)kx_k_persist<G, WPB, PAR>, lds_set);
   KxLaw L = law ? *law : KxLaw{};
   hipLaunchKernelGGL((kx_k_persist<G, WPB, PAR>), dim3(P.nwg), dim3(KX_WAVE * WPB), lds, st, d, L, law ? 1 : 0,
                      step0, step_base, nsteps, ev_row0, with_forces, P);
@@ -867,12 +873,8 @@ static int launch_mix(const KxDev &d, const KxLaw *law, int step0, const int *st
                       int with_forces, const KxPersistCfg &P, hipStream_t st) {
   size_t lds = ((size_t)P.tmpl_words + P.bm_words + P.map_words + (size_t)P.cr_words) * 4;
   if (lds > 160 * 1024) return -22;
-  static bool attr_set = false;
-  if (!attr_set) {
-    (void)hipFuncSetAttribute((const void *)kx_k_persist_mix<WPB, PAR>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              160 * 1024);
-    attr_set = true;
-  }
+  static std::atomic<uint64_t> lds_set{0};
+  allow_full_lds((const void *)kx_k_persist_mix<WPB, PAR>, lds_set);
   KxLaw L = law ? *law : KxLaw{};
   hipLaunchKernelGGL((kx_k_persist_mix<WPB, PAR>), dim3(P.nwg), dim3(KX_WAVE * WPB), lds, st, d, L, law ? 1 : 0,
                      step0, step_base, nsteps, ev_row0, with_forces, P);
