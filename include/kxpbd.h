@@ -186,6 +186,16 @@ int kx_substep(kx_handle *h, int32_t n);
 int kx_rollout(kx_handle *h, const kx_law *law, int32_t step0, int32_t nsteps,
                uint32_t *events_host);
 int kx_sync(kx_handle *h);
+/* kx_rollout that allocates nothing. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under graph capture,
+ * tight serving loops, harnesses that forbid allocation). The per-step event
+ * words go to a static device array of the library (4194304 words = 16 MiB,
+ * one per GPU, held until the process ends) instead of the handle's
+ * grow-on-demand events buffer; nsteps x creatures beyond that is KX_EINVAL.
+ * The words a call uses are cleared first; calls that overlap on one GPU
+ * race on the array. Results equal kx_rollout's. */
+int kx_rollout_noalloc(kx_handle *h, const kx_law *law, int32_t step0, int32_t nsteps,
+                       uint32_t *events_host);
 /* Control tape for rollouts under KX_LAW_TAPE: n_rows rows of n_creatures
  * records of KX_TAPE_WORDS floats (row-major). Ordered after the handle's
  * earlier work and copied before return, so the caller may reuse `tape`.

@@ -115,6 +115,7 @@ EXPORTS = {
     "kx_substep": (i32, [C.c_void_p, i32]),
     "kx_rollout": (i32, [C.c_void_p, C.POINTER(Law), i32, i32, u32p]),
     "kx_sync": (i32, [C.c_void_p]),
+    "kx_rollout_noalloc": (i32, [C.c_void_p, C.POINTER(Law), i32, i32, u32p]),
     "kx_upload_tape": (i32, [C.c_void_p, C.POINTER(C.c_float), i32]),
     "kx_server_create": (i32, [C.POINTER(ServerOpts), C.POINTER(C.c_void_p)]),
     "kx_server_submit": (i32, [C.c_void_p, C.POINTER(Request)]),

@@ -9,7 +9,8 @@ LIB = os.path.join(HERE, "libkxpbd.so")
 # sources with no device code: compiled for the host only (they use x86
 # builtins such as the run-time AVX2 check)
 HOST_ONLY = ("kx_serve.cpp",)
-SOURCES = ["kx_api.cpp", "kx_builder.cpp", "kx_sched.cpp", "kx_serve.cpp", "kx_plain.hip", "kx_persist.hip", "kx_policy.hip"]
+SOURCES = ["kx_api.cpp", "kx_builder.cpp", "kx_sched.cpp", "kx_serve.cpp", "kx_plain.hip", "kx_persist.hip", "kx_policy.hip",
+           "kx_noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
