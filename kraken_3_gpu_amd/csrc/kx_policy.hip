@@ -307,6 +307,10 @@ struct kx_policy {
   float *senses_host = nullptr;   // pinned staging of the senses (hipHostMalloc)
   int senses_cap = 0;             // floats of both
   hipEvent_t senses_done = nullptr;   // the last upload from the staging buffer
+  // the end of the last kx_policy_drive's work on the policy's buffers
+  // (senses, inputs, outputs), on whichever world stream it ran
+  hipEvent_t drive_done = nullptr;
+  bool drove = false;
 };
 
 struct kx_es {
@@ -363,6 +367,7 @@ extern "C" int kx_policy_create(int32_t n, int32_t n_in, int32_t hidden, int32_t
 extern "C" int kx_policy_destroy(kx_policy *p) {
   if (!p) return 0;
   if (p->st) (void)hipStreamSynchronize(p->st);
+  if (p->drove) (void)hipEventSynchronize(p->drive_done);   // a drive on a world stream may still read them
   (void)hipFree(p->genomes);
   (void)hipFree(p->x);
   (void)hipFree(p->out);
@@ -371,6 +376,7 @@ extern "C" int kx_policy_destroy(kx_policy *p) {
   (void)hipFree(p->senses);
   if (p->senses_host) (void)hipHostFree(p->senses_host);
   if (p->senses_done) (void)hipEventDestroy(p->senses_done);
+  if (p->drive_done) (void)hipEventDestroy(p->drive_done);
   if (p->ev0) (void)hipEventDestroy(p->ev0);
   if (p->ev1) (void)hipEventDestroy(p->ev1);
   if (p->st) (void)hipStreamDestroy(p->st);
@@ -464,10 +470,15 @@ extern "C" int kx_policy_drive(kx_policy *p, kx_handle *h, const float *senses, 
   if (d.n_out < 1 + 3 * arms) return pfail(KX_EINVAL, "decode of %d arms from %d outputs", arms, d.n_out);
   need = kx_policy_io(h, stride, &state, &tape, &st, &n_cr, &arms, 1);
   if (need < 0) return need;
+  // the previous drive may have run on another world's stream: its reads and
+  // writes of the shared senses, inputs and outputs come before this one's
+  if (!p->drive_done) PCHK(hipEventCreateWithFlags(&p->drive_done, hipEventDisableTiming));
+  if (p->drove) PCHK(hipStreamWaitEvent(st, p->drive_done, 0));
   const float *sd = nullptr;
   if (senses && sense_dim > 0) {
     const int nf = p->n * sense_dim;
     if (nf > p->senses_cap) {
+      if (p->drove) PCHK(hipEventSynchronize(p->drive_done));
       PCHK(hipStreamSynchronize(st));
       (void)hipFree(p->senses);
       if (p->senses_host) (void)hipHostFree(p->senses_host);
@@ -495,6 +506,8 @@ extern "C" int kx_policy_drive(kx_policy *p, kx_handle *h, const float *senses, 
   PCHK(hipGetLastError());
   int rc = policy_launch(p, arms, true, false, st, tape);
   if (rc) return rc;
+  PCHK(hipEventRecord(p->drive_done, st));
+  p->drove = true;
   // later work on the policy's own stream (forward, ES generation) touches
   // the same inputs, outputs and genomes: order it after this forward pass
   PCHK(hipStreamWaitEvent(p->st, p->ev1, 0));
