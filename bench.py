@@ -72,12 +72,12 @@ def algorithmic_launch_bytes(w, nsteps):
     return bin_ + bout
 
 
-def measured_traffic(n_cr, chunk):
+def measured_traffic(n_cr, chunk, workload="uniform4096"):
     """HBM bytes per launch from the committed rocprofv3 counter summaries
     (FETCH_SIZE + WRITE_SIZE, copy-calibrated), when one covers this
     workload and launch length."""
     import glob
-    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "kx_k_persist_uniform4096*_traffic.json"))):
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", f"kx_k_persist_{workload}*_traffic.json"))):
         try:
             with open(path) as f:
                 t = json.load(f)
@@ -419,6 +419,7 @@ def bench_stress(args):
     F = flops_per_step(w, g_att, min(r_near, 64.0))
     ach = F * args.chunk / (r["launch_ms"] * 1e-3) / 1e12
     lb = algorithmic_launch_bytes(w, args.chunk)
+    traffic = measured_traffic(n_cr, args.chunk, "stress65536")
     out = {
         "metric": "creature-steps/s, stress run (65536 creatures, 4096x4096 map, 32x32 corridors)",
         "value": n_cr * steps / r["wall"], "unit": "creature-steps/s", "n_gpus": 1, "steps": steps,
@@ -430,6 +431,9 @@ def bench_stress(args):
         "roofline": {"bound": "valu", "achieved": ach, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": ach / FP32_PEAK_TFLOPS, "launch_ms": r["launch_ms"],
                      "hbm_algorithmic_gbs": lb / (r["launch_ms"] * 1e-3) / 1e9, "hbm_peak_gbs": HBM_PEAK_GBS,
+                     "traffic": traffic["traffic_bytes"] if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None,
+                     "hbm_measured_gbs": (traffic["traffic_bytes"] / (r["launch_ms"] * 1e-3) / 1e9) if traffic else None,
                      "grips_attached": g_att, "rects_within_16px_mean": r_near},
         "launch_shape": G.persist_info(g),
         "p50_rollout_ms": r["p50"], "p99_rollout_ms": r["p99"], "final_stats": st,

@@ -42,11 +42,26 @@ def copy_factor(path, counter, bytes_moved):
     return bytes_moved / (kib * 1024.0) if kib > 0 else None
 
 
-def trace_durations(path, kernel):
+def trace_durations(path, kernel, min_ms=0.0):
+    """Durations (ms) of the kernel's dispatches in issue order, those shorter
+    than min_ms left out (launches of fewer steps)."""
     out = []
     for r in _rows(path):
         if kernel in r["Kernel_Name"]:
-            out.append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-6)
+            out.append((int(r["Dispatch_Id"]), (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-6))
+    return [d for _, d in sorted(out) if d >= min_ms]
+
+
+def first_dispatch_counters(path, kernel):
+    """{counter: value} of the kernel's first dispatch in one counter run."""
+    rows = [r for r in _rows(path) if kernel in r["Kernel_Name"]]
+    if not rows:
+        return {}
+    first = min(int(r["Dispatch_Id"]) for r in rows)
+    out = {}
+    for r in rows:
+        if int(r["Dispatch_Id"]) == first:
+            out[r["Counter_Name"]] = out.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
     return out
 
 
@@ -68,6 +83,10 @@ def main():
     ap.add_argument("--out", required=True)
     ap.add_argument("--json")
     ap.add_argument("--note", action="append", default=[], help="free text lines appended to the summary")
+    ap.add_argument("--min-ms", type=float, default=0.0,
+                    help="trace statistics over dispatches at least this long (the full-length launches)")
+    ap.add_argument("--last", type=int, default=0,
+                    help="also report the mean of the last N of those dispatches (a bench leg's launches)")
     a = ap.parse_args()
 
     stats = _rows(a.stats)
@@ -76,8 +95,9 @@ def main():
     write = counter_per_dispatch(a.write, a.kernel, "WRITE_SIZE") if traffic else {}
     ff = copy_factor(a.copy_fetch, "FETCH_SIZE", a.copy_bytes) if traffic else None
     fw = copy_factor(a.copy_write, "WRITE_SIZE", a.copy_bytes) if traffic else None
-    fetch_kib = max(fetch.values()) if fetch else None
-    write_kib = max(write.values()) if write else None
+    # the kernel's first dispatch in each run: the same launch in every pass
+    fetch_kib = fetch[min(fetch)] if fetch else None
+    write_kib = write[min(write)] if write else None
     fetch_b = fetch_kib * 1024 * ff if fetch_kib is not None and ff else None
     write_b = write_kib * 1024 * fw if write_kib is not None and fw else None
 
@@ -88,11 +108,15 @@ def main():
     for r in stats[:12]:
         lines.append(f"| `{r['Name'][:70]}` | {r['Calls']} | {float(r['AverageNs']) * 1e-6:.3f} | {float(r['Percentage']):.2f} |")
     if a.trace:
-        d = trace_durations(a.trace, a.kernel)
+        d = trace_durations(a.trace, a.kernel, a.min_ms)
         if d:
             s = sorted(d)
-            lines += ["", f"{a.kernel} dispatches: {len(d)}, mean {sum(d) / len(d):.3f} ms, median "
+            sel = f" of at least {a.min_ms} ms" if a.min_ms else ""
+            lines += ["", f"{a.kernel} dispatches{sel}: {len(d)}, mean {sum(d) / len(d):.3f} ms, median "
                           f"{s[len(s) // 2]:.3f} ms, min {s[0]:.3f} ms, max {s[-1]:.3f} ms"]
+            if a.last and len(d) >= a.last:
+                t = d[-a.last:]
+                lines += [f"last {a.last} of them: mean {sum(t) / len(t):.3f} ms"]
             if len(d) <= 12:
                 lines += [f"{a.kernel} dispatch durations (ms): " + ", ".join(f"{x:.3f}" for x in d)]
     if traffic:
@@ -102,13 +126,11 @@ def main():
                   f"| WRITE_SIZE | {write_kib} | {fw} | {write_b} |"]
     sq = {}
     for p in a.sq:
-        for r in _rows(p):
-            if a.kernel in r["Kernel_Name"]:
-                sq[r["Counter_Name"]] = sq.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+        sq.update(first_dispatch_counters(p, a.kernel))
     if sq:
         waves = sq.get("SQ_WAVES", 0.0)
         wc = sq.get("SQ_WAVE_CYCLES", 0.0)
-        lines += ["", "## SQ counters (one launch)", "", "| counter | total | per wave | / SQ_WAVE_CYCLES |", "|---|---|---|---|"]
+        lines += ["", "## SQ counters (the kernel's first dispatch of each counter run)", "", "| counter | total | per wave | / SQ_WAVE_CYCLES |", "|---|---|---|---|"]
         for k in sorted(sq):
             v = sq[k]
             lines.append(f"| {k} | {v:.4e} | {v / waves if waves else 0:.4e} | {v / wc if wc else 0:.4f} |")
