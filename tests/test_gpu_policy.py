@@ -229,3 +229,41 @@ def test_drive_matches_the_host_loop():
             Pd.drive(ga, law, 12)
     G.destroy(ga)
     G.destroy(gb)
+
+
+def test_drive_and_readout_reuse_one_state_buffer():
+    """The policy step reads the state at its own stride (66 floats per
+    creature) and a caller's readout at another (54): alternating them keeps
+    one readout buffer per handle, grown once, so device memory stays flat."""
+    import ctypes as C
+    from kraken_3_gpu_amd import gpuworld as G
+    from kraken_3_gpu_amd import world as W
+    n_cr = 4096
+    n_in = B.input_dim()
+    nw = B.weight_count(n_in)
+    rng = np.random.default_rng(3)
+    gen = (rng.standard_normal((n_cr, nw)) * 2 / np.sqrt(n_in)).astype(np.float32)
+    law = W.default_law(5)
+    g = G.create(W.world_uniform(n_cr, seed=2))
+    G.upload(g)
+
+    def free_bytes():
+        G.sync(g)
+        f, t = C.c_int64(), C.c_int64()
+        K.check(K.lib().kx_mem_info(C.byref(f), C.byref(t)))
+        return f.value
+
+    print(f"[case] drive / read_state alternation, {n_cr} creatures", flush=True)
+    with B.Population(n_cr, n_in) as P:
+        P.set_genomes(gen)
+        P.drive(g, law, 0)
+        G.read_state(g)
+        before = free_bytes()
+        for s in range(1, 21):
+            P.drive(g, law, s)
+            G.read_state(g)
+        after = free_bytes()
+    G.destroy(g)
+    say(f"free device memory before {before / 2**20:.1f} MiB, after 20 alternations {after / 2**20:.1f} MiB")
+    # a buffer re-allocated per call would hold 20 x ~1 MiB more
+    assert before - after < 2 * 2**20
