@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define KXPBD_ABI_VERSION 4
+#define KXPBD_ABI_VERSION 5
 
 /* error codes */
 #define KX_OK 0
@@ -221,10 +221,20 @@ int kx_schedule_info(const kx_handle *h, int32_t *rounds, int32_t max);
  * workgroups per CU the launch needs, [4] LDS words of the staged map (0:
  * map read from HBM), [5] cell-bitmap words, [6] template words, [7]
  * per-creature LDS words, [8] 1 if lane counts differ between workgroups,
- * [9] the collision fold in use (KX_FOLD_PARALLEL or KX_FOLD_OWNER).
- * Returns the number of fields (10). No counterpart in the reference (its
+ * [9] the collision fold in use (KX_FOLD_PARALLEL or KX_FOLD_OWNER), [10]
+ * rounds per iteration held in registers (0: the round slots are read from
+ * LDS each round; see kx_set_register_slots).
+ * Returns the number of fields (11). No counterpart in the reference (its
  * path has no persistent kernel). */
 int kx_persist_info(const kx_handle *h, int32_t *info, int32_t max);
+/* Persistent path: allow (1, the default) or forbid (0) register-held round
+ * slots. With them each lane keeps its slot of every round (constraint and
+ * body records, control fields, lambda) in registers and a round reads only
+ * its two poses from LDS; used when every creature's template has 12 rounds
+ * at 16 lanes per creature and the launch places at most one wavefront per
+ * SIMD. Results are identical bit for bit either way. No counterpart in the
+ * reference (a launch-shape knob, like waves_per_block). */
+int kx_set_register_slots(kx_handle *h, int32_t enable);
 /* Persistent path: how a step folds the collision points of the bodies near
  * a rect (xpbd.js:441-470 either way, with identical results).
  * KX_FOLD_PARALLEL tests the near bodies' points in parallel over the

@@ -130,6 +130,7 @@ EXPORTS = {
     "kx_set_path": (i32, [C.c_void_p, i32, i32, i32]),
     "kx_schedule_info": (i32, [C.c_void_p, i32p, i32]),
     "kx_persist_info": (i32, [C.c_void_p, i32p, i32]),
+    "kx_set_register_slots": (i32, [C.c_void_p, i32]),
     "kx_set_collision_fold": (i32, [C.c_void_p, i32]),
     "kx_read_state": (i32, [C.c_void_p, f32p, i32]),
     "kx_respawn": (i32, [C.c_void_p, i32, i32p, f64p]),

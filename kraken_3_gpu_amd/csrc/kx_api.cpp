@@ -99,10 +99,11 @@ struct kx_handle {
   std::vector<int> tmpl_rounds;
   KxPersistCfg pc{};
   void *pc_bufs[4] = {nullptr, nullptr, nullptr, nullptr};
-  int pinfo[10] = {};    // see kx_persist_info
+  int pinfo[11] = {};    // see kx_persist_info
   std::vector<int> tmpl_lanes;   // per template lanes per creature, 0 = the handle's G
   int launch_G = 16;             // lane count the uniform kernel is launched with
   int coll_mode = 0;             // collision fold: 0 = by map density, 1 = parallel, 2 = owning lane
+  int reg_mode = 1;              // 1: register-held round slots where the launch allows them
   std::string state_why;         // why the state readout is unavailable (empty: available)
   float *state_dev = nullptr;    // readout buffer [nCr * state_stride], state_cap floats allocated
   int state_stride = 0;
@@ -451,6 +452,18 @@ static int build_persist(kx_handle *h) {
   h->pinfo[4] = mapw; h->pinfo[5] = bmw; h->pinfo[6] = words; h->pinfo[7] = stride;
   h->pinfo[8] = mixed ? 1 : 0;
   h->pinfo[9] = par ? KX_FOLD_PARALLEL : KX_FOLD_OWNER;
+  // Register-held round slots: one template of KX_REG_ROUNDS rounds at 16
+  // lanes, and every workgroup resident at once at one wavefront per SIMD
+  // (the slots take the register file, so a SIMD holds no second wave).
+  int reg = 0;
+  if (h->reg_mode && !mixed && Gk == 16) {
+    bool all = true;
+    for (size_t t = 0; t < T.size(); t++)
+      if (!members[t].empty() && (T[t].R != KX_REG_ROUNDS || T[t].nb > KX_REG_BODY_REGS * 16)) all = false;
+    const size_t lds_used = base + (size_t)mapw * 4;
+    if (all && (long)nwg * WPB <= 4L * ncu && kx_persist_occupancy_reg(WPB, lds_used, par) >= need) reg = KX_REG_ROUNDS;
+  }
+  h->pinfo[10] = reg;
   std::vector<int> tables;
   tables.insert(tables.end(), wg_tmpl.begin(), wg_tmpl.end());
   tables.insert(tables.end(), wg_lanes.begin(), wg_lanes.end());
@@ -482,6 +495,7 @@ static int build_persist(kx_handle *h) {
   P.cr_words = cr_words;
   P.NB = NB; P.NC = NC; P.NP = NP;
   P.coll_par = par ? 1 : 0;
+  P.reg_rounds = reg;
   h->launch_G = mixed ? h->G : (lanes0 > 0 ? lanes0 : h->G);
   return 0;
 }
@@ -1201,8 +1215,8 @@ extern "C" int kx_schedule_info(const kx_handle *h, int32_t *rounds, int32_t max
 extern "C" int kx_persist_info(const kx_handle *h, int32_t *info, int32_t max) {
   if (!h || !info) return fail(KX_EINVAL, "null argument");
   if (!h->persist_ok) return fail(KX_ENOTSUP, "persistent path unavailable: %s", h->persist_why.c_str());
-  for (int i = 0; i < 10 && i < max; i++) info[i] = h->pinfo[i];
-  return 10;
+  for (int i = 0; i < 11 && i < max; i++) info[i] = h->pinfo[i];
+  return 11;
 }
 
 extern "C" int kx_template_info(const kx_handle *h, int32_t t, int32_t *out6) {
@@ -1331,6 +1345,18 @@ int kx_policy_io(kx_handle *h, int32_t stride, float **state_dev, float **tape_d
   *tape_dev = h->tape_dev;
   *st = h->st;
   return need;
+}
+
+extern "C" int kx_set_register_slots(kx_handle *h, int32_t enable) {
+  if (!h) return fail(KX_EINVAL, "null handle");
+  if (enable != 0 && enable != 1) return fail(KX_EINVAL, "register slots %d", enable);
+  HIPCHK(hipStreamSynchronize(h->st));
+  h->reg_mode = enable;
+  if (h->persist_ok) {
+    TRY(build_persist(h));
+    if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+  }
+  return 0;
 }
 
 extern "C" int kx_set_collision_fold(kx_handle *h, int32_t fold) {

@@ -225,7 +225,9 @@ __device__ __forceinline__ bool kx_angle_core(float &tA, float &tB, float wA, fl
                                               float target, float mn, float mx, float alpha, float rS, float &lam) {
   if (!hasB) wB = 0.0f;
   float wSum = wA + wB;
-  float rel = hasB ? kx_angle_delta(tB, tA) : tA;
+  // both sides evaluated, then selected: no branch on hasB
+  const float dBA = kx_angle_delta(tB, tA);
+  float rel = hasB ? dBA : tA;
   float dev = kx_angle_delta(rel, rest);
   float C = kx_angle_delta(dev, target);
   const bool live = wSum > 0.0f;
@@ -240,7 +242,8 @@ __device__ __forceinline__ bool kx_angle_core(float &tA, float &tB, float wA, fl
   lam = ok1 ? lam1 : lam;
   tA = ok1 ? tA1 : tA;
   tB = (ok1 && hasB) ? tB1 : tB;
-  float rel2 = hasB ? kx_angle_delta(tB, tA) : tA;
+  const float dBA2 = kx_angle_delta(tB, tA);
+  float rel2 = hasB ? dBA2 : tA;
   float dev2 = kx_angle_delta(rel2, rest);
   float over = dev2 > mx ? dev2 - mx : (dev2 < mn ? dev2 - mn : 0.0f);
   const bool ok2 = live && !bad && fabsf(over) >= KX_SLACK_ANG;

@@ -122,11 +122,17 @@ struct KxPersistCfg {
   int NB, NC, NP;           // per-creature array sizes in LDS (bodies, constraints, points)
   int coll_par;             // 1: collisions through the per-creature candidate list (phase A/B);
                             // 0: each near body folded by its owning lane
+  int reg_rounds;           // KX_REG_ROUNDS: the register-slot kernel (one template of that many
+                            // rounds at 16 lanes, at most one wavefront per SIMD); 0: the LDS-slot kernel
 };
 int kx_persist_launch(int G, int wpb, const KxDev &d, const KxLaw *law, int step0, const int *step_base,
                       int nsteps, int ev_row0, int with_forces, const KxPersistCfg &P, hipStream_t st);
 int kx_persist_cr_words(int NB, int NC, int NP, int NG, int NA, bool par);
 int kx_persist_occupancy(int G, int wpb, size_t lds, bool par);   // G = 0: the mixed-lane kernel
+// register-slot kernel: 16 lanes per creature, KX_REG_ROUNDS rounds per iteration
+#define KX_REG_ROUNDS 12
+#define KX_REG_BODY_REGS 4   // bodies per lane held in registers by the register-slot kernel
+int kx_persist_occupancy_reg(int wpb, size_t lds, bool par);
 int kx_bump_counter(int *counter, int delta, hipStream_t st);
 
 // One handle's share of a batched persistent launch: its device state and

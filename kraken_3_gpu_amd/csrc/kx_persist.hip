@@ -129,9 +129,21 @@ __device__ __forceinline__ float kx_swap_pair(float v) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
 }
 
+// Register-held round slot (RR > 0): this lane's slot of one round, its
+// records read from LDS once per launch and its dynamics once per step.
+struct KxRegSlot {
+  float4 dk;
+  kx_f2v lk;
+  kx_f3v cr;
+  kx_f2v bA, bB;
+  int A, B;
+};
+
 // The body of one workgroup `bid`: WPB wavefronts of 64/G creatures of one
-// template (workgroup tables P.wg_*).
-template <int G, int WPB, bool PAR>
+// template (workgroup tables P.wg_*). RR > 0: every template of the launch
+// has exactly RR rounds, and each lane keeps its RR round slots in registers
+// (records, dynamics, lambda), so a round reads only its two poses from LDS.
+template <int G, int WPB, bool PAR, int RR = 0>
 __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, int has_law, int step0,
                                                 const int *step_base, int nsteps, int ev_row0, int with_forces,
                                                 const KxPersistCfg &P, int bid) {
@@ -181,15 +193,16 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
 
   // ── load the creature's dynamic state (global coordinates) ──
   // pose (rx, ry, ra) and velocity (rvx, rvy, rva) of the lane's bodies
-  // sub + k*G, k < KX_BODY_REGS, in registers; every loop over them is
-  // unrolled (the host guarantees nb <= KX_BODY_REGS * G)
-  float rx[KX_BODY_REGS], ry[KX_BODY_REGS], ra[KX_BODY_REGS], rvx[KX_BODY_REGS], rvy[KX_BODY_REGS],
-      rva[KX_BODY_REGS];
+  // sub + k*G, k < BR, in registers; every loop over them is unrolled (the
+  // host guarantees nb <= BR * G: KX_BODY_REGS, or KX_REG_BODY_REGS for the
+  // register-slot kernel, whose slots leave less of the register file)
+  constexpr int BR = RR > 0 ? KX_REG_BODY_REGS : KX_BODY_REGS;
+  float rx[BR], ry[BR], ra[BR], rvx[BR], rvy[BR], rva[BR];
   // bodies per lane of this workgroup's template (wave-uniform): register
   // slots past it are skipped by a scalar branch
   const int nbl = __builtin_amdgcn_readfirstlane((T.nb + G - 1) / G);
 #pragma unroll
-  for (int k = 0; k < KX_BODY_REGS; k++) {
+  for (int k = 0; k < BR; k++) {
     const int i = sub + k * G;
     rx[k] = ry[k] = ra[k] = rvx[k] = rvy[k] = rva[k] = 0.0f;
     if (i < nb) {
@@ -206,7 +219,7 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
       const int kt = t / G;
       float x = 0.0f, y = 0.0f, an = 0.0f, vx = 0.0f, vy = 0.0f;
 #pragma unroll
-      for (int k = 0; k < KX_BODY_REGS; k++) {
+      for (int k = 0; k < BR; k++) {
         if (k >= nbl) break;
         if (k == kt) { x = rx[k]; y = ry[k]; an = ra[k]; vx = rvx[k]; vy = rvy[k]; }
       }
@@ -249,6 +262,25 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
   const uint32_t rb0 = (uint32_t)__builtin_amdgcn_readfirstlane(t_rbits[0]);
   const uint32_t rb1 = (uint32_t)__builtin_amdgcn_readfirstlane(t_rbits[1]);
   KX_WAVE_SYNC();
+
+  // register-held round slots: the slot descriptor of round r for this lane
+  // (-1 idle), and the lane's records; bit r of rflags: live and not broken
+  // (set per step), bit 16 + r: a second body
+  constexpr int NR = RR > 0 ? RR : 1;
+  [[maybe_unused]] KxRegSlot slot[NR];
+  [[maybe_unused]] uint32_t rflags = 0u;
+  auto slot_dsc = [&](int r) {
+    const int4 d4 = act ? t_sched4[(r >> 2) * G + sub] : make_int4(-1, -1, -1, -1);
+    return (r & 3) == 0 ? d4.x : (r & 3) == 1 ? d4.y : (r & 3) == 2 ? d4.z : d4.w;
+  };
+  if constexpr (RR > 0) {
+#pragma unroll
+    for (int r = 0; r < RR; r++) {
+      const KxSlot c = kx_slot_read(slot_dsc(r), dyn, lam2, t_crec, t_brec);
+      slot[r] = KxRegSlot{c.dk, c.lk, c.cr, c.bA, c.bB, c.A, c.B};
+      rflags |= (c.hasB ? 1u : 0u) << (16 + r);
+    }
+  }
 
   for (int step = 0; step < nsteps; step++) {
     const int key = (step_base ? step_base[0] : step0) + step;
@@ -299,18 +331,39 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
     KX_WAVE_SYNC();
     const float ox = act ? __uint_as_float(evw[2]) : 0.0f, oy = act ? __uint_as_float(evw[3]) : 0.0f;
 #pragma unroll
-    for (int k = 0; k < KX_BODY_REGS; k++) {
+    for (int k = 0; k < BR; k++) {
       if (k >= nbl) break;
       rx[k] -= ox; ry[k] -= oy;
     }
     kx_shift_anchors(dyn, t_crec, nc, sub, G, -ox, -oy);
     KX_WAVE_SYNC();
+    if constexpr (RR > 0) {
+      // the step's dynamics (control law, reflex, origin shift) into the
+      // slots, and which slots commit this step: live and not broken (an
+      // angle keeps its broken flag in dyn.z, a pin or grip in dyn.y)
+      uint32_t lv = 0u;
+#pragma unroll
+      for (int r = 0; r < RR; r++) {
+        const int dsc = slot_dsc(r);
+        const float4 dk = dyn[dsc >= 0 ? dsc & 255 : 0];
+        slot[r].dk = dk;
+        const bool is_ang = (((r < 32 ? rb0 : rb1) >> (r & 31)) & 1u) != 0u;
+        const int brk = __float_as_int(is_ang ? dk.z : dk.y);
+        lv |= (dsc >= 0 && brk == 0 ? 1u : 0u) << r;
+      }
+      rflags = (rflags & 0xFFFF0000u) | lv;
+    }
     for (int s = 0; s < d.S; s++) {
       // resetLambda + integrate (xpbd.js:262-287)
-      for (int k = sub; k < nc; k += G) lam2[k] = kx_f2v{0.0f, 0.0f};
+      if constexpr (RR > 0) {
+#pragma unroll
+        for (int r = 0; r < RR; r++) slot[r].lk = kx_f2v{0.0f, 0.0f};
+      } else {
+        for (int k = sub; k < nc; k += G) lam2[k] = kx_f2v{0.0f, 0.0f};
+      }
       const bool forces = with_forces && step == 0 && s == 0;
 #pragma unroll
-      for (int k = 0; k < KX_BODY_REGS; k++) {
+      for (int k = 0; k < BR; k++) {
         if (k >= nbl) break;
         const int i = sub + k * G;
         if (i >= nb) continue;
@@ -336,6 +389,50 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
         // body records: none of them is written by a round), and waits only
         // for the poses. A broken constraint is solved too and its result
         // dropped.
+        if constexpr (RR > 0) {
+          // the same solves and commits on the register-held slots; lambda
+          // stays in the slot (both lanes of a pin pair keep it). The flags
+          // word is re-read each iteration so that its per-round lane masks
+          // are formed where they are used rather than held across the loop.
+          uint32_t fl = rflags;
+          asm volatile("" : "+v"(fl));
+#pragma unroll
+          for (int r = 0; r < RR; r++) {
+            KxRegSlot &c = slot[r];
+            const bool live = ((fl >> r) & 1u) != 0u, hasB = ((fl >> (16 + r)) & 1u) != 0u;
+            kx_f3v pA = *(const kx_f3v *)&pos[c.A], pB = *(const kx_f3v *)&pos[c.B];
+            const bool is_ang = (((r < 32 ? rb0 : rb1) >> (r & 31)) & 1u) != 0u;
+            if (is_ang) {
+              float tA = pA.z, tB = pB.z, lam = c.lk.x;
+              const bool moved =
+                  kx_angle_core(tA, tB, c.bA.y, c.bB.y, hasB, c.cr.x, c.dk.y, c.cr.y, c.cr.z, c.dk.x, c.dk.w, lam);
+              if (live && moved) {
+                if (hasB) ((float *)&pos[c.B])[2] = tB;
+                ((float *)&pos[c.A])[2] = tA;
+                c.lk.x = lam;
+              }
+            } else {
+              const bool odd = (sub & 1) != 0;
+              float s1, c1;
+              kx_sincos(odd ? pB.z : pA.z, &s1, &c1);
+              const float s2 = kx_swap_pair(s1), c2 = kx_swap_pair(c1);
+              float xA = pA.x, yA = pA.y, tA = pA.z, xB = pB.x, yB = pB.y, tB = pB.z;
+              float lx = c.lk.x, ly = c.lk.y;
+              const bool ok = kx_pin_core_sc(xA, yA, tA, c.bA.x, c.bA.y, xB, yB, tB, c.bB.x, c.bB.y, hasB, c.cr.x,
+                                             c.cr.y, c.dk.z, c.dk.w, c.dk.x, lx, ly, odd ? s2 : s1, odd ? c2 : c1,
+                                             odd ? s1 : s2, odd ? c1 : c2);
+              if (live && ok) {
+                if (!odd) {
+                  // B first: for a world anchor B == A and pA must win
+                  *(kx_f3v *)&pos[c.B] = kx_f3v{xB, yB, tB};
+                  *(kx_f3v *)&pos[c.A] = kx_f3v{xA, yA, tA};
+                }
+                c.lk = kx_f2v{lx, ly};
+              }
+            }
+            KX_WAVE_SYNC();
+          }
+        } else {
         int4 d4n = act ? t_sched4[sub] : make_int4(-1, -1, -1, -1);
         // one round: its poses, then the next round's records into `nx`
         // (every load unconditional and ahead of the round-type branch, so
@@ -395,6 +492,7 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
           round(sa, sb, d4.w, (rbits >> ((r0 + 2) & 31)) & 1u);
           if (r0 + 3 >= T.R) break;
           round(sb, sa, d4n.x, (rbits >> ((r0 + 3) & 31)) & 1u);
+        }
         }
         // collisions (xpbd.js:441-470: per body, its points in order, each
         // correction moving the body before the next point is tested).
@@ -607,13 +705,24 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
         KX_WAVE_SYNC();
       }
 #pragma unroll
-      for (int k = 0; k < KX_BODY_REGS; k++) {
+      for (int k = 0; k < BR; k++) {
         if (k >= nbl) break;
         const int i = sub + k * G;
         if (i >= nb) continue;
         float4 pp = pos[i];
         kx_finalize(d, rx[k], ry[k], ra[k], rvx[k], rvy[k], rva[k], pp.x, pp.y, pp.z);
         pos[i] = pp;
+      }
+      KX_WAVE_SYNC();
+    }
+    if constexpr (RR > 0) {
+      // the last substep's lambdas back to the creature region (grip break,
+      // store); a pin's even lane writes it
+#pragma unroll
+      for (int r = 0; r < RR; r++) {
+        const int dsc = slot_dsc(r);
+        const bool is_ang = (((r < 32 ? rb0 : rb1) >> (r & 31)) & 1u) != 0u;
+        if (dsc >= 0 && (is_ang || (sub & 1) == 0)) lam2[dsc & 255] = slot[r].lk;
       }
       KX_WAVE_SYNC();
     }
@@ -636,7 +745,7 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
       if (act && sub == 0) d.events[(size_t)(ev_row0 + step) * d.nCr + cid] = evw[0];
     }
 #pragma unroll
-    for (int k = 0; k < KX_BODY_REGS; k++) {
+    for (int k = 0; k < BR; k++) {
       if (k >= nbl) break;
       const int i = sub + k * G;
       rx[k] += ox; ry[k] += oy;
@@ -653,7 +762,7 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
 
   // ── store ──
 #pragma unroll
-  for (int k = 0; k < KX_BODY_REGS; k++) {
+  for (int k = 0; k < BR; k++) {
     if (k >= nbl) break;
     const int i = sub + k * G;
     if (i >= nb) continue;
@@ -690,6 +799,16 @@ __global__ __launch_bounds__(KX_WAVE *WPB) __attribute__((amdgpu_waves_per_eu(2)
                                                              const int *step_base, int nsteps, int ev_row0,
                                                              int with_forces, KxPersistCfg P) {
   kx_persist_body<G, WPB, PAR>(d, L, has_law, step0, step_base, nsteps, ev_row0, with_forces, P, blockIdx.x);
+}
+
+// The same with RR register-held round slots per lane: at most one
+// wavefront per SIMD (the slots take the register file), for launches that
+// place no more than that.
+template <int G, int WPB, bool PAR, int RR>
+__global__ __launch_bounds__(KX_WAVE *WPB) __attribute__((amdgpu_waves_per_eu(1, 1))) void kx_k_persist_reg(
+    KxDev d, KxLaw L, int has_law, int step0, const int *step_base, int nsteps, int ev_row0, int with_forces,
+    KxPersistCfg P) {
+  kx_persist_body<G, WPB, PAR, RR>(d, L, has_law, step0, step_base, nsteps, ev_row0, with_forces, P, blockIdx.x);
 }
 
 // Lane count per workgroup (per morphology bucket): a wave-uniform switch to
@@ -807,6 +926,37 @@ static int launch_gw(const KxDev &d, const KxLaw *law, int step0, const int *ste
   return hipGetLastError() == hipSuccess ? 0 : -5;
 }
 
+template <int G, int WPB, bool PAR, int RR>
+static int launch_reg(const KxDev &d, const KxLaw *law, int step0, const int *step_base, int nsteps, int ev_row0,
+                      int with_forces, const KxPersistCfg &P, hipStream_t st) {
+  size_t lds = ((size_t)P.tmpl_words + P.bm_words + P.map_words + (size_t)P.cr_words) * 4;
+  if (lds > 160 * 1024) return -22;
+  static std::atomic<uint64_t> lds_set{0};
+  allow_full_lds((const void *)kx_k_persist_reg<G, WPB, PAR, RR>, lds_set);
+  KxLaw L = law ? *law : KxLaw{};
+  hipLaunchKernelGGL((kx_k_persist_reg<G, WPB, PAR, RR>), dim3(P.nwg), dim3(KX_WAVE * WPB), lds, st, d, L,
+                     law ? 1 : 0, step0, step_base, nsteps, ev_row0, with_forces, P);
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+template <int G, int WPB, bool PAR, int RR>
+static int occ_reg(size_t lds) {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void *)kx_k_persist_reg<G, WPB, PAR, RR>, KX_WAVE * WPB,
+                                                   lds) != hipSuccess)
+    return 0;
+  return n;
+}
+
+// Resident workgroups per CU of the register-slot kernel (16 lanes, 12
+// rounds) for wpb and fold `par`, 0 when not instantiated.
+int kx_persist_occupancy_reg(int wpb, size_t lds, bool par) {
+  if (par) return wpb == 1 ? occ_reg<16, 1, true, KX_REG_ROUNDS>(lds) : wpb == 2 ? occ_reg<16, 2, true, KX_REG_ROUNDS>(lds)
+                : wpb == 4 ? occ_reg<16, 4, true, KX_REG_ROUNDS>(lds) : 0;
+  return wpb == 1 ? occ_reg<16, 1, false, KX_REG_ROUNDS>(lds) : wpb == 2 ? occ_reg<16, 2, false, KX_REG_ROUNDS>(lds)
+         : wpb == 4 ? occ_reg<16, 4, false, KX_REG_ROUNDS>(lds) : 0;
+}
+
 template <int WPB, bool PAR>
 static int launch_mix(const KxDev &d, const KxLaw *law, int step0, const int *step_base, int nsteps, int ev_row0,
                       int with_forces, const KxPersistCfg &P, hipStream_t st) {
@@ -861,6 +1011,16 @@ int kx_persist_occupancy(int G, int wpb, size_t lds, bool par) {
 template <int G, bool PAR>
 static int launch_g(int wpb, const KxDev &d, const KxLaw *law, int step0, const int *step_base, int nsteps,
                     int ev_row0, int with_forces, const KxPersistCfg &P, hipStream_t st) {
+  if constexpr (G == 16) {
+    if (P.reg_rounds == KX_REG_ROUNDS) {
+      switch (wpb) {
+        case 1: return launch_reg<16, 1, PAR, KX_REG_ROUNDS>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
+        case 2: return launch_reg<16, 2, PAR, KX_REG_ROUNDS>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
+        case 4: return launch_reg<16, 4, PAR, KX_REG_ROUNDS>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
+        default: return -22;
+      }
+    }
+  }
   switch (wpb) {
     case 1: return launch_gw<G, 1, PAR>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
     case 2: return launch_gw<G, 2, PAR>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);

@@ -298,7 +298,13 @@ def schedule_rounds(g):
 
 
 PERSIST_INFO = ("lds_bytes", "workgroups", "resident_per_cu", "needed_per_cu", "map_words", "bitmap_words",
-                "template_words", "creature_words", "mixed_lanes", "collision_fold")
+                "template_words", "creature_words", "mixed_lanes", "collision_fold", "register_rounds")
+
+
+def set_register_slots(g, enable):
+    """Allow (True, the default) or forbid register-held round slots in the
+    persistent kernel (kx_set_register_slots); results are identical."""
+    K.check(K.lib().kx_set_register_slots(g.p, 1 if enable else 0))
 
 
 def persist_info(g):

@@ -24,7 +24,7 @@ def test_library_exports_every_declared_symbol():
     L = K.lib()
     for n in sorted(names):
         assert hasattr(L, n), n
-    assert L.kx_abi_version() == 4
+    assert L.kx_abi_version() == 5
 
 
 def _oracle_like(w):

@@ -665,3 +665,38 @@ def test_state_readout_matches_oracle(world):
             prev[c] = (w.vx[m], w.vy[m])
     say(f"    {g.nCr} creatures x 6 steps within the bar (worst {worst:.2f} of it), {loads} loaded grips")
     G.destroy(g)
+
+
+def test_register_slots_bit_identical():
+    """The persistent kernel with each lane's round slots held in registers
+    (a uniform batch of the 12-round 6 x 7 template at 16 lanes, one
+    wavefront per SIMD) against the same kernel reading its slots from LDS
+    each round and against the plain path: the same solves in the same
+    order, so every state and event agrees bit for bit over a rollout that
+    reaches the walls."""
+    law = W.default_law(31)
+    make, steps = (lambda: W.world_uniform(1024, seed=5)), 120
+    runs = {}
+    for name, path, reg in (("plain", K.KX_PATH_PLAIN, None), ("lds", K.KX_PATH_PERSISTENT, False),
+                            ("registers", K.KX_PATH_PERSISTENT, True)):
+        w = make()
+        g = G.create(w, path=path)
+        if reg is not None:
+            G.set_register_slots(g, reg)
+            assert G.persist_info(g)["register_rounds"] == (12 if reg else 0)
+        G.upload(g)
+        say(f"[case] register slots: {name}, {steps} steps")
+        ev = G.rollout(g, law, 0, steps, events=True)
+        G.download(g)
+        ctl = G.download_controls(g)
+        G.destroy(g)
+        runs[name] = (w, ev, ctl)
+    wp, evp, cp = runs["plain"]
+    for name in ("lds", "registers"):
+        wq, evq, cq = runs[name]
+        for key in ("px", "py", "ang", "vx", "vy", "av", "broken", "lambda_", "lambdaX", "lambdaY"):
+            assert np.array_equal(getattr(wp, key), getattr(wq, key)), (name, key)
+        for key in cp:
+            assert np.array_equal(cp[key], cq[key]), (name, key)
+        assert np.array_equal(evp, evq), name
+    say(f"    plain == lds == registers ({len(decode_events(evp))} events)")

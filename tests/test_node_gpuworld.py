@@ -51,7 +51,7 @@ def test_addon_loads_and_exports_the_surface():
                        capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stderr
     d = json.loads(r.stdout)
-    assert d["abi"] == 4 and d["avail"] is True
+    assert d["abi"] == 5 and d["avail"] is True
     for name in ("create", "destroy", "upload", "packControls", "step", "download", "rollout", "groupPoints",
                  "readState", "policyCreate", "policyWeights", "policyGenomes", "policyForward", "esCreate",
                  "esGeneration", "esUpdate", "esGet", "brainDestroy"):
