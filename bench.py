@@ -72,12 +72,12 @@ def algorithmic_launch_bytes(w, nsteps):
     return bin_ + bout
 
 
-def measured_traffic(n_cr, chunk, workload="uniform4096"):
+def measured_traffic(n_cr, chunk, workload="uniform4096", kernel="kx_k_persist"):
     """HBM bytes per launch from the committed rocprofv3 counter summaries
     (FETCH_SIZE + WRITE_SIZE, copy-calibrated), when one covers this
-    workload and launch length."""
+    kernel, workload and launch length."""
     import glob
-    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", f"kx_k_persist_{workload}*_traffic.json"))):
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", f"{kernel}_{workload}*_traffic.json"))):
         try:
             with open(path) as f:
                 t = json.load(f)
@@ -654,7 +654,9 @@ def main():
     timed_tflops = F * n_cr * args.steps / (dev_ms * 1e-3) / 1e12
     launch_bytes = algorithmic_launch_bytes(w, chunk)
     hbm_gbs = launch_bytes / (launch_ms * 1e-3) / 1e9
-    traffic = measured_traffic(n_cr, chunk)
+    # the kernel the launches ran: register-held round slots or slots read from LDS
+    kernel = "kx_k_persist_reg" if shape.get("register_rounds") else "kx_k_persist"
+    traffic = measured_traffic(n_cr, chunk, "uniform4096", kernel)
     plain = None
     if not args.no_plain:
         wp = W.world_uniform(args.creatures, seed=1)
@@ -701,7 +703,7 @@ def main():
                      "algorithmic_bytes_per_launch": launch_bytes,
                      "flops_per_creature_step": F, "steps_per_launch": chunk, "creatures_per_launch": n_cr,
                      "grips_attached_mean": g_att, "rects_within_16px_mean": r_near,
-                     "kernel": "kx_k_persist", "launch_ms": launch_ms,
+                     "kernel": kernel, "launch_ms": launch_ms,
                      "launch_ms_source": f"mean of {lat['samples']} launches of {chunk} steps (HIP events)",
                      "achieved_timed_region": timed_tflops,
                      "hbm_algorithmic_gbs": hbm_gbs, "hbm_peak_gbs": HBM_PEAK_GBS,
