@@ -17,6 +17,9 @@ FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
     # no packed-fp32 instructions anywhere in the device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # fused multiply-adds only where one source expression asks for them:
+    # device functions inlined into different kernels then round alike
+    "-ffp-contract=on",
     "-Wall", "-Wno-unused-function", "-Wno-unused-result",
 ]
 

@@ -413,8 +413,9 @@ def test_non_finite_state_follows_the_reference_guards(path):
     non-finite velocity (xpbd.js:280-281), an infinite spin clamps to the
     limit, a NaN pose stays NaN without spreading (every solve skips a
     non-finite gap, xpbd.js:350, 361, 407), a NaN first body does not shift
-    its creature. The device's non-finite pattern equals the oracle's after
-    each of 3 steps and every finite body stays within 1e-3 px."""
+    its creature. Over 3 steps, each one from the device's state loaded into
+    the oracle: the device's non-finite pattern equals the oracle's and every
+    finite body stays within 1e-3 px (the one-step bar)."""
     w = W.world_cpu64(8, seed=2)
     o = oracle_twin(w)
     g = G.create(w, path=PATHS[path])
@@ -438,6 +439,14 @@ def test_non_finite_state_follows_the_reference_guards(path):
     o.set_step_counter(10)
     say(f"[case] non-finite state path={path}")
     for s in range(3):
+        if s:
+            # the oracle restarts from the device's state (one-step parity)
+            ctl = G.download_controls(g)
+            w.lbx[:], w.lby[:] = ctl["lbx"], ctl["lby"]
+            w.compliance[:], w.target[:], w.breakForce[:] = ctl["compliance"], ctl["target"], ctl["breakForce"]
+            G.download_cooldown(g)
+            load_state(o, w)
+            o.set_step_counter(10 + s)
         o.apply_law(3, 10 + s)
         o.step()
         G.rollout(g, law, 10 + s, 1)

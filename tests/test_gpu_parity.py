@@ -700,3 +700,43 @@ def test_register_slots_bit_identical():
             assert np.array_equal(cp[key], cq[key]), (name, key)
         assert np.array_equal(evp, evq), name
     say(f"    plain == lds == registers ({len(decode_events(evp))} events)")
+
+
+def test_folds_bit_identical_over_a_long_rollout():
+    """Every persistent variant (register-held or LDS round slots, parallel or
+    owner collision fold) against the plain path over 1250 steps of the
+    4096-creature uniform batch, as its creatures pile against the rects. A
+    contact whose penetration sits near the 2e-3 px dead zone turns a
+    one-ulp difference in a query into a different correction, so this is
+    where an inlining context that rounds differently shows (the owner fold
+    once parted from the plain path here, at step ~1150, before the build
+    fixed where multiply-adds fuse)."""
+    law = W.default_law(2024)
+    steps = 1250
+    runs = {}
+    for name, path, reg, fold in (("plain", K.KX_PATH_PLAIN, None, None),
+                                  ("reg_parallel", K.KX_PATH_PERSISTENT, True, K.KX_FOLD_PARALLEL),
+                                  ("reg_owner", K.KX_PATH_PERSISTENT, True, K.KX_FOLD_OWNER),
+                                  ("lds_owner", K.KX_PATH_PERSISTENT, False, K.KX_FOLD_OWNER)):
+        w = W.world_uniform(4096, seed=1)
+        g = G.create(w, path=path)
+        if reg is not None:
+            G.set_register_slots(g, reg)
+            G.set_collision_fold(g, fold)
+        G.upload(g)
+        say(f"[case] long rollout: {name}, {steps} steps")
+        ev = G.rollout(g, law, 0, steps, events=True)
+        G.download(g)
+        G.destroy(g)
+        runs[name] = (w, ev)
+    wp, evp = runs["plain"]
+    for name in ("reg_parallel", "reg_owner", "lds_owner"):
+        wq, evq = runs[name]
+        for key in ("px", "py", "ang", "vx", "vy", "av", "broken", "lambda_", "lambdaX", "lambdaY"):
+            a, b = getattr(wp, key), getattr(wq, key)
+            nd = int(np.count_nonzero(a != b))
+            if nd:
+                say(f"    {name} {key}: {nd} differ, max {np.max(np.abs(a.astype(np.float64) - b)):.3e}")
+            assert nd == 0, (name, key)
+        assert np.array_equal(evp, evq), name
+    say(f"    plain == every persistent variant over {steps} steps ({len(decode_events(evp))} events)")
