@@ -345,6 +345,7 @@ static int build_grid(kx_handle *h, const float *rects, int nR, float rmax, floa
   h->d.cell_off = doff; h->d.cell_ids = dids;
   h->n_cell_ids = (int)ids.size();
   h->d.gx0 = gx0; h->d.gy0 = gy0; h->d.inv_cs = 1.0f / cs; h->d.gnx = gnx; h->d.gny = gny;
+  h->d.cull_r = R;
   return 0;
 }
 
@@ -455,11 +456,20 @@ static int build_persist(kx_handle *h) {
   // Register-held round slots: one template of KX_REG_ROUNDS rounds at 16
   // lanes, and every workgroup resident at once at one wavefront per SIMD
   // (the slots take the register file, so a SIMD holds no second wave).
+  // The kernel also fixes the round types (KX_REG_RTYPE) and solves every
+  // motor as one between two bodies with a finite rest angle.
   int reg = 0;
   if (h->reg_mode && !mixed && Gk == 16) {
     bool all = true;
-    for (size_t t = 0; t < T.size(); t++)
-      if (!members[t].empty() && (T[t].R != KX_REG_ROUNDS || T[t].nb > KX_REG_BODY_REGS * 16)) all = false;
+    for (size_t t = 0; t < T.size(); t++) {
+      if (members[t].empty()) continue;
+      if (T[t].R != KX_REG_ROUNDS || T[t].nb > KX_REG_BODY_REGS * 16 ||
+          pool[(size_t)T[t].base + T[t].o_rtype] != (uint32_t)KX_REG_RTYPE)
+        all = false;
+      const TmplHost &th = h->tmpl_host[t];
+      for (size_t k = 0; k < th.type.size(); k++)
+        if (th.type[k] == KX_ANGLE && (th.b[k] < 0 || !std::isfinite(th.rest[k]))) all = false;
+    }
     const size_t lds_used = base + (size_t)mapw * 4;
     if (all && (long)nwg * WPB <= 4L * ncu && kx_persist_occupancy_reg(WPB, lds_used, par) >= need) reg = KX_REG_ROUNDS;
   }

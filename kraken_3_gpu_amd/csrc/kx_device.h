@@ -34,6 +34,14 @@ __device__ __forceinline__ float kx_fold(float d) {
   return isfinite(d) ? r : 0.0f;
 }
 
+// kx_fold of a difference known to be finite (a fold result less a finite
+// constant): the same operations without the finiteness select
+__device__ __forceinline__ float kx_fold_finite(float d) {
+  float q = rintf(d * KX_INV_TAU);
+  float r = fmaf(-q, KX_TAU_HI, d);
+  return fmaf(-q, KX_TAU_LO, r);
+}
+
 // angleDelta(a, b) (xpbd.js:137-143): the difference folded into (-pi, pi]
 __device__ __forceinline__ float kx_angle_delta(float a, float b) { return kx_fold(a - b); }
 // wrapAngle (xpbd.js:128-134), only reached when |a| > 50
@@ -252,6 +260,39 @@ __device__ __forceinline__ bool kx_angle_core(float &tA, float &tB, float wA, fl
   float tB2 = tB + wB * dl2;
   tA = ok2 ? tA2 : tA;
   tB = (ok2 && hasB) ? tB2 : tB;
+  return ok1 || ok2;
+}
+
+// kx_angle_core for a motor between two bodies with a finite rest angle: the
+// operations of its hasB path in the same order, without the selects of a
+// world-anchored motor, and without the finiteness select of dev and dev2,
+// whose arguments (a fold result less the finite rest) are always finite.
+__device__ __forceinline__ bool kx_angle_core_ab(float &tA, float &tB, float wA, float wB, float rest, float target,
+                                                 float mn, float mx, float alpha, float rS, float &lam) {
+  const float wSum = wA + wB;
+  const float rel = kx_angle_delta(tB, tA);
+  const float dev = kx_fold_finite(rel - rest);
+  const float C = kx_angle_delta(dev, target);
+  const bool live = wSum > 0.0f;
+  const bool corr = fabsf(C) >= KX_SLACK_ANG;
+  float dl = -(C + alpha * lam) * kx_rcp(wSum + alpha);
+  const bool bad = corr && !isfinite(dl);
+  const bool ok1 = live && corr && !bad;
+  float lam1 = lam + dl;
+  float tA1 = tA - wA * dl;
+  float tB1 = tB + wB * dl;
+  lam = ok1 ? lam1 : lam;
+  tA = ok1 ? tA1 : tA;
+  tB = ok1 ? tB1 : tB;
+  const float rel2 = kx_angle_delta(tB, tA);
+  const float dev2 = kx_fold_finite(rel2 - rest);
+  float over = dev2 > mx ? dev2 - mx : (dev2 < mn ? dev2 - mn : 0.0f);
+  const bool ok2 = live && !bad && fabsf(over) >= KX_SLACK_ANG;
+  float dl2 = -over * rS;
+  float tA2 = tA - wA * dl2;
+  float tB2 = tB + wB * dl2;
+  tA = ok2 ? tA2 : tA;
+  tB = ok2 ? tB2 : tB;
   return ok1 || ok2;
 }
 

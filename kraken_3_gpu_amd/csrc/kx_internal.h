@@ -60,6 +60,7 @@ struct KxDev {
   int cell_words, near_words;    // words of each bitmap (multiples of 4)
   float gx0, gy0, inv_cs;
   int gnx, gny;
+  float cull_r;                  // every rect within cull_r + 1 px of a cull cell is registered in it
   float inv_ns;                  // near-bitmap cells per px (a power-of-two multiple of inv_cs)
   int nnx, nny;                  // near-bitmap grid, same origin as the cull grid
   // creatures / arms
@@ -131,6 +132,9 @@ int kx_persist_cr_words(int NB, int NC, int NP, int NG, int NA, bool par);
 int kx_persist_occupancy(int G, int wpb, size_t lds, bool par);   // G = 0: the mixed-lane kernel
 // register-slot kernel: 16 lanes per creature, KX_REG_ROUNDS rounds per iteration
 #define KX_REG_ROUNDS 12
+// its round types, bit r set = round r solves angle motors: the schedule of the
+// 6-arm x 7-segment creature (pins and motors alternate along the mantle chain)
+#define KX_REG_RTYPE 0xAAA
 #define KX_REG_BODY_REGS 4   // bodies per lane held in registers by the register-slot kernel
 int kx_persist_occupancy_reg(int wpb, size_t lds, bool par);
 int kx_bump_counter(int *counter, int delta, hipStream_t st);

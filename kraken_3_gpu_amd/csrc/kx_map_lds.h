@@ -23,6 +23,37 @@ __device__ __forceinline__ bool kx_sdf_cells_lds(const KxDev &d, const uint16_t 
   return isfinite(h.d);
 }
 
+// Contact test of a body centre (x, y) (relative to the origin (ox, oy))
+// against the map staged in LDS: true when the box of some rect registered in
+// the centre's cull cell lies closer than D to the centre. The box of a rect
+// is the set its clamp maps onto (kx_fold_rect: min(max(x, rx), rx + w)), a
+// convex set, so the distance to it is 1-Lipschitz: a point within reach of
+// the centre whose fold finds a rect closer than its radius r has that rect
+// within reach + r of the centre. With D at least that and no larger than the
+// cull radius, every such rect is registered in the centre's cell (cells hold
+// every rect within the cull radius + 1 px). A centre outside the grid lies
+// farther than the cull radius + one cell from every rect, and a non-finite
+// centre is never near (its points never hit either).
+__device__ __forceinline__ bool kx_centre_near_lds(const KxDev &d, const uint16_t *coff, const float4 *crect, float x,
+                                                   float y, float ox, float oy, float D) {
+  const float gx = (x + ox - d.gx0) * d.inv_cs;
+  const float gy = (y + oy - d.gy0) * d.inv_cs;
+  if (!(gx >= 0.0f && gy >= 0.0f && gx < (float)d.gnx && gy < (float)d.gny)) return false;
+  const int ix = min((int)gx, d.gnx - 1), iy = min((int)gy, d.gny - 1);
+  const int cell = iy * d.gnx + ix;
+  const int b = coff[cell], e = coff[cell + 1];
+  const float D2 = D * D;
+  bool near = false;
+  for (int k = b; k < e; k++) {
+    const float4 r = crect[k];
+    const float rx = r.x - ox, ry = r.y - oy;
+    const float cx = fminf(fmaxf(x, rx), rx + r.z), cy = fminf(fmaxf(y, ry), ry + r.w);
+    const float dx = x - cx, dy = y - cy;
+    near = near || dx * dx + dy * dy < D2;
+  }
+  return near;
+}
+
 // Grid-culled SDF query against whichever copy of the map the launch uses.
 __device__ __forceinline__ bool kx_query(const KxDev &d, const KxPersistCfg &P, const uint32_t *bits,
                                          const uint16_t *m_off, const float4 *m_crect, float wx, float wy,

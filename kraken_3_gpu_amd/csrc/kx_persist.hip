@@ -347,7 +347,7 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
         const int dsc = slot_dsc(r);
         const float4 dk = dyn[dsc >= 0 ? dsc & 255 : 0];
         slot[r].dk = dk;
-        const bool is_ang = (((r < 32 ? rb0 : rb1) >> (r & 31)) & 1u) != 0u;
+        const bool is_ang = ((KX_REG_RTYPE >> r) & 1) != 0;
         const int brk = __float_as_int(is_ang ? dk.z : dk.y);
         lv |= (dsc >= 0 && brk == 0 ? 1u : 0u) << r;
       }
@@ -401,13 +401,15 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
             KxRegSlot &c = slot[r];
             const bool live = ((fl >> r) & 1u) != 0u, hasB = ((fl >> (16 + r)) & 1u) != 0u;
             kx_f3v pA = *(const kx_f3v *)&pos[c.A], pB = *(const kx_f3v *)&pos[c.B];
-            const bool is_ang = (((r < 32 ? rb0 : rb1) >> (r & 31)) & 1u) != 0u;
-            if (is_ang) {
+            // the round types are the kernel's (the host launches it only for
+            // templates whose schedule has them, every motor between two
+            // bodies with a finite rest angle)
+            if (((KX_REG_RTYPE >> r) & 1) != 0) {   // folded by the unrolled loop
               float tA = pA.z, tB = pB.z, lam = c.lk.x;
               const bool moved =
-                  kx_angle_core(tA, tB, c.bA.y, c.bB.y, hasB, c.cr.x, c.dk.y, c.cr.y, c.cr.z, c.dk.x, c.dk.w, lam);
+                  kx_angle_core_ab(tA, tB, c.bA.y, c.bB.y, c.cr.x, c.dk.y, c.cr.y, c.cr.z, c.dk.x, c.dk.w, lam);
               if (live && moved) {
-                if (hasB) ((float *)&pos[c.B])[2] = tB;
+                ((float *)&pos[c.B])[2] = tB;
                 ((float *)&pos[c.A])[2] = tA;
                 c.lk.x = lam;
               }
@@ -558,6 +560,22 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
             int wi, bi;
             cell(pb.x, pb.y, wi, bi);
             if (!use_bm || ((nearb[wi] >> bi) & 1u) != 0u) mine |= 1u << k;
+          }
+          // Second stage, exact per body: the near bitmap bounds every body
+          // with the largest contact radius of the template and a whole cell
+          // of slack; a body whose own contact radius (brec.w: offset length
+          // + radius over its points, with a rounding margin) is within the
+          // cull radius is kept only if some rect's box lies within that
+          // radius of its centre (kx_centre_near_lds). A body dropped here
+          // has no point that corrects at its pose after the rounds, so its
+          // in-order fold would leave it unmoved.
+          if (P.map_words != 0) {
+            for (uint32_t m = mine; m; m &= m - 1u) {
+              const int k = __builtin_ctz(m), i = sub + k * G;
+              const float D = t_brec[i].w;
+              const float2 pb = *(const float2 *)&pos[i];
+              if (D <= d.cull_r && !kx_centre_near_lds(d, m_off, m_crect, pb.x, pb.y, ox, oy, D)) mine &= ~(1u << k);
+            }
           }
           // Two exact ways to fold the near bodies, chosen per launch:
           //  - parallel (launches of at most one wave per SIMD, whose time is
@@ -721,7 +739,7 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
 #pragma unroll
       for (int r = 0; r < RR; r++) {
         const int dsc = slot_dsc(r);
-        const bool is_ang = (((r < 32 ? rb0 : rb1) >> (r & 31)) & 1u) != 0u;
+        const bool is_ang = ((KX_REG_RTYPE >> r) & 1) != 0;
         if (dsc >= 0 && (is_ang || (sub & 1) == 0)) lam2[dsc & 255] = slot[r].lk;
       }
       KX_WAVE_SYNC();

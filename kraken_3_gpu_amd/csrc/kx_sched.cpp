@@ -114,7 +114,7 @@ static uint32_t fbits(float x) {
 // are 16-byte vectors so the kernel reads each with one LDS instruction:
 //   crec[k] = {lax, lay, -, cab} (pin, grip) or {rest, min, max, cab}
 //             (angle), cab = a | b << 8 | type << 16            (16 B)
-//   brec[b] = {im, ii, point offset | count << 16, reach of its points}
+//   brec[b] = {im, ii, point offset | count << 16, contact radius}
 //   prec[q] = {lx, ly, r, body}
 KxTmpl kx_pack_template(const TmplHost &t, int G, std::vector<uint32_t> &pool) {
   KxTmpl T{};
@@ -139,13 +139,21 @@ KxTmpl kx_pack_template(const TmplHost &t, int G, std::vector<uint32_t> &pool) {
   }
   T.o_brec = at();
   for (int b = 0; b < t.nb; b++) {
-    // reach: largest distance of the body's collision points from its centre
-    float reach = 0.0f;
+    // contact radius: the largest distance from the body's centre at which
+    // one of its collision points can touch a rect (offset length + radius),
+    // plus 0.05 px for the binary32 rounding of world points and distances,
+    // rounded up to binary32
+    double reach = 0.0;
     for (int j = 0; j < t.pcnt[b]; j++) {
       int q = t.poff[b] + j;
-      reach = std::max(reach, (float)std::sqrt((double)t.plx[q] * t.plx[q] + (double)t.ply[q] * t.ply[q]));
+      reach = std::max(reach, std::sqrt((double)t.plx[q] * t.plx[q] + (double)t.ply[q] * t.ply[q]) +
+                                  std::fabs((double)t.pr[q]));
     }
-    pool.insert(pool.end(), {fbits(t.im[b]), fbits(t.ii[b]), (uint32_t)(t.poff[b] | (t.pcnt[b] << 16)), fbits(reach)});
+    const double want = reach + 0.05;
+    float cr = (float)want;
+    if ((double)cr < want) cr = std::nextafter(cr, INFINITY);
+    if (!std::isfinite(reach)) cr = INFINITY;
+    pool.insert(pool.end(), {fbits(t.im[b]), fbits(t.ii[b]), (uint32_t)(t.poff[b] | (t.pcnt[b] << 16)), fbits(cr)});
   }
   T.o_prec = at();
   std::vector<int> pbody(t.plx.size(), 0);
