@@ -469,6 +469,19 @@ static int build_persist(kx_handle *h) {
       const TmplHost &th = h->tmpl_host[t];
       for (size_t k = 0; k < th.type.size(); k++)
         if (th.type[k] == KX_ANGLE && (th.b[k] < 0 || !std::isfinite(th.rest[k]))) all = false;
+      // fused rounds: every motor of an angle round sits on the even lane of
+      // the pin before it with the same two bodies
+      if (all) {
+        const uint32_t *desc = pool.data() + T[t].base + T[t].o_sched;
+        auto slot = [&](int r, int lane) { return (int)desc[((r >> 2) * 16 + lane) * 4 + (r & 3)]; };
+        for (int r = 1; r < KX_REG_ROUNDS; r += 2)
+          for (int l = 0; l < 16; l++) {
+            const int a = slot(r, l);
+            if (a < 0) continue;
+            const int p = slot(r - 1, l);
+            if ((l & 1) || p < 0 || (p >> 8) != (a >> 8)) all = false;
+          }
+      }
     }
     const size_t lds_used = base + (size_t)mapw * 4;
     if (all && (long)nwg * WPB <= 4L * ncu && kx_persist_occupancy_reg(WPB, lds_used, par) >= need) reg = KX_REG_ROUNDS;

@@ -390,47 +390,49 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
         // for the poses. A broken constraint is solved too and its result
         // dropped.
         if constexpr (RR > 0) {
-          // the same solves and commits on the register-held slots; lambda
-          // stays in the slot (both lanes of a pin pair keep it). The flags
-          // word is re-read each iteration so that its per-round lane masks
-          // are formed where they are used rather than held across the loop.
+          // the same solves on the register-held slots, a joint's hinge and
+          // its muscle fused: rounds 2f (pins and grips on lane pairs) and
+          // 2f + 1 (motors, each on the even lane of the pin with the same
+          // two bodies; the host launches this kernel only for such
+          // schedules) run as one. Both lanes of a pair solve the pin alike
+          // (same operands, so the same bits); the motor is then solved on
+          // the pose the pin leaves (its own pose if the pin does not
+          // commit), which is the pose the motor's round would read, and the
+          // even lane writes the pair's two bodies once. Lambda stays in the
+          // slot (both lanes of a pin pair keep it). The flags word is re-read
+          // each iteration so that its per-round lane masks are formed where
+          // they are used rather than held across the loop.
+          static_assert(KX_REG_RTYPE == 0xAAA && RR == 12, "fused rounds: pins on even rounds, motors on odd");
           uint32_t fl = rflags;
           asm volatile("" : "+v"(fl));
+          const bool odd = (sub & 1) != 0;
 #pragma unroll
-          for (int r = 0; r < RR; r++) {
-            KxRegSlot &c = slot[r];
-            const bool live = ((fl >> r) & 1u) != 0u, hasB = ((fl >> (16 + r)) & 1u) != 0u;
-            kx_f3v pA = *(const kx_f3v *)&pos[c.A], pB = *(const kx_f3v *)&pos[c.B];
-            // the round types are the kernel's (the host launches it only for
-            // templates whose schedule has them, every motor between two
-            // bodies with a finite rest angle)
-            if (((KX_REG_RTYPE >> r) & 1) != 0) {   // folded by the unrolled loop
-              float tA = pA.z, tB = pB.z, lam = c.lk.x;
-              const bool moved =
-                  kx_angle_core_ab(tA, tB, c.bA.y, c.bB.y, c.cr.x, c.dk.y, c.cr.y, c.cr.z, c.dk.x, c.dk.w, lam);
-              if (live && moved) {
-                ((float *)&pos[c.B])[2] = tB;
-                ((float *)&pos[c.A])[2] = tA;
-                c.lk.x = lam;
-              }
-            } else {
-              const bool odd = (sub & 1) != 0;
-              float s1, c1;
-              kx_sincos(odd ? pB.z : pA.z, &s1, &c1);
-              const float s2 = kx_swap_pair(s1), c2 = kx_swap_pair(c1);
-              float xA = pA.x, yA = pA.y, tA = pA.z, xB = pB.x, yB = pB.y, tB = pB.z;
-              float lx = c.lk.x, ly = c.lk.y;
-              const bool ok = kx_pin_core_sc(xA, yA, tA, c.bA.x, c.bA.y, xB, yB, tB, c.bB.x, c.bB.y, hasB, c.cr.x,
-                                             c.cr.y, c.dk.z, c.dk.w, c.dk.x, lx, ly, odd ? s2 : s1, odd ? c2 : c1,
-                                             odd ? s1 : s2, odd ? c1 : c2);
-              if (live && ok) {
-                if (!odd) {
-                  // B first: for a world anchor B == A and pA must win
-                  *(kx_f3v *)&pos[c.B] = kx_f3v{xB, yB, tB};
-                  *(kx_f3v *)&pos[c.A] = kx_f3v{xA, yA, tA};
-                }
-                c.lk = kx_f2v{lx, ly};
-              }
+          for (int f = 0; f < RR / 2; f++) {
+            KxRegSlot &c = slot[2 * f], &m = slot[2 * f + 1];
+            const bool livep = ((fl >> (2 * f)) & 1u) != 0u, hasB = ((fl >> (16 + 2 * f)) & 1u) != 0u;
+            const bool livem = ((fl >> (2 * f + 1)) & 1u) != 0u;
+            const kx_f3v pA = *(const kx_f3v *)&pos[c.A], pB = *(const kx_f3v *)&pos[c.B];
+            float s1, c1;
+            kx_sincos(odd ? pB.z : pA.z, &s1, &c1);
+            const float s2 = kx_swap_pair(s1), c2 = kx_swap_pair(c1);
+            float xA = pA.x, yA = pA.y, tA = pA.z, xB = pB.x, yB = pB.y, tB = pB.z;
+            float lx = c.lk.x, ly = c.lk.y;
+            // (every lane runs both solves: the live flags only select)
+            const bool okp = kx_pin_core_sc(xA, yA, tA, c.bA.x, c.bA.y, xB, yB, tB, c.bB.x, c.bB.y, hasB, c.cr.x,
+                                            c.cr.y, c.dk.z, c.dk.w, c.dk.x, lx, ly, odd ? s2 : s1, odd ? c2 : c1,
+                                            odd ? s1 : s2, odd ? c1 : c2) &&
+                             livep;
+            if (okp) c.lk = kx_f2v{lx, ly};
+            xA = okp ? xA : pA.x; yA = okp ? yA : pA.y; tA = okp ? tA : pA.z;
+            xB = okp ? xB : pB.x; yB = okp ? yB : pB.y; tB = okp ? tB : pB.z;
+            float uA = tA, uB = tB, lam = m.lk.x;
+            const bool okm =
+                kx_angle_core_ab(uA, uB, m.bA.y, m.bB.y, m.cr.x, m.dk.y, m.cr.y, m.cr.z, m.dk.x, m.dk.w, lam) && livem;
+            if (okm) { tA = uA; tB = uB; m.lk.x = lam; }
+            if (!odd && (okp || okm)) {
+              // B first: for a world anchor B == A and pA must win
+              *(kx_f3v *)&pos[c.B] = kx_f3v{xB, yB, tB};
+              *(kx_f3v *)&pos[c.A] = kx_f3v{xA, yA, tA};
             }
             KX_WAVE_SYNC();
           }

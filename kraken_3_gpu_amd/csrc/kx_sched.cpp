@@ -166,17 +166,42 @@ KxTmpl kx_pack_template(const TmplHost &t, int G, std::vector<uint32_t> &pool) {
   // round's j-th constraint occupies lanes 2j and 2j+1. Stored by groups of
   // four rounds, lane-major within a group: desc4[(grp * G + lane) * 4 + r % 4]
   // so a lane reads a group's four slots with one 16-byte load.
+  // A motor whose two bodies are those of a pin in the round before (a
+  // joint's hinge and its muscle) sits on that pin's even lane, the others on
+  // the lowest free lanes; lanes within a round are interchangeable, and this
+  // placement lets a kernel solve the motor right after the pin on the pose
+  // it holds (the register-slot kernel, kx_persist.hip).
   const int R4 = (T.R + 3) / 4;
   std::vector<int> desc((size_t)R4 * G * 4, -1);
   auto put_slot = [&](int r, int lane, int v) { desc[((size_t)(r >> 2) * G + lane) * 4 + (r & 3)] = v; };
-  for (int r = 0; r < T.R; r++)
-    for (int j = 0; j < G; j++) {
-      int k = sched[(size_t)r * G + j];
-      if (k < 0) continue;
-      int v = k | (t.a[k] << 8) | ((t.b[k] < 0 ? 255 : t.b[k]) << 16);
-      if (rtype[r]) put_slot(r, j, v);
-      else { put_slot(r, 2 * j, v); put_slot(r, 2 * j + 1, v); }
+  auto slot_of = [&](int r, int lane) { return desc[((size_t)(r >> 2) * G + lane) * 4 + (r & 3)]; };
+  auto pack = [&](int k) { return k | (t.a[k] << 8) | ((t.b[k] < 0 ? 255 : t.b[k]) << 16); };
+  for (int r = 0; r < T.R; r++) {
+    if (!rtype[r]) {
+      for (int j = 0; j < G; j++) {
+        const int k = sched[(size_t)r * G + j];
+        if (k >= 0) { put_slot(r, 2 * j, pack(k)); put_slot(r, 2 * j + 1, pack(k)); }
+      }
+      continue;
     }
+    std::vector<char> used(G, 0);
+    std::vector<int> rest;
+    for (int j = 0; j < G; j++) {
+      const int k = sched[(size_t)r * G + j];
+      if (k < 0) continue;
+      int lane = -1;
+      if (r > 0 && !rtype[r - 1] && t.b[k] >= 0)
+        for (int l = 0; l < G; l += 2) {
+          const int p = slot_of(r - 1, l);
+          if (p >= 0 && !used[l] && (p >> 8) == (pack(k) >> 8)) { lane = l; break; }
+        }
+      if (lane >= 0) { put_slot(r, lane, pack(k)); used[lane] = 1; }
+      else rest.push_back(k);
+    }
+    for (int k : rest)
+      for (int l = 0; l < G; l++)
+        if (!used[l]) { put_slot(r, l, pack(k)); used[l] = 1; break; }
+  }
   T.o_sched = at(); put(pool, desc);
   // round types as a bitmask, 32 rounds per word (bit set = angle round)
   std::vector<int> rbits(std::max<size_t>((rtype.size() + 31) / 32, 2), 0);   // at least two words

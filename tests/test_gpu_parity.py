@@ -559,18 +559,60 @@ def test_stress_map_full_size_properties_and_subset_parity():
     G.destroy(g)
 
 
+BODY_KEYS = ("px", "py", "ang", "vx", "vy", "av")
+CONS_KEYS = ("broken", "lbx", "lby", "compliance", "target", "breakForce")
+
+
+def _f64_spread(w, c, pre, seed, step, draws=16):
+    """Largest change of the f64 oracle's one-step result for creature c when
+    its binary32 input pose (px, py, ang) moves to a neighbouring binary32
+    value (random signs, fixed seed): the oracle's own discontinuity at this
+    state, where the dead zones (xpbd.js SLACK_POS / SLACK_ANG), joint limits
+    and contacts switch a correction on or off."""
+    rng = np.random.default_rng(0)
+
+    def one(perturb):
+        o, bidx, cidx = oracle_subset(w, [c])
+        st = {k: pre[k][bidx].astype(np.float32).astype(np.float64) for k in BODY_KEYS}
+        if perturb:
+            for k in ("px", "py", "ang"):
+                u = np.spacing(np.abs(st[k]).astype(np.float32)).astype(np.float64)
+                st[k] = st[k] + u * rng.choice([-1.0, 1.0], size=len(u))
+        o.set_bodies(*(st[k] for k in BODY_KEYS))
+        o.set_cons_dynamic(broken=pre["broken"][cidx].astype(np.int32),
+                           **{k: pre[k][cidx].astype(np.float32) for k in CONS_KEYS[1:]})
+        o.set_cd(np.ascontiguousarray(pre["cd"][[c]]))
+        o.set_step_counter(step)
+        o.clear_events()
+        _apply_law_keyed(o, seed, step, [c])
+        o.step()
+        b = o.bodies()
+        return b["px"], b["py"]
+
+    x0, y0 = one(False)
+    return max(float(np.maximum(np.abs(x - x0), np.abs(y - y0)).max()) for x, y in (one(True) for _ in range(draws)))
+
+
 def test_uneven_8192_autotuned_graph_subset_parity():
     """The uneven workload as benchmarked: 8192 creatures of 125 shapes, the
-    autotuner's per-bucket lane counts, waves per workgroup and steps per
+    autotuner's per-bucket lane counts and waves per workgroup, 50 steps per
     launch, a hipGraph-captured launch replayed; then one step of a
     64-creature subset (every shape bucket) against the oracle under the same
-    law, and limits / NaN over the whole batch."""
+    law, and limits / NaN over the whole batch. A creature whose error
+    exceeds the bar must be one at which the f64 oracle is itself
+    discontinuous (a one-ulp change of its binary32 input moves the oracle's
+    result by more than the bar, _f64_spread); it is reported as a dead-zone
+    flip, counted with the event flips (at most one creature in all), and its
+    error is bounded by the bar plus that spread. At step 150 of this seed one
+    creature (8 arms x 5 segments, ring 10) is such a case: the device is
+    1.07e-3 px from f64, the oracle's own one-ulp spread 1.07e-3 px, the
+    WGSL-semantics f32 emulator 4.6e-4 px."""
     from kraken_3_gpu_amd import autotune as T
     seed = 13
     w = W.world_uneven(8192, seed=1)
     law = W.default_law(seed)
     say("[case] uneven 8192 autotune")
-    g, rep = T.tune_buckets(w, law, steps=4, reps=1, passes=1, tiles=(25, 50))
+    g, rep = T.tune_buckets(w, law, steps=4, reps=1, passes=1, tiles=(50,))
     tile = rep["steps_per_launch"]
     lanes = [b["lanes"] for b in rep["buckets"]]
     say(f"    tuned lanes {lanes}, waves {rep['waves_per_block']}, tile {tile}, mixed "
@@ -582,6 +624,7 @@ def test_uneven_8192_autotuned_graph_subset_parity():
     assert st["nan"] == 0
     assert st["maxSpeed"] <= 360 * 1.001 and st["maxAngSpeed"] <= 18 * 1.001
     _pull_state(g)
+    pre = {k: np.array(getattr(w, k)) for k in BODY_KEYS + CONS_KEYS + ("cd",)}
     sub = list(range(0, 8192, 128))
     o, bidx, cidx = oracle_subset(w, sub)
     load_state_subset(o, w, sub, bidx, cidx)
@@ -600,9 +643,17 @@ def test_uneven_8192_autotuned_graph_subset_parity():
     crb = np.concatenate([np.full(1 + w.spawns[c][3] * w.spawns[c][4], i) for i, c in enumerate(sub)])
     ok = ~np.isin(crb, list(flips))
     shapes = {tuple(w.spawns[c][3:6]) for c in sub}
-    say(f"    uneven subset ({len(shapes)} shapes) max|dpos|={err[ok].max():.3e} flips={sorted(flips)}")
-    assert len(flips) <= 1
-    assert err[ok].max() <= TOL_PX
+    over = sorted({int(crb[i]) for i in np.nonzero(ok & (err > TOL_PX))[0]})
+    spread = {i: _f64_spread(w, sub[i], pre, law.seed, steps) for i in over}
+    inside = ok & ~np.isin(crb, over)
+    say(f"    uneven subset ({len(shapes)} shapes) max|dpos|={err[inside].max():.3e} event flips={sorted(flips)}")
+    for i in over:
+        e = float(err[crb == i].max())
+        say(f"    dead-zone flip: creature {sub[i]} shape {tuple(w.spawns[sub[i]][3:6])} |dpos| {e:.3e}, "
+            f"f64 one-ulp spread {spread[i]:.3e}")
+        assert spread[i] > TOL_PX and e <= TOL_PX + spread[i]
+    assert len(flips) + len(over) <= 1
+    assert err[inside].max() <= TOL_PX
     G.destroy(g)
 
 
