@@ -481,6 +481,13 @@ static int build_persist(kx_handle *h) {
             const int p = slot(r - 1, l);
             if ((l & 1) || p < 0 || (p >> 8) != (a >> 8)) all = false;
           }
+        // and the pins of the fused rounds in KX_REG_PINB all join two bodies
+        for (int f = 0; f < KX_REG_ROUNDS / 2; f++)
+          if ((KX_REG_PINB >> f) & 1)
+            for (int l = 0; l < 16; l++) {
+              const int p = slot(2 * f, l);
+              if (p >= 0 && ((p >> 16) & 255) == 255) all = false;
+            }
       }
     }
     const size_t lds_used = base + (size_t)mapw * 4;

@@ -135,7 +135,10 @@ int kx_persist_occupancy(int G, int wpb, size_t lds, bool par);   // G = 0: the 
 // its round types, bit r set = round r solves angle motors: the schedule of the
 // 6-arm x 7-segment creature (pins and motors alternate along the mantle chain)
 #define KX_REG_RTYPE 0xAAA
-#define KX_REG_BODY_REGS 4   // bodies per lane held in registers by the register-slot kernel
+// fused rounds f (pin round 2f and motor round 2f + 1) of that schedule whose pins
+// all join two bodies (the grips fall in rounds 4, 6 and 10)
+#define KX_REG_PINB 0x13
+#define KX_REG_BODY_REGS 3   // bodies per lane held in registers by the register-slot kernel (43 of the 6 x 7 creature)
 int kx_persist_occupancy_reg(int wpb, size_t lds, bool par);
 int kx_bump_counter(int *counter, int delta, hipStream_t st);
 

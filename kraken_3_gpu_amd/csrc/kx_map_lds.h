@@ -23,6 +23,25 @@ __device__ __forceinline__ bool kx_sdf_cells_lds(const KxDev &d, const uint16_t 
   return isfinite(h.d);
 }
 
+// The cull cell of a centre (x, y) relative to the origin (ox, oy): false,
+// and cell 0, when the centre lies outside the grid or is not finite.
+__device__ __forceinline__ bool kx_centre_cell(const KxDev &d, float x, float y, float ox, float oy, int &cell) {
+  const float gx = (x + ox - d.gx0) * d.inv_cs;
+  const float gy = (y + oy - d.gy0) * d.inv_cs;
+  const bool in = gx >= 0.0f && gy >= 0.0f && gx < (float)d.gnx && gy < (float)d.gny;
+  const int ix = min((int)gx, d.gnx - 1), iy = min((int)gy, d.gny - 1);
+  cell = in ? iy * d.gnx + ix : 0;
+  return in;
+}
+
+// Squared distance from (x, y) to the box a rect's clamp maps onto.
+__device__ __forceinline__ float kx_box_d2(float x, float y, float4 r, float ox, float oy) {
+  const float rx = r.x - ox, ry = r.y - oy;
+  const float cx = fminf(fmaxf(x, rx), rx + r.z), cy = fminf(fmaxf(y, ry), ry + r.w);
+  const float dx = x - cx, dy = y - cy;
+  return dx * dx + dy * dy;
+}
+
 // Contact test of a body centre (x, y) (relative to the origin (ox, oy))
 // against the map staged in LDS: true when the box of some rect registered in
 // the centre's cull cell lies closer than D to the centre. The box of a rect
@@ -36,21 +55,12 @@ __device__ __forceinline__ bool kx_sdf_cells_lds(const KxDev &d, const uint16_t 
 // centre is never near (its points never hit either).
 __device__ __forceinline__ bool kx_centre_near_lds(const KxDev &d, const uint16_t *coff, const float4 *crect, float x,
                                                    float y, float ox, float oy, float D) {
-  const float gx = (x + ox - d.gx0) * d.inv_cs;
-  const float gy = (y + oy - d.gy0) * d.inv_cs;
-  if (!(gx >= 0.0f && gy >= 0.0f && gx < (float)d.gnx && gy < (float)d.gny)) return false;
-  const int ix = min((int)gx, d.gnx - 1), iy = min((int)gy, d.gny - 1);
-  const int cell = iy * d.gnx + ix;
+  int cell;
+  if (!kx_centre_cell(d, x, y, ox, oy, cell)) return false;
   const int b = coff[cell], e = coff[cell + 1];
   const float D2 = D * D;
   bool near = false;
-  for (int k = b; k < e; k++) {
-    const float4 r = crect[k];
-    const float rx = r.x - ox, ry = r.y - oy;
-    const float cx = fminf(fmaxf(x, rx), rx + r.z), cy = fminf(fmaxf(y, ry), ry + r.w);
-    const float dx = x - cx, dy = y - cy;
-    near = near || dx * dx + dy * dy < D2;
-  }
+  for (int k = b; k < e; k++) near = near || kx_box_d2(x, y, crect[k], ox, oy) < D2;
   return near;
 }
 

@@ -246,11 +246,13 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
   double my_cd = 0.0;
   if (act && sub < T.narms) my_cd = d.cd[cid * KX_MAX_ARMS + sub];
   const double hS = d.hd * d.S;
-  // the lane's collidable bodies among sub + k*G, k < KX_NBL (dynamic, with
-  // collision points): bit k; bodies past KX_NBL*G are tested one by one
+  // the lane's collidable bodies among sub + k*G, k < NBL (dynamic, with
+  // collision points): bit k; bodies past NBL*G are tested one by one (the
+  // register-slot kernel batches all of its at most BR bodies per lane)
+  constexpr int NBL = RR > 0 ? BR : KX_NBL;
   uint32_t bp_mask = 0u;
 #pragma unroll
-  for (int k = 0; k < KX_NBL; k++) {
+  for (int k = 0; k < NBL; k++) {
     const int i = sub + k * G;
     if (act && i < nb) {
       const float4 br = t_brec[i];
@@ -409,7 +411,10 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
 #pragma unroll
           for (int f = 0; f < RR / 2; f++) {
             KxRegSlot &c = slot[2 * f], &m = slot[2 * f + 1];
-            const bool livep = ((fl >> (2 * f)) & 1u) != 0u, hasB = ((fl >> (16 + 2 * f)) & 1u) != 0u;
+            // the fused rounds whose pins all have a second body (no grip):
+            // a constant there, folded by the unrolled loop
+            const bool livep = ((fl >> (2 * f)) & 1u) != 0u;
+            const bool hasB = ((KX_REG_PINB >> f) & 1) != 0 || ((fl >> (16 + 2 * f)) & 1u) != 0u;
             const bool livem = ((fl >> (2 * f + 1)) & 1u) != 0u;
             const kx_f3v pA = *(const kx_f3v *)&pos[c.A], pB = *(const kx_f3v *)&pos[c.B];
             float s1, c1;
@@ -536,32 +541,34 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
           // read before any test (no branch per body)
           uint32_t mine = 0u;
           {
-            kx_f2v pbs[KX_NBL];
+            kx_f2v pbs[NBL];
 #pragma unroll
-            for (int k = 0; k < KX_NBL; k++) {
+            for (int k = 0; k < NBL; k++) {
               const int i = sub + k * G;
               pbs[k] = *(const kx_f2v *)&pos[i < nb ? i : 0];
             }
-            uint32_t wd[KX_NBL];
-            int bi[KX_NBL];
+            uint32_t wd[NBL];
+            int bi[NBL];
 #pragma unroll
-            for (int k = 0; k < KX_NBL; k++) {
+            for (int k = 0; k < NBL; k++) {
               int wi;
               cell(pbs[k].x, pbs[k].y, wi, bi[k]);
               wd[k] = nearb[wi];
             }
             const uint32_t all = use_bm ? 0u : 1u;   // no bitmaps: every body
 #pragma unroll
-            for (int k = 0; k < KX_NBL; k++) mine |= (((wd[k] >> bi[k]) | all) & 1u) << k;
+            for (int k = 0; k < NBL; k++) mine |= (((wd[k] >> bi[k]) | all) & 1u) << k;
             mine &= bp_mask;
           }
-          for (int k = KX_NBL, i = sub + KX_NBL * G; i < nb; k++, i += G) {
-            const float4 br = t_brec[i];
-            if (br.x == 0.0f || (__float_as_int(br.z) >> 16) <= 0) continue;
-            const float2 pb = *(const float2 *)&pos[i];
-            int wi, bi;
-            cell(pb.x, pb.y, wi, bi);
-            if (!use_bm || ((nearb[wi] >> bi) & 1u) != 0u) mine |= 1u << k;
+          if constexpr (RR == 0) {   // (the register-slot kernel batches every body)
+            for (int k = NBL, i = sub + NBL * G; i < nb; k++, i += G) {
+              const float4 br = t_brec[i];
+              if (br.x == 0.0f || (__float_as_int(br.z) >> 16) <= 0) continue;
+              const float2 pb = *(const float2 *)&pos[i];
+              int wi, bi;
+              cell(pb.x, pb.y, wi, bi);
+              if (!use_bm || ((nearb[wi] >> bi) & 1u) != 0u) mine |= 1u << k;
+            }
           }
           // Second stage, exact per body: the near bitmap bounds every body
           // with the largest contact radius of the template and a whole cell
@@ -571,6 +578,9 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
           // radius of its centre (kx_centre_near_lds). A body dropped here
           // has no point that corrects at its pose after the rounds, so its
           // in-order fold would leave it unmoved.
+          // The batched bodies go together: their contact radii and cell
+          // offsets in one LDS round trip, the first two rects of each cell
+          // in a second (a cell rarely holds more).
           if (P.map_words != 0) {
             for (uint32_t m = mine; m; m &= m - 1u) {
               const int k = __builtin_ctz(m), i = sub + k * G;
