@@ -267,15 +267,17 @@ __device__ __forceinline__ bool kx_angle_core(float &tA, float &tB, float wA, fl
 // operations of its hasB path in the same order, without the selects of a
 // world-anchored motor, and without the finiteness select of dev and dev2,
 // whose arguments (a fold result less the finite rest) are always finite.
+// rM is kx_rcp((wA + wB) + alpha), which a caller holding alpha fixed over
+// many solves computes once.
 __device__ __forceinline__ bool kx_angle_core_ab(float &tA, float &tB, float wA, float wB, float rest, float target,
-                                                 float mn, float mx, float alpha, float rS, float &lam) {
+                                                 float mn, float mx, float alpha, float rS, float rM, float &lam) {
   const float wSum = wA + wB;
   const float rel = kx_angle_delta(tB, tA);
   const float dev = kx_fold_finite(rel - rest);
   const float C = kx_angle_delta(dev, target);
   const bool live = wSum > 0.0f;
   const bool corr = fabsf(C) >= KX_SLACK_ANG;
-  float dl = -(C + alpha * lam) * kx_rcp(wSum + alpha);
+  float dl = -(C + alpha * lam) * rM;
   const bool bad = corr && !isfinite(dl);
   const bool ok1 = live && corr && !bad;
   float lam1 = lam + dl;

@@ -137,6 +137,7 @@ struct KxRegSlot {
   kx_f3v cr;
   kx_f2v bA, bB;
   int A, B;
+  float rm;   // motor: 1 / (wA + wB + alpha), refreshed with alpha once per step
 };
 
 // The body of one workgroup `bid`: WPB wavefronts of 64/G creatures of one
@@ -279,7 +280,7 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
 #pragma unroll
     for (int r = 0; r < RR; r++) {
       const KxSlot c = kx_slot_read(slot_dsc(r), dyn, lam2, t_crec, t_brec);
-      slot[r] = KxRegSlot{c.dk, c.lk, c.cr, c.bA, c.bB, c.A, c.B};
+      slot[r] = KxRegSlot{c.dk, c.lk, c.cr, c.bA, c.bB, c.A, c.B, 0.0f};
       rflags |= (c.hasB ? 1u : 0u) << (16 + r);
     }
   }
@@ -350,6 +351,7 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
         const float4 dk = dyn[dsc >= 0 ? dsc & 255 : 0];
         slot[r].dk = dk;
         const bool is_ang = ((KX_REG_RTYPE >> r) & 1) != 0;
+        if (is_ang) slot[r].rm = kx_rcp((slot[r].bA.y + slot[r].bB.y) + dk.x);
         const int brk = __float_as_int(is_ang ? dk.z : dk.y);
         lv |= (dsc >= 0 && brk == 0 ? 1u : 0u) << r;
       }
@@ -432,7 +434,8 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
             xB = okp ? xB : pB.x; yB = okp ? yB : pB.y; tB = okp ? tB : pB.z;
             float uA = tA, uB = tB, lam = m.lk.x;
             const bool okm =
-                kx_angle_core_ab(uA, uB, m.bA.y, m.bB.y, m.cr.x, m.dk.y, m.cr.y, m.cr.z, m.dk.x, m.dk.w, lam) && livem;
+                kx_angle_core_ab(uA, uB, m.bA.y, m.bB.y, m.cr.x, m.dk.y, m.cr.y, m.cr.z, m.dk.x, m.dk.w, m.rm, lam) &&
+                livem;
             if (okm) { tA = uA; tB = uB; m.lk.x = lam; }
             if (!odd && (okp || okm)) {
               // B first: for a world anchor B == A and pA must win
