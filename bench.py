@@ -134,7 +134,22 @@ def policy_cost(n, reps=50):
         alg = n * (nw + n_in + B.OUT_DIM) * 4
         res["forward"] = {"kernel": "kx_k_policy_fwd", "creatures": n, "weights_per_creature": nw,
                           "kernel_ms": k_ms, "algorithmic_bytes": alg, "achieved_gbs": alg / (k_ms * 1e6),
-                          "frac_hbm": alg / (k_ms * 1e6) / HBM_PEAK_GBS, "host_inclusive_ms": host_ms}
+                          "frac_peak": alg / (k_ms * 1e6) / HBM_PEAK_GBS, "host_inclusive_ms": host_ms,
+                          "working_set": "genomes re-read from the 256 MB Infinity Cache (%.1f MB resident)"
+                                         % (alg / 1e6)}
+    # the same kernel over a population whose genomes (358 MB) exceed the
+    # Infinity Cache, so every launch streams them from HBM
+    nbig = 24576
+    with B.Population(nbig, n_in) as P:
+        P.set_genomes((rng.standard_normal((nbig, nw)) / np.sqrt(n_in)).astype(np.float32))
+        x = rng.uniform(-1, 1, (nbig, n_in)).astype(np.float32)
+        P.forward_into(x, arms=6)
+        k_ms = P.forward_repeat(max(5, reps // 5))
+        alg = nbig * (nw + n_in + B.OUT_DIM) * 4
+        res["forward_hbm"] = {"kernel": "kx_k_policy_fwd", "creatures": nbig, "kernel_ms": k_ms,
+                              "algorithmic_bytes": alg, "achieved_gbs": alg / (k_ms * 1e6),
+                              "frac_hbm": alg / (k_ms * 1e6) / HBM_PEAK_GBS,
+                              "working_set": "%.0f MB of genomes > the 256 MB Infinity Cache" % (alg / 1e6)}
     pe = min(n, 8192)                      # the ES's population limit
     with B.ES(nw, popSize=pe) as es, B.Population(pe, n_in) as PE:
         state = B.trainer_stream_state(1)
@@ -399,6 +414,56 @@ def warm_accounting(g, w):
     return float((w.broken[grips] == 0).sum()), rects_within(w)
 
 
+def uneven_leg(steps=300):
+    """A bounded run of the uneven batch as the default line reports it: 8192
+    creatures of 125 shapes, per-bucket lanes from a short autotune, 50-step
+    launches replayed from a captured hipGraph."""
+    from kraken_3_gpu_amd import autotune as A
+    from kraken_3_gpu_amd import gpuworld as G
+    from kraken_3_gpu_amd import world as W
+    tile = 50
+    w = W.world_uneven(8192, seed=1)
+    law = law_for("uneven", 2024)
+    g, rep = A.tune_buckets(w, law, steps=4, reps=1, passes=1, tiles=(tile,))
+    G.upload(g)
+    G.rollout_graph(g, law, 0, tile, 1)
+    G.sync(g)
+    t0 = time.perf_counter()
+    G.rollout_graph(g, law, tile, tile, steps // tile)
+    G.sync(g)
+    wall = time.perf_counter() - t0
+    dev_ms, launches = G.last_timing(g)
+    st = G.download(g)
+    out = {"workload": "uneven batch: 8192 creatures, 4-8 arms x 5-9 segments, ring 6-10, makeMap 1200x800",
+           "creatures": w.n_creatures, "steps": steps, "steps_per_launch": tile,
+           "value": w.n_creatures * steps / wall, "unit": "creature-steps/s", "launch_ms": dev_ms / launches,
+           "lanes_per_bucket": [b["lanes"] for b in rep["buckets"]], "waves_per_block": rep["waves_per_block"],
+           "final_stats": st}
+    G.destroy(g)
+    return out
+
+
+def stress_leg(steps=200, warmup=20):
+    """A bounded run of the stress batch as the default line reports it:
+    65536 creatures on makeMap(4096, 4096, 32 x 32 corridors), every tip
+    adhesive with a per-tip break limit from U[50, 200] N, 50-step launches."""
+    from kraken_3_gpu_amd import _lib as K
+    from kraken_3_gpu_amd import gpuworld as G
+    from kraken_3_gpu_amd import world as W
+    w = W.world_stress(65536, seed=1)
+    law = law_for("stress", 2024)
+    g = G.create(w, path=K.KX_PATH_PERSISTENT)
+    G.upload(g)
+    r = persistent_run(g, law, steps, warmup, 50)
+    st = G.download(g)
+    out = {"workload": "stress: 65536 creatures (6 arms x 7 segments), makeMap(4096, 4096, 32 x 32 corridors)",
+           "creatures": w.n_creatures, "steps": steps, "steps_per_launch": 50,
+           "value": w.n_creatures * steps / r["wall"], "unit": "creature-steps/s", "launch_ms": r["launch_ms"],
+           "launch_shape": G.persist_info(g), "final_stats": st}
+    G.destroy(g)
+    return out
+
+
 def bench_stress(args):
     """65 536 creatures on makeMap(4096, 4096, 32 x 32 corridors), every tip
     adhesive with a per-tip break limit from U[50, 200] N."""
@@ -568,6 +633,7 @@ def main():
     ap.add_argument("--wpb", type=int, default=0, help="wavefronts per workgroup (0 = default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the bounded uneven and stress legs")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -725,6 +791,11 @@ def main():
         out["speedup_vs_plain_grid_sdf"] = value / plain["value_grid_sdf"]
     if args.serve_requests:
         out["serving"] = serving_fixed_rate(args.serve_requests, args.serve_rate)
+    if not args.no_configs:
+        # bounded runs of the uneven and stress workloads (full runs:
+        # --workload uneven / stress)
+        out["uneven"] = uneven_leg()
+        out["stress"] = stress_leg()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(device=(g, w, law, step))
         out["parity"] = out["cpu_baseline"]["device_parity"]
