@@ -484,7 +484,10 @@ def bench_stress(args):
     F = flops_per_step(w, g_att, min(r_near, 64.0))
     ach = F * args.chunk / (r["launch_ms"] * 1e-3) / 1e12
     lb = algorithmic_launch_bytes(w, args.chunk)
-    traffic = measured_traffic(n_cr, args.chunk, "stress65536")
+    shape = G.persist_info(g)
+    kernel = ("kx_k_persist_reg" if shape.get("register_rounds") else
+              "kx_k_persist_fix" if shape.get("fixed_rounds") else "kx_k_persist")
+    traffic = measured_traffic(n_cr, args.chunk, "stress65536", kernel)
     out = {
         "metric": "creature-steps/s, stress run (65536 creatures, 4096x4096 map, 32x32 corridors)",
         "value": n_cr * steps / r["wall"], "unit": "creature-steps/s", "n_gpus": 1, "steps": steps,
@@ -500,7 +503,7 @@ def bench_stress(args):
                      "traffic_source": traffic["source"] if traffic else None,
                      "hbm_measured_gbs": (traffic["traffic_bytes"] / (r["launch_ms"] * 1e-3) / 1e9) if traffic else None,
                      "grips_attached": g_att, "rects_within_16px_mean": r_near},
-        "launch_shape": G.persist_info(g),
+        "launch_shape": shape, "kernel": kernel,
         "p50_rollout_ms": r["p50"], "p99_rollout_ms": r["p99"], "final_stats": st,
     }
     print(json.dumps(out), flush=True)
@@ -721,7 +724,8 @@ def main():
     launch_bytes = algorithmic_launch_bytes(w, chunk)
     hbm_gbs = launch_bytes / (launch_ms * 1e-3) / 1e9
     # the kernel the launches ran: register-held round slots or slots read from LDS
-    kernel = "kx_k_persist_reg" if shape.get("register_rounds") else "kx_k_persist"
+    kernel = ("kx_k_persist_reg" if shape.get("register_rounds") else
+              "kx_k_persist_fix" if shape.get("fixed_rounds") else "kx_k_persist")
     traffic = measured_traffic(n_cr, chunk, "uniform4096", kernel)
     plain = None
     if not args.no_plain:

@@ -223,17 +223,23 @@ int kx_schedule_info(const kx_handle *h, int32_t *rounds, int32_t max);
  * per-creature LDS words, [8] 1 if lane counts differ between workgroups,
  * [9] the collision fold in use (KX_FOLD_PARALLEL or KX_FOLD_OWNER), [10]
  * rounds per iteration held in registers (0: the round slots are read from
- * LDS each round; see kx_set_register_slots).
- * Returns the number of fields (11). No counterpart in the reference (its
+ * LDS each round; see kx_set_register_slots), [11] rounds per iteration of
+ * the fixed schedule run with its slots read from LDS (0: not that kernel).
+ * Returns the number of fields (12). No counterpart in the reference (its
  * path has no persistent kernel). */
 int kx_persist_info(const kx_handle *h, int32_t *info, int32_t max);
-/* Persistent path: allow (1, the default) or forbid (0) register-held round
- * slots. With them each lane keeps its slot of every round (constraint and
- * body records, control fields, lambda) in registers and a round reads only
- * its two poses from LDS; used when every creature's template has 12 rounds
- * at 16 lanes per creature and the launch places at most one wavefront per
- * SIMD. Results are identical bit for bit either way. No counterpart in the
- * reference (a launch-shape knob, like waves_per_block). */
+/* Persistent path: the kernels specialised to the schedule of the default
+ * creature (6 arms x 7 segments: 12 rounds at 16 lanes per creature, pins
+ * and motors alternating, a joint's hinge and muscle solved in one fused
+ * round). enable = 1 (the default): register-held round slots when the
+ * launch places at most one wavefront per SIMD (each lane keeps its slot of
+ * every round in registers and a round reads only its two poses from LDS),
+ * else the same fused rounds with the slots read from LDS; 2: the LDS-read
+ * form even where the register form would fit; 0: neither (the kernel that
+ * schedules any morphology at run time). Templates that do not have that
+ * schedule always take the run-time kernel. Results are identical bit for
+ * bit in every mode. No counterpart in the reference (a launch-shape knob,
+ * like waves_per_block). */
 int kx_set_register_slots(kx_handle *h, int32_t enable);
 /* Persistent path: how a step folds the collision points of the bodies near
  * a rect (xpbd.js:441-470 either way, with identical results).

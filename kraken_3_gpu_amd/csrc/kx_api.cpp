@@ -99,7 +99,7 @@ struct kx_handle {
   std::vector<int> tmpl_rounds;
   KxPersistCfg pc{};
   void *pc_bufs[4] = {nullptr, nullptr, nullptr, nullptr};
-  int pinfo[11] = {};    // see kx_persist_info
+  int pinfo[12] = {};    // see kx_persist_info
   std::vector<int> tmpl_lanes;   // per template lanes per creature, 0 = the handle's G
   int launch_G = 16;             // lane count the uniform kernel is launched with
   int coll_mode = 0;             // collision fold: 0 = by map density, 1 = parallel, 2 = owning lane
@@ -453,12 +453,15 @@ static int build_persist(kx_handle *h) {
   h->pinfo[4] = mapw; h->pinfo[5] = bmw; h->pinfo[6] = words; h->pinfo[7] = stride;
   h->pinfo[8] = mixed ? 1 : 0;
   h->pinfo[9] = par ? KX_FOLD_PARALLEL : KX_FOLD_OWNER;
-  // Register-held round slots: one template of KX_REG_ROUNDS rounds at 16
-  // lanes, and every workgroup resident at once at one wavefront per SIMD
-  // (the slots take the register file, so a SIMD holds no second wave).
-  // The kernel also fixes the round types (KX_REG_RTYPE) and solves every
-  // motor as one between two bodies with a finite rest angle.
-  int reg = 0;
+  // The fixed schedule (KX_REG_ROUNDS rounds of types KX_REG_RTYPE at 16
+  // lanes): every motor between two bodies with a finite rest angle, each
+  // motor of an angle round on the even lane of the pin before it with the
+  // same two bodies (the fused rounds), the pins of the fused rounds in
+  // KX_REG_PINB all with a second body, at most KX_REG_BODY_REGS bodies per
+  // lane. With it, register-held round slots when every workgroup is
+  // resident at once at one wavefront per SIMD (the slots take the register
+  // file, so a SIMD holds no second wave), else the slots read from LDS.
+  int reg = 0, fixed = 0;
   if (h->reg_mode && !mixed && Gk == 16) {
     bool all = true;
     for (size_t t = 0; t < T.size(); t++) {
@@ -469,8 +472,6 @@ static int build_persist(kx_handle *h) {
       const TmplHost &th = h->tmpl_host[t];
       for (size_t k = 0; k < th.type.size(); k++)
         if (th.type[k] == KX_ANGLE && (th.b[k] < 0 || !std::isfinite(th.rest[k]))) all = false;
-      // fused rounds: every motor of an angle round sits on the even lane of
-      // the pin before it with the same two bodies
       if (all) {
         const uint32_t *desc = pool.data() + T[t].base + T[t].o_sched;
         auto slot = [&](int r, int lane) { return (int)desc[((r >> 2) * 16 + lane) * 4 + (r & 3)]; };
@@ -481,7 +482,6 @@ static int build_persist(kx_handle *h) {
             const int p = slot(r - 1, l);
             if ((l & 1) || p < 0 || (p >> 8) != (a >> 8)) all = false;
           }
-        // and the pins of the fused rounds in KX_REG_PINB all join two bodies
         for (int f = 0; f < KX_REG_ROUNDS / 2; f++)
           if ((KX_REG_PINB >> f) & 1)
             for (int l = 0; l < 16; l++) {
@@ -491,9 +491,13 @@ static int build_persist(kx_handle *h) {
       }
     }
     const size_t lds_used = base + (size_t)mapw * 4;
-    if (all && (long)nwg * WPB <= 4L * ncu && kx_persist_occupancy_reg(WPB, lds_used, par) >= need) reg = KX_REG_ROUNDS;
+    if (all && h->reg_mode == 1 && (long)nwg * WPB <= 4L * ncu && kx_persist_occupancy_reg(WPB, lds_used, par) >= need)
+      reg = KX_REG_ROUNDS;
+    else if (all)
+      fixed = KX_REG_ROUNDS;
   }
   h->pinfo[10] = reg;
+  h->pinfo[11] = fixed;
   std::vector<int> tables;
   tables.insert(tables.end(), wg_tmpl.begin(), wg_tmpl.end());
   tables.insert(tables.end(), wg_lanes.begin(), wg_lanes.end());
@@ -526,6 +530,7 @@ static int build_persist(kx_handle *h) {
   P.NB = NB; P.NC = NC; P.NP = NP;
   P.coll_par = par ? 1 : 0;
   P.reg_rounds = reg;
+  P.fixed_rounds = fixed;
   h->launch_G = mixed ? h->G : (lanes0 > 0 ? lanes0 : h->G);
   return 0;
 }
@@ -1245,8 +1250,8 @@ extern "C" int kx_schedule_info(const kx_handle *h, int32_t *rounds, int32_t max
 extern "C" int kx_persist_info(const kx_handle *h, int32_t *info, int32_t max) {
   if (!h || !info) return fail(KX_EINVAL, "null argument");
   if (!h->persist_ok) return fail(KX_ENOTSUP, "persistent path unavailable: %s", h->persist_why.c_str());
-  for (int i = 0; i < 11 && i < max; i++) info[i] = h->pinfo[i];
-  return 11;
+  for (int i = 0; i < 12 && i < max; i++) info[i] = h->pinfo[i];
+  return 12;
 }
 
 extern "C" int kx_template_info(const kx_handle *h, int32_t t, int32_t *out6) {
@@ -1379,7 +1384,7 @@ int kx_policy_io(kx_handle *h, int32_t stride, float **state_dev, float **tape_d
 
 extern "C" int kx_set_register_slots(kx_handle *h, int32_t enable) {
   if (!h) return fail(KX_EINVAL, "null handle");
-  if (enable != 0 && enable != 1) return fail(KX_EINVAL, "register slots %d", enable);
+  if (enable < 0 || enable > 2) return fail(KX_EINVAL, "register slots %d", enable);
   HIPCHK(hipStreamSynchronize(h->st));
   h->reg_mode = enable;
   if (h->persist_ok) {

@@ -125,6 +125,8 @@ struct KxPersistCfg {
                             // 0: each near body folded by its owning lane
   int reg_rounds;           // KX_REG_ROUNDS: the register-slot kernel (one template of that many
                             // rounds at 16 lanes, at most one wavefront per SIMD); 0: the LDS-slot kernel
+  int fixed_rounds;         // KX_REG_ROUNDS (with reg_rounds 0): the same fixed schedule with the
+                            // slots read from LDS (kx_k_persist_fix)
 };
 int kx_persist_launch(int G, int wpb, const KxDev &d, const KxLaw *law, int step0, const int *step_base,
                       int nsteps, int ev_row0, int with_forces, const KxPersistCfg &P, hipStream_t st);

@@ -142,9 +142,11 @@ struct KxRegSlot {
 
 // The body of one workgroup `bid`: WPB wavefronts of 64/G creatures of one
 // template (workgroup tables P.wg_*). RR > 0: every template of the launch
-// has exactly RR rounds, and each lane keeps its RR round slots in registers
-// (records, dynamics, lambda), so a round reads only its two poses from LDS.
-template <int G, int WPB, bool PAR, int RR = 0>
+// has the fixed schedule of RR rounds (KX_REG_RTYPE, fused hinge and muscle
+// rounds); with REG each lane keeps its RR round slots in registers
+// (records, dynamics, lambda), so a round reads only its two poses from LDS,
+// without it a fused round reads its two slots' records from LDS.
+template <int G, int WPB, bool PAR, int RR = 0, bool REG = true>
 __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, int has_law, int step0,
                                                 const int *step_base, int nsteps, int ev_row0, int with_forces,
                                                 const KxPersistCfg &P, int bid) {
@@ -276,7 +278,13 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
     const int4 d4 = act ? t_sched4[(r >> 2) * G + sub] : make_int4(-1, -1, -1, -1);
     return (r & 3) == 0 ? d4.x : (r & 3) == 1 ? d4.y : (r & 3) == 2 ? d4.z : d4.w;
   };
-  if constexpr (RR > 0) {
+  // fixed schedule with LDS-read slots: every round's descriptor, held
+  [[maybe_unused]] int fdsc[NR];
+  if constexpr (RR > 0 && !REG) {
+#pragma unroll
+    for (int r = 0; r < RR; r++) fdsc[r] = slot_dsc(r);
+  }
+  if constexpr (RR > 0 && REG) {
 #pragma unroll
     for (int r = 0; r < RR; r++) {
       const KxSlot c = kx_slot_read(slot_dsc(r), dyn, lam2, t_crec, t_brec);
@@ -340,7 +348,7 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
     }
     kx_shift_anchors(dyn, t_crec, nc, sub, G, -ox, -oy);
     KX_WAVE_SYNC();
-    if constexpr (RR > 0) {
+    if constexpr (RR > 0 && REG) {
       // the step's dynamics (control law, reflex, origin shift) into the
       // slots, and which slots commit this step: live and not broken (an
       // angle keeps its broken flag in dyn.z, a pin or grip in dyn.y)
@@ -359,7 +367,7 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
     }
     for (int s = 0; s < d.S; s++) {
       // resetLambda + integrate (xpbd.js:262-287)
-      if constexpr (RR > 0) {
+      if constexpr (RR > 0 && REG) {
 #pragma unroll
         for (int r = 0; r < RR; r++) slot[r].lk = kx_f2v{0.0f, 0.0f};
       } else {
@@ -393,7 +401,7 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
         // body records: none of them is written by a round), and waits only
         // for the poses. A broken constraint is solved too and its result
         // dropped.
-        if constexpr (RR > 0) {
+        if constexpr (RR > 0 && REG) {
           // the same solves on the register-held slots, a joint's hinge and
           // its muscle fused: rounds 2f (pins and grips on lane pairs) and
           // 2f + 1 (motors, each on the even lane of the pin with the same
@@ -437,6 +445,47 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
                 kx_angle_core_ab(uA, uB, m.bA.y, m.bB.y, m.cr.x, m.dk.y, m.cr.y, m.cr.z, m.dk.x, m.dk.w, m.rm, lam) &&
                 livem;
             if (okm) { tA = uA; tB = uB; m.lk.x = lam; }
+            if (!odd && (okp || okm)) {
+              // B first: for a world anchor B == A and pA must win
+              *(kx_f3v *)&pos[c.B] = kx_f3v{xB, yB, tB};
+              *(kx_f3v *)&pos[c.A] = kx_f3v{xA, yA, tA};
+            }
+            KX_WAVE_SYNC();
+          }
+        } else if constexpr (RR > 0) {
+          // the fused rounds of the fixed schedule with the slots' records
+          // read from LDS each round (launches of more than one wavefront per
+          // SIMD, whose register file holds no slots): the same solves,
+          // commits and lambda bookkeeping as the LDS-slot rounds below
+          static_assert(KX_REG_RTYPE == 0xAAA && RR == 12, "fused rounds: pins on even rounds, motors on odd");
+          const bool odd = (sub & 1) != 0;
+#pragma unroll
+          for (int f = 0; f < RR / 2; f++) {
+            const KxSlot c = kx_slot_read(fdsc[2 * f], dyn, lam2, t_crec, t_brec);
+            const KxSlot m = kx_slot_read(fdsc[2 * f + 1], dyn, lam2, t_crec, t_brec);
+            const kx_f3v pA = *(const kx_f3v *)&pos[c.A], pB = *(const kx_f3v *)&pos[c.B];
+            // pins and grips keep their broken flag in dyn.y, motors in dyn.z
+            const bool livep = c.live && __float_as_int(c.dk.y) == 0;
+            const bool livem = m.live && __float_as_int(m.dk.z) == 0;
+            const bool hasB = ((KX_REG_PINB >> f) & 1) != 0 || c.hasB;
+            float s1, c1;
+            kx_sincos(odd ? pB.z : pA.z, &s1, &c1);
+            const float s2 = kx_swap_pair(s1), c2 = kx_swap_pair(c1);
+            float xA = pA.x, yA = pA.y, tA = pA.z, xB = pB.x, yB = pB.y, tB = pB.z;
+            float lx = c.lk.x, ly = c.lk.y;
+            const bool okp = kx_pin_core_sc(xA, yA, tA, c.bA.x, c.bA.y, xB, yB, tB, c.bB.x, c.bB.y, hasB, c.cr.x,
+                                            c.cr.y, c.dk.z, c.dk.w, c.dk.x, lx, ly, odd ? s2 : s1, odd ? c2 : c1,
+                                            odd ? s1 : s2, odd ? c1 : c2) &&
+                             livep;
+            xA = okp ? xA : pA.x; yA = okp ? yA : pA.y; tA = okp ? tA : pA.z;
+            xB = okp ? xB : pB.x; yB = okp ? yB : pB.y; tB = okp ? tB : pB.z;
+            float uA = tA, uB = tB, lam = m.lk.x;
+            const float rm = kx_rcp((m.bA.y + m.bB.y) + m.dk.x);
+            const bool okm =
+                kx_angle_core_ab(uA, uB, m.bA.y, m.bB.y, m.cr.x, m.dk.y, m.cr.y, m.cr.z, m.dk.x, m.dk.w, rm, lam) &&
+                livem;
+            if (okm) { tA = uA; tB = uB; ((float *)&lam2[m.k])[0] = lam; }
+            if (!odd && okp) lam2[c.k] = kx_f2v{lx, ly};
             if (!odd && (okp || okm)) {
               // B first: for a world anchor B == A and pA must win
               *(kx_f3v *)&pos[c.B] = kx_f3v{xB, yB, tB};
@@ -748,7 +797,7 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
       }
       KX_WAVE_SYNC();
     }
-    if constexpr (RR > 0) {
+    if constexpr (RR > 0 && REG) {
       // the last substep's lambdas back to the creature region (grip break,
       // store); a pin's even lane writes it
 #pragma unroll
@@ -842,6 +891,16 @@ __global__ __launch_bounds__(KX_WAVE *WPB) __attribute__((amdgpu_waves_per_eu(1,
     KxDev d, KxLaw L, int has_law, int step0, const int *step_base, int nsteps, int ev_row0, int with_forces,
     KxPersistCfg P) {
   kx_persist_body<G, WPB, PAR, RR>(d, L, has_law, step0, step_base, nsteps, ev_row0, with_forces, P, blockIdx.x);
+}
+
+// The fixed schedule with the slots read from LDS: launches of the same
+// templates with more than one wavefront per SIMD.
+template <int G, int WPB, bool PAR, int RR>
+__global__ __launch_bounds__(KX_WAVE *WPB) __attribute__((amdgpu_waves_per_eu(2))) void kx_k_persist_fix(
+    KxDev d, KxLaw L, int has_law, int step0, const int *step_base, int nsteps, int ev_row0, int with_forces,
+    KxPersistCfg P) {
+  kx_persist_body<G, WPB, PAR, RR, false>(d, L, has_law, step0, step_base, nsteps, ev_row0, with_forces, P,
+                                          blockIdx.x);
 }
 
 // Lane count per workgroup (per morphology bucket): a wave-uniform switch to
@@ -973,6 +1032,19 @@ static int launch_reg(const KxDev &d, const KxLaw *law, int step0, const int *st
 }
 
 template <int G, int WPB, bool PAR, int RR>
+static int launch_fix(const KxDev &d, const KxLaw *law, int step0, const int *step_base, int nsteps, int ev_row0,
+                      int with_forces, const KxPersistCfg &P, hipStream_t st) {
+  size_t lds = ((size_t)P.tmpl_words + P.bm_words + P.map_words + (size_t)P.cr_words) * 4;
+  if (lds > 160 * 1024) return -22;
+  static std::atomic<uint64_t> lds_set{0};
+  allow_full_lds((const void *)kx_k_persist_fix<G, WPB, PAR, RR>, lds_set);
+  KxLaw L = law ? *law : KxLaw{};
+  hipLaunchKernelGGL((kx_k_persist_fix<G, WPB, PAR, RR>), dim3(P.nwg), dim3(KX_WAVE * WPB), lds, st, d, L,
+                     law ? 1 : 0, step0, step_base, nsteps, ev_row0, with_forces, P);
+  return hipGetLastError() == hipSuccess ? 0 : -5;
+}
+
+template <int G, int WPB, bool PAR, int RR>
 static int occ_reg(size_t lds) {
   int n = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void *)kx_k_persist_reg<G, WPB, PAR, RR>, KX_WAVE * WPB,
@@ -1050,6 +1122,14 @@ static int launch_g(int wpb, const KxDev &d, const KxLaw *law, int step0, const 
         case 1: return launch_reg<16, 1, PAR, KX_REG_ROUNDS>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
         case 2: return launch_reg<16, 2, PAR, KX_REG_ROUNDS>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
         case 4: return launch_reg<16, 4, PAR, KX_REG_ROUNDS>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
+        default: return -22;
+      }
+    }
+    if (P.fixed_rounds == KX_REG_ROUNDS) {
+      switch (wpb) {
+        case 1: return launch_fix<16, 1, PAR, KX_REG_ROUNDS>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
+        case 2: return launch_fix<16, 2, PAR, KX_REG_ROUNDS>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
+        case 4: return launch_fix<16, 4, PAR, KX_REG_ROUNDS>(d, law, step0, step_base, nsteps, ev_row0, with_forces, P, st);
         default: return -22;
       }
     }

@@ -298,13 +298,16 @@ def schedule_rounds(g):
 
 
 PERSIST_INFO = ("lds_bytes", "workgroups", "resident_per_cu", "needed_per_cu", "map_words", "bitmap_words",
-                "template_words", "creature_words", "mixed_lanes", "collision_fold", "register_rounds")
+                "template_words", "creature_words", "mixed_lanes", "collision_fold", "register_rounds", "fixed_rounds")
 
 
 def set_register_slots(g, enable):
-    """Allow (True, the default) or forbid register-held round slots in the
-    persistent kernel (kx_set_register_slots); results are identical."""
-    K.check(K.lib().kx_set_register_slots(g.p, 1 if enable else 0))
+    """The persistent kernels specialised to the default creature's schedule
+    (kx_set_register_slots): True or 1 (the default) register-held round
+    slots where the launch allows them, else slots read from LDS; 2 the
+    LDS-read form always; False or 0 neither. Results are identical."""
+    mode = 1 if enable is True else 0 if enable is False else int(enable)
+    K.check(K.lib().kx_set_register_slots(g.p, mode))
 
 
 def persist_info(g):

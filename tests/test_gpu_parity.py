@@ -719,22 +719,25 @@ def test_state_readout_matches_oracle(world):
 
 
 def test_register_slots_bit_identical():
-    """The persistent kernel with each lane's round slots held in registers
-    (a uniform batch of the 12-round 6 x 7 template at 16 lanes, one
-    wavefront per SIMD) against the same kernel reading its slots from LDS
-    each round and against the plain path: the same solves in the same
+    """The kernels specialised to the 6 x 7 schedule (a uniform batch of the
+    12-round template at 16 lanes, one wavefront per SIMD): round slots held
+    in registers, and the same fused hinge/muscle rounds with the slots read
+    from LDS, against the run-time kernel (slots read from LDS each round,
+    rounds unfused) and against the plain path: the same solves in the same
     order, so every state and event agrees bit for bit over a rollout that
     reaches the walls."""
     law = W.default_law(31)
     make, steps = (lambda: W.world_uniform(1024, seed=5)), 120
     runs = {}
-    for name, path, reg in (("plain", K.KX_PATH_PLAIN, None), ("lds", K.KX_PATH_PERSISTENT, False),
-                            ("registers", K.KX_PATH_PERSISTENT, True)):
+    for name, path, reg in (("plain", K.KX_PATH_PLAIN, None), ("lds", K.KX_PATH_PERSISTENT, 0),
+                            ("registers", K.KX_PATH_PERSISTENT, 1), ("fixed_lds", K.KX_PATH_PERSISTENT, 2)):
         w = make()
         g = G.create(w, path=path)
         if reg is not None:
             G.set_register_slots(g, reg)
-            assert G.persist_info(g)["register_rounds"] == (12 if reg else 0)
+            info = G.persist_info(g)
+            assert info["register_rounds"] == (12 if reg == 1 else 0)
+            assert info["fixed_rounds"] == (12 if reg == 2 else 0)
         G.upload(g)
         say(f"[case] register slots: {name}, {steps} steps")
         ev = G.rollout(g, law, 0, steps, events=True)
@@ -743,14 +746,14 @@ def test_register_slots_bit_identical():
         G.destroy(g)
         runs[name] = (w, ev, ctl)
     wp, evp, cp = runs["plain"]
-    for name in ("lds", "registers"):
+    for name in ("lds", "registers", "fixed_lds"):
         wq, evq, cq = runs[name]
         for key in ("px", "py", "ang", "vx", "vy", "av", "broken", "lambda_", "lambdaX", "lambdaY"):
             assert np.array_equal(getattr(wp, key), getattr(wq, key)), (name, key)
         for key in cp:
             assert np.array_equal(cp[key], cq[key]), (name, key)
         assert np.array_equal(evp, evq), name
-    say(f"    plain == lds == registers ({len(decode_events(evp))} events)")
+    say(f"    plain == lds == registers == fixed_lds ({len(decode_events(evp))} events)")
 
 
 def test_folds_bit_identical_over_a_long_rollout():
@@ -766,9 +769,10 @@ def test_folds_bit_identical_over_a_long_rollout():
     steps = 1250
     runs = {}
     for name, path, reg, fold in (("plain", K.KX_PATH_PLAIN, None, None),
-                                  ("reg_parallel", K.KX_PATH_PERSISTENT, True, K.KX_FOLD_PARALLEL),
-                                  ("reg_owner", K.KX_PATH_PERSISTENT, True, K.KX_FOLD_OWNER),
-                                  ("lds_owner", K.KX_PATH_PERSISTENT, False, K.KX_FOLD_OWNER)):
+                                  ("reg_parallel", K.KX_PATH_PERSISTENT, 1, K.KX_FOLD_PARALLEL),
+                                  ("reg_owner", K.KX_PATH_PERSISTENT, 1, K.KX_FOLD_OWNER),
+                                  ("lds_owner", K.KX_PATH_PERSISTENT, 0, K.KX_FOLD_OWNER),
+                                  ("fixed_lds_owner", K.KX_PATH_PERSISTENT, 2, K.KX_FOLD_OWNER)):
         w = W.world_uniform(4096, seed=1)
         g = G.create(w, path=path)
         if reg is not None:
@@ -781,7 +785,7 @@ def test_folds_bit_identical_over_a_long_rollout():
         G.destroy(g)
         runs[name] = (w, ev)
     wp, evp = runs["plain"]
-    for name in ("reg_parallel", "reg_owner", "lds_owner"):
+    for name in ("reg_parallel", "reg_owner", "lds_owner", "fixed_lds_owner"):
         wq, evq = runs[name]
         for key in ("px", "py", "ang", "vx", "vy", "av", "broken", "lambda_", "lambdaX", "lambdaY"):
             a, b = getattr(wp, key), getattr(wq, key)
