@@ -337,12 +337,20 @@ static int build_grid(kx_handle *h, const float *rects, int nR, float rmax, floa
   h->d.inv_ns = (float)m / cs;
   h->d.nnx = nnx;
   h->d.nny = nny;
-  int *doff, *dids;
+  // each cell entry holds its rect itself, so a query reads a cell's
+  // offsets and then its rects (two dependent loads, not three)
+  std::vector<float4> crects(ids.size());
+  for (size_t k = 0; k < ids.size(); k++) {
+    const float *r = rects + 4 * (size_t)ids[k];
+    crects[k] = make_float4(r[0], r[1], r[2], r[3]);
+  }
+  int *doff;
+  float4 *dcr;
   TRY(dalloc(h, &doff, off.size()));
-  TRY(dalloc(h, &dids, ids.size()));
+  TRY(dalloc(h, &dcr, crects.size()));
   TRY(dput(doff, off.data(), off.size()));
-  TRY(dput(dids, ids.data(), ids.size()));
-  h->d.cell_off = doff; h->d.cell_ids = dids;
+  if (!crects.empty()) TRY(dput((float *)dcr, (const float *)crects.data(), 4 * crects.size()));
+  h->d.cell_off = doff; h->d.cell_rects = dcr;
   h->n_cell_ids = (int)ids.size();
   h->d.gx0 = gx0; h->d.gy0 = gy0; h->d.inv_cs = 1.0f / cs; h->d.gnx = gnx; h->d.gny = gny;
   h->d.cull_r = R;

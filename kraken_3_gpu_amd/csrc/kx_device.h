@@ -137,7 +137,7 @@ __device__ __forceinline__ bool kx_sdf_grid(const KxDev &d, float x, float y, fl
   int b = d.cell_off[cell], e = d.cell_off[cell + 1];
   if (b == e) return false;
   kx_hit_init(h);
-  for (int k = b; k < e; k++) kx_fold_rect(x, y, d.rects[d.cell_ids[k]], ox, oy, h);
+  for (int k = b; k < e; k++) kx_fold_rect(x, y, d.cell_rects[k], ox, oy, h);
   return isfinite(h.d);
 }
 
@@ -154,22 +154,7 @@ __device__ __forceinline__ bool kx_sdf_grid_bm(const KxDev &d, const uint32_t *b
   if (!((bits[cell >> 5] >> (cell & 31)) & 1u)) return false;
   int b = d.cell_off[cell], e = d.cell_off[cell + 1];
   kx_hit_init(h);
-  for (int k = b; k < e; k++) kx_fold_rect(x, y, d.rects[d.cell_ids[k]], ox, oy, h);
-  return isfinite(h.d);
-}
-
-// Same fold with the map staged in LDS: cell offsets, rect ids and rects.
-__device__ __forceinline__ bool kx_sdf_grid_lds(const KxDev &d, const int *coff, const int *cids, const float4 *rects,
-                                                float x, float y, float ox, float oy, KxHit &h) {
-  float gx = (x + ox - d.gx0) * d.inv_cs;
-  float gy = (y + oy - d.gy0) * d.inv_cs;
-  if (!(gx >= 0.0f && gy >= 0.0f && gx < (float)d.gnx && gy < (float)d.gny)) return false;
-  int ix = min((int)gx, d.gnx - 1), iy = min((int)gy, d.gny - 1);
-  int cell = iy * d.gnx + ix;
-  int b = coff[cell], e = coff[cell + 1];
-  if (b == e) return false;
-  kx_hit_init(h);
-  for (int k = b; k < e; k++) kx_fold_rect(x, y, rects[cids[k]], ox, oy, h);
+  for (int k = b; k < e; k++) kx_fold_rect(x, y, d.cell_rects[k], ox, oy, h);
   return isfinite(h.d);
 }
 

@@ -55,7 +55,7 @@ struct KxDev {
   const float4 *rects;
   int nr;
   const int *cell_off;           // [gnx*gny+1]
-  const int *cell_ids;
+  const float4 *cell_rects;      // the rect of every cell entry (cell by cell, ascending rect index)
   const uint32_t *cell_bits;     // occupancy bitmap (1 bit per cull cell), then the near bitmap
   int cell_words, near_words;    // words of each bitmap (multiples of 4)
   float gx0, gy0, inv_cs;

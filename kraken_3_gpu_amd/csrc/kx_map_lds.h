@@ -106,3 +106,42 @@ __device__ __forceinline__ void kx_sdf2_cells_lds(const KxDev &d, const uint16_t
   ok1 = n1 > 0 && isfinite(h1.d);
 }
 
+// The same two queries against the map in HBM (kx_sdf_grid_bm with the
+// occupancy bitmap `bits` in LDS, kx_sdf_grid without it): both cells'
+// occupancy bits, then both cells' offsets, then the first two rects of both
+// cells are read together, two dependent global loads for both points
+// instead of two per point. Every index read is valid (0 when unused: the
+// offsets have at least two entries and the entry array at least one).
+__device__ __forceinline__ void kx_sdf2_cells_glb(const KxDev &d, const uint32_t *bits, float x0, float y0, float x1,
+                                                  float y1, float ox, float oy, bool &ok0, KxHit &h0, bool &ok1,
+                                                  KxHit &h1) {
+  auto cellof = [&](float x, float y, int &cell) {
+    const float gx = (x + ox - d.gx0) * d.inv_cs;
+    const float gy = (y + oy - d.gy0) * d.inv_cs;
+    const bool in = gx >= 0.0f && gy >= 0.0f && gx < (float)d.gnx && gy < (float)d.gny;
+    const int ix = min((int)gx, d.gnx - 1), iy = min((int)gy, d.gny - 1);
+    cell = in ? iy * d.gnx + ix : 0;
+    return in;
+  };
+  int c0, c1;
+  bool in0 = cellof(x0, y0, c0), in1 = cellof(x1, y1, c1);
+  if (bits) {
+    in0 = in0 && ((bits[c0 >> 5] >> (c0 & 31)) & 1u) != 0u;
+    in1 = in1 && ((bits[c1 >> 5] >> (c1 & 31)) & 1u) != 0u;
+  }
+  const int b0 = d.cell_off[c0], e0 = d.cell_off[c0 + 1], b1 = d.cell_off[c1], e1 = d.cell_off[c1 + 1];
+  const int n0 = in0 ? e0 - b0 : 0, n1 = in1 ? e1 - b1 : 0;
+  const float4 r00 = d.cell_rects[n0 > 0 ? b0 : 0], r01 = d.cell_rects[n0 > 1 ? b0 + 1 : 0];
+  const float4 r10 = d.cell_rects[n1 > 0 ? b1 : 0], r11 = d.cell_rects[n1 > 1 ? b1 + 1 : 0];
+  kx_hit_init(h0);
+  kx_hit_init(h1);
+  if (n0 > 0) kx_fold_rect(x0, y0, r00, ox, oy, h0);
+  if (n0 > 1) kx_fold_rect(x0, y0, r01, ox, oy, h0);
+  for (int k = 2; k < n0; k++) kx_fold_rect(x0, y0, d.cell_rects[b0 + k], ox, oy, h0);
+  if (n1 > 0) kx_fold_rect(x1, y1, r10, ox, oy, h1);
+  if (n1 > 1) kx_fold_rect(x1, y1, r11, ox, oy, h1);
+  for (int k = 2; k < n1; k++) kx_fold_rect(x1, y1, d.cell_rects[b1 + k], ox, oy, h1);
+  ok0 = n0 > 0 && isfinite(h0.d);
+  ok1 = n1 > 0 && isfinite(h1.d);
+}
+

@@ -166,7 +166,7 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
   float4 *m_crect = (float4 *)(smem + P.tmpl_words + P.bm_words);
   uint16_t *m_off = (uint16_t *)(smem + P.tmpl_words + P.bm_words + 4 * P.map_ids);
   if (P.map_words) {
-    for (int i = threadIdx.x; i < P.map_ids; i += blockDim.x) m_crect[i] = d.rects[d.cell_ids[i]];
+    for (int i = threadIdx.x; i < P.map_ids; i += blockDim.x) m_crect[i] = d.cell_rects[i];
     for (int i = threadIdx.x; i <= P.map_cells; i += blockDim.x) m_off[i] = (uint16_t)d.cell_off[i];
   }
   __syncthreads();
@@ -757,13 +757,8 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
                   kx_world_point(x, y, ca, sa, p1.x, p1.y, wx1, wy1);
                   bool q0, q1;
                   KxHit h0, h1;
-                  if (lds_map) {
-                    kx_sdf2_cells_lds(d, m_off, m_crect, wx0, wy0, wx1, wy1, ox, oy, q0, h0, q1, h1);
-                  } else {
-                    kx_hit_init(h1);
-                    q0 = kx_query(d, P, bits, m_off, m_crect, wx0, wy0, ox, oy, h0);
-                    q1 = two && kx_query(d, P, bits, m_off, m_crect, wx1, wy1, ox, oy, h1);
-                  }
+                  if (lds_map) kx_sdf2_cells_lds(d, m_off, m_crect, wx0, wy0, wx1, wy1, ox, oy, q0, h0, q1, h1);
+                  else kx_sdf2_cells_glb(d, P.bm_words ? bits : nullptr, wx0, wy0, wx1, wy1, ox, oy, q0, h0, q1, h1);
                   if (q0 && p0.z - h0.d > KX_SLACK_POS) {
                     correct(h0, p0.z - h0.d);
                     if (two) {
@@ -948,7 +943,7 @@ __global__ __launch_bounds__(KX_WAVE *WPB) __attribute__((amdgpu_waves_per_eu(2)
   d.comp = kx_glob(d.comp); d.target = kx_glob(d.target); d.lbx = kx_glob(d.lbx); d.lby = kx_glob(d.lby);
   d.bf = kx_glob(d.bf); d.broken = kx_glob(d.broken);
   d.lam = kx_glob(d.lam); d.lamx = kx_glob(d.lamx); d.lamy = kx_glob(d.lamy);
-  d.rects = kx_glob(d.rects); d.cell_off = kx_glob(d.cell_off); d.cell_ids = kx_glob(d.cell_ids);
+  d.rects = kx_glob(d.rects); d.cell_off = kx_glob(d.cell_off); d.cell_rects = kx_glob(d.cell_rects);
   d.cell_bits = kx_glob(d.cell_bits); d.cr = kx_glob(d.cr); d.cd = kx_glob(d.cd); d.tape = kx_glob(d.tape);
   d.events = nullptr;
   KxPersistCfg P = e.P;
