@@ -452,7 +452,7 @@ def stress_leg(steps=200, warmup=20):
     from kraken_3_gpu_amd import world as W
     w = W.world_stress(65536, seed=1)
     law = law_for("stress", 2024)
-    g = G.create(w, path=K.KX_PATH_PERSISTENT)
+    g = G.create(w, path=K.KX_PATH_PERSISTENT, waves_per_block=2)
     G.upload(g)
     r = persistent_run(g, law, steps, warmup, 50)
     st = G.download(g)
@@ -474,7 +474,9 @@ def bench_stress(args):
     law = law_for("stress", 2024)
     w = W.world_stress(args.creatures or 65536, seed=1)
     n_cr = w.n_creatures
-    g = G.create(w, path=K.KX_PATH_PERSISTENT, lanes_per_creature=args.lanes, waves_per_block=args.wpb)
+    # two waves per workgroup: the template and bitmaps are shared by 8
+    # creatures, so 4 workgroups (8 waves) fit a CU's LDS instead of 7
+    g = G.create(w, path=K.KX_PATH_PERSISTENT, lanes_per_creature=args.lanes, waves_per_block=args.wpb or 2)
     G.upload(g)
     G.rollout(g, law, 0, args.warmup)
     G.sync(g)
