@@ -462,7 +462,9 @@ static int build_persist(kx_handle *h) {
   h->pinfo[8] = mixed ? 1 : 0;
   h->pinfo[9] = par ? KX_FOLD_PARALLEL : KX_FOLD_OWNER;
   // The fixed schedule (KX_REG_ROUNDS rounds of types KX_REG_RTYPE at 16
-  // lanes): every motor between two bodies with a finite rest angle, each
+  // lanes): every motor between two bodies with a finite rest angle and
+  // limits that are not inverted (no min > max: the motor solve of these
+  // kernels takes the limit excess as a sum of two one-sided terms), each
   // motor of an angle round on the even lane of the pin before it with the
   // same two bodies (the fused rounds), the pins of the fused rounds in
   // KX_REG_PINB all with a second body, at most KX_REG_BODY_REGS bodies per
@@ -479,7 +481,8 @@ static int build_persist(kx_handle *h) {
         all = false;
       const TmplHost &th = h->tmpl_host[t];
       for (size_t k = 0; k < th.type.size(); k++)
-        if (th.type[k] == KX_ANGLE && (th.b[k] < 0 || !std::isfinite(th.rest[k]))) all = false;
+        if (th.type[k] == KX_ANGLE && (th.b[k] < 0 || !std::isfinite(th.rest[k]) || th.mn[k] > th.mx[k]))
+          all = false;
       if (all) {
         const uint32_t *desc = pool.data() + T[t].base + T[t].o_sched;
         auto slot = [&](int r, int lane) { return (int)desc[((r >> 2) * 16 + lane) * 4 + (r & 3)]; };

@@ -222,7 +222,11 @@ __device__ __forceinline__ bool kx_angle_core(float &tA, float &tB, float wA, fl
   const float dBA = kx_angle_delta(tB, tA);
   float rel = hasB ? dBA : tA;
   float dev = kx_angle_delta(rel, rest);
-  float C = kx_angle_delta(dev, target);
+  // angleDelta(dev, target) without its finiteness select: dev is finite,
+  // so C is non-finite only for a non-finite target, where the fold gives
+  // NaN instead of 0; both fail the slack test below, and C is used only
+  // when that test passes
+  float C = kx_fold_finite(dev - target);
   const bool live = wSum > 0.0f;
   const bool corr = fabsf(C) >= KX_SLACK_ANG;
   float dl = -(C + alpha * lam) * kx_rcp(wSum + alpha);
@@ -248,19 +252,24 @@ __device__ __forceinline__ bool kx_angle_core(float &tA, float &tB, float wA, fl
   return ok1 || ok2;
 }
 
-// kx_angle_core for a motor between two bodies with a finite rest angle: the
-// operations of its hasB path in the same order, without the selects of a
-// world-anchored motor, and without the finiteness select of dev and dev2,
-// whose arguments (a fold result less the finite rest) are always finite.
-// rM is kx_rcp((wA + wB) + alpha), which a caller holding alpha fixed over
-// many solves computes once.
+// kx_angle_core for a motor between two bodies with a finite rest angle and
+// limits mn <= mx (or NaN): the operations of its hasB path in the same
+// order, without the selects of a world-anchored motor, and without the
+// finiteness select of dev and dev2, whose arguments (a fold result less the
+// finite rest) are always finite. The limit excess is max(dev2 - mx, 0) +
+// min(dev2 - mn, 0): with mn <= mx at most one term is non-zero and the other
+// adds an exact zero, so it equals the nested select (a NaN limit drops its
+// term in both forms). rM is kx_rcp((wA + wB) + alpha), which a caller holding
+// alpha fixed over many solves computes once. LIVE: the caller has already
+// required wA + wB > 0 of every result it commits.
+template <bool LIVE = false>
 __device__ __forceinline__ bool kx_angle_core_ab(float &tA, float &tB, float wA, float wB, float rest, float target,
                                                  float mn, float mx, float alpha, float rS, float rM, float &lam) {
   const float wSum = wA + wB;
   const float rel = kx_angle_delta(tB, tA);
   const float dev = kx_fold_finite(rel - rest);
-  const float C = kx_angle_delta(dev, target);
-  const bool live = wSum > 0.0f;
+  const float C = kx_fold_finite(dev - target);   // as in kx_angle_core
+  const bool live = LIVE || wSum > 0.0f;
   const bool corr = fabsf(C) >= KX_SLACK_ANG;
   float dl = -(C + alpha * lam) * rM;
   const bool bad = corr && !isfinite(dl);
@@ -273,7 +282,7 @@ __device__ __forceinline__ bool kx_angle_core_ab(float &tA, float &tB, float wA,
   tB = ok1 ? tB1 : tB;
   const float rel2 = kx_angle_delta(tB, tA);
   const float dev2 = kx_fold_finite(rel2 - rest);
-  float over = dev2 > mx ? dev2 - mx : (dev2 < mn ? dev2 - mn : 0.0f);
+  float over = fmaxf(dev2 - mx, 0.0f) + fminf(dev2 - mn, 0.0f);
   const bool ok2 = live && !bad && fabsf(over) >= KX_SLACK_ANG;
   float dl2 = -over * rS;
   float tA2 = tA - wA * dl2;

@@ -123,10 +123,14 @@ __device__ __forceinline__ void kx_shift_anchors(float4 *dyn, const float4 *t_cr
   }
 }
 
-// Value of the neighbouring lane of an adjacent pair (lanes 2j <-> 2j+1),
-// one DPP move (quad_perm [1,0,3,2]).
-__device__ __forceinline__ float kx_swap_pair(float v) {
-  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
+// Value of the even (kx_pair_even) or odd (kx_pair_odd) lane of an adjacent
+// pair (lanes 2j, 2j+1) on both lanes, one DPP move each (quad_perm
+// [0,0,2,2] and [1,1,3,3]).
+__device__ __forceinline__ float kx_pair_even(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xA0, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float kx_pair_odd(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xF5, 0xF, 0xF, true));
 }
 
 // Register-held round slot (RR > 0): this lane's slot of one round, its
@@ -285,10 +289,16 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
     for (int r = 0; r < RR; r++) fdsc[r] = slot_dsc(r);
   }
   if constexpr (RR > 0 && REG) {
+    // The odd lane of a pin pair holds the pin's two bodies in exchanged
+    // order: of its solve only the sin/cos of its first pose (the pin's
+    // second body) is used (the even lane commits poses and lambda), so
+    // every lane takes sin/cos of its first pose.
+    const bool xch = (sub & 1) != 0;
 #pragma unroll
     for (int r = 0; r < RR; r++) {
       const KxSlot c = kx_slot_read(slot_dsc(r), dyn, lam2, t_crec, t_brec);
-      slot[r] = KxRegSlot{c.dk, c.lk, c.cr, c.bA, c.bB, c.A, c.B, 0.0f};
+      const bool x = xch && ((KX_REG_RTYPE >> r) & 1) == 0;
+      slot[r] = KxRegSlot{c.dk, c.lk, c.cr, c.bA, c.bB, x ? c.B : c.A, x ? c.A : c.B, 0.0f};
       rflags |= (c.hasB ? 1u : 0u) << (16 + r);
     }
   }
@@ -351,7 +361,8 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
     if constexpr (RR > 0 && REG) {
       // the step's dynamics (control law, reflex, origin shift) into the
       // slots, and which slots commit this step: live and not broken (an
-      // angle keeps its broken flag in dyn.z, a pin or grip in dyn.y)
+      // angle keeps its broken flag in dyn.z, a pin or grip in dyn.y), and
+      // for a motor wA + wB > 0, the test its solve would otherwise repeat
       uint32_t lv = 0u;
 #pragma unroll
       for (int r = 0; r < RR; r++) {
@@ -361,7 +372,8 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
         const bool is_ang = ((KX_REG_RTYPE >> r) & 1) != 0;
         if (is_ang) slot[r].rm = kx_rcp((slot[r].bA.y + slot[r].bB.y) + dk.x);
         const int brk = __float_as_int(is_ang ? dk.z : dk.y);
-        lv |= (dsc >= 0 && brk == 0 ? 1u : 0u) << r;
+        const bool wpos = !is_ang || (slot[r].bA.y + slot[r].bB.y) > 0.0f;
+        lv |= (dsc >= 0 && brk == 0 && wpos ? 1u : 0u) << r;
       }
       rflags = (rflags & 0xFFFF0000u) | lv;
     }
@@ -428,21 +440,21 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
             const bool livem = ((fl >> (2 * f + 1)) & 1u) != 0u;
             const kx_f3v pA = *(const kx_f3v *)&pos[c.A], pB = *(const kx_f3v *)&pos[c.B];
             float s1, c1;
-            kx_sincos(odd ? pB.z : pA.z, &s1, &c1);
-            const float s2 = kx_swap_pair(s1), c2 = kx_swap_pair(c1);
+            kx_sincos(pA.z, &s1, &c1);   // the odd lane's first body is the pin's second
+            const float sa = kx_pair_even(s1), ca = kx_pair_even(c1), sb = kx_pair_odd(s1), cb = kx_pair_odd(c1);
             float xA = pA.x, yA = pA.y, tA = pA.z, xB = pB.x, yB = pB.y, tB = pB.z;
             float lx = c.lk.x, ly = c.lk.y;
             // (every lane runs both solves: the live flags only select)
             const bool okp = kx_pin_core_sc(xA, yA, tA, c.bA.x, c.bA.y, xB, yB, tB, c.bB.x, c.bB.y, hasB, c.cr.x,
-                                            c.cr.y, c.dk.z, c.dk.w, c.dk.x, lx, ly, odd ? s2 : s1, odd ? c2 : c1,
-                                            odd ? s1 : s2, odd ? c1 : c2) &&
+                                            c.cr.y, c.dk.z, c.dk.w, c.dk.x, lx, ly, sa, ca, sb, cb) &&
                              livep;
             if (okp) c.lk = kx_f2v{lx, ly};
             xA = okp ? xA : pA.x; yA = okp ? yA : pA.y; tA = okp ? tA : pA.z;
             xB = okp ? xB : pB.x; yB = okp ? yB : pB.y; tB = okp ? tB : pB.z;
             float uA = tA, uB = tB, lam = m.lk.x;
             const bool okm =
-                kx_angle_core_ab(uA, uB, m.bA.y, m.bB.y, m.cr.x, m.dk.y, m.cr.y, m.cr.z, m.dk.x, m.dk.w, m.rm, lam) &&
+                kx_angle_core_ab<true>(uA, uB, m.bA.y, m.bB.y, m.cr.x, m.dk.y, m.cr.y, m.cr.z, m.dk.x, m.dk.w, m.rm,
+                                       lam) &&
                 livem;
             if (okm) { tA = uA; tB = uB; m.lk.x = lam; }
             if (!odd && (okp || okm)) {
@@ -470,12 +482,11 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
             const bool hasB = ((KX_REG_PINB >> f) & 1) != 0 || c.hasB;
             float s1, c1;
             kx_sincos(odd ? pB.z : pA.z, &s1, &c1);
-            const float s2 = kx_swap_pair(s1), c2 = kx_swap_pair(c1);
+            const float sa = kx_pair_even(s1), ca = kx_pair_even(c1), sb = kx_pair_odd(s1), cb = kx_pair_odd(c1);
             float xA = pA.x, yA = pA.y, tA = pA.z, xB = pB.x, yB = pB.y, tB = pB.z;
             float lx = c.lk.x, ly = c.lk.y;
             const bool okp = kx_pin_core_sc(xA, yA, tA, c.bA.x, c.bA.y, xB, yB, tB, c.bB.x, c.bB.y, hasB, c.cr.x,
-                                            c.cr.y, c.dk.z, c.dk.w, c.dk.x, lx, ly, odd ? s2 : s1, odd ? c2 : c1,
-                                            odd ? s1 : s2, odd ? c1 : c2) &&
+                                            c.cr.y, c.dk.z, c.dk.w, c.dk.x, lx, ly, sa, ca, sb, cb) &&
                              livep;
             xA = okp ? xA : pA.x; yA = okp ? yA : pA.y; tA = okp ? tA : pA.z;
             xB = okp ? xB : pB.x; yB = okp ? yB : pB.y; tB = okp ? tB : pB.z;
@@ -518,17 +529,16 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
           } else {
             // crec {lax, lay}, dyn {alpha, broken, lbx, lby}, lambda,
             // positions, {im, ii}; lanes 2j and 2j+1 hold the same pin:
-            // each evaluates sin/cos of one body's angle and the pair swaps
-            // results; the even lane commits
+            // each evaluates sin/cos of one body's angle and both lanes take
+            // the pair's two results; the even lane commits
             const bool odd = (sub & 1) != 0;
             float s1, c1;
             kx_sincos(odd ? pB.z : pA.z, &s1, &c1);
-            const float s2 = kx_swap_pair(s1), c2 = kx_swap_pair(c1);
+            const float sa = kx_pair_even(s1), ca = kx_pair_even(c1), sb = kx_pair_odd(s1), cb = kx_pair_odd(c1);
             float xA = pA.x, yA = pA.y, tA = pA.z, xB = pB.x, yB = pB.y, tB = pB.z;
             float lx = c.lk.x, ly = c.lk.y;
             const bool moved = kx_pin_core_sc(xA, yA, tA, c.bA.x, c.bA.y, xB, yB, tB, c.bB.x, c.bB.y, c.hasB, c.cr.x,
-                                              c.cr.y, c.dk.z, c.dk.w, c.dk.x, lx, ly, odd ? s2 : s1, odd ? c2 : c1,
-                                              odd ? s1 : s2, odd ? c1 : c2) &&
+                                              c.cr.y, c.dk.z, c.dk.w, c.dk.x, lx, ly, sa, ca, sb, cb) &&
                                !odd;
             if (c.live && moved && __float_as_int(c.dk.y) == 0) {
               // B first: for a world anchor B == A and pA must win
