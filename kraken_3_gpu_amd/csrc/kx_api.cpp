@@ -236,6 +236,19 @@ static int uf_find(std::vector<int> &p, int x) {
   return x;
 }
 
+// The least binary32 q with sqrtf(q) > m (sqrtf correctly rounded, so
+// monotonic in q): the speed clamp's test sqrtf(vx*vx + vy*vy) > maxSpeed is
+// then q >= threshold, and the square root is taken only to clamp. -0 when
+// every q >= 0 qualifies (m < 0), NaN when no q does (m NaN or +inf).
+static float speed_threshold(float m) {
+  if (std::isnan(m) || m == INFINITY) return NAN;
+  if (m < 0.0f) return -0.0f;
+  float q = m * m;   // within a few ulps of the threshold (inf for a huge m)
+  while (q > 0.0f && std::sqrt(std::nextafter(q, 0.0f)) > m) q = std::nextafter(q, 0.0f);
+  while (!(std::sqrt(q) > m)) q = std::nextafter(q, INFINITY);
+  return q;
+}
+
 // ── exact cull grid over the rect list ──────────────────────────────
 // Also two cell bitmaps, each padded to a multiple of four words and stored
 // back to back: occupancy (cull cells with a registered rect) and the near
@@ -845,6 +858,7 @@ static int create_impl(const kx_world_desc *w, const kx_creature_desc *cd, const
   d.linDamp = (float)std::exp(-p->lin_drag * p->h);
   d.angDamp = (float)std::exp(-p->ang_drag * p->h);
   d.maxSpeed = (float)p->max_speed;
+  d.maxSpeed2 = speed_threshold(d.maxSpeed);
   d.maxAngSpeed = (float)p->max_ang_speed;
   d.hd = p->h;
   d.S = p->substeps; d.I = p->iterations;

@@ -329,8 +329,9 @@ __device__ __forceinline__ void kx_integrate(const KxDev &d, float &vx, float &v
   vx = (vx + d.h * im * fx) * d.linDamp;
   vy = (vy + d.h * im * fy) * d.linDamp;
   av = (av + d.h * ii * tq) * d.angDamp;
-  float sp = sqrtf(vx * vx + vy * vy);
-  if (sp > d.maxSpeed) { float k = d.maxSpeed / sp; vx *= k; vy *= k; }
+  // sqrtf(q) > maxSpeed as q >= maxSpeed2 (the square root only to clamp)
+  const float q = vx * vx + vy * vy;
+  if (q >= d.maxSpeed2) { float k = d.maxSpeed / sqrtf(q); vx *= k; vy *= k; }
   if (av > d.maxAngSpeed) av = d.maxAngSpeed;
   else if (av < -d.maxAngSpeed) av = -d.maxAngSpeed;
   if (!isfinite(vx) || !isfinite(vy)) { vx = 0.0f; vy = 0.0f; }
@@ -347,8 +348,8 @@ __device__ __forceinline__ void kx_finalize(const KxDev &d, float &px, float &py
   vx = (ppx - px) * d.invH;
   vy = (ppy - py) * d.invH;
   av = kx_angle_delta(angP, ang) * d.invH;
-  float sp = sqrtf(vx * vx + vy * vy);
-  if (sp > d.maxSpeed) { float k = d.maxSpeed / sp; vx *= k; vy *= k; }
+  const float q = vx * vx + vy * vy;   // as in kx_integrate
+  if (q >= d.maxSpeed2) { float k = d.maxSpeed / sqrtf(q); vx *= k; vy *= k; }
   if (av > d.maxAngSpeed) av = d.maxAngSpeed;
   else if (av < -d.maxAngSpeed) av = -d.maxAngSpeed;
   px = ppx; py = ppy; ang = angP;
@@ -363,22 +364,29 @@ __device__ __forceinline__ uint32_t kx_mb32(uint32_t a) {
   t ^= t + (t ^ (t >> 7)) * (t | 61u);
   return t ^ (t >> 14);
 }
+// the chain's state after seed, creature and step (shared by every slot of
+// a creature's step), and the draw of one slot from it
+__device__ __forceinline__ uint32_t kx_ctl_prefix(uint32_t seed, uint32_t c, uint32_t step) {
+  return kx_mb32(kx_mb32(kx_mb32(seed) ^ c) ^ step);
+}
+__device__ __forceinline__ double kx_ctl_u_at(uint32_t prefix, uint32_t slot) {
+  return (double)kx_mb32(prefix ^ slot) * (1.0 / 4294967296.0);
+}
 __device__ __forceinline__ double kx_ctl_u(uint32_t seed, uint32_t c, uint32_t step, uint32_t slot) {
-  uint32_t h = kx_mb32(seed);
-  h = kx_mb32(h ^ c);
-  h = kx_mb32(h ^ step);
-  h = kx_mb32(h ^ slot);
-  return (double)h * (1.0 / 4294967296.0);
+  return kx_ctl_u_at(kx_ctl_prefix(seed, c, step), slot);
 }
 __device__ __forceinline__ double kx_jsmin(double a, double b) { return a < b ? a : b; }
 __device__ __forceinline__ double kx_jsmax(double a, double b) { return a > b ? a : b; }
 
 // target of a motor under the law: (2u-1)*limit, clamped (core.js:195-198)
-__device__ __forceinline__ float kx_law_target(const KxLaw &L, uint32_t cid, uint32_t step, int slot) {
-  double u = kx_ctl_u(L.seed, cid, step, (uint32_t)slot);
+__device__ __forceinline__ float kx_law_target_at(const KxLaw &L, uint32_t prefix, int slot) {
+  double u = kx_ctl_u_at(prefix, (uint32_t)slot);
   double lim = (slot & (KX_SLOTS_PER_ARM - 1)) == 0 ? L.shoulder_limit : L.joint_limit;
   double t = __dmul_rn(__dsub_rn(__dmul_rn(2.0, u), 1.0), lim);
   return (float)kx_jsmax(-lim, kx_jsmin(lim, t));
+}
+__device__ __forceinline__ float kx_law_target(const KxLaw &L, uint32_t cid, uint32_t step, int slot) {
+  return kx_law_target_at(L, kx_ctl_prefix(L.seed, cid, step), slot);
 }
 
 // Control tape: per step and creature the inputs of applyControl

@@ -78,6 +78,10 @@ struct KxDev {
   const float *tape;             // [rows][nCr][KX_TAPE_WORDS] controls under KX_LAW_TAPE, or null
   // constants
   float h, invH, invH2, linDamp, angDamp, maxSpeed, maxAngSpeed;
+  // the least binary32 q whose correctly rounded square root exceeds
+  // maxSpeed (-0 when every q qualifies, NaN when none does): for the q of
+  // a velocity, sqrtf(q) > maxSpeed exactly when q >= maxSpeed2
+  float maxSpeed2;
   double hd;                     // h in f64 (reflex cooldown arithmetic)
   int S, I, nB, nC, nCr, nArms;
   int ncol;                      // non-empty colours in handle order

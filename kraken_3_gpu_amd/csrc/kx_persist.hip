@@ -108,18 +108,32 @@ int kx_persist_cr_words(int NB, int NC, int NP, int NG, int NA, bool par) {
   return KxCrLds(NB, NC, NP, NG, NA, par).words;
 }
 
+// Whether constraint record `v` (crec.w) is a world-anchored pin or grip.
+__device__ __forceinline__ bool kx_world_anchored(int v) { return ((v >> 8) & 255) == 255 && (v >> 16) != KX_ANGLE; }
+
 // Adds (sx, sy) to the world anchors of a creature's world-anchored pins and
 // grips; with the same additions to the body poses (kx_persist_body) these
 // are the plain path's shift in binary32 (an angle constraint has no anchor).
-__device__ __forceinline__ void kx_shift_anchors(float4 *dyn, const float4 *t_crec, int nc, int sub, int G, float sx,
-                                                 float sy) {
-  for (int k = sub; k < nc; k += G) {
-    const int v = __float_as_int(t_crec[k].w);
-    if (((v >> 8) & 255) == 255 && (v >> 16) != KX_ANGLE) {
-      float4 dk = dyn[k];
-      dk.z += sx; dk.w += sy;
-      dyn[k] = dk;
+// wa: the lane's world-anchored constraints among sub + j*G (bit j), or ~0u
+// when the creature has more than 32 constraints per lane (then every
+// constraint of the lane is tested).
+__device__ __forceinline__ void kx_shift_anchors(float4 *dyn, const float4 *t_crec, int nc, int sub, int G, uint32_t wa,
+                                                 float sx, float sy) {
+  if (wa == ~0u) {
+    for (int k = sub; k < nc; k += G) {
+      if (kx_world_anchored(__float_as_int(t_crec[k].w))) {
+        float4 dk = dyn[k];
+        dk.z += sx; dk.w += sy;
+        dyn[k] = dk;
+      }
     }
+    return;
+  }
+  for (uint32_t m = wa; m; m &= m - 1u) {
+    const int k = sub + __builtin_ctz(m) * G;
+    float4 dk = dyn[k];
+    dk.z += sx; dk.w += sy;
+    dyn[k] = dk;
   }
 }
 
@@ -266,6 +280,12 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
       if (br.x != 0.0f && (__float_as_int(br.z) >> 16) > 0) bp_mask |= 1u << k;
     }
   }
+  // the lane's world-anchored pins and grips among sub + j*G (bit j)
+  uint32_t wa_mask = 0u;
+  if (nc > 32 * G) wa_mask = ~0u;
+  else
+    for (int j = 0, k = sub; k < nc; j++, k += G)
+      if (kx_world_anchored(__float_as_int(t_crec[k].w))) wa_mask |= 1u << j;
   double comp_last = L.comp;      // motor compliance of the latest control step
   // round types of the first 64 rounds (bit r set = angle round)
   const uint32_t rb0 = (uint32_t)__builtin_amdgcn_readfirstlane(t_rbits[0]);
@@ -314,18 +334,18 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
       const float *rec = tape && act ? kx_tape_rec(d, step, cid) : nullptr;
       if (rec) comp_last = kx_tape_comp(L, rec[0]);
       const float a_law = (float)comp_last * d.invH2;
+      const uint32_t ck = (uint32_t)cid + L.cbase;
+      const uint32_t hstep = kx_ctl_prefix(L.seed, ck, (uint32_t)key);   // this creature's step in the chain
       for (int j = sub; j < (act ? T.nmot : 0); j += G) {
         int k = t_mot_k[j];
         float4 dk = dyn[k];
         dk.x = a_law;
-        dk.y = rec ? kx_tape_target(L, rec, t_mot_slot[j])
-                   : kx_law_target(L, (uint32_t)cid + L.cbase, (uint32_t)key, t_mot_slot[j]);
+        dk.y = rec ? kx_tape_target(L, rec, t_mot_slot[j]) : kx_law_target_at(L, hstep, t_mot_slot[j]);
         dyn[k] = dk;
       }
       // adhesion reflex (core.js:199-218), one lane per arm
       if (act && sub < T.narms) {
-        const uint32_t ck = (uint32_t)cid + L.cbase;
-        double adh = rec ? (double)rec[3 + 3 * sub] : kx_ctl_u(L.seed, ck, (uint32_t)key, KX_SLOT_ADH + sub);
+        double adh = rec ? (double)rec[3 + 3 * sub] : kx_ctl_u_at(hstep, KX_SLOT_ADH + sub);
         double gu = (L.flags & KX_LAW_FIXED_GRIP) ? kx_ctl_u(L.seed, ck, KX_GRIP_DRAW_KEY, KX_SLOT_ADH + sub) : 0.0;
         const int g = t_arm_grip[sub], j = t_arm_gslot[sub];
         float4 dk = dyn[g], tp = tip[2 * sub], cr0 = t_crec[g];   // tip {px, py, ang, vx}; crec {lax, lay, -, cab}
@@ -356,7 +376,7 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
       if (k >= nbl) break;
       rx[k] -= ox; ry[k] -= oy;
     }
-    kx_shift_anchors(dyn, t_crec, nc, sub, G, -ox, -oy);
+    kx_shift_anchors(dyn, t_crec, nc, sub, G, wa_mask, -ox, -oy);
     KX_WAVE_SYNC();
     if constexpr (RR > 0 && REG) {
       // the step's dynamics (control law, reflex, origin shift) into the
@@ -842,7 +862,7 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
         pos[i] = pp;
       }
     }
-    kx_shift_anchors(dyn, t_crec, nc, sub, G, ox, oy);
+    kx_shift_anchors(dyn, t_crec, nc, sub, G, wa_mask, ox, oy);
     publish_tips();
     KX_WAVE_SYNC();
   }
