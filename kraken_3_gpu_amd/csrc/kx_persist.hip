@@ -309,16 +309,10 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
     for (int r = 0; r < RR; r++) fdsc[r] = slot_dsc(r);
   }
   if constexpr (RR > 0 && REG) {
-    // The odd lane of a pin pair holds the pin's two bodies in exchanged
-    // order: of its solve only the sin/cos of its first pose (the pin's
-    // second body) is used (the even lane commits poses and lambda), so
-    // every lane takes sin/cos of its first pose.
-    const bool xch = (sub & 1) != 0;
 #pragma unroll
     for (int r = 0; r < RR; r++) {
       const KxSlot c = kx_slot_read(slot_dsc(r), dyn, lam2, t_crec, t_brec);
-      const bool x = xch && ((KX_REG_RTYPE >> r) & 1) == 0;
-      slot[r] = KxRegSlot{c.dk, c.lk, c.cr, c.bA, c.bB, x ? c.B : c.A, x ? c.A : c.B, 0.0f};
+      slot[r] = KxRegSlot{c.dk, c.lk, c.cr, c.bA, c.bB, c.A, c.B, 0.0f};
       rflags |= (c.hasB ? 1u : 0u) << (16 + r);
     }
   }
@@ -460,7 +454,7 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
             const bool livem = ((fl >> (2 * f + 1)) & 1u) != 0u;
             const kx_f3v pA = *(const kx_f3v *)&pos[c.A], pB = *(const kx_f3v *)&pos[c.B];
             float s1, c1;
-            kx_sincos(pA.z, &s1, &c1);   // the odd lane's first body is the pin's second
+            kx_sincos(odd ? pB.z : pA.z, &s1, &c1);
             const float sa = kx_pair_even(s1), ca = kx_pair_even(c1), sb = kx_pair_odd(s1), cb = kx_pair_odd(c1);
             float xA = pA.x, yA = pA.y, tA = pA.z, xB = pB.x, yB = pB.y, tB = pB.z;
             float lx = c.lk.x, ly = c.lk.y;
