@@ -224,8 +224,12 @@ int kx_schedule_info(const kx_handle *h, int32_t *rounds, int32_t max);
  * [9] the collision fold in use (KX_FOLD_PARALLEL or KX_FOLD_OWNER), [10]
  * rounds per iteration held in registers (0: the round slots are read from
  * LDS each round; see kx_set_register_slots), [11] rounds per iteration of
- * the fixed schedule run with its slots read from LDS (0: not that kernel).
- * Returns the number of fields (12). No counterpart in the reference (its
+ * the fixed schedule run with its slots read from LDS (0: not that kernel),
+ * [12] creatures whose template the run-time kernels solve in fused
+ * pin/motor rounds (templates whose rounds alternate pins and motors, each
+ * motor on the even lane of the pin before it with the same two bodies, at
+ * 16 or 32 lanes; 0 with kx_set_register_slots mode 0 or under [10]/[11]).
+ * Returns the number of fields (13). No counterpart in the reference (its
  * path has no persistent kernel). */
 int kx_persist_info(const kx_handle *h, int32_t *info, int32_t max);
 /* Persistent path: the kernels specialised to the schedule of the default
@@ -236,10 +240,12 @@ int kx_persist_info(const kx_handle *h, int32_t *info, int32_t max);
  * every round in registers and a round reads only its two poses from LDS),
  * else the same fused rounds with the slots read from LDS; 2: the LDS-read
  * form even where the register form would fit; 0: neither (the kernel that
- * schedules any morphology at run time). Templates that do not have that
- * schedule always take the run-time kernel. Results are identical bit for
- * bit in every mode. No counterpart in the reference (a launch-shape knob,
- * like waves_per_block). */
+ * schedules any morphology at run time, every round solved on its own).
+ * Templates that do not have that schedule take the run-time kernel, which
+ * in modes 1 and 2 solves a template's hinge and muscle rounds fused too
+ * when its schedule allows (kx_persist_info field 12). Results are identical
+ * bit for bit in every mode. No counterpart in the reference (a launch-shape
+ * knob, like waves_per_block). */
 int kx_set_register_slots(kx_handle *h, int32_t enable);
 /* Persistent path: how a step folds the collision points of the bodies near
  * a rect (xpbd.js:441-470 either way, with identical results).

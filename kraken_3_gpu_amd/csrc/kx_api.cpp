@@ -99,7 +99,7 @@ struct kx_handle {
   std::vector<int> tmpl_rounds;
   KxPersistCfg pc{};
   void *pc_bufs[4] = {nullptr, nullptr, nullptr, nullptr};
-  int pinfo[12] = {};    // see kx_persist_info
+  int pinfo[13] = {};    // see kx_persist_info
   std::vector<int> tmpl_lanes;   // per template lanes per creature, 0 = the handle's G
   int launch_G = 16;             // lane count the uniform kernel is launched with
   int coll_mode = 0;             // collision fold: 0 = by map density, 1 = parallel, 2 = owning lane
@@ -377,6 +377,30 @@ static int tmpl_lanes_of(const kx_handle *h, size_t t) {
 
 // Pack every template's schedule for its lane count G_t and map creatures
 // onto workgroups: WPB wavefronts of 64/G_t creatures of one template each.
+// Whether a packed template's rounds can run fused: an even number of at
+// most KX_FUSED_MAX_ROUNDS rounds alternating pins (even rounds) and motors
+// (odd rounds), every motor between two bodies with a finite rest angle and
+// limits that are not inverted (no min > max: the fused motor solve takes the
+// limit excess as a sum of two one-sided terms), each on the even lane of the
+// pin before it with the same two bodies (kx_pack_template places it there
+// when it can).
+static bool fused_schedule(const KxTmpl &k, const TmplHost &th, const std::vector<uint32_t> &pool, int G) {
+  if (k.R < 2 || (k.R & 1) || k.R > KX_FUSED_MAX_ROUNDS) return false;
+  if (pool[(size_t)k.base + k.o_rtype] != (0xAAAAAAAAu & ((1u << k.R) - 1u))) return false;
+  for (size_t j = 0; j < th.type.size(); j++)
+    if (th.type[j] == KX_ANGLE && (th.b[j] < 0 || !std::isfinite(th.rest[j]) || th.mn[j] > th.mx[j])) return false;
+  const uint32_t *desc = pool.data() + k.base + k.o_sched;
+  auto slot = [&](int r, int lane) { return (int)desc[((r >> 2) * G + lane) * 4 + (r & 3)]; };
+  for (int r = 1; r < k.R; r += 2)
+    for (int l = 0; l < G; l++) {
+      const int a = slot(r, l);
+      if (a < 0) continue;
+      const int p = slot(r - 1, l);
+      if ((l & 1) || p < 0 || (p >> 8) != (a >> 8)) return false;
+    }
+  return true;
+}
+
 static int build_persist(kx_handle *h) {
   const int WPB = h->wpb;
   std::vector<uint32_t> pool;
@@ -474,38 +498,27 @@ static int build_persist(kx_handle *h) {
   h->pinfo[4] = mapw; h->pinfo[5] = bmw; h->pinfo[6] = words; h->pinfo[7] = stride;
   h->pinfo[8] = mixed ? 1 : 0;
   h->pinfo[9] = par ? KX_FOLD_PARALLEL : KX_FOLD_OWNER;
-  // The fixed schedule (KX_REG_ROUNDS rounds of types KX_REG_RTYPE at 16
-  // lanes): every motor between two bodies with a finite rest angle and
-  // limits that are not inverted (no min > max: the motor solve of these
-  // kernels takes the limit excess as a sum of two one-sided terms), each
-  // motor of an angle round on the even lane of the pin before it with the
-  // same two bodies (the fused rounds), the pins of the fused rounds in
+  // The fixed schedule: every template fused (fused_schedule) with
+  // KX_REG_ROUNDS rounds at 16 lanes, the pins of the fused rounds in
   // KX_REG_PINB all with a second body, at most KX_REG_BODY_REGS bodies per
-  // lane. With it, register-held round slots when every workgroup is
-  // resident at once at one wavefront per SIMD (the slots take the register
-  // file, so a SIMD holds no second wave), else the slots read from LDS.
+  // lane. With it, register-held round slots when every workgroup is resident
+  // at once at one wavefront per SIMD (the slots take the register file, so a
+  // SIMD holds no second wave), else the slots read from LDS. Without it, the
+  // generic kernels solve each fused template's rounds fused (KxTmpl::fused)
+  // at 16 or 32 lanes and every other template's rounds one at a time.
   int reg = 0, fixed = 0;
+  for (size_t t = 0; t < T.size(); t++) {
+    const int Gt = tmpl_lanes_of(h, t);
+    T[t].fused = h->reg_mode && (Gt == 16 || Gt == 32) && fused_schedule(T[t], h->tmpl_host[t], pool, Gt) ? 1 : 0;
+  }
   if (h->reg_mode && !mixed && Gk == 16) {
     bool all = true;
     for (size_t t = 0; t < T.size(); t++) {
       if (members[t].empty()) continue;
-      if (T[t].R != KX_REG_ROUNDS || T[t].nb > KX_REG_BODY_REGS * 16 ||
-          pool[(size_t)T[t].base + T[t].o_rtype] != (uint32_t)KX_REG_RTYPE)
-        all = false;
-      const TmplHost &th = h->tmpl_host[t];
-      for (size_t k = 0; k < th.type.size(); k++)
-        if (th.type[k] == KX_ANGLE && (th.b[k] < 0 || !std::isfinite(th.rest[k]) || th.mn[k] > th.mx[k]))
-          all = false;
+      if (!T[t].fused || T[t].R != KX_REG_ROUNDS || T[t].nb > KX_REG_BODY_REGS * 16) all = false;
       if (all) {
         const uint32_t *desc = pool.data() + T[t].base + T[t].o_sched;
         auto slot = [&](int r, int lane) { return (int)desc[((r >> 2) * 16 + lane) * 4 + (r & 3)]; };
-        for (int r = 1; r < KX_REG_ROUNDS; r += 2)
-          for (int l = 0; l < 16; l++) {
-            const int a = slot(r, l);
-            if (a < 0) continue;
-            const int p = slot(r - 1, l);
-            if ((l & 1) || p < 0 || (p >> 8) != (a >> 8)) all = false;
-          }
         for (int f = 0; f < KX_REG_ROUNDS / 2; f++)
           if ((KX_REG_PINB >> f) & 1)
             for (int l = 0; l < 16; l++) {
@@ -522,6 +535,10 @@ static int build_persist(kx_handle *h) {
   }
   h->pinfo[10] = reg;
   h->pinfo[11] = fixed;
+  int fused_cr = 0;
+  if (!reg && !fixed)
+    for (size_t t = 0; t < T.size(); t++) fused_cr += T[t].fused ? (int)members[t].size() : 0;
+  h->pinfo[12] = fused_cr;
   std::vector<int> tables;
   tables.insert(tables.end(), wg_tmpl.begin(), wg_tmpl.end());
   tables.insert(tables.end(), wg_lanes.begin(), wg_lanes.end());
@@ -1275,8 +1292,8 @@ extern "C" int kx_schedule_info(const kx_handle *h, int32_t *rounds, int32_t max
 extern "C" int kx_persist_info(const kx_handle *h, int32_t *info, int32_t max) {
   if (!h || !info) return fail(KX_EINVAL, "null argument");
   if (!h->persist_ok) return fail(KX_ENOTSUP, "persistent path unavailable: %s", h->persist_why.c_str());
-  for (int i = 0; i < 12 && i < max; i++) info[i] = h->pinfo[i];
-  return 12;
+  for (int i = 0; i < 13 && i < max; i++) info[i] = h->pinfo[i];
+  return 13;
 }
 
 extern "C" int kx_template_info(const kx_handle *h, int32_t t, int32_t *out6) {

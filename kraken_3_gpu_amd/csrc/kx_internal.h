@@ -138,6 +138,8 @@ int kx_persist_cr_words(int NB, int NC, int NP, int NG, int NA, bool par);
 int kx_persist_occupancy(int G, int wpb, size_t lds, bool par);   // G = 0: the mixed-lane kernel
 // register-slot kernel: 16 lanes per creature, KX_REG_ROUNDS rounds per iteration
 #define KX_REG_ROUNDS 12
+// the most rounds of a template whose rounds the generic kernels solve fused
+#define KX_FUSED_MAX_ROUNDS 16
 // its round types, bit r set = round r solves angle motors: the schedule of the
 // 6-arm x 7-segment creature (pins and motors alternate along the mantle chain)
 #define KX_REG_RTYPE 0xAAA

@@ -308,6 +308,44 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
 #pragma unroll
     for (int r = 0; r < RR; r++) fdsc[r] = slot_dsc(r);
   }
+  // One fused round of LDS-held slots: a pin (or grip) on lane pairs and,
+  // on the even lane, the motor with the same two bodies solved on the pose
+  // the pin leaves (its own pose if the pin does not commit), which is the
+  // pose the motor's round would read; the even lane writes the pair's two
+  // bodies once. pinB: every pin of the round has a second body (a
+  // constant of the fixed schedule).
+  [[maybe_unused]] auto fused_pair = [&](int dc, int dm, bool pinB) {
+    const KxSlot c = kx_slot_read(dc, dyn, lam2, t_crec, t_brec);
+    const KxSlot m = kx_slot_read(dm, dyn, lam2, t_crec, t_brec);
+    const kx_f3v pA = *(const kx_f3v *)&pos[c.A], pB = *(const kx_f3v *)&pos[c.B];
+    // pins and grips keep their broken flag in dyn.y, motors in dyn.z
+    const bool livep = c.live && __float_as_int(c.dk.y) == 0;
+    const bool livem = m.live && __float_as_int(m.dk.z) == 0;
+    const bool hasB = pinB || c.hasB;
+    const bool odd = (sub & 1) != 0;
+    float s1, c1;
+    kx_sincos(odd ? pB.z : pA.z, &s1, &c1);
+    const float sa = kx_pair_even(s1), ca = kx_pair_even(c1), sb = kx_pair_odd(s1), cb = kx_pair_odd(c1);
+    float xA = pA.x, yA = pA.y, tA = pA.z, xB = pB.x, yB = pB.y, tB = pB.z;
+    float lx = c.lk.x, ly = c.lk.y;
+    const bool okp = kx_pin_core_sc(xA, yA, tA, c.bA.x, c.bA.y, xB, yB, tB, c.bB.x, c.bB.y, hasB, c.cr.x, c.cr.y,
+                                    c.dk.z, c.dk.w, c.dk.x, lx, ly, sa, ca, sb, cb) &&
+                     livep;
+    xA = okp ? xA : pA.x; yA = okp ? yA : pA.y; tA = okp ? tA : pA.z;
+    xB = okp ? xB : pB.x; yB = okp ? yB : pB.y; tB = okp ? tB : pB.z;
+    float uA = tA, uB = tB, lam = m.lk.x;
+    const float rm = kx_rcp((m.bA.y + m.bB.y) + m.dk.x);
+    const bool okm =
+        kx_angle_core_ab(uA, uB, m.bA.y, m.bB.y, m.cr.x, m.dk.y, m.cr.y, m.cr.z, m.dk.x, m.dk.w, rm, lam) && livem;
+    if (okm) { tA = uA; tB = uB; ((float *)&lam2[m.k])[0] = lam; }
+    if (!odd && okp) lam2[c.k] = kx_f2v{lx, ly};
+    if (!odd && (okp || okm)) {
+      // B first: for a world anchor B == A and pA must win
+      *(kx_f3v *)&pos[c.B] = kx_f3v{xB, yB, tB};
+      *(kx_f3v *)&pos[c.A] = kx_f3v{xA, yA, tA};
+    }
+    KX_WAVE_SYNC();
+  };
   if constexpr (RR > 0 && REG) {
 #pragma unroll
     for (int r = 0; r < RR; r++) {
@@ -481,42 +519,21 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
         } else if constexpr (RR > 0) {
           // the fused rounds of the fixed schedule with the slots' records
           // read from LDS each round (launches of more than one wavefront per
-          // SIMD, whose register file holds no slots): the same solves,
-          // commits and lambda bookkeeping as the LDS-slot rounds below
+          // SIMD, whose register file holds no slots)
           static_assert(KX_REG_RTYPE == 0xAAA && RR == 12, "fused rounds: pins on even rounds, motors on odd");
-          const bool odd = (sub & 1) != 0;
 #pragma unroll
-          for (int f = 0; f < RR / 2; f++) {
-            const KxSlot c = kx_slot_read(fdsc[2 * f], dyn, lam2, t_crec, t_brec);
-            const KxSlot m = kx_slot_read(fdsc[2 * f + 1], dyn, lam2, t_crec, t_brec);
-            const kx_f3v pA = *(const kx_f3v *)&pos[c.A], pB = *(const kx_f3v *)&pos[c.B];
-            // pins and grips keep their broken flag in dyn.y, motors in dyn.z
-            const bool livep = c.live && __float_as_int(c.dk.y) == 0;
-            const bool livem = m.live && __float_as_int(m.dk.z) == 0;
-            const bool hasB = ((KX_REG_PINB >> f) & 1) != 0 || c.hasB;
-            float s1, c1;
-            kx_sincos(odd ? pB.z : pA.z, &s1, &c1);
-            const float sa = kx_pair_even(s1), ca = kx_pair_even(c1), sb = kx_pair_odd(s1), cb = kx_pair_odd(c1);
-            float xA = pA.x, yA = pA.y, tA = pA.z, xB = pB.x, yB = pB.y, tB = pB.z;
-            float lx = c.lk.x, ly = c.lk.y;
-            const bool okp = kx_pin_core_sc(xA, yA, tA, c.bA.x, c.bA.y, xB, yB, tB, c.bB.x, c.bB.y, hasB, c.cr.x,
-                                            c.cr.y, c.dk.z, c.dk.w, c.dk.x, lx, ly, sa, ca, sb, cb) &&
-                             livep;
-            xA = okp ? xA : pA.x; yA = okp ? yA : pA.y; tA = okp ? tA : pA.z;
-            xB = okp ? xB : pB.x; yB = okp ? yB : pB.y; tB = okp ? tB : pB.z;
-            float uA = tA, uB = tB, lam = m.lk.x;
-            const float rm = kx_rcp((m.bA.y + m.bB.y) + m.dk.x);
-            const bool okm =
-                kx_angle_core_ab(uA, uB, m.bA.y, m.bB.y, m.cr.x, m.dk.y, m.cr.y, m.cr.z, m.dk.x, m.dk.w, rm, lam) &&
-                livem;
-            if (okm) { tA = uA; tB = uB; ((float *)&lam2[m.k])[0] = lam; }
-            if (!odd && okp) lam2[c.k] = kx_f2v{lx, ly};
-            if (!odd && (okp || okm)) {
-              // B first: for a world anchor B == A and pA must win
-              *(kx_f3v *)&pos[c.B] = kx_f3v{xB, yB, tB};
-              *(kx_f3v *)&pos[c.A] = kx_f3v{xA, yA, tA};
-            }
-            KX_WAVE_SYNC();
+          for (int f = 0; f < RR / 2; f++) fused_pair(fdsc[2 * f], fdsc[2 * f + 1], ((KX_REG_PINB >> f) & 1) != 0);
+        } else {
+        if (T.fused) {
+          // a template the host marked fused (its rounds alternate pins and
+          // motors, each motor on the even lane of the pin before it with
+          // the same two bodies): the fixed schedule's fused rounds with the
+          // template's own round count; the slot descriptors of two fused
+          // rounds come in one 16-byte LDS read
+          for (int g = 0; 4 * g < T.R; g++) {
+            const int4 d4 = act ? t_sched4[g * G + sub] : make_int4(-1, -1, -1, -1);
+            fused_pair(d4.x, d4.y, false);
+            if (4 * g + 2 < T.R) fused_pair(d4.z, d4.w, false);
           }
         } else {
         int4 d4n = act ? t_sched4[sub] : make_int4(-1, -1, -1, -1);
@@ -577,6 +594,7 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
           round(sa, sb, d4.w, (rbits >> ((r0 + 2) & 31)) & 1u);
           if (r0 + 3 >= T.R) break;
           round(sb, sa, d4n.x, (rbits >> ((r0 + 3) & 31)) & 1u);
+        }
         }
         }
         // collisions (xpbd.js:441-470: per body, its points in order, each

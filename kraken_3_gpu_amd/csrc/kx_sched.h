@@ -17,6 +17,7 @@ struct KxTmpl {
   // o_rtype: round-type bitmask words (bit r set = angle round)
   int o_sched, o_rtype, o_coll, o_ccnt, o_arm_tip, o_arm_grip, o_mot_k, o_mot_slot, o_grip_k, o_grip_arm;
   int o_arm_gslot;                // per arm: the slot of its grip in the grip list
+  int fused;                      // 1: the generic kernels solve its rounds fused (set by the host)
 };
 
 // Host form of a template, in creature-local indices (constraints in the

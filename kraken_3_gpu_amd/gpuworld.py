@@ -298,7 +298,8 @@ def schedule_rounds(g):
 
 
 PERSIST_INFO = ("lds_bytes", "workgroups", "resident_per_cu", "needed_per_cu", "map_words", "bitmap_words",
-                "template_words", "creature_words", "mixed_lanes", "collision_fold", "register_rounds", "fixed_rounds")
+                "template_words", "creature_words", "mixed_lanes", "collision_fold", "register_rounds", "fixed_rounds",
+                "fused_creatures")
 
 
 def set_register_slots(g, enable):

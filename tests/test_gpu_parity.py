@@ -756,6 +756,45 @@ def test_register_slots_bit_identical():
     say(f"    plain == lds == registers == fixed_lds ({len(decode_events(evp))} events)")
 
 
+def test_fused_run_time_rounds_bit_identical_uneven():
+    """Uneven morphologies (4-8 arms x 5-9 segments, ring 6-10) whose rounds
+    alternate pins and motors: the run-time kernel with their hinge and
+    muscle rounds fused (the template's own round count, descriptors read
+    from LDS) against the same kernel solving every round on its own and
+    against the plain path, at 16 and at 32 lanes per creature, events
+    included, over a rollout that reaches the walls."""
+    law = W.default_law(41)
+    steps = 120
+    for lanes in (16, 32):
+        runs = {}
+        for name, path, reg in (("plain", K.KX_PATH_PLAIN, None), ("unfused", K.KX_PATH_PERSISTENT, 0),
+                                ("fused", K.KX_PATH_PERSISTENT, 1)):
+            w = W.world_uneven(384, seed=11)
+            g = G.create(w, path=path, lanes_per_creature=lanes if reg is not None else 16)
+            if reg is not None:
+                G.set_register_slots(g, reg)
+                info = G.persist_info(g)
+                assert info["register_rounds"] == 0 and info["fixed_rounds"] == 0
+                if reg == 0:
+                    assert info["fused_creatures"] == 0
+                else:
+                    assert info["fused_creatures"] >= (384 if lanes == 32 else 100), info["fused_creatures"]
+                    say(f"    {lanes} lanes: {info['fused_creatures']} of 384 creatures in fused templates")
+            G.upload(g)
+            say(f"[case] fused run-time rounds: {name}, {lanes} lanes, {steps} steps")
+            ev = G.rollout(g, law, 0, steps, events=True)
+            G.download(g)
+            G.destroy(g)
+            runs[name] = (w, ev)
+        wp, evp = runs["plain"]
+        for name in ("unfused", "fused"):
+            wq, evq = runs[name]
+            for key in ("px", "py", "ang", "vx", "vy", "av", "broken", "lambda_", "lambdaX", "lambdaY"):
+                assert np.array_equal(getattr(wp, key), getattr(wq, key)), (lanes, name, key)
+            assert np.array_equal(evp, evq), (lanes, name)
+        say(f"    {lanes} lanes: plain == unfused == fused ({len(decode_events(evp))} events)")
+
+
 def test_folds_bit_identical_over_a_long_rollout():
     """Every persistent variant (register-held or LDS round slots, parallel or
     owner collision fold) against the plain path over 1250 steps of the
