@@ -795,6 +795,38 @@ def test_fused_run_time_rounds_bit_identical_uneven():
         say(f"    {lanes} lanes: plain == unfused == fused ({len(decode_events(evp))} events)")
 
 
+def test_inverted_joint_limits_are_not_fused():
+    """A motor whose limits are inverted (min > max) is outside what the
+    fused motor solve computes (its limit excess as two one-sided terms), so
+    its creature's template must take the one-round-at-a-time solve while the
+    other creatures keep their fused rounds: the launch reports that split,
+    and every state and event agrees bit for bit with the plain path."""
+    law = W.default_law(43)
+    steps = 120
+    runs = {}
+    for name, path in (("plain", K.KX_PATH_PLAIN), ("persistent", K.KX_PATH_PERSISTENT)):
+        w = W.world_uniform(256, seed=3)
+        k = int(w.motors[w.motor_off[0] + 2])             # a joint motor of creature 0
+        w.minA[k], w.maxA[k] = 0.05, -0.05
+        g = G.create(w, path=path)
+        if path == K.KX_PATH_PERSISTENT:
+            info = G.persist_info(g)
+            assert info["register_rounds"] == 0 and info["fixed_rounds"] == 0, info
+            assert info["fused_creatures"] == 255, info
+        G.upload(g)
+        say(f"[case] inverted limits: {name}, {steps} steps")
+        ev = G.rollout(g, law, 0, steps, events=True)
+        G.download(g)
+        G.destroy(g)
+        runs[name] = (w, ev)
+    wp, evp = runs["plain"]
+    wq, evq = runs["persistent"]
+    for key in ("px", "py", "ang", "vx", "vy", "av", "broken", "lambda_", "lambdaX", "lambdaY"):
+        assert np.array_equal(getattr(wp, key), getattr(wq, key)), key
+    assert np.array_equal(evp, evq)
+    say(f"    plain == persistent with one template unfused ({len(decode_events(evp))} events)")
+
+
 def test_folds_bit_identical_over_a_long_rollout():
     """Every persistent variant (register-held or LDS round slots, parallel or
     owner collision fold) against the plain path over 1250 steps of the
