@@ -375,15 +375,13 @@ static int tmpl_lanes_of(const kx_handle *h, size_t t) {
   return t < h->tmpl_lanes.size() && h->tmpl_lanes[t] ? h->tmpl_lanes[t] : h->G;
 }
 
-// Pack every template's schedule for its lane count G_t and map creatures
-// onto workgroups: WPB wavefronts of 64/G_t creatures of one template each.
 // Whether a packed template's rounds can run fused: an even number of at
 // most KX_FUSED_MAX_ROUNDS rounds alternating pins (even rounds) and motors
 // (odd rounds), every motor between two bodies with a finite rest angle and
 // limits that are not inverted (no min > max: the fused motor solve takes the
 // limit excess as a sum of two one-sided terms), each on the even lane of the
-// pin before it with the same two bodies (kx_pack_template places it there
-// when it can).
+// pin before it (its lane when pins take single lanes) with the same two
+// bodies (kx_pack_template places it there when it can).
 static bool fused_schedule(const KxTmpl &k, const TmplHost &th, const std::vector<uint32_t> &pool, int G) {
   if (k.R < 2 || (k.R & 1) || k.R > KX_FUSED_MAX_ROUNDS) return false;
   if (pool[(size_t)k.base + k.o_rtype] != (0xAAAAAAAAu & ((1u << k.R) - 1u))) return false;
@@ -396,11 +394,13 @@ static bool fused_schedule(const KxTmpl &k, const TmplHost &th, const std::vecto
       const int a = slot(r, l);
       if (a < 0) continue;
       const int p = slot(r - 1, l);
-      if ((l & 1) || p < 0 || (p >> 8) != (a >> 8)) return false;
+      if (((l & 1) && !k.single) || p < 0 || (p >> 8) != (a >> 8)) return false;
     }
   return true;
 }
 
+// Pack every template's schedule for its lane count G_t and map creatures
+// onto workgroups: WPB wavefronts of 64/G_t creatures of one template each.
 static int build_persist(kx_handle *h) {
   const int WPB = h->wpb;
   std::vector<uint32_t> pool;
@@ -429,7 +429,24 @@ static int build_persist(kx_handle *h) {
     if (h->tmpl_host[t].nb > KX_BODY_REGS * Gt)
       return fail(KX_ENOTSUP, "a creature of %d bodies needs at least %d lanes (at most %d bodies per lane)",
                   h->tmpl_host[t].nb, (h->tmpl_host[t].nb + KX_BODY_REGS - 1) / KX_BODY_REGS, KX_BODY_REGS);
-    KxTmpl k = kx_pack_template(h->tmpl_host[t], Gt, pool);
+    // The template's rounds, fused when the schedule allows: pins on lane
+    // pairs first (16 or 32 lanes; the kernels specialised to the default
+    // creature take only that form), else on single lanes, which hold twice
+    // the pins per round (an 8-lane schedule then alternates too).
+    const TmplHost &th = h->tmpl_host[t];
+    std::vector<uint32_t> scr;
+    KxTmpl k = kx_pack_template(th, Gt, scr, false);
+    k.fused = 0;
+    if (h->reg_mode && (Gt == 16 || Gt == 32) && fused_schedule(k, th, scr, Gt)) {
+      k.fused = 1;
+    } else if (h->reg_mode && (Gt == 8 || Gt == 16 || Gt == 32)) {
+      std::vector<uint32_t> scr1;
+      KxTmpl k1 = kx_pack_template(th, Gt, scr1, true);
+      if (fused_schedule(k1, th, scr1, Gt)) { k = k1; k.fused = 1; scr.swap(scr1); }
+    }
+    while (pool.size() % 4) pool.push_back(0u);
+    k.base = (int)pool.size();
+    pool.insert(pool.end(), scr.begin(), scr.end());
     // creature regions: 16-byte records, regions offset by 16 bytes modulo
     // 128 so neighbouring creatures' same records fall in different banks
     k.cr_stride = (kx_persist_cr_words(k.nb, k.nc, k.np, k.ngrip, k.narms, par) + 31) / 32 * 32 + 4;
@@ -504,18 +521,15 @@ static int build_persist(kx_handle *h) {
   // lane. With it, register-held round slots when every workgroup is resident
   // at once at one wavefront per SIMD (the slots take the register file, so a
   // SIMD holds no second wave), else the slots read from LDS. Without it, the
-  // generic kernels solve each fused template's rounds fused (KxTmpl::fused)
-  // at 16 or 32 lanes and every other template's rounds one at a time.
+  // generic kernels solve each fused template's rounds fused (KxTmpl::fused:
+  // pins on lane pairs at 16 or 32 lanes, or on single lanes at 8, 16 or 32)
+  // and every other template's rounds one at a time.
   int reg = 0, fixed = 0;
-  for (size_t t = 0; t < T.size(); t++) {
-    const int Gt = tmpl_lanes_of(h, t);
-    T[t].fused = h->reg_mode && (Gt == 16 || Gt == 32) && fused_schedule(T[t], h->tmpl_host[t], pool, Gt) ? 1 : 0;
-  }
   if (h->reg_mode && !mixed && Gk == 16) {
     bool all = true;
     for (size_t t = 0; t < T.size(); t++) {
       if (members[t].empty()) continue;
-      if (!T[t].fused || T[t].R != KX_REG_ROUNDS || T[t].nb > KX_REG_BODY_REGS * 16) all = false;
+      if (!T[t].fused || T[t].single || T[t].R != KX_REG_ROUNDS || T[t].nb > KX_REG_BODY_REGS * 16) all = false;
       if (all) {
         const uint32_t *desc = pool.data() + T[t].base + T[t].o_sched;
         auto slot = [&](int r, int lane) { return (int)desc[((r >> 2) * 16 + lane) * 4 + (r & 3)]; };

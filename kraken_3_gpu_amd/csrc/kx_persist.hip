@@ -313,8 +313,10 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
   // the pin leaves (its own pose if the pin does not commit), which is the
   // pose the motor's round would read; the even lane writes the pair's two
   // bodies once. pinB: every pin of the round has a second body (a
-  // constant of the fixed schedule).
-  [[maybe_unused]] auto fused_pair = [&](int dc, int dm, bool pinB) {
+  // constant of the fixed schedule). single (a template packed with each pin
+  // on one lane): every lane takes both sin/cos itself and commits its own
+  // pin and motor; the values are those of the pair form.
+  [[maybe_unused]] auto fused_pair = [&](int dc, int dm, bool pinB, bool single) {
     const KxSlot c = kx_slot_read(dc, dyn, lam2, t_crec, t_brec);
     const KxSlot m = kx_slot_read(dm, dyn, lam2, t_crec, t_brec);
     const kx_f3v pA = *(const kx_f3v *)&pos[c.A], pB = *(const kx_f3v *)&pos[c.B];
@@ -322,10 +324,16 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
     const bool livep = c.live && __float_as_int(c.dk.y) == 0;
     const bool livem = m.live && __float_as_int(m.dk.z) == 0;
     const bool hasB = pinB || c.hasB;
-    const bool odd = (sub & 1) != 0;
-    float s1, c1;
-    kx_sincos(odd ? pB.z : pA.z, &s1, &c1);
-    const float sa = kx_pair_even(s1), ca = kx_pair_even(c1), sb = kx_pair_odd(s1), cb = kx_pair_odd(c1);
+    const bool odd = !single && (sub & 1) != 0;
+    float sa, ca, sb, cb;
+    if (single) {
+      kx_sincos(pA.z, &sa, &ca);
+      kx_sincos(pB.z, &sb, &cb);
+    } else {
+      float s1, c1;
+      kx_sincos(odd ? pB.z : pA.z, &s1, &c1);
+      sa = kx_pair_even(s1); ca = kx_pair_even(c1); sb = kx_pair_odd(s1); cb = kx_pair_odd(c1);
+    }
     float xA = pA.x, yA = pA.y, tA = pA.z, xB = pB.x, yB = pB.y, tB = pB.z;
     float lx = c.lk.x, ly = c.lk.y;
     const bool okp = kx_pin_core_sc(xA, yA, tA, c.bA.x, c.bA.y, xB, yB, tB, c.bB.x, c.bB.y, hasB, c.cr.x, c.cr.y,
@@ -522,18 +530,20 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
           // SIMD, whose register file holds no slots)
           static_assert(KX_REG_RTYPE == 0xAAA && RR == 12, "fused rounds: pins on even rounds, motors on odd");
 #pragma unroll
-          for (int f = 0; f < RR / 2; f++) fused_pair(fdsc[2 * f], fdsc[2 * f + 1], ((KX_REG_PINB >> f) & 1) != 0);
+          for (int f = 0; f < RR / 2; f++)
+            fused_pair(fdsc[2 * f], fdsc[2 * f + 1], ((KX_REG_PINB >> f) & 1) != 0, false);
         } else {
         if (T.fused) {
           // a template the host marked fused (its rounds alternate pins and
-          // motors, each motor on the even lane of the pin before it with
-          // the same two bodies): the fixed schedule's fused rounds with the
+          // motors, each motor on the lane of the pin before it with the
+          // same two bodies): the fixed schedule's fused rounds with the
           // template's own round count; the slot descriptors of two fused
           // rounds come in one 16-byte LDS read
+          const bool single = T.single != 0;
           for (int g = 0; 4 * g < T.R; g++) {
             const int4 d4 = act ? t_sched4[g * G + sub] : make_int4(-1, -1, -1, -1);
-            fused_pair(d4.x, d4.y, false);
-            if (4 * g + 2 < T.R) fused_pair(d4.z, d4.w, false);
+            fused_pair(d4.x, d4.y, false, single);
+            if (4 * g + 2 < T.R) fused_pair(d4.z, d4.w, false, single);
           }
         } else {
         int4 d4n = act ? t_sched4[sub] : make_int4(-1, -1, -1, -1);

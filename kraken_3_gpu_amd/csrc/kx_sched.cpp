@@ -24,7 +24,8 @@
 // pair of adjacent lanes (one sin/cos per lane), see kx_pack_template.
 int kx_pin_capacity(int G) { return G >= 2 ? G / 2 : 1; }
 
-void kx_schedule_rounds(const TmplHost &t, int G, std::vector<int> &sched, std::vector<int> &rtype, int &R) {
+void kx_schedule_rounds(const TmplHost &t, int G, std::vector<int> &sched, std::vector<int> &rtype, int &R,
+                        bool single) {
   const int nc = (int)t.type.size();
   std::vector<std::vector<int>> succ(nc);
   std::vector<int> npred(nc, 0), last(t.nb, -1);
@@ -60,7 +61,7 @@ void kx_schedule_rounds(const TmplHost &t, int G, std::vector<int> &sched, std::
     std::vector<int> cand;
     for (int k : ready) if ((t.type[k] == KX_ANGLE ? 1 : 0) == best_cls) cand.push_back(k);
     std::stable_sort(cand.begin(), cand.end(), [&](int x, int y) { return prio[x] > prio[y]; });
-    const int cap = best_cls == 0 ? kx_pin_capacity(G) : G;
+    const int cap = best_cls == 0 && !single ? kx_pin_capacity(G) : G;
     if ((int)cand.size() > cap) cand.resize(cap);
     std::vector<int> row(G, -1);
     for (int j = 0; j < (int)cand.size(); j++) row[j] = cand[j];
@@ -116,12 +117,13 @@ static uint32_t fbits(float x) {
 //             (angle), cab = a | b << 8 | type << 16            (16 B)
 //   brec[b] = {im, ii, point offset | count << 16, contact radius}
 //   prec[q] = {lx, ly, r, body}
-KxTmpl kx_pack_template(const TmplHost &t, int G, std::vector<uint32_t> &pool) {
+KxTmpl kx_pack_template(const TmplHost &t, int G, std::vector<uint32_t> &pool, bool single) {
   KxTmpl T{};
   T.nb = t.nb; T.nc = (int)t.type.size(); T.np = (int)t.plx.size();
   T.narms = (int)t.arm_tip.size(); T.nmot = (int)t.mot_k.size(); T.ngrip = (int)t.grip_k.size();
   std::vector<int> sched, rtype, lists, counts;
-  kx_schedule_rounds(t, G, sched, rtype, T.R);
+  kx_schedule_rounds(t, G, sched, rtype, T.R, single);
+  T.single = single ? 1 : 0;
   kx_schedule_collisions(t, G, lists, counts, T.maxbl);
   while (pool.size() % 4) pool.push_back(0u);
   T.base = (int)pool.size();
@@ -163,14 +165,15 @@ KxTmpl kx_pack_template(const TmplHost &t, int G, std::vector<uint32_t> &pool) {
     pool.insert(pool.end(), {fbits(t.plx[q]), fbits(t.ply[q]), fbits(t.pr[q]), (uint32_t)pbody[q]});
   // round slots as packed descriptors: k | A << 8 | B << 16 (B = 255: world
   // anchor), -1 for an idle lane, so one LDS read yields every index; a pin
-  // round's j-th constraint occupies lanes 2j and 2j+1. Stored by groups of
-  // four rounds, lane-major within a group: desc4[(grp * G + lane) * 4 + r % 4]
-  // so a lane reads a group's four slots with one 16-byte load.
+  // round's j-th constraint occupies lanes 2j and 2j+1 (lane j when single).
+  // Stored by groups of four rounds, lane-major within a group:
+  // desc4[(grp * G + lane) * 4 + r % 4], so a lane reads a group's four
+  // slots with one 16-byte load.
   // A motor whose two bodies are those of a pin in the round before (a
-  // joint's hinge and its muscle) sits on that pin's even lane, the others on
-  // the lowest free lanes; lanes within a round are interchangeable, and this
-  // placement lets a kernel solve the motor right after the pin on the pose
-  // it holds (the register-slot kernel, kx_persist.hip).
+  // joint's hinge and its muscle) sits on that pin's even lane (its lane when
+  // single), the others on the lowest free lanes; lanes within a round are
+  // interchangeable, and this placement lets a kernel solve the motor right
+  // after the pin on the pose it holds (the fused rounds, kx_persist.hip).
   const int R4 = (T.R + 3) / 4;
   std::vector<int> desc((size_t)R4 * G * 4, -1);
   auto put_slot = [&](int r, int lane, int v) { desc[((size_t)(r >> 2) * G + lane) * 4 + (r & 3)] = v; };
@@ -180,7 +183,9 @@ KxTmpl kx_pack_template(const TmplHost &t, int G, std::vector<uint32_t> &pool) {
     if (!rtype[r]) {
       for (int j = 0; j < G; j++) {
         const int k = sched[(size_t)r * G + j];
-        if (k >= 0) { put_slot(r, 2 * j, pack(k)); put_slot(r, 2 * j + 1, pack(k)); }
+        if (k < 0) continue;
+        if (single) { put_slot(r, j, pack(k)); continue; }
+        put_slot(r, 2 * j, pack(k)); put_slot(r, 2 * j + 1, pack(k));
       }
       continue;
     }
@@ -191,7 +196,7 @@ KxTmpl kx_pack_template(const TmplHost &t, int G, std::vector<uint32_t> &pool) {
       if (k < 0) continue;
       int lane = -1;
       if (r > 0 && !rtype[r - 1] && t.b[k] >= 0)
-        for (int l = 0; l < G; l += 2) {
+        for (int l = 0; l < G; l += single ? 1 : 2) {
           const int p = slot_of(r - 1, l);
           if (p >= 0 && !used[l] && (p >> 8) == (pack(k) >> 8)) { lane = l; break; }
         }

@@ -18,6 +18,7 @@ struct KxTmpl {
   int o_sched, o_rtype, o_coll, o_ccnt, o_arm_tip, o_arm_grip, o_mot_k, o_mot_slot, o_grip_k, o_grip_arm;
   int o_arm_gslot;                // per arm: the slot of its grip in the grip list
   int fused;                      // 1: the generic kernels solve its rounds fused (set by the host)
+  int single;                     // 1: each pin or grip on one lane (only with fused)
 };
 
 // Host form of a template, in creature-local indices (constraints in the
@@ -35,7 +36,9 @@ struct TmplHost {
 };
 
 int kx_pin_capacity(int G);
-void kx_schedule_rounds(const TmplHost &t, int G, std::vector<int> &sched, std::vector<int> &rtype, int &R);
+// single: a pin or grip takes one lane (G per round) instead of a lane pair
+void kx_schedule_rounds(const TmplHost &t, int G, std::vector<int> &sched, std::vector<int> &rtype, int &R,
+                        bool single = false);
 void kx_schedule_collisions(const TmplHost &t, int G, std::vector<int> &lists, std::vector<int> &counts, int &maxbl);
-KxTmpl kx_pack_template(const TmplHost &t, int G, std::vector<uint32_t> &pool);
+KxTmpl kx_pack_template(const TmplHost &t, int G, std::vector<uint32_t> &pool, bool single = false);
 std::string kx_template_key(const TmplHost &t);

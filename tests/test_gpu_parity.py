@@ -760,18 +760,23 @@ def test_fused_run_time_rounds_bit_identical_uneven():
     """Uneven morphologies (4-8 arms x 5-9 segments, ring 6-10) whose rounds
     alternate pins and motors: the run-time kernel with their hinge and
     muscle rounds fused (the template's own round count, descriptors read
-    from LDS) against the same kernel solving every round on its own and
-    against the plain path, at 16 and at 32 lanes per creature, events
-    included, over a rollout that reaches the walls."""
+    from LDS; pins on lane pairs, or on single lanes where that lets the
+    schedule alternate, as at 8 lanes) against the same kernel solving every
+    round on its own and against the plain path, at 8 (with 16 for the
+    largest shapes), 16 and 32 lanes per creature, events included, over a
+    rollout that reaches the walls."""
     law = W.default_law(41)
     steps = 120
-    for lanes in (16, 32):
+    for lanes in (8, 16, 32):
         runs = {}
         for name, path, reg in (("plain", K.KX_PATH_PLAIN, None), ("unfused", K.KX_PATH_PERSISTENT, 0),
                                 ("fused", K.KX_PATH_PERSISTENT, 1)):
             w = W.world_uneven(384, seed=11)
-            g = G.create(w, path=path, lanes_per_creature=lanes if reg is not None else 16)
+            g = G.create(w, path=path, lanes_per_creature=max(lanes, 16) if reg is not None else 16)
             if reg is not None:
+                if lanes == 8:
+                    # 8 lanes hold at most 64 bodies: the largest shapes stay at 16
+                    G.set_template_lanes(g, [8 if t["bodies"] <= 64 else 16 for t in G.template_info(g)])
                 G.set_register_slots(g, reg)
                 info = G.persist_info(g)
                 assert info["register_rounds"] == 0 and info["fixed_rounds"] == 0
