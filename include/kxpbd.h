@@ -227,8 +227,10 @@ int kx_schedule_info(const kx_handle *h, int32_t *rounds, int32_t max);
  * the fixed schedule run with its slots read from LDS (0: not that kernel),
  * [12] creatures whose template the run-time kernels solve in fused
  * pin/motor rounds (templates whose rounds alternate pins and motors, each
- * motor on the even lane of the pin before it with the same two bodies, at
- * 16 or 32 lanes; 0 with kx_set_register_slots mode 0 or under [10]/[11]).
+ * motor on the lane of the pin before it with the same two bodies: pins on
+ * lane pairs at 16 or 32 lanes, or on single lanes at 8, 16 or 32 where
+ * pairs do not alternate; 0 with kx_set_register_slots mode 0 or under
+ * [10]/[11]).
  * Returns the number of fields (13). No counterpart in the reference (its
  * path has no persistent kernel). */
 int kx_persist_info(const kx_handle *h, int32_t *info, int32_t max);
