@@ -408,7 +408,10 @@ int kx_server::build(Req *r) {
   kx_builder_cons_init(B, lbx.data(), lby.data(), comp.data(), tgt.data(), bf.data(), broken.data());
   kx_params p = o.params;
   p.path = KX_PATH_PERSISTENT;
-  p.lanes_per_creature = 16;
+  // lanes per creature as the bucket tuner settles them on the uneven batch
+  // of the same morphologies: 8 for the smallest (at most 68 constraints,
+  // whose rounds then fuse with a pin on each lane), 16 for the rest
+  p.lanes_per_creature = nc <= 68 * std::max(r->ncr, 1) ? 8 : 16;
   p.waves_per_block = 1;
   rc = kx_create(&desc, &cr, &p, &r->h);
   kx_builder_free(B);
