@@ -532,8 +532,7 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
 #pragma unroll
           for (int f = 0; f < RR / 2; f++)
             fused_pair(fdsc[2 * f], fdsc[2 * f + 1], ((KX_REG_PINB >> f) & 1) != 0, false);
-        } else {
-        if (T.fused) {
+        } else if (T.fused) {
           // a template the host marked fused (its rounds alternate pins and
           // motors, each motor on the lane of the pin before it with the
           // same two bodies): the fixed schedule's fused rounds with the
@@ -604,7 +603,6 @@ __device__ __forceinline__ void kx_persist_body(const KxDev &d, const KxLaw &L, 
           round(sa, sb, d4.w, (rbits >> ((r0 + 2) & 31)) & 1u);
           if (r0 + 3 >= T.R) break;
           round(sb, sa, d4n.x, (rbits >> ((r0 + 3) & 31)) & 1u);
-        }
         }
         }
         // collisions (xpbd.js:441-470: per body, its points in order, each
