@@ -145,11 +145,15 @@ int kx_constraint_order(const kx_handle *h, int32_t *order, int32_t *color);
 /* colour offsets/counts in handle order (colorInfo, gpuworld.js:286-297) */
 int kx_color_info(const kx_handle *h, uint32_t *off16, uint32_t *cnt16);
 /* pure host planning of the persistent path's constraint rounds for one
- * creature (local body indices, constraints in colour order): rounds of at
- * most G constraints, all pin/grip (rtype 0) or all angle (rtype 1), no two
- * sharing a body, every pair that shares a body kept in colour order.
- * sched_out[R*G] holds constraint indices or -1; returns R (rounds), or a
- * negative code if max_rounds is too small. */
+ * creature (local body indices, constraints in colour order) at G lanes:
+ * rounds all pin/grip (rtype 0) or all angle (rtype 1), no two constraints
+ * sharing a body, every pair that shares a body kept in colour order; an
+ * angle round holds at most G constraints, a pin round G / 2 (a lane pair
+ * each), or G with KX_PLAN_SINGLE_LANE or-ed into G (one lane each, the form
+ * the fused rounds take where lane pairs do not alternate). sched_out[R*G]
+ * holds constraint indices or -1; returns R (rounds), or a negative code if
+ * max_rounds is too small. */
+#define KX_PLAN_SINGLE_LANE 0x100
 int kx_plan_rounds(int32_t n_bodies, int32_t n_constraints, const int32_t *c_a, const int32_t *c_b,
                    const int32_t *c_type, int32_t G, int32_t *sched_out, int32_t *rtype_out, int32_t max_rounds);
 /* pure host colouring, what kx_create applies (xpbd.js:98-126): colours the

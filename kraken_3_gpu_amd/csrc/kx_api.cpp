@@ -179,6 +179,8 @@ static int colour_constraints(int nB, int nC, const int32_t *ca, const int32_t *
 
 extern "C" int kx_plan_rounds(int32_t nb, int32_t nc, const int32_t *ca, const int32_t *cb, const int32_t *ct,
                               int32_t G, int32_t *sched_out, int32_t *rtype_out, int32_t max_rounds) {
+  const bool single = (G & KX_PLAN_SINGLE_LANE) != 0;
+  G &= ~KX_PLAN_SINGLE_LANE;
   if (nb < 1 || nc < 0 || G < 1 || G > 64) return fail(KX_EINVAL, "bad arguments");
   TmplHost t;
   t.nb = nb;
@@ -188,7 +190,7 @@ extern "C" int kx_plan_rounds(int32_t nb, int32_t nc, const int32_t *ca, const i
   }
   std::vector<int> sched, rtype;
   int R = 0;
-  kx_schedule_rounds(t, G, sched, rtype, R);
+  kx_schedule_rounds(t, G, sched, rtype, R, single);
   if (R > max_rounds) return fail(KX_EINVAL, "needs %d rounds > %d", R, max_rounds);
   for (size_t i = 0; i < sched.size(); i++) sched_out[i] = sched[i];
   for (int r = 0; r < R; r++) rtype_out[r] = rtype[r];

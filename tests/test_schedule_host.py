@@ -11,11 +11,15 @@ from kraken_3_gpu_amd import world as W
 from oracle import pyoracle as po
 
 
-def plan(nb, a, b, t, G):
+KX_PLAN_SINGLE_LANE = 0x100
+
+
+def plan(nb, a, b, t, G, single=False):
     a, b, t = (np.ascontiguousarray(x, dtype=np.int32) for x in (a, b, t))
     sched = np.zeros(4096 * G, dtype=np.int32)
     rtype = np.zeros(4096, dtype=np.int32)
-    R = K.check(K.lib().kx_plan_rounds(nb, len(a), K.ptr(a, C.c_int32), K.ptr(b, C.c_int32), K.ptr(t, C.c_int32), G,
+    g = G | (KX_PLAN_SINGLE_LANE if single else 0)
+    R = K.check(K.lib().kx_plan_rounds(nb, len(a), K.ptr(a, C.c_int32), K.ptr(b, C.c_int32), K.ptr(t, C.c_int32), g,
                                        K.ptr(sched, C.c_int32), K.ptr(rtype, C.c_int32), 4096))
     return sched[: R * G].reshape(R, G), rtype[:R]
 
@@ -28,11 +32,16 @@ def creature_cons(arms=6, segs=7, ring=8):
     return o.n, c["a"], np.where(c["b"] >= 0, c["b"], -1), c["type"]
 
 
+@pytest.mark.parametrize("single", [False, True])
 @pytest.mark.parametrize("G", [8, 16, 32, 64])
 @pytest.mark.parametrize("morph", [(6, 7, 8), (4, 5, 6), (8, 9, 10), (5, 8, 7)])
-def test_rounds_are_valid_topological_antichains(G, morph):
+def test_rounds_are_valid_topological_antichains(G, morph, single):
     nb, a, b, t = creature_cons(*morph)
-    sched, rtype = plan(nb, a, b, t, G)
+    sched, rtype = plan(nb, a, b, t, G, single)
+    # a pin round holds a lane pair per pin, or one lane per pin when single
+    for r, row in enumerate(sched):
+        if not rtype[r]:
+            assert (row >= 0).sum() <= (G if single else G // 2)
     flat = sched[sched >= 0]
     assert sorted(flat.tolist()) == list(range(len(a)))
     where = np.empty(len(a), dtype=np.int64)
@@ -87,3 +96,17 @@ def test_schedule_order_is_bit_identical_to_colour_order(G):
     for key in ("px", "py", "ang", "vx", "vy", "av"):
         assert np.array_equal(br[key], ba[key]), key
     assert sorted(ref.events()) == sorted(alt.events())
+
+
+@pytest.mark.parametrize("morph", [(4, 5, 6), (5, 6, 7), (6, 7, 8)])
+def test_single_lane_pins_alternate_at_8_lanes(morph):
+    """With a pin on each lane an 8-lane round holds eight pins, so these
+    shapes get 2 x arms rounds alternating pins and motors (what the fused
+    rounds need), where lane pairs need extra pin rounds."""
+    nb, a, b, t = creature_cons(*morph)
+    _, rt_pair = plan(nb, a, b, t, 8)
+    _, rt_single = plan(nb, a, b, t, 8, single=True)
+    arms = morph[0]
+    assert len(rt_single) == 2 * arms
+    assert all(rt_single[r] == r % 2 for r in range(len(rt_single)))
+    assert len(rt_pair) > 2 * arms
